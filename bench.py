@@ -1,0 +1,181 @@
+"""Benchmark: MPIR solves/s at 1e-10 relative residual on the 512 x 512 x 16
+board, Q1 apply GDoF/s, per-solve latency, the roofline of the Q1 apply and
+the reference algorithm timed on the host's cores.
+
+One step is one complete MPIR solve (FP64 outer residual, update and
+convergence check; FP32 Jacobi-PCG inner solves) of the seeded 512 x 512 x 16
+copper/FR-4 board with its right-hand side already resident in HBM.  Prints
+one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT / "pcb-analysis_amd" / "src", ROOT):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "board 512x512x16 elements (4,473,873 nodes), seeded copper/FR-4, 8 sources"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def apply_roofline(operator, torch, launches: int = 200) -> dict:
+    """Time the FP32 Q1 apply with HIP events on the stream it runs on."""
+
+    n = operator.size
+    s, r, c = operator.mesh.element_grid_shape
+    e = s * r * c
+    dev = operator.device_operator
+    stream = torch.cuda.current_stream()
+    x = torch.randn(n, dtype=torch.float32, device="cuda", generator=torch.Generator("cuda").manual_seed(0))
+    y = torch.empty_like(x)
+    for _ in range(5):
+        dev.apply_f32_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(launches):
+        dev.apply_f32_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    per_launch_s = start.elapsed_time(stop) / 1e3 / launches
+    algorithmic = 13 * n + 12 * e  # x, y, robin (4 B/node each), free mask (1 B/node), coefficients (12 B/element)
+    achieved = algorithmic / per_launch_s / 1e9
+    return dict(
+        kernel="k_apply<float,8,0> (FP32 Q1 apply)",
+        per_launch_us=per_launch_s * 1e6,
+        gdof_s=n / per_launch_s / 1e9,
+        roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                      traffic=None, algorithmic_bytes=algorithmic),
+    )
+
+
+def cpu_baseline(problem, gpu_result, gpu_solve_s: float) -> dict:
+    """The reference algorithm (oracle C restatement of hex_q1.cpp's FP32
+    Jacobi PCG) on the host cores: a bounded sample of inner iterations on the
+    same board, scaled by the GPU solve's own iteration counts."""
+
+    from oracle import problem as oracle_problem
+    from oracle.native import OracleOperator, threads_default
+
+    ops = oracle_problem.operator_arrays(oracle_problem.from_problem(problem))
+    op = OracleOperator(ops)
+    threads = threads_default()
+    rhs = np.random.default_rng(0).standard_normal(op.size)
+    its = 20
+    op.pcg_f32(rhs, 1e-30, 2, threads=threads)  # warm-up
+    t0 = time.perf_counter()
+    _, done, _, _ = op.pcg_f32(rhs, 1e-30, its, threads=threads)
+    per_iter = (time.perf_counter() - t0) / max(done, 1)
+    x = rhs.copy()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        op.apply_f64(x, threads=threads)
+    per_f64 = (time.perf_counter() - t0) / 3
+    inner = gpu_result["inner_iterations"]
+    outer = gpu_result["high_operator_applications"]
+    solve_s = inner * per_iter + outer * per_f64
+    return dict(value=1.0 / solve_s, unit="solves/s", cores=threads, kind="port",
+                sample=f"{its} FP32 Jacobi-PCG iterations + 3 FP64 applies of the same board on {threads} host "
+                       f"threads (oracle C restatement of hex_q1.cpp), scaled by the GPU solve's {inner} inner / "
+                       f"{outer} outer steps: {per_iter * 1e3:.2f} ms/iteration, est. {solve_s:.1f} s/solve")
+
+
+def main() -> int:
+    parser = argparse.ArgumentParser()
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=3)
+    parser.add_argument("--warmup", type=int, default=1)
+    parser.add_argument("--size", type=int, default=512)
+    parser.add_argument("--slabs", type=int, default=16)
+    parser.add_argument("--no-cpu", action="store_true")
+    args = parser.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only"}))
+        return 2
+
+    import torch
+
+    from electrical.matrix_free_mpir_fem import MPIRConfig
+    from pcbhip import _capi, workloads
+    from thermal.matrix_free_mpir_fem import MatrixFreeThermalOperator
+
+    _capi.require_device(0)
+    torch.cuda.init()
+    log("building board")
+    problem = workloads.board_512x512x16(size=args.size, slabs=args.slabs)
+    operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    reference = operator.default_reference_temperature()
+    rhs = operator.build_rhs(reference)
+    config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000)
+    batch = _capi.Batch([operator.device_operator])
+    batch.set(0, rhs, None, config.to_c())
+
+    log("warmup")
+    for _ in range(args.warmup):
+        batch.run()
+    latencies = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        batch.run()
+        latencies.append(time.perf_counter() - s0)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    x, result = batch.result(0)
+    log(f"solve: {result['outer_iterations']} outer / {result['inner_iterations']} inner, "
+        f"rel {result['relative_residual']:.2e}, {elapsed / args.steps * 1e3:.1f} ms/solve")
+
+    roof = apply_roofline(operator, torch)
+    log(f"apply: {roof['per_launch_us']:.1f} us, {roof['roofline']['achieved']:.0f} GB/s")
+    line = {
+        "metric": "MPIR solves/s at 1e-10 rel. residual (512x512x16 board)",
+        "value": args.steps / elapsed,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 outer",
+        "data": "synthetic (seeded board generator)",
+        "config": {
+            "workload": WORKLOAD,
+            "solver": "MPIR, FP32 Jacobi-PCG inner (rtol 2e-3), FP64 outer to 1e-10",
+            "converged": result["converged"],
+            "relative_residual": result["relative_residual"],
+            "outer_iterations": result["outer_iterations"],
+            "inner_iterations": result["inner_iterations"],
+            "latency_p50_ms": float(np.percentile(latencies, 50) * 1e3),
+            "latency_p99_ms": float(np.percentile(latencies, 99) * 1e3),
+            "apply_gdof_s": roof["gdof_s"],
+            "apply_us": roof["per_launch_us"],
+            "parallelism": "single GPU",
+        },
+        "roofline": roof["roofline"],
+    }
+    if not args.no_cpu:
+        log("cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(problem, result, elapsed / args.steps)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,135 @@
+/*
+ * pcbhip -- C ABI of the MI355X (gfx950) matrix-free hexahedral Q1
+ * conduction solver: the thermal steady-conduction and hex DC-conduction
+ * operator, its Jacobi diagonal and its mixed-precision iterative-refinement
+ * (MPIR) Jacobi-PCG solve.
+ *
+ * Everything is extern "C" with plain pointers and sizes.  Handles are
+ * opaque.  Every function returns PCBHIP_OK or an error code; the message of
+ * the last error on the calling thread is pcbhip_last_error().  The caller
+ * owns every host buffer for the duration of a call only; a handle owns its
+ * device copies.  Functions ending in _dev take device pointers and a HIP
+ * stream (NULL: the handle's own stream) and do not synchronise.
+ *
+ * Node layout is the reference's flat C order ((z * (rows + 1) + y) *
+ * (cols + 1) + x), elements ((ez * rows + ey) * cols + ex), coefficients
+ * (3, slabs, rows, cols) = (a_x, a_y, a_z) with a_x = k_in h_y h_z / h_x,
+ * a_y = k_in h_x h_z / h_y, a_z = k_z h_x h_y / h_z
+ * (reference: src/thermal/matrix_free_mpir_fem/mesh.py:276-293).
+ */
+#ifndef PCBHIP_H
+#define PCBHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pcbhip_status {
+    PCBHIP_OK = 0,
+    PCBHIP_INVALID_INPUT = 1, /* ValueError in the Python layer */
+    PCBHIP_RUNTIME = 2,       /* RuntimeError: HIP failure, not SPD */
+    PCBHIP_NON_FINITE = 3,    /* FloatingPointError */
+    PCBHIP_NO_MEMORY = 4      /* MemoryError */
+};
+
+typedef struct pcbhip_operator pcbhip_operator;
+typedef struct pcbhip_batch pcbhip_batch;
+
+/* Replaces cpp/include/pcbcore/fem/mpir.hpp:11-17 (MpirConfig). */
+typedef struct pcbhip_mpir_config {
+    double relative_tolerance;       /* 1e-10 */
+    double absolute_tolerance;       /* 0 */
+    double inner_relative_tolerance; /* 2e-3 */
+    int max_outer_iterations;        /* 8 (16 for the thermal solve) */
+    int max_inner_iterations;        /* 200 */
+} pcbhip_mpir_config;
+
+/* Replaces cpp/include/pcbcore/fem/mpir.hpp:27-35 (MpirResult); the history
+ * rows go to a separate caller buffer of 4 doubles per outer step:
+ * (outer step, FP64 relative residual before it, inner iterations, inner
+ * relative residual), as cpp/bindings/mpir_convert.hpp:28-44 returns them. */
+typedef struct pcbhip_mpir_result {
+    int converged;
+    int outer_iterations;
+    int inner_iterations;
+    int high_operator_applications;
+    int low_operator_applications;
+    int not_spd;
+    double relative_residual;
+} pcbhip_mpir_result;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *pcbhip_last_error(void);
+
+/* Number of visible HIP devices (replaces cpp/bindings/cuda_module.cpp:90-93,
+ * device_count). */
+int pcbhip_device_count(int *count);
+
+/* A prepared operator resident on ``device``: replaces the reference's
+ * ThermalSystem construction (cpp/src/thermal/thermal_system.cpp:252-363)
+ * followed by the device upload of DeviceSystem::thermal
+ * (cpp/bindings/cuda_module.cpp:52-62, cpp/cuda/inner_pcg.cu:619-640).
+ *   coef        (3, slabs, rows, cols) FP64 element coefficients
+ *   robin       (nodes) FP64 lumped Robin conductance plus C/dt
+ *   free_nodes  (nodes) 1 free, 0 fixed (fixed rows are the identity)
+ * The Jacobi diagonal is formed on the device; a free node without positive
+ * conductance is PCBHIP_INVALID_INPUT. */
+int pcbhip_operator_create(int device, int slabs, int rows, int cols, const double *coef, const double *robin,
+                           const uint8_t *free_nodes, pcbhip_operator **out);
+int pcbhip_operator_destroy(pcbhip_operator *op);
+/* Node count of the operator. */
+int64_t pcbhip_operator_size(const pcbhip_operator *op);
+/* FP64 Jacobi diagonal (replaces operator.py:289-306 / thermal_system.cpp:329-349). */
+int pcbhip_operator_diagonal(const pcbhip_operator *op, double *diag);
+
+/* y = A x with fixed rows the identity and Robin on the diagonal, FP32
+ * (replaces hex_q1.cpp:131-193 apply_lines<float>, pcbcore::fem::thermal_hex::
+ * apply_low, thermal_hex.hpp:33) and FP64 (apply_high, thermal_hex.hpp:32). */
+int pcbhip_apply_f32(const pcbhip_operator *op, const float *x, float *y);
+int pcbhip_apply_f64(const pcbhip_operator *op, const double *x, double *y);
+/* y = K x, the unconstrained stiffness with no Robin term
+ * (replaces ThermalSystem::stiffness, thermal_system.cpp:381-391). */
+int pcbhip_stiffness_f64(const pcbhip_operator *op, const double *x, double *y);
+int pcbhip_apply_f32_dev(const pcbhip_operator *op, const float *x, float *y, void *stream);
+int pcbhip_apply_f64_dev(const pcbhip_operator *op, const double *x, double *y, void *stream);
+int pcbhip_stiffness_f64_dev(const pcbhip_operator *op, const double *x, double *y, void *stream);
+
+/* One MPIR solve: replaces thermal_hex::solve_mpir (hex_q1.cpp:837-925,
+ * thermal_hex.hpp:37-39) and DeviceSystem.solve (cuda_module.cpp:64-86).
+ * x holds the initial guess on entry (zeros for none) and the solution on
+ * return.  inner_fp64 = 0 is the FP32 inner Jacobi PCG; 1 runs the inner PCG
+ * in FP64 (a plain FP64 Jacobi PCG when inner_relative_tolerance equals the
+ * outer tolerance).  history: NULL or 4 * max_outer_iterations doubles.
+ * A non-SPD operator returns PCBHIP_RUNTIME with result->not_spd = 1. */
+int pcbhip_solve_mpir(const pcbhip_operator *op, const double *rhs, double *x, const pcbhip_mpir_config *config,
+                      int inner_fp64, pcbhip_mpir_result *result, double *history);
+
+/* Many right-hand sides of possibly different operators (of possibly
+ * different sizes) solved together: every kernel launch covers every slot
+ * still iterating and each slot's arithmetic does not depend on the others.
+ * No reference counterpart beyond the per-solve contract above: the
+ * reference solves these one at a time (cpp/src/fem/dc_ports.cpp:26-49).
+ * ops[i] may repeat (many right-hand sides of one operator). */
+int pcbhip_batch_create(int count, const pcbhip_operator *const *ops, int inner_fp64, pcbhip_batch **out);
+int pcbhip_batch_destroy(pcbhip_batch *batch);
+/* Host rhs and initial guess (NULL: zero) and the slot's limits. */
+int pcbhip_batch_set(pcbhip_batch *batch, int slot, const double *rhs, const double *x0,
+                     const pcbhip_mpir_config *config);
+/* Device pointers to a slot's FP64 right-hand side (write it before run). */
+int pcbhip_batch_rhs_dev(pcbhip_batch *batch, int slot, double **rhs_dev);
+/* Reset every slot to its initial guess and iterate until every slot has
+ * converged, reached its outer limit or failed. */
+int pcbhip_batch_run(pcbhip_batch *batch);
+/* Solution (host, may be NULL), result and history (NULL or 4 * max_outer). */
+int pcbhip_batch_result(pcbhip_batch *batch, int slot, double *x, pcbhip_mpir_result *result, double *history);
+
+/* Bandwidth reference: copy n_floats (multiple of 4) device floats. */
+int pcbhip_copy_dev(const float *src, float *dst, int64_t n_floats, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

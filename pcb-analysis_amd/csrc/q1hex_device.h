@@ -1,0 +1,119 @@
+// Device-side data layout shared by the gfx950 kernels and the host runtime
+// of the hexahedral Q1 conduction solver.
+//
+// Nodes of one problem are the flat C-order grid ((z * (R + 1) + y) * (C + 1)
+// + x) with x fastest, elements ((ez * R + ey) * C + ex); local corners are
+// ordered 4 dz + 2 dy + dx.  This is the reference's own layout
+// (cpp/include/pcbcore/thermal/thermal_system.hpp:39-46), so host arrays cross
+// the boundary without a permutation.
+//
+// A "slot" is one right-hand side of one operator: its FP64 solution and
+// residual, its inner Krylov vectors and its scalar state all live in HBM, and
+// every kernel finds its work through a tile table, so one launch serves one
+// solve, a batch of uneven problems or the requests in flight of a server.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pcbhip {
+
+// Lanes of a wavefront and the node columns one wave strip writes: lane l
+// holds element/node column x0 - 1 + l, the two edge lanes are halo.
+constexpr int kWave = 64;
+constexpr int kStripCols = kWave - 2;
+// Waves per workgroup of the stencil kernels (stacked in y).
+constexpr int kWavesPerGroup = 4;
+constexpr int kGroupThreads = kWave * kWavesPerGroup;
+// Nodes per workgroup of the pointwise kernel.
+constexpr int kFlatPerThread = 8;
+constexpr int kFlatChunk = kGroupThreads * kFlatPerThread;
+// Rows per wave strip of the FP64 outer-residual kernel and of the inner
+// kernels in each precision.
+constexpr int kRowsOuter = 4;
+constexpr int kRowsInner32 = 8;
+constexpr int kRowsInner64 = 4;
+
+enum Phase : int { kPhaseOuter = 0, kPhaseInner = 1, kPhaseDone = 2 };
+enum Status : int { kStatusOk = 0, kStatusNotSpd = 1 };
+// Outer step bookkeeping: no correction pending, a PCG correction pending,
+// or a zero correction pending (the inner right-hand side was zero).
+enum Pending : int { kPendingNone = 0, kPendingCorrection = 1, kPendingZero = 2 };
+
+// The operator of one problem, read-only on the device.  Coefficients are
+// stored pre-divided by 36: with M = W / 6, W = [[2, 1], [1, 2]], the element
+// action a U_x u is (a / 36) times integer-weighted sums of edge differences.
+// ``robin`` holds the lumped Robin conductance of a free node and -1 on a
+// fixed node (the free mask and the Robin term in one array).
+struct OperatorDev {
+    int S, R, C, pad;
+    long long N;  // nodes
+    const float* coef32;    // (3, S, R, C) a/36, b/36, c/36
+    const double* coef64;
+    const float* robin32;   // (N)
+    const double* robin64;
+    const float* dinv32;    // (N) 1 / diagonal, 1 on fixed rows
+    const double* dinv64;
+};
+
+// One right-hand side.  ``x64`` and ``p`` are ping-pong pairs: a stencil
+// kernel reads one buffer at its halo while neighbouring workgroups write the
+// other, so no workgroup ever reads a value another rewrites in the same
+// launch.  r/q/c are the inner-precision residual, operator-times-direction
+// and correction.
+struct SlotDev {
+    OperatorDev op;
+    const double* b64;  // (N) right-hand side
+    double* x64[2];     // (N) solution pair
+    void* r;
+    void* p[2];
+    void* q;
+    void* c;
+    double* part;       // 4 * max(tiles) partial sums
+    double* history;    // 4 * max_outer: (outer, fp64 rel before, inner its, inner rel)
+    int tiles_outer;    // workgroups of the FP64 residual kernel for this slot
+    int tiles_inner;    // workgroups of the inner stencil kernel
+    int tiles_flat;     // workgroups of the pointwise kernel
+    int part_stride;
+};
+
+struct SlotConfig {
+    double rtol, atol, inner_tol;
+    int max_outer, max_inner;
+};
+
+// Mutable scalars of one slot.  Written only by the last workgroup of a
+// kernel to finish the slot's tiles; read by every workgroup of later launches.
+struct SlotState {
+    int phase, first, pending, outer;
+    int inner_its, total_inner, high_apps, low_apps;
+    int status, converged, xcur, pcur;
+    int started, pad0, pad1, pad2;
+    double rz, rhs_norm, rel_outer, rel_inner;
+    double alpha, beta, scale, target;
+};
+
+// Tickets of the last-workgroup reductions, one counter per kernel and slot.
+struct SlotTickets {
+    int outer, inner, update, pad;
+};
+
+// One workgroup of work: slot, x strip, y group, index of the tile within the
+// slot (its partial-sum slot).
+struct Tile {
+    int slot, sx, sy, index;
+};
+
+namespace launch {
+// Host-side launchers of the kernels (q1hex_kernels.hip).
+template <typename T>
+hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, int C, int mode, hipStream_t stream);
+hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S, int R,
+                    int C, int* bad, hipStream_t stream);
+hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream);
+hipError_t mpir_step(bool inner64, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
+                     SlotTickets* tickets, const Tile* outer_tiles, int n_outer, const Tile* inner_tiles, int n_inner,
+                     const Tile* flat_tiles, int n_flat, int* active, hipStream_t stream);
+}  // namespace launch
+
+}  // namespace pcbhip

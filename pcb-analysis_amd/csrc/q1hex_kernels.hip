@@ -1,0 +1,672 @@
+// gfx950 kernels of the matrix-free hexahedral Q1 conduction solve.
+//
+// The operator is the reference's node-owned Q1 action
+// (src/thermal/matrix_free_mpir_fem/native/hex_q1.cpp:90-193,
+// src/thermal/matrix_free_mpir_fem/cuda.py:19-94): for every free node the
+// sum over its adjacent elements of (a U_x + b U_y + c U_z) restricted to free
+// columns, plus the lumped Robin conductance times the node value; fixed rows
+// are the identity.  Instead of the reference's 64 weight products per
+// element and node, the element action is sum-factorised: with
+// U_x = M(z) M(y) S(x), S u is an edge difference and M = W / 6 a 2-point
+// average, so each element costs edge differences, integer-weighted sums and
+// three scalings (the 1/36 is folded into the stored coefficients).
+//
+// Layout and mapping (see q1hex_device.h): one wavefront owns a strip of 64
+// node columns (two halo lanes) by ROWS node rows and sweeps the z node
+// planes, keeping two planes in registers.  x-neighbours move between lanes
+// with cross-lane shuffles, y-neighbours stay in the same lane's registers,
+// so the stencil needs no LDS and no barrier; loads are 64-wide coalesced
+// rows of x.  Every output node is written by exactly one lane; reductions
+// are wave butterflies, a fixed-order sum over the workgroup's waves, one
+// partial per workgroup and a fixed-order sum over a slot's partials by the
+// workgroup that finishes last, so results never depend on scheduling, batch
+// composition or launch order.
+#include <hip/hip_runtime.h>
+
+#include "q1hex_device.h"
+
+namespace pcbhip {
+
+// ------------------------------------------------------- buffer access
+//
+// Every stencil access is a raw buffer load or store: a wave-uniform base
+// (the node plane or element layer, in scalar registers) plus a per-lane
+// 32-bit element offset that is fixed for the whole sweep.  Offsets of nodes
+// outside the grid are set to kOutside, beyond the descriptor's range, so the
+// hardware returns zero for them without a branch or a select.
+constexpr unsigned kOutside = 0x10000000u;
+constexpr int kRange = 0x3FFFFFFF;
+
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+__device__ __forceinline__ Rsrc rsrc(const void* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, kRange, 0x00020000);
+}
+__device__ __forceinline__ float ld(const Rsrc r, const unsigned off, float) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off * 4u, 0, 0));
+}
+__device__ __forceinline__ double ld(const Rsrc r, const unsigned off, double) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off * 8u, 0, 0));
+}
+template <typename T>
+__device__ __forceinline__ T ldt(const T* base, const unsigned off) {
+    return ld(rsrc(base), off, T());
+}
+__device__ __forceinline__ void st(const float v, const Rsrc r, const unsigned off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off * 4u, 0, 0);
+}
+__device__ __forceinline__ void st(const double v, const Rsrc r, const unsigned off) {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, off * 8u, 0, 0);
+}
+template <typename T>
+__device__ __forceinline__ void stt(T* base, const unsigned off, const T v) {
+    st(v, rsrc(base), off);
+}
+
+template <typename T>
+struct NodeVal {
+    T u;      // operand value at the node (unmasked)
+    T robin;  // Robin conductance of a free node, negative on a fixed node
+};
+
+template <typename T>
+__device__ __forceinline__ T masked(const NodeVal<T>& v) {
+    return v.robin >= T(0) ? v.u : T(0);
+}
+
+// The sum-factorised Q1 action over one wave strip.
+//
+// ``load(base, off, owned)`` returns the operand at flat node base + off
+// (``base`` the wave-uniform start of the node plane, ``off`` the lane's
+// offset in it, kOutside outside the grid, where loads return zero); it is
+// called for the ROWS + 2 rows of every plane.  ``owned`` marks the nodes
+// this lane writes.  ``emit(base, off, conduction, value)`` receives, for
+// every owned node, the conduction sum K u restricted to free columns (no
+// Robin term).
+template <typename T, int ROWS, class Load, class Emit>
+__device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const int S, const int R, const int C,
+                                            const int x0, const int y0, Load&& load, Emit&& emit) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int xc = x0 - 1 + lane;
+    const bool col_in = xc >= 0 && xc <= C;
+    const bool ecol_in = xc >= 0 && xc < C && lane < kWave - 1;
+    const bool out_lane = lane >= 1 && lane <= kWave - 2 && col_in;
+    const long long NC = C + 1;
+    const long long NR = R + 1;
+    const long long E = (long long)S * R * C;
+
+    // Lane offsets of the strip's node rows inside a plane and of its element
+    // rows inside a layer, fixed for the whole sweep.
+    unsigned noff[ROWS + 2], eoff[ROWS + 1];
+#pragma unroll
+    for (int i = 0; i < ROWS + 2; ++i) {
+        const int y = y0 - 1 + i;
+        noff[i] = (col_in && y >= 0 && y <= R) ? (unsigned)(y * (int)NC + xc) : kOutside;
+    }
+#pragma unroll
+    for (int j = 0; j < ROWS + 1; ++j) {
+        const int ye = y0 - 1 + j;
+        eoff[j] = (ecol_in && ye >= 0 && ye < R) ? (unsigned)(ye * C + xc) : kOutside;
+    }
+
+    T mlo[ROWS + 2], nlo[ROWS + 2];
+    NodeVal<T> vlo[ROWS];
+    T Llo[ROWS], Rlo[ROWS];
+
+    auto load_plane = [&](const int z, T (&m)[ROWS + 2], T (&n)[ROWS + 2], NodeVal<T> (&v)[ROWS]) {
+        const long long pb = (long long)z * NR * NC;
+#pragma unroll
+        for (int i = 0; i < ROWS + 2; ++i) {
+            const bool owned = out_lane && i >= 1 && i <= ROWS && y0 - 1 + i <= R;
+            const NodeVal<T> nv = load(pb, noff[i], owned);
+            m[i] = masked(nv);
+            if (i >= 1 && i <= ROWS) v[i - 1] = nv;
+        }
+#pragma unroll
+        for (int i = 0; i < ROWS + 2; ++i) n[i] = __shfl_down(m[i], 1);
+    };
+
+    auto finish_plane = [&](const int z, const T (&L)[ROWS], const T (&Rt)[ROWS], const NodeVal<T> (&v)[ROWS]) {
+        const long long pb = (long long)z * NR * NC;
+#pragma unroll
+        for (int i = 0; i < ROWS; ++i) {
+            const T total = L[i] + __shfl_up(Rt[i], 1);
+            if (out_lane && y0 + i <= R) emit(pb, noff[i + 1], total, v[i]);
+        }
+    };
+
+    load_plane(0, mlo, nlo, vlo);
+#pragma unroll
+    for (int i = 0; i < ROWS; ++i) {
+        Llo[i] = T(0);
+        Rlo[i] = T(0);
+    }
+
+    for (int ez = 0; ez < S; ++ez) {
+        T mhi[ROWS + 2], nhi[ROWS + 2];
+        NodeVal<T> vhi[ROWS];
+        load_plane(ez + 1, mhi, nhi, vhi);
+
+        T ca[ROWS + 1], cb[ROWS + 1], cc[ROWS + 1];
+        const T* layer = coef + (long long)ez * R * C;
+        const Rsrc ra = rsrc(layer), rb = rsrc(layer + E), rc = rsrc(layer + 2 * E);
+#pragma unroll
+        for (int j = 0; j < ROWS + 1; ++j) {
+            ca[j] = ld(ra, eoff[j], T());
+            cb[j] = ld(rb, eoff[j], T());
+            cc[j] = ld(rc, eoff[j], T());
+        }
+
+        // Per node row: x differences in both planes, z differences in both
+        // columns and their 2-point x average.
+        T dxl[ROWS + 2], dxh[ROWS + 2], zw0[ROWS + 2], zw1[ROWS + 2];
+#pragma unroll
+        for (int i = 0; i < ROWS + 2; ++i) {
+            dxl[i] = nlo[i] - mlo[i];
+            dxh[i] = nhi[i] - mhi[i];
+            const T dz = mhi[i] - mlo[i];
+            const T dzn = nhi[i] - nlo[i];
+            zw0[i] = T(2) * dz + dzn;
+            zw1[i] = dz + T(2) * dzn;
+        }
+
+        T Lhi[ROWS], Rhi[ROWS];
+#pragma unroll
+        for (int i = 0; i < ROWS; ++i) {
+            Lhi[i] = T(0);
+            Rhi[i] = T(0);
+        }
+
+#pragma unroll
+        for (int j = 0; j < ROWS + 1; ++j) {
+            // a U_x: y average of the x differences, then z average.
+            const T wl0 = T(2) * dxl[j] + dxl[j + 1];
+            const T wl1 = dxl[j] + T(2) * dxl[j + 1];
+            const T wh0 = T(2) * dxh[j] + dxh[j + 1];
+            const T wh1 = dxh[j] + T(2) * dxh[j + 1];
+            const T rx00 = ca[j] * (T(2) * wl0 + wh0);
+            const T rx01 = ca[j] * (T(2) * wl1 + wh1);
+            const T rx10 = ca[j] * (wl0 + T(2) * wh0);
+            const T rx11 = ca[j] * (wl1 + T(2) * wh1);
+            // b U_y: y differences in both columns, x average, z average.
+            const T dyl0 = mlo[j + 1] - mlo[j];
+            const T dyl1 = nlo[j + 1] - nlo[j];
+            const T dyh0 = mhi[j + 1] - mhi[j];
+            const T dyh1 = nhi[j + 1] - nhi[j];
+            const T vl0 = T(2) * dyl0 + dyl1;
+            const T vl1 = dyl0 + T(2) * dyl1;
+            const T vh0 = T(2) * dyh0 + dyh1;
+            const T vh1 = dyh0 + T(2) * dyh1;
+            const T ry00 = cb[j] * (T(2) * vl0 + vh0);
+            const T ry01 = cb[j] * (T(2) * vl1 + vh1);
+            const T ry10 = cb[j] * (vl0 + T(2) * vh0);
+            const T ry11 = cb[j] * (vl1 + T(2) * vh1);
+            // c U_z: z differences, x average (per node row), y average.
+            const T rz00 = cc[j] * (T(2) * zw0[j] + zw0[j + 1]);
+            const T rz01 = cc[j] * (T(2) * zw1[j] + zw1[j + 1]);
+            const T rz10 = cc[j] * (zw0[j] + T(2) * zw0[j + 1]);
+            const T rz11 = cc[j] * (zw1[j] + T(2) * zw1[j + 1]);
+            // Corner (z, y, x) receives s(x) RX[z][y] + s(y) RY[z][x] + s(z) RZ[y][x],
+            // s(0) = -1, s(1) = +1.  Row j is output row j - 1, row j + 1 is output row j.
+            if (j >= 1) {
+                Llo[j - 1] += (-rx00 - ry00) - rz00;
+                Rlo[j - 1] += (rx00 - ry01) - rz01;
+                Lhi[j - 1] += (-rx10 - ry10) + rz00;
+                Rhi[j - 1] += (rx10 - ry11) + rz01;
+            }
+            if (j < ROWS) {
+                Llo[j] += (-rx01 + ry00) - rz10;
+                Rlo[j] += (rx01 + ry01) - rz11;
+                Lhi[j] += (-rx11 + ry10) + rz10;
+                Rhi[j] += (rx11 + ry11) + rz11;
+            }
+        }
+
+        finish_plane(ez, Llo, Rlo, vlo);
+
+#pragma unroll
+        for (int i = 0; i < ROWS + 2; ++i) {
+            mlo[i] = mhi[i];
+            nlo[i] = nhi[i];
+        }
+#pragma unroll
+        for (int i = 0; i < ROWS; ++i) {
+            vlo[i] = vhi[i];
+            Llo[i] = Lhi[i];
+            Rlo[i] = Rhi[i];
+        }
+    }
+    finish_plane(S, Llo, Rlo, vlo);
+}
+
+// ------------------------------------------------------------ reductions
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int offset = kWave / 2; offset > 0; offset >>= 1) v += __shfl_xor(v, offset);
+    return v;
+}
+
+// Reduces NV per-thread values over the workgroup, stores them as this
+// workgroup's partials and takes a ticket.  Returns true in the workgroup
+// that arrived last, with ``total`` holding the fixed-order sums over all
+// ``count`` partials of the slot.
+template <int NV>
+__device__ bool reduce_last(double (&v)[NV], double* __restrict__ part, const int stride, const int index,
+                            const int count, int* ticket, double (&total)[NV]) {
+    __shared__ double red[kGroupThreads];
+    __shared__ double waves[kWavesPerGroup][NV];
+    __shared__ int last;
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double s = wave_sum(v[k]);
+        if ((tid & (kWave - 1)) == 0) waves[wave][k] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < kWavesPerGroup; ++w) s += waves[w][k];
+            part[k * stride + index] = s;
+        }
+        __threadfence();
+        const int t = atomicAdd(ticket, 1);
+        last = (t == count - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (int i = tid; i < count; i += kGroupThreads) {
+            s += __hip_atomic_load(part + k * stride + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        red[tid] = s;
+        __syncthreads();
+        for (int width = kGroupThreads / 2; width > 0; width >>= 1) {
+            if (tid < width) red[tid] += red[tid + width];
+            __syncthreads();
+        }
+        total[k] = red[0];
+        __syncthreads();
+    }
+    if (tid == 0) __hip_atomic_store(ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// --------------------------------------------------- standalone applies
+
+// y = A x (mode 0: fixed rows identity, Robin on the diagonal) or y = K x
+// (mode 1: unconstrained stiffness, every node free, no Robin).
+template <typename T, int ROWS, int MODE>
+__global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ coef, const T* __restrict__ robin,
+                                                          const T* __restrict__ x, T* __restrict__ y, const int S,
+                                                          const int R, const int C, const int nsx) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int sx = blockIdx.x % nsx;
+    const int sy = (blockIdx.x / nsx) * kWavesPerGroup + wave;
+    const int x0 = sx * kStripCols;
+    const int y0 = sy * ROWS;
+    if (y0 > R) return;
+    strip_apply<T, ROWS>(
+        coef, S, R, C, x0, y0,
+        [&](const long long pb, const unsigned off, const bool) -> NodeVal<T> {
+            if (MODE == 1) return {ldt(x + pb, off), T(0)};
+            return {ldt(x + pb, off), ldt(robin + pb, off)};
+        },
+        [&](const long long pb, const unsigned off, const T conduction, const NodeVal<T>& v) {
+            stt(y + pb, off, v.robin >= T(0) ? conduction + v.robin * v.u : v.u);
+        });
+}
+
+// Jacobi diagonal: sum of the element diagonals ((a + b + c) / 9 for every
+// corner of the trilinear unit tensors) over the adjacent elements plus the
+// Robin conductance; 1 on fixed rows (operator.py:289-306).  ``bad`` counts
+// free nodes whose conductance is not positive.
+__global__ __launch_bounds__(256) void k_diagonal(const double* __restrict__ coef, const double* __restrict__ robin,
+                                                  double* __restrict__ diag, double* __restrict__ dinv64,
+                                                  float* __restrict__ dinv32, const int S, const int R, const int C,
+                                                  int* bad) {
+    const long long NC = C + 1, NR = R + 1;
+    const long long N = (long long)(S + 1) * NR * NC;
+    const long long E = (long long)S * R * C;
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int x = (int)(n % NC);
+    const int y = (int)((n / NC) % NR);
+    const int z = (int)(n / (NC * NR));
+    double d = 0.0;
+    for (int ez = z - 1; ez <= z; ++ez) {
+        if (ez < 0 || ez >= S) continue;
+        for (int ey = y - 1; ey <= y; ++ey) {
+            if (ey < 0 || ey >= R) continue;
+            for (int ex = x - 1; ex <= x; ++ex) {
+                if (ex < 0 || ex >= C) continue;
+                const long long e = ((long long)ez * R + ey) * C + ex;
+                d += 4.0 * ((coef[e] + coef[E + e]) + coef[2 * E + e]);
+            }
+        }
+    }
+    const double rb = robin[n];
+    double value = 1.0;
+    if (rb >= 0.0) {
+        value = d + rb;
+        if (!(value > 0.0)) atomicAdd(bad, 1);
+    }
+    diag[n] = value;
+    dinv64[n] = 1.0 / value;
+    dinv32[n] = (float)(1.0 / value);
+}
+
+// Plain copy, the reference point for bandwidth and for the hardware byte
+// counters.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                              const long long n4) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------- MPIR kernels
+
+template <typename TI>
+struct InnerOf;
+template <>
+struct InnerOf<float> {
+    __device__ static const float* dinv(const OperatorDev& op) { return op.dinv32; }
+    __device__ static const float* coef(const OperatorDev& op) { return op.coef32; }
+    __device__ static const float* robin(const OperatorDev& op) { return op.robin32; }
+};
+template <>
+struct InnerOf<double> {
+    __device__ static const double* dinv(const OperatorDev& op) { return op.dinv64; }
+    __device__ static const double* coef(const OperatorDev& op) { return op.coef64; }
+    __device__ static const double* robin(const OperatorDev& op) { return op.robin64; }
+};
+
+// Outer step (hex_q1.cpp:870-911 and solver.py:638-686): apply a pending
+// correction x += (double) c, form the FP64 residual r = b - A x and its norm,
+// decide convergence, and start the inner solve on r rounded to the inner
+// precision (with the first r.z of the Jacobi PCG, hex_q1.cpp:434-446).
+template <typename TI>
+__global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restrict__ slots, SlotState* states,
+                                                         const SlotConfig* __restrict__ configs,
+                                                         SlotTickets* tickets, const Tile* __restrict__ tiles,
+                                                         int* active) {
+    const Tile tile = tiles[blockIdx.x];
+    SlotState& st = states[tile.slot];
+    if (st.phase != kPhaseOuter) return;
+    const SlotDev& sd = slots[tile.slot];
+    const OperatorDev& op = sd.op;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int x0 = tile.sx * kStripCols;
+    const int y0 = (tile.sy * kWavesPerGroup + wave) * kRowsOuter;
+    const int pending = st.pending;
+    const int xcur = st.xcur;
+    const double* xin = sd.x64[xcur];
+    double* xout = sd.x64[1 - xcur];
+    const TI* __restrict__ cin = static_cast<const TI*>(sd.c);
+    TI* __restrict__ rout = static_cast<TI*>(sd.r);
+    const TI* __restrict__ dinv = InnerOf<TI>::dinv(op);
+    const double* __restrict__ robin = op.robin64;
+    const double* __restrict__ b = sd.b64;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};  // b.b, r.r (FP64), r.r and r.z (inner precision)
+    if (y0 <= op.R) {
+        strip_apply<double, kRowsOuter>(
+            op.coef64, op.S, op.R, op.C, x0, y0,
+            [&](const long long pb, const unsigned off, const bool owned) -> NodeVal<double> {
+                double xv = ldt(xin + pb, off);
+                if (pending == kPendingCorrection) xv += (double)ldt(cin + pb, off);
+                if (owned && pending != kPendingNone) stt(xout + pb, off, xv);
+                return {xv, ldt(robin + pb, off)};
+            },
+            [&](const long long pb, const unsigned off, const double conduction, const NodeVal<double>& v) {
+                const double ax = v.robin >= 0.0 ? conduction + v.robin * v.u : v.u;
+                const double bv = ldt(b + pb, off);
+                const double r = bv - ax;
+                const TI ri = (TI)r;
+                stt(rout + pb, off, ri);
+                acc[0] += bv * bv;
+                acc[1] += r * r;
+                acc[2] += (double)ri * (double)ri;
+                acc[3] += (double)ri * (double)(ri * ldt(dinv + pb, off));
+            });
+    }
+    double total[4];
+    if (!reduce_last<4>(acc, sd.part, sd.part_stride, tile.index, sd.tiles_outer, &tickets[tile.slot].outer, total))
+        return;
+    if (threadIdx.x != 0) return;
+    const SlotConfig cfg = configs[tile.slot];
+    if (!st.started) {
+        const double bn = sqrt(total[0]);
+        st.scale = bn > 0.0 ? bn : 1.0;
+        st.target = cfg.atol + cfg.rtol * st.scale;
+        st.started = 1;
+    }
+    if (pending != kPendingNone) st.xcur = 1 - xcur;
+    st.pending = kPendingNone;
+    st.high_apps += 1;
+    const double rn = sqrt(total[1]);
+    st.rel_outer = rn / st.scale;
+    if (rn <= st.target) {
+        st.converged = 1;
+        st.phase = kPhaseDone;
+        atomicSub(active, 1);
+        return;
+    }
+    if (st.outer >= cfg.max_outer) {
+        st.phase = kPhaseDone;
+        atomicSub(active, 1);
+        return;
+    }
+    const double rhs_norm = sqrt(total[2]);
+    if (rhs_norm == 0.0) {
+        // A zero inner right-hand side: the correction is zero (solver.py:188-189).
+        double* h = sd.history + 4 * st.outer;
+        h[0] = st.outer + 1;
+        h[1] = st.rel_outer;
+        h[2] = 0.0;
+        h[3] = 0.0;
+        st.outer += 1;
+        st.pending = kPendingZero;
+        return;
+    }
+    st.rhs_norm = rhs_norm;
+    st.rz = total[3];
+    st.first = 1;
+    st.inner_its = 0;
+    st.phase = kPhaseInner;
+}
+
+// Inner step, part 1: the direction p = z + beta p (z = r / diag; p = z on
+// the first step) fused into the operator apply q = A p, and p.q.  The last
+// workgroup checks the curvature and forms alpha = rz / p.q
+// (hex_q1.cpp:454-464).
+template <typename TI, int ROWS>
+__global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __restrict__ slots, SlotState* states,
+                                                               SlotTickets* tickets, const Tile* __restrict__ tiles,
+                                                               int* active) {
+    const Tile tile = tiles[blockIdx.x];
+    SlotState& st = states[tile.slot];
+    if (st.phase != kPhaseInner) return;
+    const SlotDev& sd = slots[tile.slot];
+    const OperatorDev& op = sd.op;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int x0 = tile.sx * kStripCols;
+    const int y0 = (tile.sy * kWavesPerGroup + wave) * ROWS;
+    const bool first = st.first != 0;
+    const TI beta = (TI)st.beta;
+    const int pcur = st.pcur;
+    const TI* __restrict__ pin = static_cast<const TI*>(sd.p[pcur]);
+    TI* __restrict__ pout = static_cast<TI*>(sd.p[1 - pcur]);
+    const TI* __restrict__ r = static_cast<const TI*>(sd.r);
+    TI* __restrict__ q = static_cast<TI*>(sd.q);
+    const TI* __restrict__ dinv = InnerOf<TI>::dinv(op);
+    const TI* __restrict__ robin = InnerOf<TI>::robin(op);
+    double acc[1] = {0.0};
+    if (y0 <= op.R) {
+        strip_apply<TI, ROWS>(
+            InnerOf<TI>::coef(op), op.S, op.R, op.C, x0, y0,
+            [&](const long long pb, const unsigned off, const bool owned) -> NodeVal<TI> {
+                const TI z = ldt(r + pb, off) * ldt(dinv + pb, off);
+                const TI pv = first ? z : z + beta * ldt(pin + pb, off);
+                if (owned) stt(pout + pb, off, pv);
+                return {pv, ldt(robin + pb, off)};
+            },
+            [&](const long long pb, const unsigned off, const TI conduction, const NodeVal<TI>& v) {
+                const TI ap = v.robin >= TI(0) ? conduction + v.robin * v.u : v.u;
+                stt(q + pb, off, ap);
+                acc[0] += (double)v.u * (double)ap;
+            });
+    }
+    double total[1];
+    if (!reduce_last<1>(acc, sd.part, sd.part_stride, tile.index, sd.tiles_inner, &tickets[tile.slot].inner, total))
+        return;
+    if (threadIdx.x != 0) return;
+    st.pcur = 1 - pcur;
+    st.first = 0;
+    st.low_apps += 1;
+    const double curvature = total[0];
+    if (!isfinite(curvature) || curvature <= 0.0) {
+        st.status = kStatusNotSpd;
+        st.phase = kPhaseDone;
+        atomicSub(active, 1);
+        return;
+    }
+    st.alpha = (double)(TI)(st.rz / curvature);
+}
+
+// Inner step, part 2: c += alpha p, r -= alpha q, |r|^2 and r.z
+// (hex_q1.cpp:465-489).  The last workgroup tests the inner tolerance and
+// either ends the inner solve (the next outer kernel applies c) or forms
+// beta = r.z' / r.z.
+template <typename TI>
+__global__ __launch_bounds__(kGroupThreads) void k_inner_update(const SlotDev* __restrict__ slots, SlotState* states,
+                                                                const SlotConfig* __restrict__ configs,
+                                                                SlotTickets* tickets, const Tile* __restrict__ tiles) {
+    const Tile tile = tiles[blockIdx.x];
+    SlotState& st = states[tile.slot];
+    if (st.phase != kPhaseInner) return;
+    const SlotDev& sd = slots[tile.slot];
+    const long long N = sd.op.N;
+    const TI alpha = (TI)st.alpha;
+    const bool fresh = st.inner_its == 0;
+    const TI* __restrict__ p = static_cast<const TI*>(sd.p[st.pcur]);
+    const TI* __restrict__ q = static_cast<const TI*>(sd.q);
+    TI* __restrict__ r = static_cast<TI*>(sd.r);
+    TI* __restrict__ c = static_cast<TI*>(sd.c);
+    const TI* __restrict__ dinv = InnerOf<TI>::dinv(sd.op);
+    double acc[2] = {0.0, 0.0};
+    const long long base = (long long)tile.index * kFlatChunk;
+#pragma unroll
+    for (int k = 0; k < kFlatPerThread; ++k) {
+        const long long i = base + (long long)k * kGroupThreads + threadIdx.x;
+        if (i < N) {
+            const TI pv = p[i];
+            const TI cv = fresh ? alpha * pv : c[i] + alpha * pv;
+            const TI rv = r[i] - alpha * q[i];
+            c[i] = cv;
+            r[i] = rv;
+            acc[0] += (double)rv * (double)rv;
+            acc[1] += (double)rv * (double)(rv * dinv[i]);
+        }
+    }
+    double total[2];
+    if (!reduce_last<2>(acc, sd.part, sd.part_stride, tile.index, sd.tiles_flat, &tickets[tile.slot].update, total))
+        return;
+    if (threadIdx.x != 0) return;
+    const SlotConfig cfg = configs[tile.slot];
+    st.inner_its += 1;
+    const double rel = sqrt(total[0]) / st.rhs_norm;
+    st.rel_inner = rel;
+    bool end = rel <= cfg.inner_tol || st.inner_its >= cfg.max_inner;
+    if (!end) {
+        const double next_rz = total[1];
+        if (!isfinite(next_rz) || st.rz == 0.0) {
+            end = true;
+        } else {
+            st.beta = (double)(TI)(next_rz / st.rz);
+            st.rz = next_rz;
+        }
+    }
+    if (end) {
+        double* h = sd.history + 4 * st.outer;
+        h[0] = st.outer + 1;
+        h[1] = st.rel_outer;
+        h[2] = st.inner_its;
+        h[3] = rel;
+        st.total_inner += st.inner_its;
+        st.outer += 1;
+        st.pending = kPendingCorrection;
+        st.phase = kPhaseOuter;
+    }
+}
+
+// ------------------------------------------------------ host launchers
+
+namespace launch {
+
+template <typename T>
+hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, int C, int mode,
+                 hipStream_t stream) {
+    constexpr int rows = sizeof(T) == 4 ? kRowsInner32 : kRowsInner64;
+    const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
+    const int nsy = (R + 1 + rows * kWavesPerGroup - 1) / (rows * kWavesPerGroup);
+    const dim3 grid(nsx * nsy), block(kGroupThreads);
+    if (mode == 0) {
+        hipLaunchKernelGGL((k_apply<T, rows, 0>), grid, block, 0, stream, coef, robin, x, y, S, R, C, nsx);
+    } else {
+        hipLaunchKernelGGL((k_apply<T, rows, 1>), grid, block, 0, stream, coef, robin, x, y, S, R, C, nsx);
+    }
+    return hipGetLastError();
+}
+
+template hipError_t apply<float>(const float*, const float*, const float*, float*, int, int, int, int, hipStream_t);
+template hipError_t apply<double>(const double*, const double*, const double*, double*, int, int, int, int,
+                                  hipStream_t);
+
+hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S,
+                    int R, int C, int* bad, hipStream_t stream) {
+    const long long N = (long long)(S + 1) * (R + 1) * (C + 1);
+    const int blocks = (int)((N + 255) / 256);
+    hipLaunchKernelGGL(k_diagonal, dim3(blocks), dim3(256), 0, stream, coef, robin, diag, dinv64, dinv32, S, R, C,
+                       bad);
+    return hipGetLastError();
+}
+
+hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream) {
+    const long long n4 = n_floats / 4;
+    const int blocks = 4096;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, stream, reinterpret_cast<const float4*>(src),
+                       reinterpret_cast<float4*>(dst), n4);
+    return hipGetLastError();
+}
+
+hipError_t mpir_step(bool inner64, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
+                     SlotTickets* tickets, const Tile* outer_tiles, int n_outer, const Tile* inner_tiles,
+                     int n_inner, const Tile* flat_tiles, int n_flat, int* active, hipStream_t stream) {
+    const dim3 block(kGroupThreads);
+    if (inner64) {
+        hipLaunchKernelGGL(k_outer<double>, dim3(n_outer), block, 0, stream, slots, states, configs, tickets,
+                           outer_tiles, active);
+        hipLaunchKernelGGL((k_inner_apply<double, kRowsInner64>), dim3(n_inner), block, 0, stream, slots, states,
+                           tickets, inner_tiles, active);
+        hipLaunchKernelGGL(k_inner_update<double>, dim3(n_flat), block, 0, stream, slots, states, configs,
+                           tickets, flat_tiles);
+    } else {
+        hipLaunchKernelGGL(k_outer<float>, dim3(n_outer), block, 0, stream, slots, states, configs, tickets,
+                           outer_tiles, active);
+        hipLaunchKernelGGL((k_inner_apply<float, kRowsInner32>), dim3(n_inner), block, 0, stream, slots, states,
+                           tickets, inner_tiles, active);
+        hipLaunchKernelGGL(k_inner_update<float>, dim3(n_flat), block, 0, stream, slots, states, configs, tickets,
+                           flat_tiles);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace launch
+}  // namespace pcbhip

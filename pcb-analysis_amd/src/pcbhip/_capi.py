@@ -1,0 +1,289 @@
+"""ctypes binding of ``libpcbhip.so``, the C ABI declared in ``include/pcbhip.h``.
+
+This is the thin layer between the Python solvers and the gfx950 kernels
+(the counterpart of the reference's pybind modules ``cpp/bindings/thermal.cpp``
+and ``cpp/bindings/cuda_module.cpp``).  There is no CPU fallback: when the
+library is missing, or no HIP device is visible, every call raises.
+
+Status codes map onto the Python exceptions of the reference's translator
+(``cpp/include/pcbcore/errors.hpp:1-31``, ``cpp/bindings/module.cpp:571-583``):
+invalid input -> ValueError, runtime failure -> RuntimeError, non-finite ->
+FloatingPointError, allocation failure -> MemoryError.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libpcbhip.so"
+
+OK, INVALID_INPUT, RUNTIME, NON_FINITE, NO_MEMORY = 0, 1, 2, 3, 4
+_ERRORS = {INVALID_INPUT: ValueError, RUNTIME: RuntimeError, NON_FINITE: FloatingPointError, NO_MEMORY: MemoryError}
+
+# Every symbol include/pcbhip.h declares.
+EXPORTS = (
+    "pcbhip_last_error",
+    "pcbhip_device_count",
+    "pcbhip_operator_create",
+    "pcbhip_operator_destroy",
+    "pcbhip_operator_size",
+    "pcbhip_operator_diagonal",
+    "pcbhip_apply_f32",
+    "pcbhip_apply_f64",
+    "pcbhip_stiffness_f64",
+    "pcbhip_apply_f32_dev",
+    "pcbhip_apply_f64_dev",
+    "pcbhip_stiffness_f64_dev",
+    "pcbhip_solve_mpir",
+    "pcbhip_batch_create",
+    "pcbhip_batch_destroy",
+    "pcbhip_batch_set",
+    "pcbhip_batch_rhs_dev",
+    "pcbhip_batch_run",
+    "pcbhip_batch_result",
+    "pcbhip_copy_dev",
+)
+
+
+class MpirConfigC(ctypes.Structure):
+    _fields_ = [
+        ("relative_tolerance", ctypes.c_double),
+        ("absolute_tolerance", ctypes.c_double),
+        ("inner_relative_tolerance", ctypes.c_double),
+        ("max_outer_iterations", ctypes.c_int),
+        ("max_inner_iterations", ctypes.c_int),
+    ]
+
+
+class MpirResultC(ctypes.Structure):
+    _fields_ = [
+        ("converged", ctypes.c_int),
+        ("outer_iterations", ctypes.c_int),
+        ("inner_iterations", ctypes.c_int),
+        ("high_operator_applications", ctypes.c_int),
+        ("low_operator_applications", ctypes.c_int),
+        ("not_spd", ctypes.c_int),
+        ("relative_residual", ctypes.c_double),
+    ]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_int = ctypes.c_int
+
+
+def load() -> ctypes.CDLL:
+    """The library, loaded once; raises ImportError when it is not built."""
+
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"the HIP library {LIB_PATH} is not built; run python -c 'import __graft_entry__ as g; g.build()'"
+        )
+    lib = ctypes.CDLL(str(LIB_PATH))
+    lib.pcbhip_last_error.restype = ctypes.c_char_p
+    lib.pcbhip_device_count.argtypes = [ctypes.POINTER(_int)]
+    lib.pcbhip_operator_create.argtypes = [_int, _int, _int, _int, _vp, _vp, _vp, ctypes.POINTER(_vp)]
+    lib.pcbhip_operator_destroy.argtypes = [_vp]
+    lib.pcbhip_operator_size.argtypes = [_vp]
+    lib.pcbhip_operator_size.restype = ctypes.c_int64
+    lib.pcbhip_operator_diagonal.argtypes = [_vp, _vp]
+    for name in ("pcbhip_apply_f32", "pcbhip_apply_f64", "pcbhip_stiffness_f64"):
+        getattr(lib, name).argtypes = [_vp, _vp, _vp]
+    for name in ("pcbhip_apply_f32_dev", "pcbhip_apply_f64_dev", "pcbhip_stiffness_f64_dev"):
+        getattr(lib, name).argtypes = [_vp, _vp, _vp, _vp]
+    lib.pcbhip_solve_mpir.argtypes = [_vp, _vp, _vp, ctypes.POINTER(MpirConfigC), _int, ctypes.POINTER(MpirResultC), _vp]
+    lib.pcbhip_batch_create.argtypes = [_int, ctypes.POINTER(_vp), _int, ctypes.POINTER(_vp)]
+    lib.pcbhip_batch_destroy.argtypes = [_vp]
+    lib.pcbhip_batch_set.argtypes = [_vp, _int, _vp, _vp, ctypes.POINTER(MpirConfigC)]
+    lib.pcbhip_batch_rhs_dev.argtypes = [_vp, _int, ctypes.POINTER(_vp)]
+    lib.pcbhip_batch_run.argtypes = [_vp]
+    lib.pcbhip_batch_result.argtypes = [_vp, _int, _vp, ctypes.POINTER(MpirResultC), _vp]
+    lib.pcbhip_copy_dev.argtypes = [_vp, _vp, ctypes.c_int64, _vp]
+    _lib = lib
+    return lib
+
+
+def check(code: int) -> None:
+    if code != OK:
+        message = load().pcbhip_last_error().decode() or f"pcbhip error {code}"
+        raise _ERRORS.get(code, RuntimeError)(message)
+
+
+def device_count() -> int:
+    n = _int(0)
+    check(load().pcbhip_device_count(ctypes.byref(n)))
+    return int(n.value)
+
+
+def require_device(device: int = 0) -> None:
+    count = device_count()
+    if count < 1:
+        raise RuntimeError("HIP backend requested, but no HIP device is visible")
+    if not 0 <= device < count:
+        raise ValueError(f"HIP device_id {device} is outside the visible range [0, {count - 1}]")
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def config_c(relative_tolerance, absolute_tolerance, inner_relative_tolerance, max_outer, max_inner) -> MpirConfigC:
+    return MpirConfigC(float(relative_tolerance), float(absolute_tolerance), float(inner_relative_tolerance),
+                       int(max_outer), int(max_inner))
+
+
+class Operator:
+    """Owning handle of one device-resident operator (``pcbhip_operator``)."""
+
+    def __init__(self, device: int, element_shape, coef, robin, free):
+        s, r, c = (int(v) for v in element_shape)
+        n = (s + 1) * (r + 1) * (c + 1)
+        coef = np.ascontiguousarray(coef, dtype=np.float64).reshape(-1)
+        robin = np.ascontiguousarray(robin, dtype=np.float64).reshape(-1)
+        free = np.ascontiguousarray(free, dtype=np.uint8).reshape(-1)
+        if coef.size != 3 * s * r * c or robin.size != n or free.size != n:
+            raise ValueError("operator arrays have the wrong size")
+        lib = load()
+        self._lib = lib
+        self.handle = None
+        self.device = int(device)
+        self.element_shape = (s, r, c)
+        self.size = n
+        h = _vp()
+        check(lib.pcbhip_operator_create(self.device, s, r, c, _ptr(coef), _ptr(robin), _ptr(free), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle is not None:
+            self._lib.pcbhip_operator_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def diagonal(self) -> np.ndarray:
+        out = np.empty(self.size)
+        check(self._lib.pcbhip_operator_diagonal(self.handle, _ptr(out)))
+        return out
+
+    def _vec(self, x, dtype):
+        v = np.ascontiguousarray(x, dtype=dtype).reshape(-1)
+        if v.size != self.size:
+            raise ValueError(f"vector has size {v.size}, expected {self.size}")
+        return v
+
+    def apply_f32(self, x) -> np.ndarray:
+        v = self._vec(x, np.float32)
+        out = np.empty(self.size, dtype=np.float32)
+        check(self._lib.pcbhip_apply_f32(self.handle, _ptr(v), _ptr(out)))
+        return out
+
+    def apply_f64(self, x) -> np.ndarray:
+        v = self._vec(x, np.float64)
+        out = np.empty(self.size)
+        check(self._lib.pcbhip_apply_f64(self.handle, _ptr(v), _ptr(out)))
+        return out
+
+    def stiffness_f64(self, x) -> np.ndarray:
+        v = self._vec(x, np.float64)
+        out = np.empty(self.size)
+        check(self._lib.pcbhip_stiffness_f64(self.handle, _ptr(v), _ptr(out)))
+        return out
+
+    def apply_f32_dev(self, x_ptr: int, y_ptr: int, stream: int | None = None) -> None:
+        check(self._lib.pcbhip_apply_f32_dev(self.handle, x_ptr, y_ptr, stream))
+
+    def apply_f64_dev(self, x_ptr: int, y_ptr: int, stream: int | None = None) -> None:
+        check(self._lib.pcbhip_apply_f64_dev(self.handle, x_ptr, y_ptr, stream))
+
+
+def result_dict(res: MpirResultC, history: np.ndarray) -> dict:
+    rows = history.reshape(-1, 4)[: res.outer_iterations]
+    return dict(
+        converged=bool(res.converged),
+        outer_iterations=int(res.outer_iterations),
+        inner_iterations=int(res.inner_iterations),
+        relative_residual=float(res.relative_residual),
+        high_operator_applications=int(res.high_operator_applications),
+        low_operator_applications=int(res.low_operator_applications),
+        history=[(int(a), float(b), int(c), float(d)) for a, b, c, d in rows],
+    )
+
+
+class Batch:
+    """Owning handle of a set of right-hand sides solved together (``pcbhip_batch``)."""
+
+    def __init__(self, operators, inner_fp64: bool = False):
+        ops = list(operators)
+        lib = load()
+        self._lib = lib
+        self.handle = None
+        self.operators = ops  # keeps the operator handles alive
+        arr = (_vp * len(ops))(*[op.handle for op in ops])
+        h = _vp()
+        check(lib.pcbhip_batch_create(len(ops), arr, int(bool(inner_fp64)), ctypes.byref(h)))
+        self.handle = h
+        self.max_outer = [8] * len(ops)
+
+    def close(self) -> None:
+        if self.handle is not None:
+            self._lib.pcbhip_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def __len__(self):
+        return len(self.operators)
+
+    def set(self, slot: int, rhs, x0=None, config: MpirConfigC | None = None) -> None:
+        op = self.operators[slot]
+        b = None if rhs is None else op._vec(rhs, np.float64)
+        x = None if x0 is None else op._vec(x0, np.float64)
+        check(self._lib.pcbhip_batch_set(self.handle, slot, None if b is None else _ptr(b),
+                                         None if x is None else _ptr(x),
+                                         None if config is None else ctypes.byref(config)))
+        if config is not None:
+            self.max_outer[slot] = int(config.max_outer_iterations)
+
+    def rhs_device_pointer(self, slot: int) -> int:
+        p = _vp()
+        check(self._lib.pcbhip_batch_rhs_dev(self.handle, slot, ctypes.byref(p)))
+        return int(p.value)
+
+    def run(self) -> None:
+        check(self._lib.pcbhip_batch_run(self.handle))
+
+    def result(self, slot: int, *, solution: bool = True):
+        op = self.operators[slot]
+        x = np.empty(op.size) if solution else None
+        res = MpirResultC()
+        history = np.zeros(4 * max(1, self.max_outer[slot]))
+        check(self._lib.pcbhip_batch_result(self.handle, slot, None if x is None else _ptr(x), ctypes.byref(res),
+                                            _ptr(history)))
+        return x, result_dict(res, history)
+
+
+def solve_mpir(op: Operator, rhs, x0, config: MpirConfigC, inner_fp64: bool = False):
+    """One solve through ``pcbhip_solve_mpir``; returns (solution, result dict)."""
+
+    b = op._vec(rhs, np.float64)
+    x = np.zeros(op.size) if x0 is None else np.array(op._vec(x0, np.float64))
+    res = MpirResultC()
+    history = np.zeros(4 * max(1, config.max_outer_iterations))
+    code = load().pcbhip_solve_mpir(op.handle, _ptr(b), _ptr(x), ctypes.byref(config), int(bool(inner_fp64)),
+                                    ctypes.byref(res), _ptr(history))
+    check(code)
+    return x, result_dict(res, history)

@@ -1,0 +1,122 @@
+"""Seeded synthetic boards: the workloads the tests and the benchmark run.
+
+Every board is a 0.25 mm-pitch hexahedral stack, 1.6 mm thick split evenly
+into ``S`` slabs, FR-4 (0.3 W/m K, isotropic) with copper (385 W/m K) on the
+bottom and top slabs under a random occupancy mask of 4 x 4-cell blocks,
+the bottom node plane held at 298.15 K and the top face cooled by h = 10
+W/m^2 K to 298.15 K.  Heat enters as element heat spread uniformly over
+square patches of top-slab cells.  Draw order is fixed (bottom mask, top
+mask, then sources), so a seed names a board.
+
+* ``board_64x64x8``       -- 8 x 64 x 64 elements, 40 % copper, 1 W on a 4 x 4 patch (seed 1).
+* ``board_512x512x16``    -- 16 x 512 x 512, 30-60 % copper per plane, 8 sources of
+                             0.1-2 W (log-uniform) on 3..8-cell patches (seed 2).
+* ``uneven_boards``       -- 200 boards, sides log-uniform 32..384, depth 4..16,
+                             20-70 % copper, 1..8 sources of 0.1-3 W (seed 4).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from thermal.matrix_free_mpir_fem import ConvectionBoundary, LayeredThermalMesh, ThermalConductionProblem
+
+PITCH_M = 0.25e-3
+STACK_M = 1.6e-3
+FR4_K = 0.3
+COPPER_K = 385.0
+AMBIENT_K = 298.15
+FILM_W_PER_M2_K = 10.0
+
+
+def copper_mask(rng: np.random.Generator, rows: int, cols: int, fill: float) -> np.ndarray:
+    """Occupancy of 4 x 4-cell blocks, cropped to (rows, cols)."""
+
+    blocks = rng.random((-(-rows // 4), -(-cols // 4))) < fill
+    return np.kron(blocks, np.ones((4, 4), dtype=bool))[:rows, :cols]
+
+
+@dataclass(frozen=True)
+class Source:
+    row: int
+    col: int
+    side: int
+    power_w: float
+
+
+def layered_board(slabs: int, rows: int, cols: int, masks: tuple[np.ndarray, np.ndarray],
+                  sources: list[Source]) -> ThermalConductionProblem:
+    k = np.full((slabs, rows, cols), FR4_K)
+    k[0] = np.where(masks[0], COPPER_K, FR4_K)
+    k[-1] = np.where(masks[1], COPPER_K, FR4_K)
+    mesh = LayeredThermalMesh(
+        slab_thickness_m=(STACK_M / slabs,) * slabs,
+        pitch_x_m=PITCH_M,
+        pitch_y_m=PITCH_M,
+        conductivity_w_per_m_k=k,
+    )
+    heat = np.zeros(mesh.element_grid_shape)
+    for s in sources:
+        heat[-1, s.row : s.row + s.side, s.col : s.col + s.side] += s.power_w / (s.side * s.side)
+    fixed = np.zeros(mesh.node_shape, dtype=bool)
+    fixed[0] = True
+    return ThermalConductionProblem(
+        mesh,
+        convection=(ConvectionBoundary("top", FILM_W_PER_M2_K, AMBIENT_K),),
+        fixed_temperature_mask=fixed,
+        fixed_temperature_k=AMBIENT_K,
+        element_heat_w=heat,
+    )
+
+
+def board_64x64x8(seed: int = 1) -> ThermalConductionProblem:
+    rng = np.random.default_rng(seed)
+    masks = (copper_mask(rng, 64, 64, 0.40), copper_mask(rng, 64, 64, 0.40))
+    row, col = (int(v) for v in rng.integers(0, 61, 2))
+    return layered_board(8, 64, 64, masks, [Source(row, col, 4, 1.0)])
+
+
+def random_sources(rng: np.random.Generator, rows: int, cols: int, count: int, p_lo: float, p_hi: float,
+                   side_lo: int = 3, side_hi: int = 8) -> list[Source]:
+    out = []
+    for _ in range(count):
+        power = float(np.exp(rng.uniform(np.log(p_lo), np.log(p_hi))))
+        side = int(rng.integers(side_lo, side_hi + 1))
+        side = min(side, rows, cols)
+        row = int(rng.integers(0, rows - side + 1))
+        col = int(rng.integers(0, cols - side + 1))
+        out.append(Source(row, col, side, power))
+    return out
+
+
+def board_512x512x16(seed: int = 2, size: int = 512, slabs: int = 16) -> ThermalConductionProblem:
+    rng = np.random.default_rng(seed)
+    fills = rng.uniform(0.3, 0.6, 2)
+    masks = (copper_mask(rng, size, size, fills[0]), copper_mask(rng, size, size, fills[1]))
+    return layered_board(slabs, size, size, masks, random_sources(rng, size, size, 8, 0.1, 2.0))
+
+
+def uneven_board_shapes(count: int = 200, seed: int = 4) -> list[tuple[int, int, int]]:
+    """(slabs, rows, cols) of the uneven batch, without building the boards."""
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        rows, cols = (int(round(float(np.exp(v)))) for v in rng.uniform(np.log(32), np.log(384), 2))
+        slabs = int(rng.integers(4, 17))
+        out.append((slabs, rows, cols))
+    return out
+
+
+def uneven_boards(count: int = 200, seed: int = 4) -> list[ThermalConductionProblem]:
+    shapes = uneven_board_shapes(count, seed)
+    rng = np.random.default_rng(seed + 1000)
+    boards = []
+    for slabs, rows, cols in shapes:
+        fills = rng.uniform(0.2, 0.7, 2)
+        masks = (copper_mask(rng, rows, cols, fills[0]), copper_mask(rng, rows, cols, fills[1]))
+        n_src = int(rng.integers(1, 9))
+        boards.append(layered_board(slabs, rows, cols, masks, random_sources(rng, rows, cols, n_src, 0.1, 3.0)))
+    return boards

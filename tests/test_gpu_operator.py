@@ -1,0 +1,101 @@
+"""Device operator parity: the gfx950 sum-factorised Q1 action, its FP64
+twin, the unconstrained stiffness and the Jacobi diagonal against the
+oracle's element-assembled FP64 matrix (tests/test_native_thermal.py:55-96
+is the reference's version of these checks: FP32 within a few eps32 of the
+FP64 action, FP64 within 64 eps64, fixed rows exactly the identity)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import assembled as A
+from oracle import problem as P
+from pcbhip import _capi
+from tests.helpers import DOUBLE_EPS, SINGLE_EPS, random_body, stack, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+# (slabs, rows, cols): single elements, strips narrower and wider than one
+# wave strip (62 output columns), several workgroups in y.
+SHAPES = [(1, 1, 1), (2, 1, 4), (3, 2, 3), (1, 1, 61), (1, 2, 62), (2, 3, 63), (3, 12, 20), (2, 40, 130),
+          (5, 70, 63), (4, 33, 125)]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    _capi.require_device(0)
+
+
+def _device_and_matrix(problem):
+    ops = P.operator_arrays(to_oracle(problem))
+    dev = _capi.Operator(0, problem.mesh.element_grid_shape, ops["coef"], ops["robin_total"], ops["free"])
+    a = A.system_matrix(ops["coef"], ops["unit"], ops["robin_total"], ops["free"], problem.mesh.element_grid_shape)
+    k = A.stiffness_matrix(ops["coef"], ops["unit"], problem.mesh.element_grid_shape)
+    return dev, ops, a, k
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("void", [False, True])
+def test_device_actions_match_the_assembled_operator(shape, void):
+    print(f"apply parity shape={shape} void={void}", flush=True)
+    problem = random_body(shape, seed=sum(shape), void=void)
+    dev, ops, a, k = _device_and_matrix(problem)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(problem.mesh.size)
+    scale = float((abs(a) @ np.abs(x)).max())
+
+    y64 = dev.apply_f64(x)
+    np.testing.assert_allclose(y64, a @ x, rtol=0, atol=64 * DOUBLE_EPS * scale)
+
+    x32 = x.astype(np.float32)
+    y32 = dev.apply_f32(x32)
+    assert y32.dtype == np.float32
+    exact32 = a @ x32.astype(np.float64)
+    assert np.max(np.abs(y32 - exact32)) <= 32 * SINGLE_EPS * scale
+    fixed = ops["fixed"]
+    np.testing.assert_array_equal(y32[fixed], x32[fixed])
+    np.testing.assert_array_equal(y64[fixed], x[fixed])
+
+    ks = dev.stiffness_f64(x)
+    np.testing.assert_allclose(ks, k @ x, rtol=0, atol=64 * DOUBLE_EPS * float((abs(k) @ np.abs(x)).max()))
+
+    np.testing.assert_allclose(dev.diagonal(), a.diagonal(), rtol=4 * DOUBLE_EPS, atol=0)
+    # Bitwise repeatable: every node is written by one lane in a fixed order.
+    assert dev.apply_f32(x32).tobytes() == y32.tobytes()
+    assert dev.apply_f64(x).tobytes() == y64.tobytes()
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 4), (3, 2), (12, 20)])
+@pytest.mark.parametrize("fixed", [False, True])
+def test_reference_stack_fixture(shape, fixed):
+    print(f"stack fixture shape={shape} fixed={fixed}", flush=True)
+    problem = stack(*shape, fixed=fixed)
+    dev, ops, a, _ = _device_and_matrix(problem)
+    x = np.random.default_rng(3).standard_normal(problem.mesh.size).astype(np.float32)
+    y = dev.apply_f32(x)
+    expected = a @ x.astype(np.float64)
+    assert np.linalg.norm(y - expected) <= 8.0 * SINGLE_EPS * np.linalg.norm(expected) + 8.0 * SINGLE_EPS * float(
+        (abs(a) @ np.abs(x.astype(np.float64))).max())
+
+
+def test_pure_conduction_annihilates_constants_on_the_device():
+    print("constant kernel", flush=True)
+    problem = random_body((3, 9, 70), seed=3)
+    ops = P.operator_arrays(to_oracle(problem))
+    dev = _capi.Operator(0, problem.mesh.element_grid_shape, ops["coef"], ops["robin_total"], ops["free"])
+    ones = np.ones(problem.mesh.size)
+    k = A.stiffness_matrix(ops["coef"], ops["unit"], problem.mesh.element_grid_shape)
+    # The sum-factorised action works on edge differences, so a constant
+    # field gives exactly zero.
+    assert np.all(dev.stiffness_f64(ones) == 0.0)
+    assert np.abs(k @ ones).max() < 1e-10 * abs(k).max()
+
+
+def test_a_free_node_without_conductance_is_refused():
+    print("invalid operator", flush=True)
+    coef = np.zeros(3)
+    robin = np.zeros(8)
+    free = np.ones(8, dtype=np.uint8)
+    with pytest.raises(ValueError, match="positive conductance"):
+        _capi.Operator(0, (1, 1, 1), coef, robin, free)

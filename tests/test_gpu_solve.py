@@ -1,0 +1,209 @@
+"""Device MPIR parity: the whole FP64-refined Jacobi-PCG solve on the GPU
+against direct FP64 solves of the oracle's assembled matrix, closed-form
+temperature fields (tests/test_thermal_conduction.py:79-194), the 64 x 64 x 8
+board against the oracle's sparse direct solve, and batch independence."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from electrical.matrix_free_mpir_fem import MPIRConfig, solve_mpir
+from oracle import assembled as A
+from oracle import problem as P
+from pcbhip import _capi, workloads
+from tests.helpers import random_body, stack, to_oracle
+from thermal.matrix_free_mpir_fem import (
+    ConvectionBoundary,
+    HeatSource,
+    LayeredThermalMesh,
+    MatrixFreeThermalOperator,
+    ThermalConductionProblem,
+    solve_thermal_conduction,
+)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    _capi.require_device(0)
+
+
+def _oracle_solution(problem):
+    """FP64 direct solve of the rise and its reference temperature."""
+
+    op = to_oracle(problem)
+    ops = P.operator_arrays(op)
+    a = A.system_matrix(ops["coef"], ops["unit"], ops["robin_total"], ops["free"], op.element_shape)
+    k = A.stiffness_matrix(ops["coef"], ops["unit"], op.element_shape)
+    ref = P.default_reference(ops, op)
+    rhs = P.build_rhs(ops, ref, lambda v: k @ v)
+    return A.solve_direct(a, rhs), rhs, ref
+
+
+def _rise_error(rise, exact):
+    return float(np.max(np.abs(rise - exact)) / np.max(np.abs(exact)))
+
+
+@pytest.mark.parametrize("seed,void", [(2, False), (3, True)])
+def test_mpir_matches_the_direct_solve(seed, void):
+    print(f"mpir vs direct seed={seed} void={void}", flush=True)
+    problem = random_body((3, 5, 70), seed=seed, void=void)
+    exact, rhs, _ = _oracle_solution(problem)
+    operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    result = solve_mpir(operator, rhs, config=MPIRConfig(relative_tolerance=1e-11, max_outer_iterations=16,
+                                                         max_inner_iterations=2000))
+    assert result.converged
+    assert result.low_runtime == "hip-fp32"
+    assert result.low_operator_applications > result.high_operator_applications
+    assert _rise_error(result.solution, exact) < 1e-9
+    # One record per correction, numbered from 1, starting from the residual
+    # of the zero guess and falling (tests/test_native_mpir.py:84-92).
+    assert [s.outer_iteration for s in result.history] == list(range(1, result.outer_iterations + 1))
+    assert result.history[0].high_relative_residual == pytest.approx(1.0)
+    residuals = [s.high_relative_residual for s in result.history]
+    assert all(b < a for a, b in zip(residuals, residuals[1:]))
+    assert sum(s.inner_iterations for s in result.history) == result.inner_iterations
+    assert result.relative_residual <= 1e-11
+    # The device's final residual is the oracle's residual of the device answer.
+    true_rel = np.linalg.norm(rhs - (A.system_matrix(*_arrays(problem)) @ result.solution)) / np.linalg.norm(rhs)
+    assert true_rel == pytest.approx(result.relative_residual, rel=1e-3, abs=1e-14)
+    # Bitwise repeatable.
+    again = solve_mpir(operator, rhs, config=MPIRConfig(relative_tolerance=1e-11, max_outer_iterations=16,
+                                                        max_inner_iterations=2000))
+    assert again.solution.tobytes() == result.solution.tobytes()
+
+
+def _arrays(problem):
+    ops = P.operator_arrays(to_oracle(problem))
+    return ops["coef"], ops["unit"], ops["robin_total"], ops["free"], problem.mesh.element_grid_shape
+
+
+def test_fp64_inner_pcg_is_the_plain_fp64_solve():
+    print("plain fp64 pcg", flush=True)
+    problem = random_body((2, 7, 9), seed=11)
+    exact, rhs, _ = _oracle_solution(problem)
+    operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    cfg = MPIRConfig(relative_tolerance=1e-12, inner_relative_tolerance=1e-12, max_outer_iterations=4,
+                     max_inner_iterations=5000)
+    x, result = _capi.solve_mpir(operator.device_operator, rhs, None, cfg.to_c(), inner_fp64=True)
+    assert result["converged"]
+    assert result["outer_iterations"] <= 2
+    assert _rise_error(x, exact) < 1e-9
+
+
+def test_solve_reaches_the_closed_form_parabola():
+    print("parabola", flush=True)
+    mesh = LayeredThermalMesh((0.25e-3,) * 8, 1.0e-3, 1.0e-3, 0.3, element_shape=(4, 5))
+    problem = ThermalConductionProblem(
+        mesh, convection=(ConvectionBoundary("top", 100.0, 300.0),),
+        element_heat_w=np.full(mesh.element_grid_shape, 1.0 / 160.0),
+    )
+    solution = solve_thermal_conduction(problem, initial_temperature_k=300.0, preconditioner="jacobi",
+                                        config=MPIRConfig(max_outer_iterations=16, max_inner_iterations=1000))
+    flux = 1.0 / (4.0e-3 * 5.0e-3)
+    top = 300.0 + flux / 100.0
+    bottom = top + flux * 2.0e-3 / (2.0 * 0.3)
+    assert solution.solve.converged
+    assert solution.temperature_k[-1, 2, 2] == pytest.approx(top, rel=1e-9)
+    assert solution.temperature_k[0, 2, 2] == pytest.approx(bottom, rel=1e-9)
+    assert solution.total_heat_input_w == pytest.approx(1.0)
+    assert solution.convective_heat_w[0] == pytest.approx(1.0, rel=1e-9)
+    assert solution.heat_balance_error_w == pytest.approx(0.0, abs=1e-9)
+
+
+def test_fixed_faces_give_a_linear_profile_and_exact_flux():
+    print("linear profile", flush=True)
+    mesh = LayeredThermalMesh((0.5e-3,) * 4, 1.0e-3, 1.0e-3, 1.0, element_shape=(2, 3))
+    mask = np.zeros(mesh.node_shape, dtype=bool)
+    mask[0] = mask[-1] = True
+    values = np.zeros(mesh.node_shape)
+    values[0], values[-1] = 300.0, 350.0
+    solution = solve_thermal_conduction(ThermalConductionProblem(mesh, fixed_temperature_mask=mask,
+                                                                 fixed_temperature_k=values),
+                                        preconditioner="jacobi")
+    assert solution.solve.converged
+    for plane, expected in zip(solution.temperature_k, 300.0 + 12.5 * np.arange(5)):
+        np.testing.assert_allclose(plane, expected, rtol=1e-8)
+    np.testing.assert_allclose(solution.heat_flux_w_per_m2[..., 2], -50.0 / 2.0e-3, rtol=0,
+                               atol=2 * 1e-8 * 350 / 0.5e-3)
+    assert solution.fixed_temperature_heat_w == pytest.approx(0.0, abs=1e-9)
+    assert solution.heat_balance_error_w == pytest.approx(0.0, abs=1e-9)
+
+
+def test_nodal_source_heat_is_conserved():
+    print("conservation", flush=True)
+    mesh = LayeredThermalMesh((0.4e-3,) * 2, 1.0e-3, 1.0e-3, 5.0, element_shape=(3, 3))
+    mask = np.zeros(mesh.node_shape, dtype=bool)
+    mask[0, 0, :] = True
+    problem = ThermalConductionProblem(
+        mesh, convection=(ConvectionBoundary("bottom", 20.0, 290.0),), fixed_temperature_mask=mask,
+        fixed_temperature_k=290.0, heat_sources=(HeatSource(((2, 1, 1), (2, 1, 2)), 0.3, "chip"),),
+    )
+    solution = solve_thermal_conduction(problem, preconditioner="jacobi")
+    assert solution.solve.converged
+    assert solution.total_heat_input_w == pytest.approx(0.3)
+    assert solution.convective_heat_w[0] + solution.fixed_temperature_heat_w == pytest.approx(0.3, rel=1e-8)
+    assert solution.min_temperature_k == pytest.approx(290.0)
+
+
+def test_board_64x64x8_plain_fp64_and_mpir_match_the_oracle():
+    print("board 64x64x8", flush=True)
+    problem = workloads.board_64x64x8()
+    exact, rhs, ref = _oracle_solution(problem)
+    operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    plain = MPIRConfig(relative_tolerance=1e-10, inner_relative_tolerance=1e-10, max_outer_iterations=4,
+                       max_inner_iterations=20000)
+    x64, r64 = _capi.solve_mpir(operator.device_operator, rhs, None, plain.to_c(), inner_fp64=True)
+    assert r64["converged"]
+    assert _rise_error(x64, exact) < 1e-9
+    mixed = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=30, max_inner_iterations=20000)
+    result = solve_mpir(operator, rhs, config=mixed)
+    assert result.converged
+    assert _rise_error(result.solution, exact) < 1e-9
+
+
+def test_batched_slots_equal_their_single_solves_bit_for_bit():
+    print("batch independence", flush=True)
+    problems = [random_body((2, 6, 9), seed=1), stack(12, 20, fixed=True), random_body((3, 40, 70), seed=4),
+                workloads.board_64x64x8()]
+    cfg = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=30, max_inner_iterations=4000)
+    ops = [MatrixFreeThermalOperator(p, preconditioner="jacobi") for p in problems]
+    rhss = [o.build_rhs(o.default_reference_temperature()) for o in ops]
+    singles = [solve_mpir(o, b, config=cfg).solution for o, b in zip(ops, rhss)]
+    batch = _capi.Batch([o.device_operator for o in ops] + [ops[0].device_operator])
+    for i, b in enumerate(rhss + [rhss[0]]):
+        batch.set(i, b, None, cfg.to_c())
+    batch.run()
+    for i, single in enumerate(singles + [singles[0]]):
+        x, res = batch.result(i)
+        assert res["converged"]
+        assert x.tobytes() == single.tobytes(), i
+
+
+def test_a_capped_first_pass_re_references_and_converges():
+    print("re-reference restart", flush=True)
+    # tests/test_native_thermal_system.py:188-201.
+    problem = stack(9, 11, fixed=True)
+    solution = solve_thermal_conduction(problem, preconditioner="jacobi",
+                                        config=MPIRConfig(relative_tolerance=1e-12, max_outer_iterations=2,
+                                                          max_inner_iterations=3000))
+    capped = solution.solve.outer_iterations
+    assert capped >= 2
+    exact, _, ref = _oracle_solution(problem)
+    full = solve_thermal_conduction(problem, preconditioner="jacobi",
+                                    config=MPIRConfig(relative_tolerance=1e-11, max_outer_iterations=30,
+                                                      max_inner_iterations=3000))
+    assert full.solve.converged
+    np.testing.assert_allclose(full.temperature_k.reshape(-1) - ref, exact, rtol=0, atol=1e-9 * np.abs(exact).max())
+
+
+def test_exact_initial_guess_converges_without_inner_work():
+    print("exact warm start", flush=True)
+    problem = random_body((2, 5, 6), seed=8)
+    exact, rhs, _ = _oracle_solution(problem)
+    operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    result = solve_mpir(operator, rhs, config=MPIRConfig(relative_tolerance=1e-8), initial_guess=exact)
+    assert result.converged and result.outer_iterations == 0 and result.inner_iterations == 0
+    assert result.high_operator_applications == 1
