@@ -32,32 +32,65 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def apply_roofline(operator, torch, launches: int = 200) -> dict:
-    """Time the FP32 Q1 apply with HIP events on the stream it runs on."""
+def _time_launches(torch, stream, launch, launches: int) -> float:
+    """Average seconds per launch, HIP events recorded on the launch stream."""
+
+    for _ in range(3):
+        launch()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    stream.synchronize()
+    start.record(stream)
+    for _ in range(launches):
+        launch()
+    stop.record(stream)
+    stop.synchronize()
+    return start.elapsed_time(stop) / 1e3 / launches
+
+
+def copy_bandwidth(torch, stream, nbytes: int = 1 << 30) -> float:
+    """Stream-copy GB/s of the box (read + write bytes over time)."""
+
+    from pcbhip import _capi
+
+    n = nbytes // 4
+    src = torch.empty(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    lib = _capi.load()
+    per = _time_launches(torch, stream, lambda: _capi.check(
+        lib.pcbhip_copy_dev(src.data_ptr(), dst.data_ptr(), n, stream.cuda_stream)), 20)
+    return 2 * nbytes / per / 1e9
+
+
+def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
+    """Tile-shape sweep of the FP32 Q1 apply (the autotuner's candidates),
+    timed with HIP events on the stream it runs on; the fastest shape is kept."""
+
+    from pcbhip import autotune
 
     n = operator.size
     s, r, c = operator.mesh.element_grid_shape
     e = s * r * c
     dev = operator.device_operator
-    stream = torch.cuda.current_stream()
     x = torch.randn(n, dtype=torch.float32, device="cuda", generator=torch.Generator("cuda").manual_seed(0))
     y = torch.empty_like(x)
-    for _ in range(5):
-        dev.apply_f32_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record(stream)
-    for _ in range(launches):
-        dev.apply_f32_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
-    stop.record(stream)
-    torch.cuda.synchronize()
-    per_launch_s = start.elapsed_time(stop) / 1e3 / launches
+    table = []
+    for rows, zb in autotune.apply_candidates(operator.mesh.element_grid_shape):
+        dev.set_shape(rows, zb)
+        per = _time_launches(torch, stream, lambda: dev.apply_f32_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream),
+                             launches)
+        table.append((per, rows, zb))
+        log(f"apply rows={rows} zb={zb}: {per * 1e6:.1f} us")
+    per, rows, zb = min(table)
+    dev.set_shape(rows, zb)
     algorithmic = 13 * n + 12 * e  # x, y, robin (4 B/node each), free mask (1 B/node), coefficients (12 B/element)
-    achieved = algorithmic / per_launch_s / 1e9
+    achieved = algorithmic / per / 1e9
     return dict(
-        kernel="k_apply<float,8,0> (FP32 Q1 apply)",
-        per_launch_us=per_launch_s * 1e6,
-        gdof_s=n / per_launch_s / 1e9,
+        kernel=f"k_apply<float,{rows},0> (FP32 Q1 apply), {zb} planes per z block",
+        rows=rows,
+        zb=zb,
+        per_launch_us=per * 1e6,
+        gdof_s=n / per / 1e9,
+        sweep=[dict(rows=r_, zb=z_, us=p_ * 1e6) for p_, r_, z_ in table],
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                       traffic=None, algorithmic_bytes=algorithmic),
     )
@@ -102,6 +135,7 @@ def main() -> int:
     parser.add_argument("--size", type=int, default=512)
     parser.add_argument("--slabs", type=int, default=16)
     parser.add_argument("--no-cpu", action="store_true")
+    parser.add_argument("--rows", type=int, default=4, help="rows per wave strip of the FP32 inner stencil")
     args = parser.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only"}))
@@ -122,6 +156,7 @@ def main() -> int:
     rhs = operator.build_rhs(reference)
     config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000)
     batch = _capi.Batch([operator.device_operator])
+    batch.set_shape(args.rows, 1024)
     batch.set(0, rhs, None, config.to_c())
 
     log("warmup")
@@ -140,8 +175,11 @@ def main() -> int:
     log(f"solve: {result['outer_iterations']} outer / {result['inner_iterations']} inner, "
         f"rel {result['relative_residual']:.2e}, {elapsed / args.steps * 1e3:.1f} ms/solve")
 
-    roof = apply_roofline(operator, torch)
+    stream = torch.cuda.Stream()
+    roof = apply_roofline(operator, torch, stream)
     log(f"apply: {roof['per_launch_us']:.1f} us, {roof['roofline']['achieved']:.0f} GB/s")
+    copy_gbs = copy_bandwidth(torch, stream)
+    log(f"copy: {copy_gbs:.0f} GB/s")
     line = {
         "metric": "MPIR solves/s at 1e-10 rel. residual (512x512x16 board)",
         "value": args.steps / elapsed,
@@ -166,6 +204,9 @@ def main() -> int:
             "latency_p99_ms": float(np.percentile(latencies, 99) * 1e3),
             "apply_gdof_s": roof["gdof_s"],
             "apply_us": roof["per_launch_us"],
+            "apply_kernel": roof["kernel"],
+            "apply_sweep": roof["sweep"],
+            "stream_copy_gbs": copy_gbs,
             "parallelism": "single GPU",
         },
         "roofline": roof["roofline"],

@@ -118,6 +118,15 @@ int pcbhip_batch_destroy(pcbhip_batch *batch);
 /* Host rhs and initial guess (NULL: zero) and the slot's limits. */
 int pcbhip_batch_set(pcbhip_batch *batch, int slot, const double *rhs, const double *x0,
                      const pcbhip_mpir_config *config);
+/* Tile shape of the batch's kernels: node rows per wave strip of the FP32
+ * inner stencil (4 or 8) and the number of stencil workgroups a launch should
+ * reach before each board's z sweep is split into blocks.  The autotuner
+ * times candidates through this call.  No reference counterpart (the
+ * reference's CPU team splits node lines statically, hex_q1.cpp:195-205). */
+int pcbhip_batch_set_shape(pcbhip_batch *batch, int rows_inner, int target_groups);
+/* Tile shape of the standalone applies: rows per wave strip (FP32: 4 or 8,
+ * FP64: 2 or 4) and node planes per z block. */
+int pcbhip_operator_set_shape(pcbhip_operator *op, int rows32, int zb32, int rows64, int zb64);
 /* Device pointers to a slot's FP64 right-hand side (write it before run). */
 int pcbhip_batch_rhs_dev(pcbhip_batch *batch, int slot, double **rhs_dev);
 /* Reset every slot to its initial guess and iterate until every slot has

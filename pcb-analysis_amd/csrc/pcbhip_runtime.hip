@@ -71,6 +71,18 @@ size_t align256(size_t bytes) { return (bytes + 255) & ~size_t(255); }
 int strips_x(int C) { return (C + 1 + kStripCols - 1) / kStripCols; }
 int groups_y(int R, int rows) { return (R + 1 + rows * kWavesPerGroup - 1) / (rows * kWavesPerGroup); }
 
+// Planes per z block: the whole depth when the launch already has
+// ``target`` workgroups, else split into the blocks needed to reach it
+// (each block re-reads one halo plane on each side, so never fewer than 2
+// planes).
+int planes_per_block(int S, long long groups_xy, long long target) {
+    const int planes = S + 1;
+    if (groups_xy >= target) return planes;
+    const long long split = (target + groups_xy - 1) / groups_xy;
+    int zb = (int)((planes + split - 1) / split);
+    return zb < 2 ? 2 : (zb > planes ? planes : zb);
+}
+
 }  // namespace
 
 struct pcbhip_operator {
@@ -86,6 +98,7 @@ struct pcbhip_operator {
     float* dinv32 = nullptr;
     double* dinv64 = nullptr;
     double* diag64 = nullptr;
+    int rows32 = 4, zb32 = 0, rows64 = 4, zb64 = 0;  // standalone apply tile shapes
 
     OperatorDev view() const {
         OperatorDev v{};
@@ -141,6 +154,8 @@ struct pcbhip_batch {
     Tile* d_inner = nullptr;
     Tile* d_flat = nullptr;
     int n_outer = 0, n_inner = 0, n_flat = 0;
+    int rows_inner = 4;
+    long long target_groups = kTargetGroups;
     int* d_active = nullptr;
     int* h_active = nullptr;
     hipGraph_t graph = nullptr;
@@ -195,7 +210,9 @@ void apply_dev(const pcbhip_operator* op, const T* x, T* y, int mode, hipStream_
         coef = op->coef64;
         robin = op->robin64;
     }
-    check(launch::apply<T>(coef, robin, x, y, op->S, op->R, op->C, mode, stream), "apply launch");
+    const int rows = sizeof(T) == 4 ? op->rows32 : op->rows64;
+    const int zb = sizeof(T) == 4 ? op->zb32 : op->zb64;
+    check(launch::apply<T>(coef, robin, x, y, op->S, op->R, op->C, mode, rows, zb, stream), "apply launch");
 }
 
 template <typename T>
@@ -217,12 +234,60 @@ void apply_host(const pcbhip_operator* op, const T* x, T* y, int mode) {
     check(hipFree(dx), "hipFree");
 }
 
-// Tiles of one slot: stencil workgroups in (y group, x strip) order and the
-// pointwise chunks.
-void add_tiles(std::vector<Tile>& out, int slot, int nsx, int nsy) {
+// Tiles of one slot: stencil workgroups in (z block, y group, x strip)
+// order, so a contiguous range of tiles is a compact piece of the board.
+int add_tiles(std::vector<Tile>& out, int slot, int nsx, int nsy, int S, int zb) {
+    const int nsz = (S + 1 + zb - 1) / zb;
     int index = 0;
-    for (int sy = 0; sy < nsy; ++sy)
-        for (int sx = 0; sx < nsx; ++sx) out.push_back(Tile{slot, sx, sy, index++});
+    for (int sz = 0; sz < nsz; ++sz)
+        for (int sy = 0; sy < nsy; ++sy)
+            for (int sx = 0; sx < nsx; ++sx)
+                out.push_back(Tile{slot, index++, (short)sx, (short)sy, (short)sz, (short)zb});
+    return index;
+}
+
+void upload_tiles(pcbhip_batch* b, const std::vector<Tile>& t, Tile** d, int* n) {
+    if (*d) check(hipFree(*d), "hipFree");
+    *n = (int)t.size();
+    *d = static_cast<Tile*>(device_alloc(sizeof(Tile) * t.size()));
+    check(hipMemcpyAsync(*d, t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, b->stream), "upload");
+}
+
+// (Re)builds the three tile tables for the batch's current shape.  The
+// z split is chosen over the whole batch: many slots fill the device without
+// splitting any of them.
+void build_tiles(pcbhip_batch* b) {
+    const int rows_inner = b->inner64 ? kRowsInner64 : b->rows_inner;
+    long long xy_outer = 0, xy_inner = 0;
+    for (const SlotHost& s : b->slots) {
+        xy_outer += (long long)strips_x(s.op->C) * groups_y(s.op->R, kRowsOuter);
+        xy_inner += (long long)strips_x(s.op->C) * groups_y(s.op->R, rows_inner);
+    }
+    std::vector<Tile> outer, inner, flat;
+    for (int i = 0; i < (int)b->slots.size(); ++i) {
+        SlotHost& s = b->slots[(size_t)i];
+        const pcbhip_operator* op = s.op;
+        const int nsx = strips_x(op->C);
+        const int nyo = groups_y(op->R, kRowsOuter), nyi = groups_y(op->R, rows_inner);
+        const long long share_o = xy_outer > 0 ? (long long)nsx * nyo * b->target_groups / xy_outer : 0;
+        const long long share_i = xy_inner > 0 ? (long long)nsx * nyi * b->target_groups / xy_inner : 0;
+        const int zbo = planes_per_block(op->S, (long long)nsx * nyo, std::max(share_o, (long long)nsx * nyo));
+        const int zbi = planes_per_block(op->S, (long long)nsx * nyi, std::max(share_i, (long long)nsx * nyi));
+        s.tiles_outer = add_tiles(outer, i, nsx, nyo, op->S, zbo);
+        s.tiles_inner = add_tiles(inner, i, nsx, nyi, op->S, zbi);
+        s.tiles_flat = (int)((op->N + kFlatChunk - 1) / kFlatChunk);
+        for (int k = 0; k < s.tiles_flat; ++k) flat.push_back(Tile{i, k, 0, 0, 0, 0});
+        if (std::max({s.tiles_outer, s.tiles_inner, s.tiles_flat}) > s.part_stride)
+            fail(PCBHIP_RUNTIME, "tile count exceeds the partial-sum area");
+    }
+    upload_tiles(b, outer, &b->d_outer, &b->n_outer);
+    upload_tiles(b, inner, &b->d_inner, &b->n_inner);
+    upload_tiles(b, flat, &b->d_flat, &b->n_flat);
+    b->configs_dirty = true;
+    if (b->exec) {
+        check(hipGraphExecDestroy(b->exec), "graph exec destroy");
+        b->exec = nullptr;
+    }
 }
 
 void build_graph(pcbhip_batch* b, int steps) {
@@ -237,7 +302,7 @@ void build_graph(pcbhip_batch* b, int steps) {
     check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     for (int s = 0; s < steps; ++s) {
         const hipError_t err =
-            launch::mpir_step(b->inner64 != 0, b->d_slots, b->d_states, b->d_configs, b->d_tickets, b->d_outer,
+            launch::mpir_step(b->inner64 != 0, b->rows_inner, b->d_slots, b->d_states, b->d_configs, b->d_tickets, b->d_outer,
                               b->n_outer, b->d_inner, b->n_inner, b->d_flat, b->n_flat, b->d_active, b->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
@@ -303,6 +368,8 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
         op->N = N;
         op->E = E;
         check(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking), "stream create");
+        op->zb32 = planes_per_block(slabs, (long long)strips_x(cols) * groups_y(rows, op->rows32), kTargetGroups);
+        op->zb64 = planes_per_block(slabs, (long long)strips_x(cols) * groups_y(rows, op->rows64), kTargetGroups);
         const size_t s_c32 = align256(sizeof(float) * 3 * E), s_c64 = align256(sizeof(double) * 3 * E);
         const size_t s_n32 = align256(sizeof(float) * N), s_n64 = align256(sizeof(double) * N);
         char* base = static_cast<char*>(device_alloc(s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + 256));
@@ -404,24 +471,18 @@ int pcbhip_batch_create(int count, const pcbhip_operator* const* ops, int inner_
         b->device = device;
         b->inner64 = inner_fp64 ? 1 : 0;
         check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream create");
-        const int rows_inner = b->inner64 ? kRowsInner64 : kRowsInner32;
         const size_t ti = b->inner64 ? sizeof(double) : sizeof(float);
-        std::vector<Tile> outer, inner, flat;
         b->slots.resize((size_t)count);
         for (int i = 0; i < count; ++i) {
             SlotHost& s = b->slots[(size_t)i];
             const pcbhip_operator* op = ops[i];
             s.op = op;
-            const int nsx = strips_x(op->C);
-            const int nyo = groups_y(op->R, kRowsOuter), nyi = groups_y(op->R, rows_inner);
-            s.tiles_outer = nsx * nyo;
-            s.tiles_inner = nsx * nyi;
-            s.tiles_flat = (int)((op->N + kFlatChunk - 1) / kFlatChunk);
-            s.part_stride = std::max({s.tiles_outer, s.tiles_inner, s.tiles_flat});
+            // Partial-sum room for the finest tiling any shape can ask for:
+            // 2-row strips, 2-plane z blocks.
+            const long long finest = (long long)strips_x(op->C) * groups_y(op->R, 2) * ((op->S + 2) / 2);
+            const long long flat = (op->N + kFlatChunk - 1) / kFlatChunk;
+            s.part_stride = (int)std::max(finest, flat);
             s.history_rows = s.config.max_outer_iterations;
-            add_tiles(outer, i, nsx, nyo);
-            add_tiles(inner, i, nsx, nyi);
-            for (int k = 0; k < s.tiles_flat; ++k) flat.push_back(Tile{i, 0, 0, k});
             // Vectors of the slot in one allocation.  History room is
             // allocated on demand in pcbhip_batch_set.
             const size_t n64 = align256(sizeof(double) * op->N), nti = align256(ti * op->N);
@@ -442,14 +503,7 @@ int pcbhip_batch_create(int count, const pcbhip_operator* const* ops, int inner_
             s.part = reinterpret_cast<double*>(base);
             check(hipMemsetAsync(s.b, 0, 4 * n64, b->stream), "memset");
         }
-        auto upload_tiles = [&](const std::vector<Tile>& t, Tile** d, int* n) {
-            *n = (int)t.size();
-            *d = static_cast<Tile*>(device_alloc(sizeof(Tile) * t.size()));
-            check(hipMemcpyAsync(*d, t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, b->stream), "upload");
-        };
-        upload_tiles(outer, &b->d_outer, &b->n_outer);
-        upload_tiles(inner, &b->d_inner, &b->n_inner);
-        upload_tiles(flat, &b->d_flat, &b->n_flat);
+        build_tiles(b.get());
         b->d_slots = static_cast<SlotDev*>(device_alloc(sizeof(SlotDev) * count));
         b->d_states = static_cast<SlotState*>(device_alloc(sizeof(SlotState) * count));
         b->d_configs = static_cast<SlotConfig*>(device_alloc(sizeof(SlotConfig) * count));
@@ -500,6 +554,32 @@ int pcbhip_batch_set(pcbhip_batch* b, int slot, const double* rhs, const double*
         }
         check(hipStreamSynchronize(b->stream), "batch set");
         b->configs_dirty = true;
+    });
+}
+
+int pcbhip_batch_set_shape(pcbhip_batch* b, int rows_inner, int target_groups) {
+    return guarded([&] {
+        if (!b) fail(PCBHIP_INVALID_INPUT, "null batch");
+        if (rows_inner != 4 && rows_inner != 8) fail(PCBHIP_INVALID_INPUT, "rows_inner must be 4 or 8");
+        if (target_groups < 1) fail(PCBHIP_INVALID_INPUT, "target_groups must be positive");
+        set_device(b->device);
+        b->rows_inner = rows_inner;
+        b->target_groups = target_groups;
+        build_tiles(b);
+        check(hipStreamSynchronize(b->stream), "set shape");
+    });
+}
+
+int pcbhip_operator_set_shape(pcbhip_operator* op, int rows32, int zb32, int rows64, int zb64) {
+    return guarded([&] {
+        if (!op) fail(PCBHIP_INVALID_INPUT, "null operator");
+        if ((rows32 != 4 && rows32 != 8) || (rows64 != 2 && rows64 != 4))
+            fail(PCBHIP_INVALID_INPUT, "rows must be 4 or 8 (FP32) and 2 or 4 (FP64)");
+        if (zb32 < 1 || zb64 < 1) fail(PCBHIP_INVALID_INPUT, "planes per z block must be positive");
+        op->rows32 = rows32;
+        op->zb32 = std::min(zb32, op->S + 1);
+        op->rows64 = rows64;
+        op->zb64 = std::min(zb64, op->S + 1);
     });
 }
 

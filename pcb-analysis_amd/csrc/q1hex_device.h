@@ -28,11 +28,14 @@ constexpr int kGroupThreads = kWave * kWavesPerGroup;
 // Nodes per workgroup of the pointwise kernel.
 constexpr int kFlatPerThread = 8;
 constexpr int kFlatChunk = kGroupThreads * kFlatPerThread;
-// Rows per wave strip of the FP64 outer-residual kernel and of the inner
-// kernels in each precision.
+// Rows per wave strip of the FP64 outer-residual kernel and of the FP64
+// inner kernel; the FP32 inner kernel and the standalone apply are built for
+// 4 and 8 rows and the tile shape picks one.
 constexpr int kRowsOuter = 4;
-constexpr int kRowsInner32 = 8;
 constexpr int kRowsInner64 = 4;
+// Stencil workgroups wanted per launch before the z sweep is split into
+// blocks (256 CUs, several waves each).
+constexpr int kTargetGroups = 1024;
 
 enum Phase : int { kPhaseOuter = 0, kPhaseInner = 1, kPhaseDone = 2 };
 enum Status : int { kStatusOk = 0, kStatusNotSpd = 1 };
@@ -98,20 +101,22 @@ struct SlotTickets {
     int outer, inner, update, pad;
 };
 
-// One workgroup of work: slot, x strip, y group, index of the tile within the
-// slot (its partial-sum slot).
+// One workgroup of work: slot, index of the tile within the slot (its
+// partial-sum slot), x strip, y group, z block and planes per z block.
 struct Tile {
-    int slot, sx, sy, index;
+    int slot, index;
+    short sx, sy, sz, zb;
 };
 
 namespace launch {
 // Host-side launchers of the kernels (q1hex_kernels.hip).
 template <typename T>
-hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, int C, int mode, hipStream_t stream);
+hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, int C, int mode, int rows, int zb,
+                 hipStream_t stream);
 hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S, int R,
                     int C, int* bad, hipStream_t stream);
 hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream);
-hipError_t mpir_step(bool inner64, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
+hipError_t mpir_step(bool inner64, int rows_inner, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
                      SlotTickets* tickets, const Tile* outer_tiles, int n_outer, const Tile* inner_tiles, int n_inner,
                      const Tile* flat_tiles, int n_flat, int* active, hipStream_t stream);
 }  // namespace launch

@@ -39,8 +39,8 @@ constexpr int kRange = 0x3FFFFFFF;
 
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 
-__device__ __forceinline__ Rsrc rsrc(const void* base) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, kRange, 0x00020000);
+__device__ __forceinline__ Rsrc rsrc(const void* base, const int bytes = kRange) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
 }
 __device__ __forceinline__ float ld(const Rsrc r, const unsigned off, float) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off * 4u, 0, 0));
@@ -52,16 +52,16 @@ template <typename T>
 __device__ __forceinline__ T ldt(const T* base, const unsigned off) {
     return ld(rsrc(base), off, T());
 }
-__device__ __forceinline__ void st(const float v, const Rsrc r, const unsigned off) {
+__device__ __forceinline__ void store(const float v, const Rsrc r, const unsigned off) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off * 4u, 0, 0);
 }
-__device__ __forceinline__ void st(const double v, const Rsrc r, const unsigned off) {
+__device__ __forceinline__ void store(const double v, const Rsrc r, const unsigned off) {
     typedef unsigned u2 __attribute__((ext_vector_type(2)));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, off * 8u, 0, 0);
 }
 template <typename T>
 __device__ __forceinline__ void stt(T* base, const unsigned off, const T v) {
-    st(v, rsrc(base), off);
+    store(v, rsrc(base), off);
 }
 
 template <typename T>
@@ -86,7 +86,8 @@ __device__ __forceinline__ T masked(const NodeVal<T>& v) {
 // Robin term).
 template <typename T, int ROWS, class Load, class Emit>
 __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const int S, const int R, const int C,
-                                            const int x0, const int y0, Load&& load, Emit&& emit) {
+                                            const int x0, const int y0, const int zlo, const int zhi, Load&& load,
+                                            Emit&& emit) {
     const int lane = threadIdx.x & (kWave - 1);
     const int xc = x0 - 1 + lane;
     const bool col_in = xc >= 0 && xc <= C;
@@ -136,14 +137,18 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
         }
     };
 
-    load_plane(0, mlo, nlo, vlo);
+    // Output planes [zlo, zhi) need element layers zlo - 1 .. zhi - 1 (those
+    // that exist); the first layer only completes a plane that is not output.
+    const int first_layer = zlo > 0 ? zlo - 1 : 0;
+    const int last_layer = zhi - 1 < S - 1 ? zhi - 1 : S - 1;
+    load_plane(first_layer, mlo, nlo, vlo);
 #pragma unroll
     for (int i = 0; i < ROWS; ++i) {
         Llo[i] = T(0);
         Rlo[i] = T(0);
     }
 
-    for (int ez = 0; ez < S; ++ez) {
+    for (int ez = first_layer; ez <= last_layer; ++ez) {
         T mhi[ROWS + 2], nhi[ROWS + 2];
         NodeVal<T> vhi[ROWS];
         load_plane(ez + 1, mhi, nhi, vhi);
@@ -223,7 +228,7 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
             }
         }
 
-        finish_plane(ez, Llo, Rlo, vlo);
+        if (ez >= zlo) finish_plane(ez, Llo, Rlo, vlo);
 
 #pragma unroll
         for (int i = 0; i < ROWS + 2; ++i) {
@@ -237,7 +242,17 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
             Rlo[i] = Rhi[i];
         }
     }
-    finish_plane(S, Llo, Rlo, vlo);
+    if (zhi == S + 1) finish_plane(S, Llo, Rlo, vlo);
+}
+
+// Workgroups are dealt to the eight XCDs round-robin; give each XCD a
+// contiguous range of tiles so neighbouring strips (which share halo rows)
+// meet in the same L2.
+__device__ __forceinline__ int xcd_tile(const int block, const int count) {
+    constexpr int kXcds = 8;
+    const int q = count / kXcds, r = count % kXcds;
+    const int k = block % kXcds, j = block / kXcds;
+    return k * q + (k < r ? k : r) + j;
 }
 
 // ------------------------------------------------------------ reductions
@@ -306,15 +321,20 @@ __device__ bool reduce_last(double (&v)[NV], double* __restrict__ part, const in
 template <typename T, int ROWS, int MODE>
 __global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ coef, const T* __restrict__ robin,
                                                           const T* __restrict__ x, T* __restrict__ y, const int S,
-                                                          const int R, const int C, const int nsx) {
+                                                          const int R, const int C, const int nsx, const int nsy,
+                                                          const int zb) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int sx = blockIdx.x % nsx;
-    const int sy = (blockIdx.x / nsx) * kWavesPerGroup + wave;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int sx = t % nsx;
+    const int sy = (t / nsx) % nsy;
+    const int sz = t / (nsx * nsy);
     const int x0 = sx * kStripCols;
-    const int y0 = sy * ROWS;
+    const int y0 = (sy * kWavesPerGroup + wave) * ROWS;
+    const int zlo = sz * zb;
+    const int zhi = zlo + zb < S + 1 ? zlo + zb : S + 1;
     if (y0 > R) return;
     strip_apply<T, ROWS>(
-        coef, S, R, C, x0, y0,
+        coef, S, R, C, x0, y0, zlo, zhi,
         [&](const long long pb, const unsigned off, const bool) -> NodeVal<T> {
             if (MODE == 1) return {ldt(x + pb, off), T(0)};
             return {ldt(x + pb, off), ldt(robin + pb, off)};
@@ -397,7 +417,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
                                                          const SlotConfig* __restrict__ configs,
                                                          SlotTickets* tickets, const Tile* __restrict__ tiles,
                                                          int* active) {
-    const Tile tile = tiles[blockIdx.x];
+    const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     SlotState& st = states[tile.slot];
     if (st.phase != kPhaseOuter) return;
     const SlotDev& sd = slots[tile.slot];
@@ -405,6 +425,8 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int x0 = tile.sx * kStripCols;
     const int y0 = (tile.sy * kWavesPerGroup + wave) * kRowsOuter;
+    const int zlo = tile.sz * tile.zb;
+    const int zhi = zlo + tile.zb < op.S + 1 ? zlo + tile.zb : op.S + 1;
     const int pending = st.pending;
     const int xcur = st.xcur;
     const double* xin = sd.x64[xcur];
@@ -417,7 +439,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
     double acc[4] = {0.0, 0.0, 0.0, 0.0};  // b.b, r.r (FP64), r.r and r.z (inner precision)
     if (y0 <= op.R) {
         strip_apply<double, kRowsOuter>(
-            op.coef64, op.S, op.R, op.C, x0, y0,
+            op.coef64, op.S, op.R, op.C, x0, y0, zlo, zhi,
             [&](const long long pb, const unsigned off, const bool owned) -> NodeVal<double> {
                 double xv = ldt(xin + pb, off);
                 if (pending == kPendingCorrection) xv += (double)ldt(cin + pb, off);
@@ -490,7 +512,7 @@ template <typename TI, int ROWS>
 __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __restrict__ slots, SlotState* states,
                                                                SlotTickets* tickets, const Tile* __restrict__ tiles,
                                                                int* active) {
-    const Tile tile = tiles[blockIdx.x];
+    const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     SlotState& st = states[tile.slot];
     if (st.phase != kPhaseInner) return;
     const SlotDev& sd = slots[tile.slot];
@@ -498,6 +520,8 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int x0 = tile.sx * kStripCols;
     const int y0 = (tile.sy * kWavesPerGroup + wave) * ROWS;
+    const int zlo = tile.sz * tile.zb;
+    const int zhi = zlo + tile.zb < op.S + 1 ? zlo + tile.zb : op.S + 1;
     const bool first = st.first != 0;
     const TI beta = (TI)st.beta;
     const int pcur = st.pcur;
@@ -510,7 +534,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
     double acc[1] = {0.0};
     if (y0 <= op.R) {
         strip_apply<TI, ROWS>(
-            InnerOf<TI>::coef(op), op.S, op.R, op.C, x0, y0,
+            InnerOf<TI>::coef(op), op.S, op.R, op.C, x0, y0, zlo, zhi,
             [&](const long long pb, const unsigned off, const bool owned) -> NodeVal<TI> {
                 const TI z = ldt(r + pb, off) * ldt(dinv + pb, off);
                 const TI pv = first ? z : z + beta * ldt(pin + pb, off);
@@ -548,32 +572,42 @@ template <typename TI>
 __global__ __launch_bounds__(kGroupThreads) void k_inner_update(const SlotDev* __restrict__ slots, SlotState* states,
                                                                 const SlotConfig* __restrict__ configs,
                                                                 SlotTickets* tickets, const Tile* __restrict__ tiles) {
-    const Tile tile = tiles[blockIdx.x];
+    const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     SlotState& st = states[tile.slot];
     if (st.phase != kPhaseInner) return;
     const SlotDev& sd = slots[tile.slot];
-    const long long N = sd.op.N;
     const TI alpha = (TI)st.alpha;
     const bool fresh = st.inner_its == 0;
-    const TI* __restrict__ p = static_cast<const TI*>(sd.p[st.pcur]);
-    const TI* __restrict__ q = static_cast<const TI*>(sd.q);
-    TI* __restrict__ r = static_cast<TI*>(sd.r);
-    TI* __restrict__ c = static_cast<TI*>(sd.c);
-    const TI* __restrict__ dinv = InnerOf<TI>::dinv(sd.op);
-    double acc[2] = {0.0, 0.0};
+    // The chunk's buffers end at the last node, so loads past it read zero
+    // and stores past it are dropped: no bounds branch.
     const long long base = (long long)tile.index * kFlatChunk;
+    const long long left = sd.op.N - base;
+    const int bytes = (int)((left < kFlatChunk ? left : kFlatChunk) * (long long)sizeof(TI));
+    const Rsrc rp = rsrc(static_cast<const TI*>(sd.p[st.pcur]) + base, bytes);
+    const Rsrc rq = rsrc(static_cast<const TI*>(sd.q) + base, bytes);
+    const Rsrc rr = rsrc(static_cast<TI*>(sd.r) + base, bytes);
+    const Rsrc rc = rsrc(static_cast<TI*>(sd.c) + base, bytes);
+    const Rsrc rd = rsrc(InnerOf<TI>::dinv(sd.op) + base, bytes);
+    TI pv[kFlatPerThread], qv[kFlatPerThread], rv[kFlatPerThread], cv[kFlatPerThread], dv[kFlatPerThread];
 #pragma unroll
     for (int k = 0; k < kFlatPerThread; ++k) {
-        const long long i = base + (long long)k * kGroupThreads + threadIdx.x;
-        if (i < N) {
-            const TI pv = p[i];
-            const TI cv = fresh ? alpha * pv : c[i] + alpha * pv;
-            const TI rv = r[i] - alpha * q[i];
-            c[i] = cv;
-            r[i] = rv;
-            acc[0] += (double)rv * (double)rv;
-            acc[1] += (double)rv * (double)(rv * dinv[i]);
-        }
+        const unsigned off = (unsigned)(k * kGroupThreads) + threadIdx.x;
+        pv[k] = ld(rp, off, TI());
+        qv[k] = ld(rq, off, TI());
+        rv[k] = ld(rr, off, TI());
+        cv[k] = fresh ? TI(0) : ld(rc, off, TI());
+        dv[k] = ld(rd, off, TI());
+    }
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kFlatPerThread; ++k) {
+        const unsigned off = (unsigned)(k * kGroupThreads) + threadIdx.x;
+        const TI c = fresh ? alpha * pv[k] : cv[k] + alpha * pv[k];
+        const TI r = rv[k] - alpha * qv[k];
+        store(c, rc, off);
+        store(r, rr, off);
+        acc[0] += (double)r * (double)r;
+        acc[1] += (double)r * (double)(r * dv[k]);
     }
     double total[2];
     if (!reduce_last<2>(acc, sd.part, sd.part_stride, tile.index, sd.tiles_flat, &tickets[tile.slot].update, total))
@@ -611,22 +645,34 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_update(const SlotDev* _
 namespace launch {
 
 template <typename T>
-hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, int C, int mode,
+hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, int C, int mode, int rows, int zb,
                  hipStream_t stream) {
-    constexpr int rows = sizeof(T) == 4 ? kRowsInner32 : kRowsInner64;
     const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
     const int nsy = (R + 1 + rows * kWavesPerGroup - 1) / (rows * kWavesPerGroup);
-    const dim3 grid(nsx * nsy), block(kGroupThreads);
-    if (mode == 0) {
-        hipLaunchKernelGGL((k_apply<T, rows, 0>), grid, block, 0, stream, coef, robin, x, y, S, R, C, nsx);
+    const int nsz = (S + 1 + zb - 1) / zb;
+    const dim3 grid(nsx * nsy * nsz), block(kGroupThreads);
+#define PCBHIP_APPLY(ROWS, MODE)                                                                                   \
+    hipLaunchKernelGGL((k_apply<T, ROWS, MODE>), grid, block, 0, stream, coef, robin, x, y, S, R, C, nsx, nsy, zb)
+    if constexpr (sizeof(T) == 4) {
+        if (rows == 8) {
+            if (mode == 0) PCBHIP_APPLY(8, 0); else PCBHIP_APPLY(8, 1);
+        } else {
+            if (mode == 0) PCBHIP_APPLY(4, 0); else PCBHIP_APPLY(4, 1);
+        }
     } else {
-        hipLaunchKernelGGL((k_apply<T, rows, 1>), grid, block, 0, stream, coef, robin, x, y, S, R, C, nsx);
+        if (rows == 2) {
+            if (mode == 0) PCBHIP_APPLY(2, 0); else PCBHIP_APPLY(2, 1);
+        } else {
+            if (mode == 0) PCBHIP_APPLY(4, 0); else PCBHIP_APPLY(4, 1);
+        }
     }
+#undef PCBHIP_APPLY
     return hipGetLastError();
 }
 
-template hipError_t apply<float>(const float*, const float*, const float*, float*, int, int, int, int, hipStream_t);
-template hipError_t apply<double>(const double*, const double*, const double*, double*, int, int, int, int,
+template hipError_t apply<float>(const float*, const float*, const float*, float*, int, int, int, int, int, int,
+                                 hipStream_t);
+template hipError_t apply<double>(const double*, const double*, const double*, double*, int, int, int, int, int, int,
                                   hipStream_t);
 
 hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S,
@@ -646,7 +692,7 @@ hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t st
     return hipGetLastError();
 }
 
-hipError_t mpir_step(bool inner64, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
+hipError_t mpir_step(bool inner64, int rows_inner, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
                      SlotTickets* tickets, const Tile* outer_tiles, int n_outer, const Tile* inner_tiles,
                      int n_inner, const Tile* flat_tiles, int n_flat, int* active, hipStream_t stream) {
     const dim3 block(kGroupThreads);
@@ -660,8 +706,13 @@ hipError_t mpir_step(bool inner64, const SlotDev* slots, SlotState* states, cons
     } else {
         hipLaunchKernelGGL(k_outer<float>, dim3(n_outer), block, 0, stream, slots, states, configs, tickets,
                            outer_tiles, active);
-        hipLaunchKernelGGL((k_inner_apply<float, kRowsInner32>), dim3(n_inner), block, 0, stream, slots, states,
-                           tickets, inner_tiles, active);
+        if (rows_inner == 8) {
+            hipLaunchKernelGGL((k_inner_apply<float, 8>), dim3(n_inner), block, 0, stream, slots, states, tickets,
+                               inner_tiles, active);
+        } else {
+            hipLaunchKernelGGL((k_inner_apply<float, 4>), dim3(n_inner), block, 0, stream, slots, states, tickets,
+                               inner_tiles, active);
+        }
         hipLaunchKernelGGL(k_inner_update<float>, dim3(n_flat), block, 0, stream, slots, states, configs, tickets,
                            flat_tiles);
     }

@@ -41,6 +41,8 @@ EXPORTS = (
     "pcbhip_batch_create",
     "pcbhip_batch_destroy",
     "pcbhip_batch_set",
+    "pcbhip_batch_set_shape",
+    "pcbhip_operator_set_shape",
     "pcbhip_batch_rhs_dev",
     "pcbhip_batch_run",
     "pcbhip_batch_result",
@@ -102,6 +104,8 @@ def load() -> ctypes.CDLL:
     lib.pcbhip_batch_destroy.argtypes = [_vp]
     lib.pcbhip_batch_set.argtypes = [_vp, _int, _vp, _vp, ctypes.POINTER(MpirConfigC)]
     lib.pcbhip_batch_rhs_dev.argtypes = [_vp, _int, ctypes.POINTER(_vp)]
+    lib.pcbhip_batch_set_shape.argtypes = [_vp, _int, _int]
+    lib.pcbhip_operator_set_shape.argtypes = [_vp, _int, _int, _int, _int]
     lib.pcbhip_batch_run.argtypes = [_vp]
     lib.pcbhip_batch_result.argtypes = [_vp, _int, _vp, ctypes.POINTER(MpirResultC), _vp]
     lib.pcbhip_copy_dev.argtypes = [_vp, _vp, ctypes.c_int64, _vp]
@@ -199,6 +203,9 @@ class Operator:
         check(self._lib.pcbhip_stiffness_f64(self.handle, _ptr(v), _ptr(out)))
         return out
 
+    def set_shape(self, rows32: int, zb32: int, rows64: int = 4, zb64: int = 1 << 20) -> None:
+        check(self._lib.pcbhip_operator_set_shape(self.handle, rows32, zb32, rows64, zb64))
+
     def apply_f32_dev(self, x_ptr: int, y_ptr: int, stream: int | None = None) -> None:
         check(self._lib.pcbhip_apply_f32_dev(self.handle, x_ptr, y_ptr, stream))
 
@@ -257,6 +264,9 @@ class Batch:
                                          None if config is None else ctypes.byref(config)))
         if config is not None:
             self.max_outer[slot] = int(config.max_outer_iterations)
+
+    def set_shape(self, rows_inner: int, target_groups: int) -> None:
+        check(self._lib.pcbhip_batch_set_shape(self.handle, int(rows_inner), int(target_groups)))
 
     def rhs_device_pointer(self, slot: int) -> int:
         p = _vp()
