@@ -149,7 +149,6 @@ struct pcbhip_batch {
     SlotDev* d_slots = nullptr;
     SlotState* d_states = nullptr;
     SlotConfig* d_configs = nullptr;
-    SlotTickets* d_tickets = nullptr;
     Tile* d_outer = nullptr;
     Tile* d_inner = nullptr;
     Tile* d_flat = nullptr;
@@ -170,7 +169,7 @@ struct pcbhip_batch {
             if (s.block) (void)hipFree(s.block);
             if (s.history) (void)hipFree(s.history);
         }
-        for (void* p : {(void*)d_slots, (void*)d_states, (void*)d_configs, (void*)d_tickets, (void*)d_outer,
+        for (void* p : {(void*)d_slots, (void*)d_states, (void*)d_configs, (void*)d_outer,
                         (void*)d_inner, (void*)d_flat, (void*)d_active})
             if (p) (void)hipFree(p);
         if (h_active) (void)hipHostFree(h_active);
@@ -302,8 +301,9 @@ void build_graph(pcbhip_batch* b, int steps) {
     check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     for (int s = 0; s < steps; ++s) {
         const hipError_t err =
-            launch::mpir_step(b->inner64 != 0, b->rows_inner, b->d_slots, b->d_states, b->d_configs, b->d_tickets, b->d_outer,
-                              b->n_outer, b->d_inner, b->n_inner, b->d_flat, b->n_flat, b->d_active, b->stream);
+            launch::mpir_step(b->inner64 != 0, b->rows_inner, (int)b->slots.size(), b->d_slots, b->d_states,
+                              b->d_configs, b->d_outer, b->n_outer, b->d_inner, b->n_inner, b->d_flat, b->n_flat,
+                              b->d_active, b->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(b->stream, &dropped);
@@ -507,11 +507,9 @@ int pcbhip_batch_create(int count, const pcbhip_operator* const* ops, int inner_
         b->d_slots = static_cast<SlotDev*>(device_alloc(sizeof(SlotDev) * count));
         b->d_states = static_cast<SlotState*>(device_alloc(sizeof(SlotState) * count));
         b->d_configs = static_cast<SlotConfig*>(device_alloc(sizeof(SlotConfig) * count));
-        b->d_tickets = static_cast<SlotTickets*>(device_alloc(sizeof(SlotTickets) * count));
         b->d_active = static_cast<int*>(device_alloc(sizeof(int)));
         check(hipHostMalloc(reinterpret_cast<void**>(&b->h_active), sizeof(int), hipHostMallocDefault),
               "pinned alloc");
-        check(hipMemsetAsync(b->d_tickets, 0, sizeof(SlotTickets) * count, b->stream), "memset");
         check(hipStreamSynchronize(b->stream), "batch create");
         *out = b.release();
     });

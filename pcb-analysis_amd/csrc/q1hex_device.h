@@ -36,6 +36,8 @@ constexpr int kRowsInner64 = 4;
 // Stencil workgroups wanted per launch before the z sweep is split into
 // blocks (256 CUs, several waves each).
 constexpr int kTargetGroups = 1024;
+// Workgroups of the persistent outer-residual kernel.
+constexpr int kOuterGroups = 512;
 
 enum Phase : int { kPhaseOuter = 0, kPhaseInner = 1, kPhaseDone = 2 };
 enum Status : int { kStatusOk = 0, kStatusNotSpd = 1 };
@@ -85,8 +87,8 @@ struct SlotConfig {
     int max_outer, max_inner;
 };
 
-// Mutable scalars of one slot.  Written only by the last workgroup of a
-// kernel to finish the slot's tiles; read by every workgroup of later launches.
+// Mutable scalars of one slot.  Written only by the one-workgroup "finish"
+// kernels that follow each compute kernel; read by every later launch.
 struct SlotState {
     int phase, first, pending, outer;
     int inner_its, total_inner, high_apps, low_apps;
@@ -94,11 +96,6 @@ struct SlotState {
     int started, pad0, pad1, pad2;
     double rz, rhs_norm, rel_outer, rel_inner;
     double alpha, beta, scale, target;
-};
-
-// Tickets of the last-workgroup reductions, one counter per kernel and slot.
-struct SlotTickets {
-    int outer, inner, update, pad;
 };
 
 // One workgroup of work: slot, index of the tile within the slot (its
@@ -116,9 +113,9 @@ hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, 
 hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S, int R,
                     int C, int* bad, hipStream_t stream);
 hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream);
-hipError_t mpir_step(bool inner64, int rows_inner, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
-                     SlotTickets* tickets, const Tile* outer_tiles, int n_outer, const Tile* inner_tiles, int n_inner,
-                     const Tile* flat_tiles, int n_flat, int* active, hipStream_t stream);
+hipError_t mpir_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
+                     const SlotConfig* configs, const Tile* outer_tiles, int n_outer, const Tile* inner_tiles,
+                     int n_inner, const Tile* flat_tiles, int n_flat, int* active, hipStream_t stream);
 }  // namespace launch
 
 }  // namespace pcbhip

@@ -263,16 +263,13 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Reduces NV per-thread values over the workgroup, stores them as this
-// workgroup's partials and takes a ticket.  Returns true in the workgroup
-// that arrived last, with ``total`` holding the fixed-order sums over all
-// ``count`` partials of the slot.
+// Reduces NV per-thread values over the workgroup (wave butterflies, then
+// the waves in order) and stores them as partial ``index`` of the slot.  The
+// partials become visible to the next kernel at the launch boundary, so no
+// workgroup needs a device-scope fence.
 template <int NV>
-__device__ bool reduce_last(double (&v)[NV], double* __restrict__ part, const int stride, const int index,
-                            const int count, int* ticket, double (&total)[NV]) {
-    __shared__ double red[kGroupThreads];
+__device__ void group_partials(const double (&v)[NV], double* __restrict__ part, const int stride, const int index) {
     __shared__ double waves[kWavesPerGroup][NV];
-    __shared__ int last;
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
 #pragma unroll
@@ -288,19 +285,20 @@ __device__ bool reduce_last(double (&v)[NV], double* __restrict__ part, const in
             for (int w = 0; w < kWavesPerGroup; ++w) s += waves[w][k];
             part[k * stride + index] = s;
         }
-        __threadfence();
-        const int t = atomicAdd(ticket, 1);
-        last = (t == count - 1) ? 1 : 0;
     }
     __syncthreads();
-    if (!last) return false;
-    __threadfence();
+}
+
+// Fixed-order sum of ``count`` partials of one slot by one workgroup:
+// thread t adds partials t, t + 256, ... in order, then a fixed tree.
+template <int NV>
+__device__ void slot_sum(const double* __restrict__ part, const int stride, const int count, double (&total)[NV]) {
+    __shared__ double red[kGroupThreads];
+    const int tid = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         double s = 0.0;
-        for (int i = tid; i < count; i += kGroupThreads) {
-            s += __hip_atomic_load(part + k * stride + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        for (int i = tid; i < count; i += kGroupThreads) s += part[k * stride + i];
         red[tid] = s;
         __syncthreads();
         for (int width = kGroupThreads / 2; width > 0; width >>= 1) {
@@ -310,8 +308,6 @@ __device__ bool reduce_last(double (&v)[NV], double* __restrict__ part, const in
         total[k] = red[0];
         __syncthreads();
     }
-    if (tid == 0) __hip_atomic_store(ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
 }
 
 // --------------------------------------------------- standalone applies
@@ -413,63 +409,81 @@ struct InnerOf<double> {
 // decide convergence, and start the inner solve on r rounded to the inner
 // precision (with the first r.z of the Jacobi PCG, hex_q1.cpp:434-446).
 template <typename TI>
-__global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restrict__ slots, SlotState* states,
-                                                         const SlotConfig* __restrict__ configs,
-                                                         SlotTickets* tickets, const Tile* __restrict__ tiles,
-                                                         int* active) {
-    const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    SlotState& st = states[tile.slot];
-    if (st.phase != kPhaseOuter) return;
-    const SlotDev& sd = slots[tile.slot];
-    const OperatorDev& op = sd.op;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int x0 = tile.sx * kStripCols;
-    const int y0 = (tile.sy * kWavesPerGroup + wave) * kRowsOuter;
-    const int zlo = tile.sz * tile.zb;
-    const int zhi = zlo + tile.zb < op.S + 1 ? zlo + tile.zb : op.S + 1;
-    const int pending = st.pending;
-    const int xcur = st.xcur;
-    const double* xin = sd.x64[xcur];
-    double* xout = sd.x64[1 - xcur];
-    const TI* __restrict__ cin = static_cast<const TI*>(sd.c);
-    TI* __restrict__ rout = static_cast<TI*>(sd.r);
-    const TI* __restrict__ dinv = InnerOf<TI>::dinv(op);
-    const double* __restrict__ robin = op.robin64;
-    const double* __restrict__ b = sd.b64;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};  // b.b, r.r (FP64), r.r and r.z (inner precision)
-    if (y0 <= op.R) {
-        strip_apply<double, kRowsOuter>(
-            op.coef64, op.S, op.R, op.C, x0, y0, zlo, zhi,
-            [&](const long long pb, const unsigned off, const bool owned) -> NodeVal<double> {
-                double xv = ldt(xin + pb, off);
-                if (pending == kPendingCorrection) xv += (double)ldt(cin + pb, off);
-                if (owned && pending != kPendingNone) stt(xout + pb, off, xv);
-                return {xv, ldt(robin + pb, off)};
-            },
-            [&](const long long pb, const unsigned off, const double conduction, const NodeVal<double>& v) {
-                const double ax = v.robin >= 0.0 ? conduction + v.robin * v.u : v.u;
-                const double bv = ldt(b + pb, off);
-                const double r = bv - ax;
-                const TI ri = (TI)r;
-                stt(rout + pb, off, ri);
-                acc[0] += bv * bv;
-                acc[1] += r * r;
-                acc[2] += (double)ri * (double)ri;
-                acc[3] += (double)ri * (double)(ri * ldt(dinv + pb, off));
-            });
+__global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restrict__ slots,
+                                                         const SlotState* __restrict__ states,
+                                                         const Tile* __restrict__ tiles, const int count) {
+    // Persistent over the tile table: in most steps no slot is in its outer
+    // phase and each workgroup only reads phases.
+    for (int t = blockIdx.x; t < count; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const SlotState& st = states[tile.slot];
+        if (st.phase != kPhaseOuter) continue;
+        const SlotDev& sd = slots[tile.slot];
+        const OperatorDev& op = sd.op;
+        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+        const int x0 = tile.sx * kStripCols;
+        const int y0 = (tile.sy * kWavesPerGroup + wave) * kRowsOuter;
+        const int zlo = tile.sz * tile.zb;
+        const int zhi = zlo + tile.zb < op.S + 1 ? zlo + tile.zb : op.S + 1;
+        const int pending = st.pending;
+        const int xcur = st.xcur;
+        const double* xin = sd.x64[xcur];
+        double* xout = sd.x64[1 - xcur];
+        const TI* __restrict__ cin = static_cast<const TI*>(sd.c);
+        TI* __restrict__ rout = static_cast<TI*>(sd.r);
+        const TI* __restrict__ dinv = InnerOf<TI>::dinv(op);
+        const double* __restrict__ robin = op.robin64;
+        const double* __restrict__ b = sd.b64;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};  // b.b, r.r (FP64), r.r and r.z (inner precision)
+        if (y0 <= op.R) {
+            strip_apply<double, kRowsOuter>(
+                op.coef64, op.S, op.R, op.C, x0, y0, zlo, zhi,
+                [&](const long long pb, const unsigned off, const bool owned) -> NodeVal<double> {
+                    double xv = ldt(xin + pb, off);
+                    if (pending == kPendingCorrection) xv += (double)ldt(cin + pb, off);
+                    if (owned && pending != kPendingNone) stt(xout + pb, off, xv);
+                    return {xv, ldt(robin + pb, off)};
+                },
+                [&](const long long pb, const unsigned off, const double conduction, const NodeVal<double>& v) {
+                    const double ax = v.robin >= 0.0 ? conduction + v.robin * v.u : v.u;
+                    const double bv = ldt(b + pb, off);
+                    const double r = bv - ax;
+                    const TI ri = (TI)r;
+                    stt(rout + pb, off, ri);
+                    acc[0] += bv * bv;
+                    acc[1] += r * r;
+                    acc[2] += (double)ri * (double)ri;
+                    acc[3] += (double)ri * (double)(ri * ldt(dinv + pb, off));
+                });
+        }
+        group_partials<4>(acc, sd.part, sd.part_stride, tile.index);
     }
+}
+
+// Outer step decision, one workgroup per slot: ||b|| (first call), ||r||,
+// convergence, and the start of the inner solve (hex_q1.cpp:870-911,
+// solver.py:638-686).
+template <typename TI>
+__global__ __launch_bounds__(kGroupThreads) void k_finish_outer(const SlotDev* __restrict__ slots,
+                                                                SlotState* states,
+                                                                const SlotConfig* __restrict__ configs,
+                                                                int* active) {
+    const int slot = blockIdx.x;
+    SlotState& st = states[slot];
+    if (st.phase != kPhaseOuter) return;
+    const SlotDev& sd = slots[slot];
     double total[4];
-    if (!reduce_last<4>(acc, sd.part, sd.part_stride, tile.index, sd.tiles_outer, &tickets[tile.slot].outer, total))
-        return;
+    slot_sum<4>(sd.part, sd.part_stride, sd.tiles_outer, total);
     if (threadIdx.x != 0) return;
-    const SlotConfig cfg = configs[tile.slot];
+    const SlotConfig cfg = configs[slot];
+    const int pending = st.pending;
     if (!st.started) {
         const double bn = sqrt(total[0]);
         st.scale = bn > 0.0 ? bn : 1.0;
         st.target = cfg.atol + cfg.rtol * st.scale;
         st.started = 1;
     }
-    if (pending != kPendingNone) st.xcur = 1 - xcur;
+    if (pending != kPendingNone) st.xcur = 1 - st.xcur;
     st.pending = kPendingNone;
     st.high_apps += 1;
     const double rn = sqrt(total[1]);
@@ -509,11 +523,11 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
 // workgroup checks the curvature and forms alpha = rz / p.q
 // (hex_q1.cpp:454-464).
 template <typename TI, int ROWS>
-__global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __restrict__ slots, SlotState* states,
-                                                               SlotTickets* tickets, const Tile* __restrict__ tiles,
-                                                               int* active) {
+__global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __restrict__ slots,
+                                                               const SlotState* __restrict__ states,
+                                                               const Tile* __restrict__ tiles) {
     const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    SlotState& st = states[tile.slot];
+    const SlotState& st = states[tile.slot];
     if (st.phase != kPhaseInner) return;
     const SlotDev& sd = slots[tile.slot];
     const OperatorDev& op = sd.op;
@@ -547,11 +561,21 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
                 acc[0] += (double)v.u * (double)ap;
             });
     }
+    group_partials<1>(acc, sd.part, sd.part_stride, tile.index);
+}
+
+// Curvature check and alpha = rz / p.q, one workgroup per slot.
+template <typename TI>
+__global__ __launch_bounds__(kGroupThreads) void k_finish_apply(const SlotDev* __restrict__ slots, SlotState* states,
+                                                                int* active) {
+    const int slot = blockIdx.x;
+    SlotState& st = states[slot];
+    if (st.phase != kPhaseInner) return;
+    const SlotDev& sd = slots[slot];
     double total[1];
-    if (!reduce_last<1>(acc, sd.part, sd.part_stride, tile.index, sd.tiles_inner, &tickets[tile.slot].inner, total))
-        return;
+    slot_sum<1>(sd.part, sd.part_stride, sd.tiles_inner, total);
     if (threadIdx.x != 0) return;
-    st.pcur = 1 - pcur;
+    st.pcur = 1 - st.pcur;
     st.first = 0;
     st.low_apps += 1;
     const double curvature = total[0];
@@ -569,11 +593,11 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
 // either ends the inner solve (the next outer kernel applies c) or forms
 // beta = r.z' / r.z.
 template <typename TI>
-__global__ __launch_bounds__(kGroupThreads) void k_inner_update(const SlotDev* __restrict__ slots, SlotState* states,
-                                                                const SlotConfig* __restrict__ configs,
-                                                                SlotTickets* tickets, const Tile* __restrict__ tiles) {
+__global__ __launch_bounds__(kGroupThreads) void k_inner_update(const SlotDev* __restrict__ slots,
+                                                                const SlotState* __restrict__ states,
+                                                                const Tile* __restrict__ tiles) {
     const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    SlotState& st = states[tile.slot];
+    const SlotState& st = states[tile.slot];
     if (st.phase != kPhaseInner) return;
     const SlotDev& sd = slots[tile.slot];
     const TI alpha = (TI)st.alpha;
@@ -609,11 +633,22 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_update(const SlotDev* _
         acc[0] += (double)r * (double)r;
         acc[1] += (double)r * (double)(r * dv[k]);
     }
+    group_partials<2>(acc, sd.part, sd.part_stride, tile.index);
+}
+
+// Inner tolerance test and beta, or the end of the inner solve, one
+// workgroup per slot (hex_q1.cpp:469-489).
+template <typename TI>
+__global__ __launch_bounds__(kGroupThreads) void k_finish_update(const SlotDev* __restrict__ slots, SlotState* states,
+                                                                 const SlotConfig* __restrict__ configs) {
+    const int slot = blockIdx.x;
+    SlotState& st = states[slot];
+    if (st.phase != kPhaseInner) return;
+    const SlotDev& sd = slots[slot];
     double total[2];
-    if (!reduce_last<2>(acc, sd.part, sd.part_stride, tile.index, sd.tiles_flat, &tickets[tile.slot].update, total))
-        return;
+    slot_sum<2>(sd.part, sd.part_stride, sd.tiles_flat, total);
     if (threadIdx.x != 0) return;
-    const SlotConfig cfg = configs[tile.slot];
+    const SlotConfig cfg = configs[slot];
     st.inner_its += 1;
     const double rel = sqrt(total[0]) / st.rhs_norm;
     st.rel_inner = rel;
@@ -692,29 +727,32 @@ hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t st
     return hipGetLastError();
 }
 
-hipError_t mpir_step(bool inner64, int rows_inner, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
-                     SlotTickets* tickets, const Tile* outer_tiles, int n_outer, const Tile* inner_tiles,
+hipError_t mpir_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
+                     const SlotConfig* configs, const Tile* outer_tiles, int n_outer, const Tile* inner_tiles,
                      int n_inner, const Tile* flat_tiles, int n_flat, int* active, hipStream_t stream) {
     const dim3 block(kGroupThreads);
+    const dim3 outer_grid(n_outer < kOuterGroups ? n_outer : kOuterGroups);
     if (inner64) {
-        hipLaunchKernelGGL(k_outer<double>, dim3(n_outer), block, 0, stream, slots, states, configs, tickets,
-                           outer_tiles, active);
+        hipLaunchKernelGGL(k_outer<double>, outer_grid, block, 0, stream, slots, states, outer_tiles, n_outer);
+        hipLaunchKernelGGL(k_finish_outer<double>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
         hipLaunchKernelGGL((k_inner_apply<double, kRowsInner64>), dim3(n_inner), block, 0, stream, slots, states,
-                           tickets, inner_tiles, active);
-        hipLaunchKernelGGL(k_inner_update<double>, dim3(n_flat), block, 0, stream, slots, states, configs,
-                           tickets, flat_tiles);
+                           inner_tiles);
+        hipLaunchKernelGGL(k_finish_apply<double>, dim3(n_slots), block, 0, stream, slots, states, active);
+        hipLaunchKernelGGL(k_inner_update<double>, dim3(n_flat), block, 0, stream, slots, states, flat_tiles);
+        hipLaunchKernelGGL(k_finish_update<double>, dim3(n_slots), block, 0, stream, slots, states, configs);
     } else {
-        hipLaunchKernelGGL(k_outer<float>, dim3(n_outer), block, 0, stream, slots, states, configs, tickets,
-                           outer_tiles, active);
+        hipLaunchKernelGGL(k_outer<float>, outer_grid, block, 0, stream, slots, states, outer_tiles, n_outer);
+        hipLaunchKernelGGL(k_finish_outer<float>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
         if (rows_inner == 8) {
-            hipLaunchKernelGGL((k_inner_apply<float, 8>), dim3(n_inner), block, 0, stream, slots, states, tickets,
-                               inner_tiles, active);
+            hipLaunchKernelGGL((k_inner_apply<float, 8>), dim3(n_inner), block, 0, stream, slots, states,
+                               inner_tiles);
         } else {
-            hipLaunchKernelGGL((k_inner_apply<float, 4>), dim3(n_inner), block, 0, stream, slots, states, tickets,
-                               inner_tiles, active);
+            hipLaunchKernelGGL((k_inner_apply<float, 4>), dim3(n_inner), block, 0, stream, slots, states,
+                               inner_tiles);
         }
-        hipLaunchKernelGGL(k_inner_update<float>, dim3(n_flat), block, 0, stream, slots, states, configs, tickets,
-                           flat_tiles);
+        hipLaunchKernelGGL(k_finish_apply<float>, dim3(n_slots), block, 0, stream, slots, states, active);
+        hipLaunchKernelGGL(k_inner_update<float>, dim3(n_flat), block, 0, stream, slots, states, flat_tiles);
+        hipLaunchKernelGGL(k_finish_update<float>, dim3(n_slots), block, 0, stream, slots, states, configs);
     }
     return hipGetLastError();
 }
