@@ -154,6 +154,7 @@ struct pcbhip_batch {
     Tile* d_flat = nullptr;
     int n_outer = 0, n_inner = 0, n_flat = 0;
     int rows_inner = 4;
+    int outer_every = 8;
     long long target_groups = kTargetGroups;
     int* d_active = nullptr;
     int* h_active = nullptr;
@@ -299,11 +300,18 @@ void build_graph(pcbhip_batch* b, int steps) {
         b->graph = nullptr;
     }
     check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    const int n_slots = (int)b->slots.size();
     for (int s = 0; s < steps; ++s) {
-        const hipError_t err =
-            launch::mpir_step(b->inner64 != 0, b->rows_inner, (int)b->slots.size(), b->d_slots, b->d_states,
-                              b->d_configs, b->d_outer, b->n_outer, b->d_inner, b->n_inner, b->d_flat, b->n_flat,
-                              b->d_active, b->stream);
+        hipError_t err = hipSuccess;
+        // The outer step runs every outer_every inner iterations: a slot whose
+        // inner solve ended waits at most that long, and the idle outer
+        // kernels stay out of most steps.
+        if (s % b->outer_every == 0)
+            err = launch::outer_step(b->inner64 != 0, n_slots, b->d_slots, b->d_states, b->d_configs, b->d_outer,
+                                     b->n_outer, b->d_active, b->stream);
+        if (err == hipSuccess)
+            err = launch::inner_step(b->inner64 != 0, b->rows_inner, n_slots, b->d_slots, b->d_states, b->d_configs,
+                                     b->d_inner, b->n_inner, b->d_flat, b->n_flat, b->d_active, b->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(b->stream, &dropped);
@@ -640,7 +648,8 @@ int pcbhip_batch_run(pcbhip_batch* b) {
                   "reset x");
             std::memset(&states[(size_t)i], 0, sizeof(SlotState));
             states[(size_t)i].phase = kPhaseOuter;
-            cap = std::max(cap, (long long)s.config.max_outer_iterations * (s.config.max_inner_iterations + 1) + 8);
+            cap = std::max(cap, (long long)(s.config.max_outer_iterations + 1) *
+                                    (s.config.max_inner_iterations + b->outer_every) + 2 * b->outer_every);
         }
         check(hipMemcpyAsync(b->d_states, states.data(), sizeof(SlotState) * count, hipMemcpyHostToDevice,
                              b->stream),
@@ -650,7 +659,7 @@ int pcbhip_batch_run(pcbhip_batch* b) {
         // Graph length: enough steps that the wait per replay is a small
         // share of the work, few enough that steps launched after the last
         // slot finished cost little.
-        const int steps = 16;
+        const int steps = 32;
         if (!b->exec || b->graph_steps != steps) build_graph(b, steps);
         long long done_steps = 0;
         for (;;) {

@@ -113,9 +113,13 @@ hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, 
 hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S, int R,
                     int C, int* bad, hipStream_t stream);
 hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream);
-hipError_t mpir_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
-                     const SlotConfig* configs, const Tile* outer_tiles, int n_outer, const Tile* inner_tiles,
-                     int n_inner, const Tile* flat_tiles, int n_flat, int* active, hipStream_t stream);
+// One outer step (FP64 residual and decision) for the slots in their outer
+// phase, and one inner Jacobi-PCG iteration for the slots in their inner phase.
+hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
+                      const Tile* outer_tiles, int n_outer, int* active, hipStream_t stream);
+hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
+                      const SlotConfig* configs, const Tile* inner_tiles, int n_inner, const Tile* flat_tiles,
+                      int n_flat, int* active, hipStream_t stream);
 }  // namespace launch
 
 }  // namespace pcbhip

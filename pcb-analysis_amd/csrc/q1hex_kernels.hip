@@ -438,13 +438,14 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
         if (y0 <= op.R) {
             strip_apply<double, kRowsOuter>(
                 op.coef64, op.S, op.R, op.C, x0, y0, zlo, zhi,
-                [&](const long long pb, const unsigned off, const bool owned) -> NodeVal<double> {
+                [&](const long long pb, const unsigned off, const bool) -> NodeVal<double> {
                     double xv = ldt(xin + pb, off);
                     if (pending == kPendingCorrection) xv += (double)ldt(cin + pb, off);
-                    if (owned && pending != kPendingNone) stt(xout + pb, off, xv);
                     return {xv, ldt(robin + pb, off)};
                 },
                 [&](const long long pb, const unsigned off, const double conduction, const NodeVal<double>& v) {
+                    // Stores wait for their plane's emit, so a plane's loads issue back to back.
+                    if (pending != kPendingNone) stt(xout + pb, off, v.u);
                     const double ax = v.robin >= 0.0 ? conduction + v.robin * v.u : v.u;
                     const double bv = ldt(b + pb, off);
                     const double r = bv - ax;
@@ -549,13 +550,14 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
     if (y0 <= op.R) {
         strip_apply<TI, ROWS>(
             InnerOf<TI>::coef(op), op.S, op.R, op.C, x0, y0, zlo, zhi,
-            [&](const long long pb, const unsigned off, const bool owned) -> NodeVal<TI> {
+            [&](const long long pb, const unsigned off, const bool) -> NodeVal<TI> {
                 const TI z = ldt(r + pb, off) * ldt(dinv + pb, off);
                 const TI pv = first ? z : z + beta * ldt(pin + pb, off);
-                if (owned) stt(pout + pb, off, pv);
                 return {pv, ldt(robin + pb, off)};
             },
             [&](const long long pb, const unsigned off, const TI conduction, const NodeVal<TI>& v) {
+                // p is stored with q at the plane's emit, so a plane's loads issue back to back.
+                stt(pout + pb, off, v.u);
                 const TI ap = v.robin >= TI(0) ? conduction + v.robin * v.u : v.u;
                 stt(q + pb, off, ap);
                 acc[0] += (double)v.u * (double)ap;
@@ -727,22 +729,32 @@ hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t st
     return hipGetLastError();
 }
 
-hipError_t mpir_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
-                     const SlotConfig* configs, const Tile* outer_tiles, int n_outer, const Tile* inner_tiles,
-                     int n_inner, const Tile* flat_tiles, int n_flat, int* active, hipStream_t stream) {
+hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState* states,
+                      const SlotConfig* configs, const Tile* outer_tiles, int n_outer, int* active,
+                      hipStream_t stream) {
     const dim3 block(kGroupThreads);
-    const dim3 outer_grid(n_outer < kOuterGroups ? n_outer : kOuterGroups);
+    const dim3 grid(n_outer < kOuterGroups ? n_outer : kOuterGroups);
     if (inner64) {
-        hipLaunchKernelGGL(k_outer<double>, outer_grid, block, 0, stream, slots, states, outer_tiles, n_outer);
+        hipLaunchKernelGGL(k_outer<double>, grid, block, 0, stream, slots, states, outer_tiles, n_outer);
         hipLaunchKernelGGL(k_finish_outer<double>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
+    } else {
+        hipLaunchKernelGGL(k_outer<float>, grid, block, 0, stream, slots, states, outer_tiles, n_outer);
+        hipLaunchKernelGGL(k_finish_outer<float>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
+    }
+    return hipGetLastError();
+}
+
+hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
+                      const SlotConfig* configs, const Tile* inner_tiles, int n_inner, const Tile* flat_tiles,
+                      int n_flat, int* active, hipStream_t stream) {
+    const dim3 block(kGroupThreads);
+    if (inner64) {
         hipLaunchKernelGGL((k_inner_apply<double, kRowsInner64>), dim3(n_inner), block, 0, stream, slots, states,
                            inner_tiles);
         hipLaunchKernelGGL(k_finish_apply<double>, dim3(n_slots), block, 0, stream, slots, states, active);
         hipLaunchKernelGGL(k_inner_update<double>, dim3(n_flat), block, 0, stream, slots, states, flat_tiles);
         hipLaunchKernelGGL(k_finish_update<double>, dim3(n_slots), block, 0, stream, slots, states, configs);
     } else {
-        hipLaunchKernelGGL(k_outer<float>, outer_grid, block, 0, stream, slots, states, outer_tiles, n_outer);
-        hipLaunchKernelGGL(k_finish_outer<float>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
         if (rows_inner == 8) {
             hipLaunchKernelGGL((k_inner_apply<float, 8>), dim3(n_inner), block, 0, stream, slots, states,
                                inner_tiles);
