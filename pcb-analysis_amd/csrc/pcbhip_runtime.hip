@@ -509,7 +509,9 @@ int pcbhip_batch_create(int count, const pcbhip_operator* const* ops, int inner_
             s.c = base + 4 * nti;
             base += 5 * nti;
             s.part = reinterpret_cast<double*>(base);
-            check(hipMemsetAsync(s.b, 0, 4 * n64, b->stream), "memset");
+            // Zeroed vectors: the kernels load p, c and x unconditionally and
+            // weight them by 0 where the algorithm starts from zero.
+            check(hipMemsetAsync(s.b, 0, 4 * n64 + 5 * nti, b->stream), "memset");
         }
         build_tiles(b.get());
         b->d_slots = static_cast<SlotDev*>(device_alloc(sizeof(SlotDev) * count));

@@ -383,8 +383,19 @@ __global__ __launch_bounds__(256) void k_diagonal(const double* __restrict__ coe
 // counters.
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
                                               const long long n4) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+    // Four 16-byte loads in flight per thread before the stores.
+    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+    float4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long long i = base + k * 256;
+        if (i < n4) v[k] = src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long long i = base + k * 256;
+        if (i < n4) dst[i] = v[k];
+    }
 }
 
 // ---------------------------------------------------------- MPIR kernels
@@ -427,6 +438,9 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
         const int zhi = zlo + tile.zb < op.S + 1 ? zlo + tile.zb : op.S + 1;
         const int pending = st.pending;
         const int xcur = st.xcur;
+        // Weight of the pending correction: loaded always, added with weight
+        // 0 or 1 (the correction buffer starts zeroed), so no load is branched.
+        const double cw = pending == kPendingCorrection ? 1.0 : 0.0;
         const double* xin = sd.x64[xcur];
         double* xout = sd.x64[1 - xcur];
         const TI* __restrict__ cin = static_cast<const TI*>(sd.c);
@@ -439,8 +453,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
             strip_apply<double, kRowsOuter>(
                 op.coef64, op.S, op.R, op.C, x0, y0, zlo, zhi,
                 [&](const long long pb, const unsigned off, const bool) -> NodeVal<double> {
-                    double xv = ldt(xin + pb, off);
-                    if (pending == kPendingCorrection) xv += (double)ldt(cin + pb, off);
+                    const double xv = ldt(xin + pb, off) + cw * (double)ldt(cin + pb, off);
                     return {xv, ldt(robin + pb, off)};
                 },
                 [&](const long long pb, const unsigned off, const double conduction, const NodeVal<double>& v) {
@@ -537,8 +550,9 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
     const int y0 = (tile.sy * kWavesPerGroup + wave) * ROWS;
     const int zlo = tile.sz * tile.zb;
     const int zhi = zlo + tile.zb < op.S + 1 ? zlo + tile.zb : op.S + 1;
-    const bool first = st.first != 0;
-    const TI beta = (TI)st.beta;
+    // beta is zero on the first iteration (p = z); p_old is always loaded
+    // (the buffers start zeroed), so no load sits behind a branch.
+    const TI beta = st.first != 0 ? TI(0) : (TI)st.beta;
     const int pcur = st.pcur;
     const TI* __restrict__ pin = static_cast<const TI*>(sd.p[pcur]);
     TI* __restrict__ pout = static_cast<TI*>(sd.p[1 - pcur]);
@@ -552,7 +566,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
             InnerOf<TI>::coef(op), op.S, op.R, op.C, x0, y0, zlo, zhi,
             [&](const long long pb, const unsigned off, const bool) -> NodeVal<TI> {
                 const TI z = ldt(r + pb, off) * ldt(dinv + pb, off);
-                const TI pv = first ? z : z + beta * ldt(pin + pb, off);
+                const TI pv = z + beta * ldt(pin + pb, off);
                 return {pv, ldt(robin + pb, off)};
             },
             [&](const long long pb, const unsigned off, const TI conduction, const NodeVal<TI>& v) {
@@ -603,7 +617,8 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_update(const SlotDev* _
     if (st.phase != kPhaseInner) return;
     const SlotDev& sd = slots[tile.slot];
     const TI alpha = (TI)st.alpha;
-    const bool fresh = st.inner_its == 0;
+    // The correction restarts from zero with each inner solve (weight 0).
+    const TI keep = st.inner_its == 0 ? TI(0) : TI(1);
     // The chunk's buffers end at the last node, so loads past it read zero
     // and stores past it are dropped: no bounds branch.
     const long long base = (long long)tile.index * kFlatChunk;
@@ -621,14 +636,14 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_update(const SlotDev* _
         pv[k] = ld(rp, off, TI());
         qv[k] = ld(rq, off, TI());
         rv[k] = ld(rr, off, TI());
-        cv[k] = fresh ? TI(0) : ld(rc, off, TI());
+        cv[k] = ld(rc, off, TI());
         dv[k] = ld(rd, off, TI());
     }
     double acc[2] = {0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < kFlatPerThread; ++k) {
         const unsigned off = (unsigned)(k * kGroupThreads) + threadIdx.x;
-        const TI c = fresh ? alpha * pv[k] : cv[k] + alpha * pv[k];
+        const TI c = keep * cv[k] + alpha * pv[k];
         const TI r = rv[k] - alpha * qv[k];
         store(c, rc, off);
         store(r, rr, off);
@@ -723,7 +738,7 @@ hipError_t diagonal(const double* coef, const double* robin, double* diag, doubl
 
 hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream) {
     const long long n4 = n_floats / 4;
-    const int blocks = 4096;
+    const int blocks = (int)((n4 + 1023) / 1024);
     hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, stream, reinterpret_cast<const float4*>(src),
                        reinterpret_cast<float4*>(dst), n4);
     return hipGetLastError();
