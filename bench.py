@@ -61,6 +61,21 @@ def copy_bandwidth(torch, stream, nbytes: int = 1 << 30) -> float:
     return 2 * nbytes / per / 1e9
 
 
+def profiled_traffic(element_shape, rows: int, zb: int):
+    """HBM bytes per launch of the FP32 apply at this tile shape, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or
+    None when that shape was not profiled."""
+
+    s, r, c = element_shape
+    path = ROOT / "profiles" / f"q1hex_apply_f32_board_{c}x{r}x{s}.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith(f"traffic_bytes rows={rows} zb={zb}:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
     """Tile-shape sweep of the FP32 Q1 apply (the autotuner's candidates),
     timed with HIP events on the stream it runs on; the fastest shape is kept."""
@@ -82,6 +97,7 @@ def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
         log(f"apply rows={rows} zb={zb}: {per * 1e6:.1f} us")
     per, rows, zb = min(table)
     dev.set_shape(rows, zb)
+    traffic = profiled_traffic(operator.mesh.element_grid_shape, rows, zb)
     algorithmic = 13 * n + 12 * e  # x, y, robin (4 B/node each), free mask (1 B/node), coefficients (12 B/element)
     achieved = algorithmic / per / 1e9
     return dict(
@@ -92,7 +108,7 @@ def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
         gdof_s=n / per / 1e9,
         sweep=[dict(rows=r_, zb=z_, us=p_ * 1e6) for p_, r_, z_ in table],
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=None, algorithmic_bytes=algorithmic),
+                      traffic=traffic, algorithmic_bytes=algorithmic),
     )
 
 
