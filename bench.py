@@ -143,6 +143,76 @@ def cpu_baseline(problem, gpu_result, gpu_solve_s: float) -> dict:
                        f"{outer} outer steps: {per_iter * 1e3:.2f} ms/iteration, est. {solve_s:.1f} s/solve")
 
 
+def run_batch200(args, torch) -> dict:
+    """200 uneven boards solved together in one batch (every launch covers
+    every board still iterating) against the same 200 solved one at a time
+    through the single-solve path."""
+
+    from electrical.matrix_free_mpir_fem import MPIRConfig
+    from pcbhip import _capi, workloads
+    from thermal.matrix_free_mpir_fem import MatrixFreeThermalOperator
+
+    count = args.count
+    log(f"building {count} uneven boards")
+    boards = workloads.uneven_boards(count)
+    ops = [MatrixFreeThermalOperator(b, preconditioner="jacobi") for b in boards]
+    rhss = [o.build_rhs(o.default_reference_temperature()) for o in ops]
+    nodes = sum(o.size for o in ops)
+    config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000)
+    batch = _capi.Batch([o.device_operator for o in ops])
+    batch.set_shape(args.rows, 1024)
+    for i, b in enumerate(rhss):
+        batch.set(i, b, None, config.to_c())
+    log("batched warm-up")
+    for _ in range(args.warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    batched_s = (time.perf_counter() - t0) / args.steps
+    results = [batch.result(i, solution=False)[1] for i in range(count)]
+    converged = sum(r["converged"] for r in results)
+    log(f"batched: {batched_s * 1e3:.1f} ms for {count} solves ({converged} converged)")
+    singles = [_capi.Batch([o.device_operator]) for o in ops]
+    for sb, b in zip(singles, rhss):
+        sb.set_shape(args.rows, 1024)
+        sb.set(0, b, None, config.to_c())
+        sb.run()  # warm-up: graph capture of every single-solve batch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sb in singles:
+        sb.run()
+    single_s = time.perf_counter() - t0
+    log(f"one at a time: {single_s * 1e3:.1f} ms")
+    # The batched answers are the single answers bit for bit.
+    same = all(batch.result(i)[0].tobytes() == singles[i].result(0)[0].tobytes() for i in range(0, count, 17))
+    return {
+        "metric": "MPIR solves/s at 1e-10 rel. residual (200 uneven boards, one batch)",
+        "value": count / batched_s,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": batched_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 outer",
+        "data": "synthetic (seeded uneven-board generator, seed 4)",
+        "config": {
+            "workload": f"{count} boards, sides log-uniform 32..384, depth 4..16, {nodes:,} nodes in total",
+            "converged": converged,
+            "single_solve_path_solves_per_s": count / single_s,
+            "speedup_vs_single_solve_path": single_s / batched_s,
+            "bitwise_equal_alone_and_in_batch": bool(same),
+            "total_inner_iterations": int(sum(r["inner_iterations"] for r in results)),
+            "max_inner_iterations": int(max(r["inner_iterations"] for r in results)),
+            "parallelism": "single GPU",
+        },
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -152,6 +222,8 @@ def main() -> int:
     parser.add_argument("--slabs", type=int, default=16)
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--rows", type=int, default=4, help="rows per wave strip of the FP32 inner stencil")
+    parser.add_argument("--workload", default="board512", choices=["board512", "batch200"])
+    parser.add_argument("--count", type=int, default=200, help="boards in the batch200 workload")
     args = parser.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only"}))
@@ -165,6 +237,9 @@ def main() -> int:
 
     _capi.require_device(0)
     torch.cuda.init()
+    if args.workload == "batch200":
+        print(json.dumps(run_batch200(args, torch)), flush=True)
+        return 0
     log("building board")
     problem = workloads.board_512x512x16(size=args.size, slabs=args.slabs)
     operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
