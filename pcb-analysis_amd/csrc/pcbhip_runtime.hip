@@ -80,6 +80,7 @@ int planes_per_block(int S, long long groups_xy, long long target) {
     if (groups_xy >= target) return planes;
     const long long split = (target + groups_xy - 1) / groups_xy;
     int zb = (int)((planes + split - 1) / split);
+    zb += zb & 1;  // whole plane pairs: z blocks never cut a reduction cell
     return zb < 2 ? 2 : (zb > planes ? planes : zb);
 }
 
@@ -136,7 +137,7 @@ struct SlotHost {
     double* history = nullptr;
     int history_rows = 0;
     pcbhip_mpir_config config{1e-10, 0.0, 2e-3, 8, 200};
-    int tiles_outer = 0, tiles_inner = 0, tiles_flat = 0, part_stride = 0;
+    int tiles_outer = 0, tiles_inner = 0, tiles_flat = 0, cells = 0, part_stride = 0;
 };
 
 }  // namespace
@@ -155,6 +156,7 @@ struct pcbhip_batch {
     int n_outer = 0, n_inner = 0, n_flat = 0;
     int rows_inner = 4;
     int outer_every = 8;
+    int tiled_slots = 0;  // slots whose tiles are in the tables
     long long target_groups = kTargetGroups;
     int* d_active = nullptr;
     int* h_active = nullptr;
@@ -256,10 +258,12 @@ void upload_tiles(pcbhip_batch* b, const std::vector<Tile>& t, Tile** d, int* n)
 // (Re)builds the three tile tables for the batch's current shape.  The
 // z split is chosen over the whole batch: many slots fill the device without
 // splitting any of them.
-void build_tiles(pcbhip_batch* b) {
+void build_tiles(pcbhip_batch* b, const std::vector<char>* keep = nullptr) {
     const int rows_inner = b->inner64 ? kRowsInner64 : b->rows_inner;
     long long xy_outer = 0, xy_inner = 0;
-    for (const SlotHost& s : b->slots) {
+    for (size_t i = 0; i < b->slots.size(); ++i) {
+        if (keep && !(*keep)[i]) continue;
+        const SlotHost& s = b->slots[i];
         xy_outer += (long long)strips_x(s.op->C) * groups_y(s.op->R, kRowsOuter);
         xy_inner += (long long)strips_x(s.op->C) * groups_y(s.op->R, rows_inner);
     }
@@ -273,13 +277,23 @@ void build_tiles(pcbhip_batch* b) {
         const long long share_i = xy_inner > 0 ? (long long)nsx * nyi * b->target_groups / xy_inner : 0;
         const int zbo = planes_per_block(op->S, (long long)nsx * nyo, std::max(share_o, (long long)nsx * nyo));
         const int zbi = planes_per_block(op->S, (long long)nsx * nyi, std::max(share_i, (long long)nsx * nyi));
-        s.tiles_outer = add_tiles(outer, i, nsx, nyo, op->S, zbo);
-        s.tiles_inner = add_tiles(inner, i, nsx, nyi, op->S, zbi);
         s.tiles_flat = (int)((op->N + kFlatChunk - 1) / kFlatChunk);
-        for (int k = 0; k < s.tiles_flat; ++k) flat.push_back(Tile{i, k, 0, 0, 0, 0});
-        if (std::max({s.tiles_outer, s.tiles_inner, s.tiles_flat}) > s.part_stride)
-            fail(PCBHIP_RUNTIME, "tile count exceeds the partial-sum area");
+        if (!keep || (*keep)[(size_t)i]) {
+            // A finished slot's tiles leave the tables, so its early exits
+            // stop costing launch slots; its partial layout does not change.
+            s.tiles_outer = add_tiles(outer, i, nsx, nyo, op->S, zbo);
+            s.tiles_inner = add_tiles(inner, i, nsx, nyi, op->S, zbi);
+            for (int k = 0; k < s.tiles_flat; ++k) flat.push_back(Tile{i, k, 0, 0, 0, 0});
+        }
+        s.cells = cell_count(op->S, op->R, op->C);
+        if (std::max(s.cells, s.tiles_flat) > s.part_stride)
+            fail(PCBHIP_RUNTIME, "partial count exceeds the partial-sum area");
     }
+    b->tiled_slots = 0;
+    for (size_t i = 0; i < b->slots.size(); ++i) b->tiled_slots += (!keep || (*keep)[i]) ? 1 : 0;
+    if (outer.empty()) outer.push_back(Tile{0, 0, 0, 0, 0, 1});
+    if (inner.empty()) inner.push_back(Tile{0, 0, 0, 0, 0, 1});
+    if (flat.empty()) flat.push_back(Tile{0, 0, 0, 0, 0, 0});
     upload_tiles(b, outer, &b->d_outer, &b->n_outer);
     upload_tiles(b, inner, &b->d_inner, &b->n_inner);
     upload_tiles(b, flat, &b->d_flat, &b->n_flat);
@@ -485,11 +499,10 @@ int pcbhip_batch_create(int count, const pcbhip_operator* const* ops, int inner_
             SlotHost& s = b->slots[(size_t)i];
             const pcbhip_operator* op = ops[i];
             s.op = op;
-            // Partial-sum room for the finest tiling any shape can ask for:
-            // 2-row strips, 2-plane z blocks.
-            const long long finest = (long long)strips_x(op->C) * groups_y(op->R, 2) * ((op->S + 2) / 2);
+            // One partial per reduction cell of the stencils or per chunk of
+            // the pointwise kernel, whichever is more.
             const long long flat = (op->N + kFlatChunk - 1) / kFlatChunk;
-            s.part_stride = (int)std::max(finest, flat);
+            s.part_stride = (int)std::max((long long)cell_count(op->S, op->R, op->C), flat);
             s.history_rows = s.config.max_outer_iterations;
             // Vectors of the slot in one allocation.  History room is
             // allocated on demand in pcbhip_batch_set.
@@ -585,9 +598,9 @@ int pcbhip_operator_set_shape(pcbhip_operator* op, int rows32, int zb32, int row
             fail(PCBHIP_INVALID_INPUT, "rows must be 4 or 8 (FP32) and 2 or 4 (FP64)");
         if (zb32 < 1 || zb64 < 1) fail(PCBHIP_INVALID_INPUT, "planes per z block must be positive");
         op->rows32 = rows32;
-        op->zb32 = std::min(zb32, op->S + 1);
+        op->zb32 = std::min(zb32 + (zb32 & 1), op->S + 1);
         op->rows64 = rows64;
-        op->zb64 = std::min(zb64, op->S + 1);
+        op->zb64 = std::min(zb64 + (zb64 & 1), op->S + 1);
     });
 }
 
@@ -625,8 +638,7 @@ int pcbhip_batch_run(pcbhip_batch* b) {
                 d.c = s.c;
                 d.part = s.part;
                 d.history = s.history;
-                d.tiles_outer = s.tiles_outer;
-                d.tiles_inner = s.tiles_inner;
+                d.cells = s.cells;
                 d.tiles_flat = s.tiles_flat;
                 d.part_stride = s.part_stride;
                 cfgs[(size_t)i] = SlotConfig{s.config.relative_tolerance, s.config.absolute_tolerance,
@@ -664,13 +676,26 @@ int pcbhip_batch_run(pcbhip_batch* b) {
         const int steps = 32;
         if (!b->exec || b->graph_steps != steps) build_graph(b, steps);
         long long done_steps = 0;
+        std::vector<SlotState> host_states;
         for (;;) {
             check(hipGraphLaunch(b->exec, b->stream), "graph launch");
             check(hipStreamSynchronize(b->stream), "mpir steps");
             done_steps += steps;
-            if (*b->h_active <= 0) break;
+            const int active = *b->h_active;
+            if (active <= 0) break;
             if (done_steps > cap) fail(PCBHIP_RUNTIME, "MPIR iteration did not terminate within its limits");
+            if (count > 1 && active * 2 <= b->tiled_slots) {
+                // Half the tiled slots have finished: drop their tiles.
+                host_states.resize((size_t)count);
+                check(hipMemcpy(host_states.data(), b->d_states, sizeof(SlotState) * count, hipMemcpyDeviceToHost),
+                      "copy states");
+                std::vector<char> keep((size_t)count);
+                for (int i = 0; i < count; ++i) keep[(size_t)i] = host_states[(size_t)i].phase != kPhaseDone;
+                build_tiles(b, &keep);
+                build_graph(b, steps);
+            }
         }
+        if (b->tiled_slots != count) build_tiles(b);
     });
 }
 

@@ -31,13 +31,26 @@ constexpr int kFlatChunk = kGroupThreads * kFlatPerThread;
 // Rows per wave strip of the FP64 outer-residual kernel and of the FP64
 // inner kernel; the FP32 inner kernel and the standalone apply are built for
 // 4 and 8 rows and the tile shape picks one.
-constexpr int kRowsOuter = 4;
+constexpr int kRowsOuter = 4;  // multiples of the 4-row reduction cell
 constexpr int kRowsInner64 = 4;
 // Stencil workgroups wanted per launch before the z sweep is split into
 // blocks (256 CUs, several waves each).
 constexpr int kTargetGroups = 1024;
 // Workgroups of the persistent outer-residual kernel.
 constexpr int kOuterGroups = 512;
+
+// Reduction cells of the stencil kernels: every inner product is summed per
+// (node-plane pair, 4 node rows, one strip of 62 node columns) and stored as
+// one partial, whatever tile shape computed it.
+constexpr int kCellRows = 4;
+constexpr int kCellPlanes = 2;
+
+__host__ __device__ inline int cell_count(const int S, const int R, const int C) {
+    const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
+    const int nrg = (R + 1 + kCellRows - 1) / kCellRows;
+    const int nzp = (S + 1 + kCellPlanes - 1) / kCellPlanes;
+    return nsx * nrg * nzp;
+}
 
 enum Phase : int { kPhaseOuter = 0, kPhaseInner = 1, kPhaseDone = 2 };
 enum Status : int { kStatusOk = 0, kStatusNotSpd = 1 };
@@ -76,10 +89,10 @@ struct SlotDev {
     void* c;
     double* part;       // 4 * max(tiles) partial sums
     double* history;    // 4 * max_outer: (outer, fp64 rel before, inner its, inner rel)
-    int tiles_outer;    // workgroups of the FP64 residual kernel for this slot
-    int tiles_inner;    // workgroups of the inner stencil kernel
+    int cells;          // reduction cells of the stencil kernels (see cell_count)
     int tiles_flat;     // workgroups of the pointwise kernel
     int part_stride;
+    int pad;
 };
 
 struct SlotConfig {

@@ -81,13 +81,14 @@ __device__ __forceinline__ T masked(const NodeVal<T>& v) {
 // (``base`` the wave-uniform start of the node plane, ``off`` the lane's
 // offset in it, kOutside outside the grid, where loads return zero); it is
 // called for the ROWS + 2 rows of every plane.  ``owned`` marks the nodes
-// this lane writes.  ``emit(base, off, conduction, value)`` receives, for
-// every owned node, the conduction sum K u restricted to free columns (no
-// Robin term).
-template <typename T, int ROWS, class Load, class Emit>
+// this lane writes.  ``emit(base, off, row, conduction, value)`` receives,
+// for every owned node (``row`` its row in the strip), the conduction sum
+// K u restricted to free columns (no Robin term); ``plane_done(z)`` follows
+// every output plane, in uniform control flow.
+template <typename T, int ROWS, class Load, class Emit, class PlaneDone>
 __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const int S, const int R, const int C,
                                             const int x0, const int y0, const int zlo, const int zhi, Load&& load,
-                                            Emit&& emit) {
+                                            Emit&& emit, PlaneDone&& plane_done) {
     const int lane = threadIdx.x & (kWave - 1);
     const int xc = x0 - 1 + lane;
     const bool col_in = xc >= 0 && xc <= C;
@@ -133,8 +134,9 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
 #pragma unroll
         for (int i = 0; i < ROWS; ++i) {
             const T total = L[i] + __shfl_up(Rt[i], 1);
-            if (out_lane && y0 + i <= R) emit(pb, noff[i + 1], total, v[i]);
+            if (out_lane && y0 + i <= R) emit(pb, noff[i + 1], i, total, v[i]);
         }
+        plane_done(z);
     };
 
     // Output planes [zlo, zhi) need element layers zlo - 1 .. zhi - 1 (those
@@ -263,10 +265,48 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Reduces NV per-thread values over the workgroup (wave butterflies, then
-// the waves in order) and stores them as partial ``index`` of the slot.  The
-// partials become visible to the next kernel at the launch boundary, so no
-// workgroup needs a device-scope fence.
+// Reduction cells of the stencil kernels.  Every inner product is summed
+// per cell of (node-plane pair, 4 node rows, one 62-column strip) -- lanes in
+// a fixed order, then a wave butterfly -- and one partial per cell is stored.
+// The cells do not depend on the tile shape (rows per wave, planes per z
+// block) or on which other slots share the launch, so a slot's bits depend
+// only on its own data.  Partials become visible to the finish kernel at the
+// launch boundary; no workgroup needs a device-scope fence.
+// Per-lane accumulators of NV sums for the ROWS / 4 row cells of a strip.
+template <int NV, int ROWS>
+struct CellSums {
+    double v[NV][ROWS / kCellRows];
+
+    __device__ void clear() {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int g = 0; g < ROWS / kCellRows; ++g) v[k][g] = 0.0;
+    }
+    // After output plane z: a plane pair (or the last plane) is complete, so
+    // store one partial per row cell and start again.
+    __device__ void plane_done(const int z, const int S, const int R, const int C, const int x0, const int y0,
+                               double* __restrict__ part, const int stride) {
+        if (!((z & 1) || z == S)) return;
+        const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
+        const int nrg = (R + 1 + kCellRows - 1) / kCellRows;
+        const int sx = x0 / kStripCols;
+#pragma unroll
+        for (int g = 0; g < ROWS / kCellRows; ++g) {
+            const int rg = y0 / kCellRows + g;
+            const int cell = ((z / kCellPlanes) * nrg + rg) * nsx + sx;
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                const double s = wave_sum(v[k][g]);
+                if (rg < nrg && (threadIdx.x & (kWave - 1)) == 0) part[k * stride + cell] = s;
+            }
+        }
+        clear();
+    }
+};
+
+// Workgroup sum for the pointwise kernel: waves in order, one partial per
+// workgroup (its chunk index).
 template <int NV>
 __device__ void group_partials(const double (&v)[NV], double* __restrict__ part, const int stride, const int index) {
     __shared__ double waves[kWavesPerGroup][NV];
@@ -335,9 +375,10 @@ __global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ c
             if (MODE == 1) return {ldt(x + pb, off), T(0)};
             return {ldt(x + pb, off), ldt(robin + pb, off)};
         },
-        [&](const long long pb, const unsigned off, const T conduction, const NodeVal<T>& v) {
+        [&](const long long pb, const unsigned off, const int, const T conduction, const NodeVal<T>& v) {
             stt(y + pb, off, v.robin >= T(0) ? conduction + v.robin * v.u : v.u);
-        });
+        },
+        [](const int) {});
 }
 
 // Jacobi diagonal: sum of the element diagonals ((a + b + c) / 9 for every
@@ -448,7 +489,9 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
         const TI* __restrict__ dinv = InnerOf<TI>::dinv(op);
         const double* __restrict__ robin = op.robin64;
         const double* __restrict__ b = sd.b64;
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};  // b.b, r.r (FP64), r.r and r.z (inner precision)
+        // b.b, r.r (FP64), r.r and r.z (inner precision) per reduction cell.
+        CellSums<4, kRowsOuter> acc;
+        acc.clear();
         if (y0 <= op.R) {
             strip_apply<double, kRowsOuter>(
                 op.coef64, op.S, op.R, op.C, x0, y0, zlo, zhi,
@@ -456,7 +499,8 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
                     const double xv = ldt(xin + pb, off) + cw * (double)ldt(cin + pb, off);
                     return {xv, ldt(robin + pb, off)};
                 },
-                [&](const long long pb, const unsigned off, const double conduction, const NodeVal<double>& v) {
+                [&](const long long pb, const unsigned off, const int row, const double conduction,
+                    const NodeVal<double>& v) {
                     // Stores wait for their plane's emit, so a plane's loads issue back to back.
                     if (pending != kPendingNone) stt(xout + pb, off, v.u);
                     const double ax = v.robin >= 0.0 ? conduction + v.robin * v.u : v.u;
@@ -464,13 +508,14 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
                     const double r = bv - ax;
                     const TI ri = (TI)r;
                     stt(rout + pb, off, ri);
-                    acc[0] += bv * bv;
-                    acc[1] += r * r;
-                    acc[2] += (double)ri * (double)ri;
-                    acc[3] += (double)ri * (double)(ri * ldt(dinv + pb, off));
-                });
+                    const int g = row / kCellRows;
+                    acc.v[0][g] += bv * bv;
+                    acc.v[1][g] += r * r;
+                    acc.v[2][g] += (double)ri * (double)ri;
+                    acc.v[3][g] += (double)ri * (double)(ri * ldt(dinv + pb, off));
+                },
+                [&](const int z) { acc.plane_done(z, op.S, op.R, op.C, x0, y0, sd.part, sd.part_stride); });
         }
-        group_partials<4>(acc, sd.part, sd.part_stride, tile.index);
     }
 }
 
@@ -487,7 +532,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_finish_outer(const SlotDev* _
     if (st.phase != kPhaseOuter) return;
     const SlotDev& sd = slots[slot];
     double total[4];
-    slot_sum<4>(sd.part, sd.part_stride, sd.tiles_outer, total);
+    slot_sum<4>(sd.part, sd.part_stride, sd.cells, total);
     if (threadIdx.x != 0) return;
     const SlotConfig cfg = configs[slot];
     const int pending = st.pending;
@@ -560,7 +605,8 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
     TI* __restrict__ q = static_cast<TI*>(sd.q);
     const TI* __restrict__ dinv = InnerOf<TI>::dinv(op);
     const TI* __restrict__ robin = InnerOf<TI>::robin(op);
-    double acc[1] = {0.0};
+    CellSums<1, ROWS> acc;  // p.q per reduction cell
+    acc.clear();
     if (y0 <= op.R) {
         strip_apply<TI, ROWS>(
             InnerOf<TI>::coef(op), op.S, op.R, op.C, x0, y0, zlo, zhi,
@@ -569,15 +615,15 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
                 const TI pv = z + beta * ldt(pin + pb, off);
                 return {pv, ldt(robin + pb, off)};
             },
-            [&](const long long pb, const unsigned off, const TI conduction, const NodeVal<TI>& v) {
+            [&](const long long pb, const unsigned off, const int row, const TI conduction, const NodeVal<TI>& v) {
                 // p is stored with q at the plane's emit, so a plane's loads issue back to back.
                 stt(pout + pb, off, v.u);
                 const TI ap = v.robin >= TI(0) ? conduction + v.robin * v.u : v.u;
                 stt(q + pb, off, ap);
-                acc[0] += (double)v.u * (double)ap;
-            });
+                acc.v[0][row / kCellRows] += (double)v.u * (double)ap;
+            },
+            [&](const int z) { acc.plane_done(z, op.S, op.R, op.C, x0, y0, sd.part, sd.part_stride); });
     }
-    group_partials<1>(acc, sd.part, sd.part_stride, tile.index);
 }
 
 // Curvature check and alpha = rz / p.q, one workgroup per slot.
@@ -589,7 +635,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_finish_apply(const SlotDev* _
     if (st.phase != kPhaseInner) return;
     const SlotDev& sd = slots[slot];
     double total[1];
-    slot_sum<1>(sd.part, sd.part_stride, sd.tiles_inner, total);
+    slot_sum<1>(sd.part, sd.part_stride, sd.cells, total);
     if (threadIdx.x != 0) return;
     st.pcur = 1 - st.pcur;
     st.first = 0;

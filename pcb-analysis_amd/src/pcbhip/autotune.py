@@ -17,7 +17,10 @@ def apply_candidates(element_shape) -> list[tuple[int, int]]:
 
     s = int(element_shape[0])
     planes = s + 1
-    blocks = sorted({planes, -(-planes // 2), -(-planes // 3), -(-planes // 4), 4, 3, 2})
+    # z blocks hold whole plane pairs (the reduction cells); the last block
+    # may be odd.
+    even = lambda n: n + (n & 1)  # noqa: E731
+    blocks = sorted({planes, even(-(-planes // 2)), even(-(-planes // 3)), even(-(-planes // 4)), 4, 2})
     out = []
     for rows in (4, 8):
         for zb in blocks:

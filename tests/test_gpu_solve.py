@@ -182,6 +182,45 @@ def test_batched_slots_equal_their_single_solves_bit_for_bit():
         assert x.tobytes() == single.tobytes(), i
 
 
+def test_tile_shape_does_not_change_a_single_bit():
+    print("tile-shape independence", flush=True)
+    # Inner products are summed per fixed reduction cell, so rows per wave,
+    # z blocks and batch composition leave the bits alone.
+    problem = random_body((7, 45, 70), seed=12)
+    operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    rhs = operator.build_rhs(operator.default_reference_temperature())
+    cfg = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=30, max_inner_iterations=4000)
+    answers = set()
+    for rows, target in ((4, 1), (4, 4096), (8, 64), (8, 100000)):
+        batch = _capi.Batch([operator.device_operator])
+        batch.set_shape(rows, target)
+        batch.set(0, rhs, None, cfg.to_c())
+        batch.run()
+        x, res = batch.result(0)
+        assert res["converged"]
+        answers.add(x.tobytes())
+    assert len(answers) == 1
+
+
+def test_finished_slots_leave_the_batch_without_changing_the_rest():
+    print("compaction", flush=True)
+    # Eight boards of very different difficulty: the easy ones finish early,
+    # the batch drops their tiles, the hard ones continue unchanged.
+    problems = [random_body((2, 4 + 3 * i, 5 + 7 * i), seed=20 + i) for i in range(8)]
+    cfg = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=30, max_inner_iterations=4000)
+    ops = [MatrixFreeThermalOperator(p, preconditioner="jacobi") for p in problems]
+    rhss = [o.build_rhs(o.default_reference_temperature()) for o in ops]
+    batch = _capi.Batch([o.device_operator for o in ops])
+    for i, b in enumerate(rhss):
+        batch.set(i, b, None, cfg.to_c())
+    batch.run()
+    for i, (o, b) in enumerate(zip(ops, rhss)):
+        single = solve_mpir(o, b, config=cfg).solution
+        assert batch.result(i)[0].tobytes() == single.tobytes(), i
+    batch.run()  # a second run after compaction starts from the full tables
+    assert all(batch.result(i)[1]["converged"] for i in range(len(ops)))
+
+
 def test_a_capped_first_pass_re_references_and_converges():
     print("re-reference restart", flush=True)
     # tests/test_native_thermal_system.py:188-201.
