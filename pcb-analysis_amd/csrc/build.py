@@ -22,6 +22,11 @@ FLAGS = [
     "-fPIC",
     "-shared",
     "-Wall",
+    # No FMA contraction: every rounding is written in the source, so the
+    # same element gives the same bits whatever tile shape or template
+    # instantiation computes it (the reference builds its kernels the same
+    # way: -ffp-contract=off, --fmad=false).
+    "-ffp-contract=off",
     "-Xclang",
     "-target-feature",
     "-Xclang",
