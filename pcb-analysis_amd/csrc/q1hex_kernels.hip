@@ -64,6 +64,11 @@ __device__ __forceinline__ void stt(T* base, const unsigned off, const T v) {
     store(v, rsrc(base), off);
 }
 
+__device__ __forceinline__ float fmadd(const float a, const float b, const float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(const double a, const double b, const double c) {
+    return __builtin_fma(a, b, c);
+}
+
 template <typename T>
 struct NodeVal {
     T u;      // operand value at the node (unmasked)
@@ -166,7 +171,11 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
         }
 
         // Per node row: x differences in both planes, z differences in both
-        // columns and their 2-point x average.
+        // columns and their 2-point x average.  Every rounding is explicit
+        // (the file is built without contraction): 2 a + b is one fused op
+        // and exact in its doubling, and each corner accumulates its three
+        // terms with fused multiply-adds, so the bits do not depend on the
+        // tile shape that computes an element.
         T dxl[ROWS + 2], dxh[ROWS + 2], zw0[ROWS + 2], zw1[ROWS + 2];
 #pragma unroll
         for (int i = 0; i < ROWS + 2; ++i) {
@@ -174,8 +183,8 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
             dxh[i] = nhi[i] - mhi[i];
             const T dz = mhi[i] - mlo[i];
             const T dzn = nhi[i] - nlo[i];
-            zw0[i] = T(2) * dz + dzn;
-            zw1[i] = dz + T(2) * dzn;
+            zw0[i] = fmadd(T(2), dz, dzn);
+            zw1[i] = fmadd(T(2), dzn, dz);
         }
 
         T Lhi[ROWS], Rhi[ROWS];
@@ -187,46 +196,47 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
 
 #pragma unroll
         for (int j = 0; j < ROWS + 1; ++j) {
-            // a U_x: y average of the x differences, then z average.
-            const T wl0 = T(2) * dxl[j] + dxl[j + 1];
-            const T wl1 = dxl[j] + T(2) * dxl[j + 1];
-            const T wh0 = T(2) * dxh[j] + dxh[j + 1];
-            const T wh1 = dxh[j] + T(2) * dxh[j + 1];
-            const T rx00 = ca[j] * (T(2) * wl0 + wh0);
-            const T rx01 = ca[j] * (T(2) * wl1 + wh1);
-            const T rx10 = ca[j] * (wl0 + T(2) * wh0);
-            const T rx11 = ca[j] * (wl1 + T(2) * wh1);
-            // b U_y: y differences in both columns, x average, z average.
+            // a U_x: y average of the x differences, then z average: tx[z][y].
+            const T wl0 = fmadd(T(2), dxl[j], dxl[j + 1]);
+            const T wl1 = fmadd(T(2), dxl[j + 1], dxl[j]);
+            const T wh0 = fmadd(T(2), dxh[j], dxh[j + 1]);
+            const T wh1 = fmadd(T(2), dxh[j + 1], dxh[j]);
+            const T tx00 = fmadd(T(2), wl0, wh0);
+            const T tx01 = fmadd(T(2), wl1, wh1);
+            const T tx10 = fmadd(T(2), wh0, wl0);
+            const T tx11 = fmadd(T(2), wh1, wl1);
+            // b U_y: y differences in both columns, x average, z average: ty[z][x].
             const T dyl0 = mlo[j + 1] - mlo[j];
             const T dyl1 = nlo[j + 1] - nlo[j];
             const T dyh0 = mhi[j + 1] - mhi[j];
             const T dyh1 = nhi[j + 1] - nhi[j];
-            const T vl0 = T(2) * dyl0 + dyl1;
-            const T vl1 = dyl0 + T(2) * dyl1;
-            const T vh0 = T(2) * dyh0 + dyh1;
-            const T vh1 = dyh0 + T(2) * dyh1;
-            const T ry00 = cb[j] * (T(2) * vl0 + vh0);
-            const T ry01 = cb[j] * (T(2) * vl1 + vh1);
-            const T ry10 = cb[j] * (vl0 + T(2) * vh0);
-            const T ry11 = cb[j] * (vl1 + T(2) * vh1);
-            // c U_z: z differences, x average (per node row), y average.
-            const T rz00 = cc[j] * (T(2) * zw0[j] + zw0[j + 1]);
-            const T rz01 = cc[j] * (T(2) * zw1[j] + zw1[j + 1]);
-            const T rz10 = cc[j] * (zw0[j] + T(2) * zw0[j + 1]);
-            const T rz11 = cc[j] * (zw1[j] + T(2) * zw1[j + 1]);
-            // Corner (z, y, x) receives s(x) RX[z][y] + s(y) RY[z][x] + s(z) RZ[y][x],
+            const T vl0 = fmadd(T(2), dyl0, dyl1);
+            const T vl1 = fmadd(T(2), dyl1, dyl0);
+            const T vh0 = fmadd(T(2), dyh0, dyh1);
+            const T vh1 = fmadd(T(2), dyh1, dyh0);
+            const T ty00 = fmadd(T(2), vl0, vh0);
+            const T ty01 = fmadd(T(2), vl1, vh1);
+            const T ty10 = fmadd(T(2), vh0, vl0);
+            const T ty11 = fmadd(T(2), vh1, vl1);
+            // c U_z: z differences, x average (per node row), y average: tz[y][x].
+            const T tz00 = fmadd(T(2), zw0[j], zw0[j + 1]);
+            const T tz01 = fmadd(T(2), zw1[j], zw1[j + 1]);
+            const T tz10 = fmadd(T(2), zw0[j + 1], zw0[j]);
+            const T tz11 = fmadd(T(2), zw1[j + 1], zw1[j]);
+            const T a = ca[j], b = cb[j], c = cc[j];
+            // Corner (z, y, x) receives s(x) a tx[z][y] + s(y) b ty[z][x] + s(z) c tz[y][x],
             // s(0) = -1, s(1) = +1.  Row j is output row j - 1, row j + 1 is output row j.
             if (j >= 1) {
-                Llo[j - 1] += (-rx00 - ry00) - rz00;
-                Rlo[j - 1] += (rx00 - ry01) - rz01;
-                Lhi[j - 1] += (-rx10 - ry10) + rz00;
-                Rhi[j - 1] += (rx10 - ry11) + rz01;
+                Llo[j - 1] = fmadd(-a, tx00, fmadd(-b, ty00, fmadd(-c, tz00, Llo[j - 1])));
+                Rlo[j - 1] = fmadd(a, tx00, fmadd(-b, ty01, fmadd(-c, tz01, Rlo[j - 1])));
+                Lhi[j - 1] = fmadd(-a, tx10, fmadd(-b, ty10, fmadd(c, tz00, Lhi[j - 1])));
+                Rhi[j - 1] = fmadd(a, tx10, fmadd(-b, ty11, fmadd(c, tz01, Rhi[j - 1])));
             }
             if (j < ROWS) {
-                Llo[j] += (-rx01 + ry00) - rz10;
-                Rlo[j] += (rx01 + ry01) - rz11;
-                Lhi[j] += (-rx11 + ry10) + rz10;
-                Rhi[j] += (rx11 + ry11) + rz11;
+                Llo[j] = fmadd(-a, tx01, fmadd(b, ty00, fmadd(-c, tz10, Llo[j])));
+                Rlo[j] = fmadd(a, tx01, fmadd(b, ty01, fmadd(-c, tz11, Rlo[j])));
+                Lhi[j] = fmadd(-a, tx11, fmadd(b, ty10, fmadd(c, tz10, Lhi[j])));
+                Rhi[j] = fmadd(a, tx11, fmadd(b, ty11, fmadd(c, tz11, Rhi[j])));
             }
         }
 
