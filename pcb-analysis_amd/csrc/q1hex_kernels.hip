@@ -364,15 +364,8 @@ __device__ void slot_sum(const double* __restrict__ part, const int stride, cons
 
 // y = A x (mode 0: fixed rows identity, Robin on the diagonal) or y = K x
 // (mode 1: unconstrained stiffness, every node free, no Robin).
-// Workgroups per CU the register budget must allow: four-row FP32 strips fit
-// four waves per SIMD (at most 128 registers), the rest two.
-template <typename T, int ROWS>
-constexpr int groups_per_cu() {
-    return (sizeof(T) == 4 && ROWS == 4) ? 4 : 2;
-}
-
 template <typename T, int ROWS, int MODE>
-__global__ __launch_bounds__(kGroupThreads, (groups_per_cu<T, ROWS>())) void k_apply(const T* __restrict__ coef, const T* __restrict__ robin,
+__global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ coef, const T* __restrict__ robin,
                                                           const T* __restrict__ x, T* __restrict__ y, const int S,
                                                           const int R, const int C, const int nsx, const int nsy,
                                                           const int zb) {
@@ -599,7 +592,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_finish_outer(const SlotDev* _
 // workgroup checks the curvature and forms alpha = rz / p.q
 // (hex_q1.cpp:454-464).
 template <typename TI, int ROWS>
-__global__ __launch_bounds__(kGroupThreads, (groups_per_cu<TI, ROWS>())) void k_inner_apply(const SlotDev* __restrict__ slots,
+__global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __restrict__ slots,
                                                                const SlotState* __restrict__ states,
                                                                const Tile* __restrict__ tiles) {
     const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
