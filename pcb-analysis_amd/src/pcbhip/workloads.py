@@ -120,3 +120,65 @@ def uneven_boards(count: int = 200, seed: int = 4) -> list[ThermalConductionProb
         n_src = int(rng.integers(1, 9))
         boards.append(layered_board(slabs, rows, cols, masks, random_sources(rng, rows, cols, n_src, 0.1, 3.0)))
     return boards
+
+
+COPPER_SIGMA_S_PER_M = 5.8e7
+DIELECTRIC_SIGMA_S_PER_M = 1.0e-12
+
+
+@dataclass(frozen=True)
+class DCBoard:
+    """A volumetric DC-conduction board: mesh (conductivity in S/m), the
+    terminal-pair excitations, the gauge node and the top-slab copper cluster
+    the terminals sit in (element mask)."""
+
+    mesh: LayeredThermalMesh
+    excitations: tuple
+    reference_node: tuple[int, int, int]
+    cluster: np.ndarray
+
+
+def dc_board(seed: int = 3, size: int = 256, slabs: int = 12, pairs: int = 64, rows: int | None = None) -> DCBoard:
+    """Copper (5.8e7 S/m) on the bottom and top slabs at U(0.3, 0.6) block
+    fill, 1e-12 S/m elsewhere; ``pairs`` excitations of +1 A / -1 A on two
+    3 x 3-cell fully-copper patches of the top slab, both inside the largest
+    connected copper cluster of that slab (copper islands that float would
+    leave the potential undetermined), 1/16 A on each of a patch's 16
+    top-face nodes; the gauge is the cluster's first top-face node."""
+
+    from scipy import ndimage
+
+    from electrical.matrix_free_mpir_fem.hex_dc import CurrentTerminal
+
+    rows = size if rows is None else rows
+    rng = np.random.default_rng(seed)
+    fills = rng.uniform(0.3, 0.6, 2)
+    masks = (copper_mask(rng, rows, size, fills[0]), copper_mask(rng, rows, size, fills[1]))
+    sigma = np.full((slabs, rows, size), DIELECTRIC_SIGMA_S_PER_M)
+    sigma[0] = np.where(masks[0], COPPER_SIGMA_S_PER_M, DIELECTRIC_SIGMA_S_PER_M)
+    sigma[-1] = np.where(masks[1], COPPER_SIGMA_S_PER_M, DIELECTRIC_SIGMA_S_PER_M)
+    mesh = LayeredThermalMesh(slab_thickness_m=(STACK_M / slabs,) * slabs, pitch_x_m=PITCH_M, pitch_y_m=PITCH_M,
+                              conductivity_w_per_m_k=sigma)
+    # Elements sharing a node conduct into each other: 8-connectivity.
+    labels, count = ndimage.label(masks[1], structure=np.ones((3, 3), dtype=int))
+    sizes = ndimage.sum(np.ones_like(labels), labels, index=np.arange(1, count + 1))
+    cluster = labels == (1 + int(np.argmax(sizes)))
+    # 3 x 3 windows made only of cluster elements.
+    inside = ndimage.minimum_filter(cluster.astype(np.uint8), size=3, mode="constant", cval=0)[1:-1, 1:-1] > 0
+    rows_ok, cols_ok = np.nonzero(inside)  # window top-left corners (row, col)
+    top = slabs
+    cells = np.argwhere(cluster)
+    ref = (top, int(cells[0][0]), int(cells[0][1]))
+    excitations = []
+    for _ in range(pairs):
+        while True:
+            a, b = rng.choice(rows_ok.size, 2, replace=False)
+            pa = (int(rows_ok[a]), int(cols_ok[a]))
+            pb = (int(rows_ok[b]), int(cols_ok[b]))
+            nodes_a = tuple((top, pa[0] + i, pa[1] + j) for i in range(4) for j in range(4))
+            nodes_b = tuple((top, pb[0] + i, pb[1] + j) for i in range(4) for j in range(4))
+            if ref in nodes_a or ref in nodes_b or set(nodes_a) & set(nodes_b):
+                continue
+            break
+        excitations.append((CurrentTerminal(nodes_a, 1.0, "source"), CurrentTerminal(nodes_b, -1.0, "sink")))
+    return DCBoard(mesh, tuple(excitations), ref, cluster)
