@@ -32,11 +32,13 @@ enum pcbhip_status {
     PCBHIP_INVALID_INPUT = 1, /* ValueError in the Python layer */
     PCBHIP_RUNTIME = 2,       /* RuntimeError: HIP failure, not SPD */
     PCBHIP_NON_FINITE = 3,    /* FloatingPointError */
-    PCBHIP_NO_MEMORY = 4      /* MemoryError */
+    PCBHIP_NO_MEMORY = 4,     /* MemoryError */
+    PCBHIP_FULL = 5           /* a server has no free slot (the caller queues the request) */
 };
 
 typedef struct pcbhip_operator pcbhip_operator;
 typedef struct pcbhip_batch pcbhip_batch;
+typedef struct pcbhip_server pcbhip_server;
 
 /* Replaces cpp/include/pcbcore/fem/mpir.hpp:11-17 (MpirConfig). */
 typedef struct pcbhip_mpir_config {
@@ -134,6 +136,61 @@ int pcbhip_batch_rhs_dev(pcbhip_batch *batch, int slot, double **rhs_dev);
 int pcbhip_batch_run(pcbhip_batch *batch);
 /* Solution (host, may be NULL), result and history (NULL or 4 * max_outer). */
 int pcbhip_batch_result(pcbhip_batch *batch, int slot, double *x, pcbhip_mpir_result *result, double *history);
+
+/* Device bytes an operator handle holds (coefficients, Robin terms and
+ * Jacobi inverses in both precisions): the admission footprint of a served
+ * request's operator. */
+int pcbhip_operator_bytes(const pcbhip_operator *op, int64_t *bytes);
+
+/* Continuous batching of MPIR solves (the "serving submit/poll" export of the
+ * drop-in boundary; the reference has no serving loop -- its callers solve one
+ * right-hand side at a time, src/multiphysics/staggered_coupling/
+ * electro_thermal.py:403-423, cpp/src/fem/dc_ports.cpp:26-49).
+ *
+ * A server owns up to max_slots slots.  submit() uploads a right-hand side of
+ * a resident operator into a free slot (its vectors come from a pool of
+ * released blocks); step() retires the oldest replay in flight, admits the
+ * submitted slots into the tile tables and launches the next replay of a
+ * captured graph of steps_per_replay MPIR steps, keeping at most two replays
+ * in flight, so admission and retirement happen between replays while the
+ * GPU keeps iterating.  finished() lists slots whose solve ended and
+ * collect() copies one result out and frees its slot.  Each slot's
+ * arithmetic is that of pcbhip_solve_mpir: a result does not depend on the
+ * requests it shared the device with.  Calls on one server are not
+ * thread-safe; distinct servers are independent. */
+typedef struct pcbhip_server_info {
+    int queued;         /* submitted, waiting for the next step */
+    int running;
+    int finished;       /* ended, not yet collected */
+    int free_slots;
+    int64_t bytes_in_use;  /* slot vectors of queued, running and finished slots */
+    int64_t bytes_pooled;  /* released blocks kept for reuse */
+    int64_t replays;
+    int64_t graph_builds;
+    int steps_per_replay;
+} pcbhip_server_info;
+
+int pcbhip_server_create(int device, int max_slots, int inner_fp64, int steps_per_replay, pcbhip_server **out);
+int pcbhip_server_destroy(pcbhip_server *server);
+/* Inner stencil rows (4 or 8) and target workgroups, as pcbhip_batch_set_shape. */
+int pcbhip_server_set_shape(pcbhip_server *server, int rows_inner, int target_groups);
+/* Device bytes one slot of this operator takes (admission footprint). */
+int pcbhip_server_slot_bytes(const pcbhip_server *server, const pcbhip_operator *op, int max_outer, int64_t *bytes);
+/* Queue one solve; PCBHIP_FULL when every slot is in use.  The operator must
+ * stay alive until the slot is collected.  x0: NULL for zero. */
+int pcbhip_server_submit(pcbhip_server *server, const pcbhip_operator *op, const double *rhs, const double *x0,
+                         const pcbhip_mpir_config *config, int64_t tag, int *slot);
+/* One replay boundary (see above); running/finished may be NULL. */
+int pcbhip_server_step(pcbhip_server *server, int *running, int *finished);
+/* Up to capacity finished slots and their tags. */
+int pcbhip_server_finished(const pcbhip_server *server, int *slots, int64_t *tags, int capacity, int *count);
+/* Solution (NULL to skip), result and history (NULL or 4 * max_outer) of a
+ * finished slot; frees the slot.  A non-SPD solve returns PCBHIP_RUNTIME
+ * after filling result. */
+int pcbhip_server_collect(pcbhip_server *server, int slot, double *x, pcbhip_mpir_result *result, double *history);
+int pcbhip_server_stats(const pcbhip_server *server, pcbhip_server_info *info);
+/* Free the pooled blocks. */
+int pcbhip_server_trim(pcbhip_server *server);
 
 /* Bandwidth reference: copy n_floats (multiple of 4) device floats. */
 int pcbhip_copy_dev(const float *src, float *dst, int64_t n_floats, void *stream);

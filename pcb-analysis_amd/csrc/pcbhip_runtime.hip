@@ -137,7 +137,7 @@ struct SlotHost {
     double* history = nullptr;
     int history_rows = 0;
     pcbhip_mpir_config config{1e-10, 0.0, 2e-3, 8, 200};
-    int tiles_outer = 0, tiles_inner = 0, tiles_flat = 0, cells = 0, part_stride = 0;
+    int tiles_flat = 0, cells = 0, part_stride = 0;
 };
 
 }  // namespace
@@ -153,6 +153,7 @@ struct pcbhip_batch {
     Tile* d_outer = nullptr;
     Tile* d_inner = nullptr;
     Tile* d_flat = nullptr;
+    int* d_counts = nullptr;  // live table lengths: outer, inner, pointwise
     int n_outer = 0, n_inner = 0, n_flat = 0;
     int rows_inner = 4;
     int outer_every = 8;
@@ -164,6 +165,7 @@ struct pcbhip_batch {
     hipGraphExec_t exec = nullptr;
     int graph_steps = 0;
     bool configs_dirty = true;
+    int h_counts[4] = {0, 0, 0, 0};  // source of the last counts upload (synchronised before reuse)
 
     ~pcbhip_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -173,7 +175,7 @@ struct pcbhip_batch {
             if (s.history) (void)hipFree(s.history);
         }
         for (void* p : {(void*)d_slots, (void*)d_states, (void*)d_configs, (void*)d_outer,
-                        (void*)d_inner, (void*)d_flat, (void*)d_active})
+                        (void*)d_inner, (void*)d_flat, (void*)d_active, (void*)d_counts})
             if (p) (void)hipFree(p);
         if (h_active) (void)hipHostFree(h_active);
         if (stream) (void)hipStreamDestroy(stream);
@@ -236,6 +238,56 @@ void apply_host(const pcbhip_operator* op, const T* x, T* y, int mode) {
     check(hipFree(dx), "hipFree");
 }
 
+SlotDev slot_dev(const SlotHost& s) {
+    SlotDev d{};
+    d.op = s.op->view();
+    d.b64 = s.b;
+    d.x64[0] = s.x[0];
+    d.x64[1] = s.x[1];
+    d.r = s.r;
+    d.p[0] = s.p[0];
+    d.p[1] = s.p[1];
+    d.q = s.q;
+    d.c = s.c;
+    d.part = s.part;
+    d.history = s.history;
+    d.cells = s.cells;
+    d.tiles_flat = s.tiles_flat;
+    d.part_stride = s.part_stride;
+    return d;
+}
+
+SlotConfig slot_config(const pcbhip_mpir_config& c) {
+    return SlotConfig{c.relative_tolerance, c.absolute_tolerance, c.inner_relative_tolerance, c.max_outer_iterations,
+                      c.max_inner_iterations};
+}
+
+// Device bytes of one slot's vectors: b and the x pair in FP64, r, the p pair,
+// q and c in the inner precision, and the partial sums.
+size_t slot_vector_bytes(const SlotHost& s, int inner64) {
+    const size_t ti = inner64 ? sizeof(double) : sizeof(float);
+    return 3 * align256(sizeof(double) * s.op->N) + 5 * align256(ti * s.op->N) +
+           align256(sizeof(double) * 4 * (size_t)s.part_stride);
+}
+
+// Points the slot's vectors into one block laid out as slot_vector_bytes
+// counts them (the batch adds an x0 buffer behind its own block).
+void carve_slot(SlotHost& s, char* base, int inner64) {
+    const size_t ti = inner64 ? sizeof(double) : sizeof(float);
+    const size_t n64 = align256(sizeof(double) * s.op->N), nti = align256(ti * s.op->N);
+    s.block = base;
+    s.b = reinterpret_cast<double*>(base);
+    s.x[0] = reinterpret_cast<double*>(base + n64);
+    s.x[1] = reinterpret_cast<double*>(base + 2 * n64);
+    base += 3 * n64;
+    s.r = base;
+    s.p[0] = base + nti;
+    s.p[1] = base + 2 * nti;
+    s.q = base + 3 * nti;
+    s.c = base + 4 * nti;
+    s.part = reinterpret_cast<double*>(base + 5 * nti);
+}
+
 // Tiles of one slot: stencil workgroups in (z block, y group, x strip)
 // order, so a contiguous range of tiles is a compact piece of the board.
 int add_tiles(std::vector<Tile>& out, int slot, int nsx, int nsy, int S, int zb) {
@@ -255,48 +307,72 @@ void upload_tiles(pcbhip_batch* b, const std::vector<Tile>& t, Tile** d, int* n)
     check(hipMemcpyAsync(*d, t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, b->stream), "upload");
 }
 
-// (Re)builds the three tile tables for the batch's current shape.  The
-// z split is chosen over the whole batch: many slots fill the device without
+// Appends the tiles of the slots that ``slots`` holds (nullptr entries are
+// skipped; the index in the vector is the slot number).  The z split is chosen
+// over all listed slots together: many slots fill the device without
 // splitting any of them.
-void build_tiles(pcbhip_batch* b, const std::vector<char>* keep = nullptr) {
-    const int rows_inner = b->inner64 ? kRowsInner64 : b->rows_inner;
+void collect_tiles(const std::vector<const SlotHost*>& slots, int rows_inner, long long target_groups,
+                   std::vector<Tile>& outer, std::vector<Tile>& inner, std::vector<Tile>& flat) {
     long long xy_outer = 0, xy_inner = 0;
-    for (size_t i = 0; i < b->slots.size(); ++i) {
-        if (keep && !(*keep)[i]) continue;
-        const SlotHost& s = b->slots[i];
-        xy_outer += (long long)strips_x(s.op->C) * groups_y(s.op->R, kRowsOuter);
-        xy_inner += (long long)strips_x(s.op->C) * groups_y(s.op->R, rows_inner);
+    for (const SlotHost* s : slots) {
+        if (!s) continue;
+        xy_outer += (long long)strips_x(s->op->C) * groups_y(s->op->R, kRowsOuter);
+        xy_inner += (long long)strips_x(s->op->C) * groups_y(s->op->R, rows_inner);
     }
-    std::vector<Tile> outer, inner, flat;
-    for (int i = 0; i < (int)b->slots.size(); ++i) {
-        SlotHost& s = b->slots[(size_t)i];
-        const pcbhip_operator* op = s.op;
+    for (int i = 0; i < (int)slots.size(); ++i) {
+        const SlotHost* s = slots[(size_t)i];
+        if (!s) continue;
+        const pcbhip_operator* op = s->op;
         const int nsx = strips_x(op->C);
         const int nyo = groups_y(op->R, kRowsOuter), nyi = groups_y(op->R, rows_inner);
-        const long long share_o = xy_outer > 0 ? (long long)nsx * nyo * b->target_groups / xy_outer : 0;
-        const long long share_i = xy_inner > 0 ? (long long)nsx * nyi * b->target_groups / xy_inner : 0;
+        const long long share_o = (long long)nsx * nyo * target_groups / xy_outer;
+        const long long share_i = (long long)nsx * nyi * target_groups / xy_inner;
         const int zbo = planes_per_block(op->S, (long long)nsx * nyo, std::max(share_o, (long long)nsx * nyo));
         const int zbi = planes_per_block(op->S, (long long)nsx * nyi, std::max(share_i, (long long)nsx * nyi));
-        s.tiles_flat = (int)((op->N + kFlatChunk - 1) / kFlatChunk);
-        if (!keep || (*keep)[(size_t)i]) {
-            // A finished slot's tiles leave the tables, so its early exits
-            // stop costing launch slots; its partial layout does not change.
-            s.tiles_outer = add_tiles(outer, i, nsx, nyo, op->S, zbo);
-            s.tiles_inner = add_tiles(inner, i, nsx, nyi, op->S, zbi);
-            for (int k = 0; k < s.tiles_flat; ++k) flat.push_back(Tile{i, k, 0, 0, 0, 0});
-        }
-        s.cells = cell_count(op->S, op->R, op->C);
-        if (std::max(s.cells, s.tiles_flat) > s.part_stride)
-            fail(PCBHIP_RUNTIME, "partial count exceeds the partial-sum area");
+        add_tiles(outer, i, nsx, nyo, op->S, zbo);
+        add_tiles(inner, i, nsx, nyi, op->S, zbi);
+        for (int k = 0; k < s->tiles_flat; ++k) flat.push_back(Tile{i, k, 0, 0, 0, 0});
     }
+}
+
+// Partial-sum layout of a slot: one partial per reduction cell of the
+// stencils or per chunk of the pointwise kernel, whichever is more.
+void set_partial_layout(SlotHost& s) {
+    const pcbhip_operator* op = s.op;
+    s.tiles_flat = (int)((op->N + kFlatChunk - 1) / kFlatChunk);
+    s.cells = cell_count(op->S, op->R, op->C);
+    s.part_stride = std::max(s.cells, s.tiles_flat);
+}
+
+// (Re)builds the three tile tables for the batch's current shape; with
+// ``keep``, only the slots it marks (finished slots leave the tables, so their
+// early exits stop costing launch slots; their partial layout does not change).
+void build_tiles(pcbhip_batch* b, const std::vector<char>* keep = nullptr) {
+    const int rows_inner = b->inner64 ? kRowsInner64 : b->rows_inner;
+    std::vector<const SlotHost*> listed(b->slots.size(), nullptr);
     b->tiled_slots = 0;
-    for (size_t i = 0; i < b->slots.size(); ++i) b->tiled_slots += (!keep || (*keep)[i]) ? 1 : 0;
+    for (size_t i = 0; i < b->slots.size(); ++i) {
+        if (keep && !(*keep)[i]) continue;
+        listed[i] = &b->slots[i];
+        b->tiled_slots += 1;
+    }
+    std::vector<Tile> outer, inner, flat;
+    collect_tiles(listed, rows_inner, b->target_groups, outer, inner, flat);
     if (outer.empty()) outer.push_back(Tile{0, 0, 0, 0, 0, 1});
     if (inner.empty()) inner.push_back(Tile{0, 0, 0, 0, 0, 1});
     if (flat.empty()) flat.push_back(Tile{0, 0, 0, 0, 0, 0});
     upload_tiles(b, outer, &b->d_outer, &b->n_outer);
     upload_tiles(b, inner, &b->d_inner, &b->n_inner);
     upload_tiles(b, flat, &b->d_flat, &b->n_flat);
+    if (!b->d_counts) b->d_counts = static_cast<int*>(device_alloc(4 * sizeof(int)));
+    b->h_counts[0] = b->n_outer;
+    b->h_counts[1] = b->n_inner;
+    b->h_counts[2] = b->n_flat;
+    b->h_counts[3] = 0;
+    check(hipMemcpyAsync(b->d_counts, b->h_counts, 4 * sizeof(int), hipMemcpyHostToDevice, b->stream), "counts");
+    // Tables and counts are read through stream order; the sources above are
+    // host memory that changes at the next build, so wait here.
+    check(hipStreamSynchronize(b->stream), "tile upload");
     b->configs_dirty = true;
     if (b->exec) {
         check(hipGraphExecDestroy(b->exec), "graph exec destroy");
@@ -322,10 +398,11 @@ void build_graph(pcbhip_batch* b, int steps) {
         // kernels stay out of most steps.
         if (s % b->outer_every == 0)
             err = launch::outer_step(b->inner64 != 0, n_slots, b->d_slots, b->d_states, b->d_configs, b->d_outer,
-                                     b->n_outer, b->d_active, b->stream);
+                                     b->d_counts, b->n_outer, b->d_active, b->stream);
         if (err == hipSuccess)
             err = launch::inner_step(b->inner64 != 0, b->rows_inner, n_slots, b->d_slots, b->d_states, b->d_configs,
-                                     b->d_inner, b->n_inner, b->d_flat, b->n_flat, b->d_active, b->stream);
+                                     b->d_inner, b->d_flat, b->d_counts, b->n_inner, b->n_flat, b->d_active,
+                                     b->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(b->stream, &dropped);
@@ -337,6 +414,300 @@ void build_graph(pcbhip_batch* b, int steps) {
     check(hipStreamEndCapture(b->stream, &b->graph), "end capture");
     check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
     b->graph_steps = steps;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ server
+
+namespace {
+
+enum SlotStatus : int { kSlotFree = 0, kSlotQueued = 1, kSlotRunning = 2, kSlotFinished = 3 };
+
+struct ServerSlot {
+    SlotHost h;
+    size_t block_bytes = 0;
+    int status = kSlotFree;
+    int64_t tag = 0;
+    long long first_replay = 0;  // sequence number of the first replay that runs it
+    long long cap_steps = 0;     // steps after which the slot must have ended
+    SlotState final_state{};
+};
+
+// Pinned sources of one set of table uploads.  Two sets are used in turn;
+// ``used`` is recorded after the uploads that read a set, and the host waits
+// on it before writing the set again.
+struct Staging {
+    int* counts = nullptr;
+    Tile* tiles = nullptr;
+    size_t tile_cap = 0;
+    SlotDev* devs = nullptr;
+    SlotConfig* cfgs = nullptr;
+    SlotState* fresh = nullptr;
+    int* fresh_index = nullptr;
+    hipEvent_t used = nullptr;
+    bool recorded = false;
+};
+
+struct Replay {
+    long long seq;
+    int parity;
+};
+
+}  // namespace
+
+struct pcbhip_server {
+    int device = 0;
+    int inner64 = 0;
+    int rows_inner = 4;
+    int outer_every = 8;
+    int steps = 16;
+    int max_slots = 0;
+    long long target_groups = kTargetGroups;
+    hipStream_t stream = nullptr;       // the MPIR replays and table uploads
+    hipStream_t copy_stream = nullptr;  // admissions and results
+    std::vector<ServerSlot> slots;
+    std::vector<std::pair<void*, size_t>> pool;  // released slot blocks
+    size_t bytes_in_use = 0, bytes_pooled = 0;
+    SlotDev* d_slots = nullptr;
+    SlotState* d_states = nullptr;
+    SlotConfig* d_configs = nullptr;
+    SlotState* d_fresh = nullptr;
+    int* d_fresh_index = nullptr;
+    int* d_active = nullptr;  // decremented by the finish kernels; the server reads states instead
+    int* d_counts = nullptr;
+    Tile* d_tiles = nullptr;  // outer | inner | pointwise tables at their capacities
+    int cap_outer = 0, cap_inner = 0, cap_flat = 0;
+    Staging stage[2];
+    int stage_next = 0;
+    SlotState* h_states = nullptr;  // pinned, two snapshots of max_slots states
+    hipEvent_t replay_done[2] = {nullptr, nullptr};
+    std::vector<Replay> inflight;  // oldest first, at most two
+    long long seq = 0;             // replays launched
+    long long graph_builds = 0;
+    bool tables_dirty = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~pcbhip_server() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (ServerSlot& s : slots)
+            if (s.h.block) (void)hipFree(s.h.block);
+        for (auto& b : pool) (void)hipFree(b.first);
+        for (void* p : {(void*)d_slots, (void*)d_states, (void*)d_configs, (void*)d_fresh, (void*)d_fresh_index,
+                        (void*)d_active, (void*)d_counts, (void*)d_tiles})
+            if (p) (void)hipFree(p);
+        for (Staging& st : stage) {
+            for (void* p : {(void*)st.counts, (void*)st.tiles, (void*)st.devs, (void*)st.cfgs, (void*)st.fresh,
+                            (void*)st.fresh_index})
+                if (p) (void)hipHostFree(p);
+            if (st.used) (void)hipEventDestroy(st.used);
+        }
+        if (h_states) (void)hipHostFree(h_states);
+        for (hipEvent_t e : replay_done)
+            if (e) (void)hipEventDestroy(e);
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T>
+T* pinned_alloc(size_t count) {
+    void* p = nullptr;
+    check(hipHostMalloc(&p, sizeof(T) * (count ? count : 1), hipHostMallocDefault), "pinned alloc");
+    return static_cast<T*>(p);
+}
+
+// Waits for the oldest replay in flight and takes its snapshot of the slot
+// states: slots it shows finished leave the tables at the next upload.
+void server_retire_oldest(pcbhip_server* sv) {
+    const Replay r = sv->inflight.front();
+    sv->inflight.erase(sv->inflight.begin());
+    check(hipEventSynchronize(sv->replay_done[r.parity]), "replay");
+    const SlotState* snap = sv->h_states + (size_t)r.parity * sv->max_slots;
+    for (int i = 0; i < sv->max_slots; ++i) {
+        ServerSlot& s = sv->slots[(size_t)i];
+        if (s.status != kSlotRunning || s.first_replay > r.seq) continue;
+        if (snap[i].phase == kPhaseDone) {
+            s.status = kSlotFinished;
+            s.final_state = snap[i];
+            sv->tables_dirty = true;
+        } else if ((r.seq - s.first_replay + 1) * sv->steps > s.cap_steps) {
+            fail(PCBHIP_RUNTIME, "MPIR iteration did not terminate within its limits");
+        }
+    }
+}
+
+void server_drain(pcbhip_server* sv) {
+    while (!sv->inflight.empty()) server_retire_oldest(sv);
+}
+
+void server_build_graph(pcbhip_server* sv) {
+    if (sv->exec) {
+        check(hipGraphExecDestroy(sv->exec), "graph exec destroy");
+        sv->exec = nullptr;
+    }
+    if (sv->graph) {
+        check(hipGraphDestroy(sv->graph), "graph destroy");
+        sv->graph = nullptr;
+    }
+    Tile* outer = sv->d_tiles;
+    Tile* inner = outer + sv->cap_outer;
+    Tile* flat = inner + sv->cap_inner;
+    check(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    for (int s = 0; s < sv->steps; ++s) {
+        hipError_t err = hipSuccess;
+        if (s % sv->outer_every == 0)
+            err = launch::outer_step(sv->inner64 != 0, sv->max_slots, sv->d_slots, sv->d_states, sv->d_configs,
+                                     outer, sv->d_counts, sv->cap_outer, sv->d_active, sv->stream);
+        if (err == hipSuccess)
+            err = launch::inner_step(sv->inner64 != 0, sv->rows_inner, sv->max_slots, sv->d_slots, sv->d_states,
+                                     sv->d_configs, inner, flat, sv->d_counts, sv->cap_inner, sv->cap_flat,
+                                     sv->d_active, sv->stream);
+        if (err != hipSuccess) {
+            hipGraph_t dropped = nullptr;
+            (void)hipStreamEndCapture(sv->stream, &dropped);
+            if (dropped) (void)hipGraphDestroy(dropped);
+            check(err, "capture launch");
+        }
+    }
+    check(hipStreamEndCapture(sv->stream, &sv->graph), "end capture");
+    check(hipGraphInstantiate(&sv->exec, sv->graph, nullptr, nullptr, 0), "graph instantiate");
+    sv->graph_builds += 1;
+}
+
+int round_capacity(size_t n) { return (int)std::max<size_t>(256, ((n + n / 4 + 255) / 256) * 256); }
+
+// Grows the device tables when they are too short and shrinks them when
+// they are mostly empty (workgroups past the live count still cost dispatch);
+// either way the replays in flight are drained and the graph is captured
+// again with the new grids.  Returns whether it drained.
+bool server_reserve(pcbhip_server* sv, size_t n_outer, size_t n_inner, size_t n_flat) {
+    const bool fits = n_outer <= (size_t)sv->cap_outer && n_inner <= (size_t)sv->cap_inner &&
+                      n_flat <= (size_t)sv->cap_flat;
+    const size_t live = n_outer + n_inner + n_flat;
+    const size_t cap = (size_t)sv->cap_outer + sv->cap_inner + sv->cap_flat;
+    const bool sparse = cap > 3 * 1024 && live * 3 < cap;
+    if (sv->exec && fits && !sparse) return false;
+    server_drain(sv);
+    check(hipStreamSynchronize(sv->stream), "drain");
+    if (!fits || sparse || !sv->d_tiles) {
+        if (sv->d_tiles) check(hipFree(sv->d_tiles), "hipFree");
+        sv->d_tiles = nullptr;
+        sv->cap_outer = round_capacity(n_outer);
+        sv->cap_inner = round_capacity(n_inner);
+        sv->cap_flat = round_capacity(n_flat);
+        sv->d_tiles = static_cast<Tile*>(
+            device_alloc(sizeof(Tile) * ((size_t)sv->cap_outer + sv->cap_inner + sv->cap_flat)));
+    }
+    server_build_graph(sv);
+    return true;
+}
+
+// Uploads the tile tables of the running slots, every slot's descriptor and
+// configuration, and fresh states for the slots admitted in this call; all
+// of it runs on the replay stream between two replays.
+void server_upload(pcbhip_server* sv, const std::vector<int>& admitted) {
+    const int rows_inner = sv->inner64 ? kRowsInner64 : sv->rows_inner;
+    std::vector<Tile> outer, inner, flat;
+    // A drain inside the reserve can finish slots: list again after one.
+    for (int pass = 0; pass < 2; ++pass) {
+        std::vector<const SlotHost*> listed((size_t)sv->max_slots, nullptr);
+        for (int i = 0; i < sv->max_slots; ++i)
+            if (sv->slots[(size_t)i].status == kSlotRunning) listed[(size_t)i] = &sv->slots[(size_t)i].h;
+        outer.clear();
+        inner.clear();
+        flat.clear();
+        collect_tiles(listed, rows_inner, sv->target_groups, outer, inner, flat);
+        if (!server_reserve(sv, outer.size(), inner.size(), flat.size())) break;
+    }
+    Staging& st = sv->stage[sv->stage_next];
+    sv->stage_next ^= 1;
+    if (st.recorded) check(hipEventSynchronize(st.used), "staging");
+    const size_t n_tiles = outer.size() + inner.size() + flat.size();
+    if (n_tiles > st.tile_cap) {
+        if (st.tiles) check(hipHostFree(st.tiles), "pinned free");
+        st.tile_cap = n_tiles + n_tiles / 4;
+        st.tiles = pinned_alloc<Tile>(st.tile_cap);
+    }
+    st.counts[0] = (int)outer.size();
+    st.counts[1] = (int)inner.size();
+    st.counts[2] = (int)flat.size();
+    st.counts[3] = 0;
+    std::copy(outer.begin(), outer.end(), st.tiles);
+    std::copy(inner.begin(), inner.end(), st.tiles + outer.size());
+    std::copy(flat.begin(), flat.end(), st.tiles + outer.size() + inner.size());
+    for (int i = 0; i < sv->max_slots; ++i) {
+        const ServerSlot& s = sv->slots[(size_t)i];
+        const bool bound = s.status == kSlotRunning || s.status == kSlotFinished;
+        st.devs[i] = bound ? slot_dev(s.h) : SlotDev{};
+        st.cfgs[i] = bound ? slot_config(s.h.config) : SlotConfig{};
+    }
+    for (size_t k = 0; k < admitted.size(); ++k) {
+        std::memset(&st.fresh[k], 0, sizeof(SlotState));
+        st.fresh[k].phase = kPhaseOuter;
+        st.fresh_index[k] = admitted[k];
+    }
+    Tile* d_outer = sv->d_tiles;
+    Tile* d_inner = d_outer + sv->cap_outer;
+    Tile* d_flat = d_inner + sv->cap_inner;
+    const hipMemcpyKind h2d = hipMemcpyHostToDevice;
+    if (!outer.empty())
+        check(hipMemcpyAsync(d_outer, st.tiles, sizeof(Tile) * outer.size(), h2d, sv->stream), "tiles");
+    if (!inner.empty())
+        check(hipMemcpyAsync(d_inner, st.tiles + outer.size(), sizeof(Tile) * inner.size(), h2d, sv->stream),
+              "tiles");
+    if (!flat.empty())
+        check(hipMemcpyAsync(d_flat, st.tiles + outer.size() + inner.size(), sizeof(Tile) * flat.size(), h2d,
+                             sv->stream),
+              "tiles");
+    check(hipMemcpyAsync(sv->d_counts, st.counts, 4 * sizeof(int), h2d, sv->stream), "counts");
+    check(hipMemcpyAsync(sv->d_slots, st.devs, sizeof(SlotDev) * sv->max_slots, h2d, sv->stream), "slots");
+    check(hipMemcpyAsync(sv->d_configs, st.cfgs, sizeof(SlotConfig) * sv->max_slots, h2d, sv->stream), "configs");
+    if (!admitted.empty()) {
+        check(hipMemcpyAsync(sv->d_fresh, st.fresh, sizeof(SlotState) * admitted.size(), h2d, sv->stream), "fresh");
+        check(hipMemcpyAsync(sv->d_fresh_index, st.fresh_index, sizeof(int) * admitted.size(), h2d, sv->stream),
+              "fresh index");
+        check(launch::scatter_states(sv->d_fresh_index, sv->d_fresh, sv->d_states, (int)admitted.size(),
+                                     sv->stream),
+              "scatter states");
+    }
+    check(hipEventRecord(st.used, sv->stream), "event");
+    st.recorded = true;
+    sv->tables_dirty = false;
+}
+
+void server_launch(pcbhip_server* sv) {
+    const int parity = (int)(sv->seq & 1);
+    check(hipGraphLaunch(sv->exec, sv->stream), "graph launch");
+    check(hipMemcpyAsync(sv->h_states + (size_t)parity * sv->max_slots, sv->d_states,
+                         sizeof(SlotState) * sv->max_slots, hipMemcpyDeviceToHost, sv->stream),
+          "state snapshot");
+    check(hipEventRecord(sv->replay_done[parity], sv->stream), "event");
+    sv->inflight.push_back(Replay{sv->seq, parity});
+    sv->seq += 1;
+}
+
+void* server_take_block(pcbhip_server* sv, size_t bytes, size_t* got) {
+    // Best fit among the released blocks, else a new allocation.
+    size_t best = sv->pool.size();
+    for (size_t i = 0; i < sv->pool.size(); ++i)
+        if (sv->pool[i].second >= bytes && (best == sv->pool.size() || sv->pool[i].second < sv->pool[best].second))
+            best = i;
+    if (best < sv->pool.size()) {
+        void* p = sv->pool[best].first;
+        *got = sv->pool[best].second;
+        sv->pool.erase(sv->pool.begin() + (long)best);
+        sv->bytes_pooled -= *got;
+        return p;
+    }
+    *got = bytes;
+    return device_alloc(bytes);
 }
 
 }  // namespace
@@ -493,38 +864,22 @@ int pcbhip_batch_create(int count, const pcbhip_operator* const* ops, int inner_
         b->device = device;
         b->inner64 = inner_fp64 ? 1 : 0;
         check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream create");
-        const size_t ti = b->inner64 ? sizeof(double) : sizeof(float);
         b->slots.resize((size_t)count);
         for (int i = 0; i < count; ++i) {
             SlotHost& s = b->slots[(size_t)i];
             const pcbhip_operator* op = ops[i];
             s.op = op;
-            // One partial per reduction cell of the stencils or per chunk of
-            // the pointwise kernel, whichever is more.
-            const long long flat = (op->N + kFlatChunk - 1) / kFlatChunk;
-            s.part_stride = (int)std::max((long long)cell_count(op->S, op->R, op->C), flat);
+            set_partial_layout(s);
             s.history_rows = s.config.max_outer_iterations;
-            // Vectors of the slot in one allocation.  History room is
-            // allocated on demand in pcbhip_batch_set.
-            const size_t n64 = align256(sizeof(double) * op->N), nti = align256(ti * op->N);
-            const size_t part = align256(sizeof(double) * 4 * (size_t)s.part_stride);
-            char* base = static_cast<char*>(device_alloc(4 * n64 + 5 * nti + part));
-            s.block = base;
-            s.b = reinterpret_cast<double*>(base);
-            s.x[0] = reinterpret_cast<double*>(base + n64);
-            s.x[1] = reinterpret_cast<double*>(base + 2 * n64);
-            s.x0 = reinterpret_cast<double*>(base + 3 * n64);
-            base += 4 * n64;
-            s.r = base;
-            s.p[0] = base + nti;
-            s.p[1] = base + 2 * nti;
-            s.q = base + 3 * nti;
-            s.c = base + 4 * nti;
-            base += 5 * nti;
-            s.part = reinterpret_cast<double*>(base);
+            // Vectors of the slot in one allocation, x0 behind them.  History
+            // room is allocated on demand in pcbhip_batch_set.
+            const size_t vec = slot_vector_bytes(s, b->inner64);
+            char* base = static_cast<char*>(device_alloc(vec + align256(sizeof(double) * op->N)));
+            carve_slot(s, base, b->inner64);
+            s.x0 = reinterpret_cast<double*>(base + vec);
             // Zeroed vectors: the kernels load p, c and x unconditionally and
             // weight them by 0 where the algorithm starts from zero.
-            check(hipMemsetAsync(s.b, 0, 4 * n64 + 5 * nti, b->stream), "memset");
+            check(hipMemsetAsync(base, 0, vec, b->stream), "memset");
         }
         build_tiles(b.get());
         b->d_slots = static_cast<SlotDev*>(device_alloc(sizeof(SlotDev) * count));
@@ -626,24 +981,8 @@ int pcbhip_batch_run(pcbhip_batch* b) {
                     s.history_rows = s.config.max_outer_iterations;
                     s.history = static_cast<double*>(device_alloc(sizeof(double) * 4 * (size_t)s.history_rows));
                 }
-                SlotDev& d = devs[(size_t)i];
-                d.op = s.op->view();
-                d.b64 = s.b;
-                d.x64[0] = s.x[0];
-                d.x64[1] = s.x[1];
-                d.r = s.r;
-                d.p[0] = s.p[0];
-                d.p[1] = s.p[1];
-                d.q = s.q;
-                d.c = s.c;
-                d.part = s.part;
-                d.history = s.history;
-                d.cells = s.cells;
-                d.tiles_flat = s.tiles_flat;
-                d.part_stride = s.part_stride;
-                cfgs[(size_t)i] = SlotConfig{s.config.relative_tolerance, s.config.absolute_tolerance,
-                                             s.config.inner_relative_tolerance, s.config.max_outer_iterations,
-                                             s.config.max_inner_iterations};
+                devs[(size_t)i] = slot_dev(s);
+                cfgs[(size_t)i] = slot_config(s.config);
             }
             check(hipMemcpyAsync(b->d_slots, devs.data(), sizeof(SlotDev) * count, hipMemcpyHostToDevice,
                                  b->stream),
@@ -749,6 +1088,271 @@ int pcbhip_copy_dev(const float* src, float* dst, int64_t n_floats, void* stream
     return guarded([&] {
         if (!src || !dst || n_floats % 4 != 0) fail(PCBHIP_INVALID_INPUT, "copy needs a multiple of 4 floats");
         check(launch::copy(src, dst, n_floats, static_cast<hipStream_t>(stream)), "copy launch");
+    });
+}
+
+int pcbhip_operator_bytes(const pcbhip_operator* op, int64_t* bytes) {
+    return guarded([&] {
+        if (!op || !bytes) fail(PCBHIP_INVALID_INPUT, "null argument");
+        const size_t e = (size_t)op->E, n = (size_t)op->N;
+        *bytes = (int64_t)(align256(12 * e) + align256(24 * e) + align256(4 * n) + 2 * align256(8 * n) +
+                           align256(4 * n) + align256(8 * n));
+    });
+}
+
+int pcbhip_server_create(int device, int max_slots, int inner_fp64, int steps_per_replay, pcbhip_server** out) {
+    return guarded([&] {
+        if (!out) fail(PCBHIP_INVALID_INPUT, "null argument");
+        *out = nullptr;
+        if (max_slots < 1 || max_slots > 65536) fail(PCBHIP_INVALID_INPUT, "max_slots must be in 1..65536");
+        if (steps_per_replay < 1 || steps_per_replay > 1024)
+            fail(PCBHIP_INVALID_INPUT, "steps_per_replay must be in 1..1024");
+        set_device(device);
+        auto sv = std::make_unique<pcbhip_server>();
+        sv->device = device;
+        sv->inner64 = inner_fp64 ? 1 : 0;
+        sv->max_slots = max_slots;
+        sv->steps = steps_per_replay;
+        sv->slots.resize((size_t)max_slots);
+        check(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking), "stream create");
+        check(hipStreamCreateWithFlags(&sv->copy_stream, hipStreamNonBlocking), "stream create");
+        const size_t n = (size_t)max_slots;
+        sv->d_slots = static_cast<SlotDev*>(device_alloc(sizeof(SlotDev) * n));
+        sv->d_states = static_cast<SlotState*>(device_alloc(sizeof(SlotState) * n));
+        sv->d_configs = static_cast<SlotConfig*>(device_alloc(sizeof(SlotConfig) * n));
+        sv->d_fresh = static_cast<SlotState*>(device_alloc(sizeof(SlotState) * n));
+        sv->d_fresh_index = static_cast<int*>(device_alloc(sizeof(int) * n));
+        sv->d_active = static_cast<int*>(device_alloc(sizeof(int)));
+        sv->d_counts = static_cast<int*>(device_alloc(4 * sizeof(int)));
+        for (Staging& st : sv->stage) {
+            st.counts = pinned_alloc<int>(4);
+            st.devs = pinned_alloc<SlotDev>(n);
+            st.cfgs = pinned_alloc<SlotConfig>(n);
+            st.fresh = pinned_alloc<SlotState>(n);
+            st.fresh_index = pinned_alloc<int>(n);
+            check(hipEventCreateWithFlags(&st.used, hipEventDisableTiming), "event create");
+        }
+        sv->h_states = pinned_alloc<SlotState>(2 * n);
+        for (hipEvent_t& e : sv->replay_done) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        // Every slot starts finished, so the finish kernels skip empty slots.
+        std::vector<SlotState> done(n);
+        for (SlotState& st : done) {
+            std::memset(&st, 0, sizeof(SlotState));
+            st.phase = kPhaseDone;
+        }
+        check(hipMemcpy(sv->d_states, done.data(), sizeof(SlotState) * n, hipMemcpyHostToDevice), "states");
+        check(hipMemset(sv->d_slots, 0, sizeof(SlotDev) * n), "memset");
+        check(hipMemset(sv->d_configs, 0, sizeof(SlotConfig) * n), "memset");
+        check(hipMemset(sv->d_counts, 0, 4 * sizeof(int)), "memset");
+        check(hipMemset(sv->d_active, 0, sizeof(int)), "memset");
+        check(hipDeviceSynchronize(), "server create");
+        *out = sv.release();
+    });
+}
+
+int pcbhip_server_destroy(pcbhip_server* server) {
+    return guarded([&] {
+        if (server) {
+            set_device(server->device);
+            delete server;
+        }
+    });
+}
+
+int pcbhip_server_set_shape(pcbhip_server* sv, int rows_inner, int target_groups) {
+    return guarded([&] {
+        if (!sv) fail(PCBHIP_INVALID_INPUT, "null server");
+        if (rows_inner != 4 && rows_inner != 8) fail(PCBHIP_INVALID_INPUT, "rows_inner must be 4 or 8");
+        if (target_groups < 1) fail(PCBHIP_INVALID_INPUT, "target_groups must be positive");
+        set_device(sv->device);
+        server_drain(sv);
+        check(hipStreamSynchronize(sv->stream), "drain");
+        sv->rows_inner = rows_inner;
+        sv->target_groups = target_groups;
+        if (sv->exec) {
+            check(hipGraphExecDestroy(sv->exec), "graph exec destroy");
+            sv->exec = nullptr;
+        }
+        sv->tables_dirty = true;
+    });
+}
+
+int pcbhip_server_slot_bytes(const pcbhip_server* sv, const pcbhip_operator* op, int max_outer, int64_t* bytes) {
+    return guarded([&] {
+        if (!sv || !op || !bytes || max_outer < 1) fail(PCBHIP_INVALID_INPUT, "invalid argument");
+        SlotHost h;
+        h.op = op;
+        set_partial_layout(h);
+        *bytes = (int64_t)(slot_vector_bytes(h, sv->inner64) + align256(sizeof(double) * 4 * (size_t)max_outer));
+    });
+}
+
+int pcbhip_server_submit(pcbhip_server* sv, const pcbhip_operator* op, const double* rhs, const double* x0,
+                         const pcbhip_mpir_config* config, int64_t tag, int* slot_out) {
+    return guarded([&] {
+        if (!sv || !op || !rhs || !config || !slot_out) fail(PCBHIP_INVALID_INPUT, "null argument");
+        if (op->device != sv->device) fail(PCBHIP_INVALID_INPUT, "operator and server live on different devices");
+        check_config(*config);
+        for (long long i = 0; i < op->N; ++i)
+            if (!std::isfinite(rhs[i])) fail(PCBHIP_INVALID_INPUT, "rhs must contain only finite values");
+        if (x0)
+            for (long long i = 0; i < op->N; ++i)
+                if (!std::isfinite(x0[i])) fail(PCBHIP_INVALID_INPUT, "initial guess must be finite");
+        int index = -1;
+        for (int i = 0; i < sv->max_slots && index < 0; ++i)
+            if (sv->slots[(size_t)i].status == kSlotFree) index = i;
+        if (index < 0) fail(PCBHIP_FULL, "every slot of the server is in use");
+        set_device(sv->device);
+        ServerSlot& s = sv->slots[(size_t)index];
+        SlotHost h;
+        h.op = op;
+        h.config = *config;
+        set_partial_layout(h);
+        h.history_rows = config->max_outer_iterations;
+        const size_t vec = slot_vector_bytes(h, sv->inner64);
+        const size_t bytes = vec + align256(sizeof(double) * 4 * (size_t)h.history_rows);
+        size_t got = 0;
+        char* block = static_cast<char*>(server_take_block(sv, bytes, &got));
+        carve_slot(h, block, sv->inner64);
+        h.history = reinterpret_cast<double*>(block + vec);
+        const size_t n_bytes = sizeof(double) * (size_t)op->N;
+        // Zeroed vectors (the kernels weight p, c and x by 0 where the
+        // algorithm starts from zero, and a reused block holds old values).
+        hipError_t err = hipMemsetAsync(block, 0, bytes, sv->copy_stream);
+        if (err == hipSuccess) err = hipMemcpyAsync(h.b, rhs, n_bytes, hipMemcpyHostToDevice, sv->copy_stream);
+        if (err == hipSuccess && x0) err = hipMemcpyAsync(h.x[0], x0, n_bytes, hipMemcpyHostToDevice, sv->copy_stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(sv->copy_stream);
+        if (err != hipSuccess) {
+            sv->pool.emplace_back(block, got);
+            sv->bytes_pooled += got;
+            check(err, "slot upload");
+        }
+        s.h = h;
+        s.block_bytes = got;
+        s.status = kSlotQueued;
+        s.tag = tag;
+        s.cap_steps = (long long)(config->max_outer_iterations + 1) *
+                          (config->max_inner_iterations + sv->outer_every) + 2LL * sv->outer_every;
+        sv->bytes_in_use += got;
+        *slot_out = index;
+    });
+}
+
+int pcbhip_server_step(pcbhip_server* sv, int* running, int* finished) {
+    return guarded([&] {
+        if (!sv) fail(PCBHIP_INVALID_INPUT, "null server");
+        set_device(sv->device);
+        if (sv->inflight.size() >= 2) server_retire_oldest(sv);
+        std::vector<int> admitted;
+        for (int i = 0; i < sv->max_slots; ++i) {
+            ServerSlot& s = sv->slots[(size_t)i];
+            if (s.status != kSlotQueued) continue;
+            s.status = kSlotRunning;
+            s.first_replay = sv->seq;
+            admitted.push_back(i);
+            sv->tables_dirty = true;
+        }
+        int n_running = 0;
+        for (const ServerSlot& s : sv->slots) n_running += s.status == kSlotRunning ? 1 : 0;
+        if (n_running == 0) {
+            server_drain(sv);
+        } else {
+            if (sv->tables_dirty || !sv->exec) {
+                server_upload(sv, admitted);
+                // server_upload may have drained: slots it finished leave at the next call.
+                for (int i : admitted) sv->slots[(size_t)i].first_replay = sv->seq;
+            }
+            server_launch(sv);
+        }
+        int r = 0, f = 0;
+        for (const ServerSlot& s : sv->slots) {
+            r += s.status == kSlotRunning ? 1 : 0;
+            f += s.status == kSlotFinished ? 1 : 0;
+        }
+        if (running) *running = r;
+        if (finished) *finished = f;
+    });
+}
+
+int pcbhip_server_finished(const pcbhip_server* sv, int* slots, int64_t* tags, int capacity, int* count) {
+    return guarded([&] {
+        if (!sv || !count || capacity < 0 || (capacity > 0 && (!slots || !tags)))
+            fail(PCBHIP_INVALID_INPUT, "invalid argument");
+        int n = 0;
+        for (int i = 0; i < sv->max_slots && n < capacity; ++i) {
+            if (sv->slots[(size_t)i].status != kSlotFinished) continue;
+            slots[n] = i;
+            tags[n] = sv->slots[(size_t)i].tag;
+            ++n;
+        }
+        *count = n;
+    });
+}
+
+int pcbhip_server_collect(pcbhip_server* sv, int slot, double* x, pcbhip_mpir_result* result, double* history) {
+    return guarded([&] {
+        if (!sv || slot < 0 || slot >= sv->max_slots) fail(PCBHIP_INVALID_INPUT, "slot out of range");
+        ServerSlot& s = sv->slots[(size_t)slot];
+        if (s.status != kSlotFinished) fail(PCBHIP_INVALID_INPUT, "slot has no finished result");
+        set_device(sv->device);
+        const SlotState& st = s.final_state;
+        // The slot's vectors are final once its state reads done: no later
+        // kernel touches them, so the copy does not wait for the replays.
+        hipError_t err = hipSuccess;
+        if (x)
+            err = hipMemcpyAsync(x, s.h.x[st.xcur], sizeof(double) * (size_t)s.h.op->N, hipMemcpyDeviceToHost,
+                                 sv->copy_stream);
+        if (err == hipSuccess && history && st.outer > 0)
+            err = hipMemcpyAsync(history, s.h.history, sizeof(double) * 4 * (size_t)st.outer, hipMemcpyDeviceToHost,
+                                 sv->copy_stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(sv->copy_stream);
+        check(err, "result copy");
+        if (result) {
+            result->converged = st.converged;
+            result->outer_iterations = st.outer;
+            result->inner_iterations = st.total_inner;
+            result->high_operator_applications = st.high_apps;
+            result->low_operator_applications = st.low_apps;
+            result->not_spd = st.status == kStatusNotSpd ? 1 : 0;
+            result->relative_residual = st.rel_outer;
+        }
+        // Release: replays still in flight may list the slot, but its device
+        // state stays "done" until an admission rewrites it in stream order.
+        sv->pool.emplace_back(s.h.block, s.block_bytes);
+        sv->bytes_pooled += s.block_bytes;
+        sv->bytes_in_use -= s.block_bytes;
+        const bool not_spd = st.status == kStatusNotSpd;
+        s = ServerSlot{};
+        if (not_spd) fail(PCBHIP_RUNTIME, "inner PCG requires a finite symmetric positive-definite operator");
+    });
+}
+
+int pcbhip_server_stats(const pcbhip_server* sv, pcbhip_server_info* out) {
+    return guarded([&] {
+        if (!sv || !out) fail(PCBHIP_INVALID_INPUT, "null argument");
+        pcbhip_server_info st{};
+        for (const ServerSlot& s : sv->slots) {
+            st.queued += s.status == kSlotQueued ? 1 : 0;
+            st.running += s.status == kSlotRunning ? 1 : 0;
+            st.finished += s.status == kSlotFinished ? 1 : 0;
+            st.free_slots += s.status == kSlotFree ? 1 : 0;
+        }
+        st.bytes_in_use = (int64_t)sv->bytes_in_use;
+        st.bytes_pooled = (int64_t)sv->bytes_pooled;
+        st.replays = sv->seq;
+        st.graph_builds = sv->graph_builds;
+        st.steps_per_replay = sv->steps;
+        *out = st;
+    });
+}
+
+int pcbhip_server_trim(pcbhip_server* sv) {
+    return guarded([&] {
+        if (!sv) fail(PCBHIP_INVALID_INPUT, "null server");
+        set_device(sv->device);
+        for (auto& b : sv->pool) check(hipFree(b.first), "hipFree");
+        sv->pool.clear();
+        sv->bytes_pooled = 0;
     });
 }
 

@@ -127,12 +127,17 @@ hipError_t diagonal(const double* coef, const double* robin, double* diag, doubl
                     int C, int* bad, hipStream_t stream);
 hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream);
 // One outer step (FP64 residual and decision) for the slots in their outer
-// phase, and one inner Jacobi-PCG iteration for the slots in their inner phase.
+// phase, and one inner Jacobi-PCG iteration for the slots in their inner
+// phase.  ``counts`` (device) holds the live lengths of the outer, inner and
+// pointwise tile tables; the grids may be longer (workgroups past the count
+// return at once), so a captured graph survives changes of the tables.
 hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
-                      const Tile* outer_tiles, int n_outer, int* active, hipStream_t stream);
+                      const Tile* outer_tiles, const int* counts, int grid_outer, int* active, hipStream_t stream);
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
-                      const SlotConfig* configs, const Tile* inner_tiles, int n_inner, const Tile* flat_tiles,
-                      int n_flat, int* active, hipStream_t stream);
+                      const SlotConfig* configs, const Tile* inner_tiles, const Tile* flat_tiles, const int* counts,
+                      int grid_inner, int grid_flat, int* active, hipStream_t stream);
+// states[index[i]] = fresh[i] for i < count (device arrays).
+hipError_t scatter_states(const int* index, const SlotState* fresh, SlotState* states, int count, hipStream_t stream);
 }  // namespace launch
 
 }  // namespace pcbhip

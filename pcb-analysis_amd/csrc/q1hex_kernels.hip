@@ -473,9 +473,11 @@ struct InnerOf<double> {
 template <typename TI>
 __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restrict__ slots,
                                                          const SlotState* __restrict__ states,
-                                                         const Tile* __restrict__ tiles, const int count) {
+                                                         const Tile* __restrict__ tiles,
+                                                         const int* __restrict__ counts) {
     // Persistent over the tile table: in most steps no slot is in its outer
     // phase and each workgroup only reads phases.
+    const int count = counts[0];
     for (int t = blockIdx.x; t < count; t += gridDim.x) {
         const Tile tile = tiles[t];
         const SlotState& st = states[tile.slot];
@@ -594,8 +596,12 @@ __global__ __launch_bounds__(kGroupThreads) void k_finish_outer(const SlotDev* _
 template <typename TI, int ROWS>
 __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __restrict__ slots,
                                                                const SlotState* __restrict__ states,
-                                                               const Tile* __restrict__ tiles) {
-    const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+                                                               const Tile* __restrict__ tiles,
+                                                               const int* __restrict__ counts) {
+    // The grid may be longer than the table (a server sizes it for growth).
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= counts[1]) return;
+    const Tile tile = tiles[t];
     const SlotState& st = states[tile.slot];
     if (st.phase != kPhaseInner) return;
     const SlotDev& sd = slots[tile.slot];
@@ -667,8 +673,11 @@ __global__ __launch_bounds__(kGroupThreads) void k_finish_apply(const SlotDev* _
 template <typename TI>
 __global__ __launch_bounds__(kGroupThreads) void k_inner_update(const SlotDev* __restrict__ slots,
                                                                 const SlotState* __restrict__ states,
-                                                                const Tile* __restrict__ tiles) {
-    const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+                                                                const Tile* __restrict__ tiles,
+                                                                const int* __restrict__ counts) {
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= counts[2]) return;
+    const Tile tile = tiles[t];
     const SlotState& st = states[tile.slot];
     if (st.phase != kPhaseInner) return;
     const SlotDev& sd = slots[tile.slot];
@@ -748,6 +757,15 @@ __global__ __launch_bounds__(kGroupThreads) void k_finish_update(const SlotDev* 
     }
 }
 
+// Writes fresh states of newly admitted slots of a server: one thread per
+// listed slot.  Slots already iterating keep the states their finish kernels
+// wrote.
+__global__ void k_scatter_states(const int* __restrict__ index, const SlotState* __restrict__ fresh,
+                                 SlotState* states, const int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) states[index[i]] = fresh[i];
+}
+
 // ------------------------------------------------------ host launchers
 
 namespace launch {
@@ -801,42 +819,51 @@ hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t st
 }
 
 hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState* states,
-                      const SlotConfig* configs, const Tile* outer_tiles, int n_outer, int* active,
-                      hipStream_t stream) {
+                      const SlotConfig* configs, const Tile* outer_tiles, const int* counts, int grid_outer,
+                      int* active, hipStream_t stream) {
     const dim3 block(kGroupThreads);
-    const dim3 grid(n_outer < kOuterGroups ? n_outer : kOuterGroups);
+    const dim3 grid(grid_outer < kOuterGroups ? grid_outer : kOuterGroups);
     if (inner64) {
-        hipLaunchKernelGGL(k_outer<double>, grid, block, 0, stream, slots, states, outer_tiles, n_outer);
+        hipLaunchKernelGGL(k_outer<double>, grid, block, 0, stream, slots, states, outer_tiles, counts);
         hipLaunchKernelGGL(k_finish_outer<double>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
     } else {
-        hipLaunchKernelGGL(k_outer<float>, grid, block, 0, stream, slots, states, outer_tiles, n_outer);
+        hipLaunchKernelGGL(k_outer<float>, grid, block, 0, stream, slots, states, outer_tiles, counts);
         hipLaunchKernelGGL(k_finish_outer<float>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
     }
     return hipGetLastError();
 }
 
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
-                      const SlotConfig* configs, const Tile* inner_tiles, int n_inner, const Tile* flat_tiles,
-                      int n_flat, int* active, hipStream_t stream) {
+                      const SlotConfig* configs, const Tile* inner_tiles, const Tile* flat_tiles, const int* counts,
+                      int grid_inner, int grid_flat, int* active, hipStream_t stream) {
     const dim3 block(kGroupThreads);
     if (inner64) {
-        hipLaunchKernelGGL((k_inner_apply<double, kRowsInner64>), dim3(n_inner), block, 0, stream, slots, states,
-                           inner_tiles);
+        hipLaunchKernelGGL((k_inner_apply<double, kRowsInner64>), dim3(grid_inner), block, 0, stream, slots, states,
+                           inner_tiles, counts);
         hipLaunchKernelGGL(k_finish_apply<double>, dim3(n_slots), block, 0, stream, slots, states, active);
-        hipLaunchKernelGGL(k_inner_update<double>, dim3(n_flat), block, 0, stream, slots, states, flat_tiles);
+        hipLaunchKernelGGL(k_inner_update<double>, dim3(grid_flat), block, 0, stream, slots, states, flat_tiles,
+                           counts);
         hipLaunchKernelGGL(k_finish_update<double>, dim3(n_slots), block, 0, stream, slots, states, configs);
     } else {
         if (rows_inner == 8) {
-            hipLaunchKernelGGL((k_inner_apply<float, 8>), dim3(n_inner), block, 0, stream, slots, states,
-                               inner_tiles);
+            hipLaunchKernelGGL((k_inner_apply<float, 8>), dim3(grid_inner), block, 0, stream, slots, states,
+                               inner_tiles, counts);
         } else {
-            hipLaunchKernelGGL((k_inner_apply<float, 4>), dim3(n_inner), block, 0, stream, slots, states,
-                               inner_tiles);
+            hipLaunchKernelGGL((k_inner_apply<float, 4>), dim3(grid_inner), block, 0, stream, slots, states,
+                               inner_tiles, counts);
         }
         hipLaunchKernelGGL(k_finish_apply<float>, dim3(n_slots), block, 0, stream, slots, states, active);
-        hipLaunchKernelGGL(k_inner_update<float>, dim3(n_flat), block, 0, stream, slots, states, flat_tiles);
+        hipLaunchKernelGGL(k_inner_update<float>, dim3(grid_flat), block, 0, stream, slots, states, flat_tiles,
+                           counts);
         hipLaunchKernelGGL(k_finish_update<float>, dim3(n_slots), block, 0, stream, slots, states, configs);
     }
+    return hipGetLastError();
+}
+
+hipError_t scatter_states(const int* index, const SlotState* fresh, SlotState* states, int count,
+                          hipStream_t stream) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_states, dim3((count + 63) / 64), dim3(64), 0, stream, index, fresh, states, count);
     return hipGetLastError();
 }
 

@@ -20,8 +20,15 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libpcbhip.so"
 
-OK, INVALID_INPUT, RUNTIME, NON_FINITE, NO_MEMORY = 0, 1, 2, 3, 4
-_ERRORS = {INVALID_INPUT: ValueError, RUNTIME: RuntimeError, NON_FINITE: FloatingPointError, NO_MEMORY: MemoryError}
+OK, INVALID_INPUT, RUNTIME, NON_FINITE, NO_MEMORY, FULL = 0, 1, 2, 3, 4, 5
+
+
+class ServerFull(RuntimeError):
+    """Every slot of a server is in use (the caller keeps the request queued)."""
+
+
+_ERRORS = {INVALID_INPUT: ValueError, RUNTIME: RuntimeError, NON_FINITE: FloatingPointError, NO_MEMORY: MemoryError,
+           FULL: ServerFull}
 
 # Every symbol include/pcbhip.h declares.
 EXPORTS = (
@@ -46,6 +53,17 @@ EXPORTS = (
     "pcbhip_batch_rhs_dev",
     "pcbhip_batch_run",
     "pcbhip_batch_result",
+    "pcbhip_operator_bytes",
+    "pcbhip_server_create",
+    "pcbhip_server_destroy",
+    "pcbhip_server_set_shape",
+    "pcbhip_server_slot_bytes",
+    "pcbhip_server_submit",
+    "pcbhip_server_step",
+    "pcbhip_server_finished",
+    "pcbhip_server_collect",
+    "pcbhip_server_stats",
+    "pcbhip_server_trim",
     "pcbhip_copy_dev",
 )
 
@@ -57,6 +75,20 @@ class MpirConfigC(ctypes.Structure):
         ("inner_relative_tolerance", ctypes.c_double),
         ("max_outer_iterations", ctypes.c_int),
         ("max_inner_iterations", ctypes.c_int),
+    ]
+
+
+class ServerInfoC(ctypes.Structure):
+    _fields_ = [
+        ("queued", ctypes.c_int),
+        ("running", ctypes.c_int),
+        ("finished", ctypes.c_int),
+        ("free_slots", ctypes.c_int),
+        ("bytes_in_use", ctypes.c_int64),
+        ("bytes_pooled", ctypes.c_int64),
+        ("replays", ctypes.c_int64),
+        ("graph_builds", ctypes.c_int64),
+        ("steps_per_replay", ctypes.c_int),
     ]
 
 
@@ -109,6 +141,19 @@ def load() -> ctypes.CDLL:
     lib.pcbhip_batch_run.argtypes = [_vp]
     lib.pcbhip_batch_result.argtypes = [_vp, _int, _vp, ctypes.POINTER(MpirResultC), _vp]
     lib.pcbhip_copy_dev.argtypes = [_vp, _vp, ctypes.c_int64, _vp]
+    _i64p = ctypes.POINTER(ctypes.c_int64)
+    lib.pcbhip_operator_bytes.argtypes = [_vp, _i64p]
+    lib.pcbhip_server_create.argtypes = [_int, _int, _int, _int, ctypes.POINTER(_vp)]
+    lib.pcbhip_server_destroy.argtypes = [_vp]
+    lib.pcbhip_server_set_shape.argtypes = [_vp, _int, _int]
+    lib.pcbhip_server_slot_bytes.argtypes = [_vp, _vp, _int, _i64p]
+    lib.pcbhip_server_submit.argtypes = [_vp, _vp, _vp, _vp, ctypes.POINTER(MpirConfigC), ctypes.c_int64,
+                                         ctypes.POINTER(_int)]
+    lib.pcbhip_server_step.argtypes = [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int)]
+    lib.pcbhip_server_finished.argtypes = [_vp, _vp, _vp, _int, ctypes.POINTER(_int)]
+    lib.pcbhip_server_collect.argtypes = [_vp, _int, _vp, ctypes.POINTER(MpirResultC), _vp]
+    lib.pcbhip_server_stats.argtypes = [_vp, ctypes.POINTER(ServerInfoC)]
+    lib.pcbhip_server_trim.argtypes = [_vp]
     _lib = lib
     return lib
 
@@ -212,6 +257,11 @@ class Operator:
     def apply_f64_dev(self, x_ptr: int, y_ptr: int, stream: int | None = None) -> None:
         check(self._lib.pcbhip_apply_f64_dev(self.handle, x_ptr, y_ptr, stream))
 
+    def device_bytes(self) -> int:
+        n = ctypes.c_int64(0)
+        check(self._lib.pcbhip_operator_bytes(self.handle, ctypes.byref(n)))
+        return int(n.value)
+
 
 def result_dict(res: MpirResultC, history: np.ndarray) -> dict:
     rows = history.reshape(-1, 4)[: res.outer_iterations]
@@ -297,3 +347,85 @@ def solve_mpir(op: Operator, rhs, x0, config: MpirConfigC, inner_fp64: bool = Fa
                                     ctypes.byref(res), _ptr(history))
     check(code)
     return x, result_dict(res, history)
+
+
+class Server:
+    """Owning handle of a continuous-batching server (``pcbhip_server``).
+
+    ``submit`` queues one right-hand side of a resident operator, ``step``
+    is one replay boundary (retire, admit, launch), ``finished`` lists ended
+    slots and ``collect`` copies a result out and frees its slot."""
+
+    def __init__(self, device: int = 0, max_slots: int = 256, inner_fp64: bool = False, steps_per_replay: int = 16):
+        lib = load()
+        self._lib = lib
+        self.handle = None
+        h = _vp()
+        check(lib.pcbhip_server_create(int(device), int(max_slots), int(bool(inner_fp64)), int(steps_per_replay),
+                                       ctypes.byref(h)))
+        self.handle = h
+        self.max_slots = int(max_slots)
+        self._slots: dict[int, tuple[Operator, int]] = {}  # slot -> (operator kept alive, max_outer)
+        self._ids = np.zeros(self.max_slots, dtype=np.int32)
+        self._tags = np.zeros(self.max_slots, dtype=np.int64)
+
+    def close(self) -> None:
+        if self.handle is not None:
+            self._lib.pcbhip_server_destroy(self.handle)
+            self.handle = None
+            self._slots.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def set_shape(self, rows_inner: int, target_groups: int) -> None:
+        check(self._lib.pcbhip_server_set_shape(self.handle, int(rows_inner), int(target_groups)))
+
+    def slot_bytes(self, op: Operator, max_outer: int) -> int:
+        n = ctypes.c_int64(0)
+        check(self._lib.pcbhip_server_slot_bytes(self.handle, op.handle, int(max_outer), ctypes.byref(n)))
+        return int(n.value)
+
+    def submit(self, op: Operator, rhs, x0=None, config: MpirConfigC | None = None, tag: int = 0) -> int:
+        b = op._vec(rhs, np.float64)
+        x = None if x0 is None else op._vec(x0, np.float64)
+        cfg = config if config is not None else config_c(1e-10, 0.0, 2e-3, 8, 200)
+        slot = _int(-1)
+        check(self._lib.pcbhip_server_submit(self.handle, op.handle, _ptr(b), None if x is None else _ptr(x),
+                                             ctypes.byref(cfg), int(tag), ctypes.byref(slot)))
+        self._slots[int(slot.value)] = (op, int(cfg.max_outer_iterations))
+        return int(slot.value)
+
+    def step(self) -> tuple[int, int]:
+        running, finished = _int(0), _int(0)
+        check(self._lib.pcbhip_server_step(self.handle, ctypes.byref(running), ctypes.byref(finished)))
+        return int(running.value), int(finished.value)
+
+    def finished(self) -> list[tuple[int, int]]:
+        n = _int(0)
+        check(self._lib.pcbhip_server_finished(self.handle, _ptr(self._ids), _ptr(self._tags), self.max_slots,
+                                               ctypes.byref(n)))
+        return [(int(self._ids[i]), int(self._tags[i])) for i in range(n.value)]
+
+    def collect(self, slot: int, *, solution: bool = True):
+        op, max_outer = self._slots[slot]
+        x = np.empty(op.size) if solution else None
+        res = MpirResultC()
+        history = np.zeros(4 * max(1, max_outer))
+        try:
+            check(self._lib.pcbhip_server_collect(self.handle, int(slot), None if x is None else _ptr(x),
+                                                  ctypes.byref(res), _ptr(history)))
+        finally:
+            self._slots.pop(slot, None)
+        return x, result_dict(res, history)
+
+    def info(self) -> dict:
+        st = ServerInfoC()
+        check(self._lib.pcbhip_server_stats(self.handle, ctypes.byref(st)))
+        return {name: int(getattr(st, name)) for name, _ in ServerInfoC._fields_}
+
+    def trim(self) -> None:
+        check(self._lib.pcbhip_server_trim(self.handle))

@@ -182,3 +182,66 @@ def dc_board(seed: int = 3, size: int = 256, slabs: int = 12, pairs: int = 64, r
             break
         excitations.append((CurrentTerminal(nodes_a, 1.0, "source"), CurrentTerminal(nodes_b, -1.0, "sink")))
     return DCBoard(mesh, tuple(excitations), ref, cluster)
+
+
+@dataclass
+class PoolBoard:
+    """One board of the serving pool: its resident device operator, up to
+    ``max_rhs`` right-hand sides and the front-end operator that built them."""
+
+    kind: str
+    operator: object  # _capi.Operator
+    rhs: list
+    front_end: object
+
+
+def element_heat_load(mesh: LayeredThermalMesh, element_heat_w: np.ndarray) -> np.ndarray:
+    """Element heat lumped an eighth onto each corner (the nodal load of
+    element heat, reference operator.py:374-387)."""
+
+    from thermal.matrix_free_mpir_fem.mesh import corner_views
+
+    load = np.zeros(mesh.node_shape)
+    eighth = np.asarray(element_heat_w, dtype=np.float64) / 8.0
+    for view in corner_views(load):
+        view[...] += eighth
+    return load.reshape(-1)
+
+
+def serving_pool(seed: int = 5, thermal_boards: int = 16, dc_boards: int = 8, max_rhs: int = 16,
+                 max_side: int = 384) -> list[PoolBoard]:
+    """Boards for the serving stream: thermal boards of the uneven-batch size
+    and material distribution whose right-hand sides are independent draws of
+    1..8 heat sources, and DC boards of the copper/dielectric materials on
+    the same size distribution whose right-hand sides are terminal pairs in
+    the largest copper cluster.  Operators are built on device 0."""
+
+    from electrical.matrix_free_mpir_fem.hex_dc import MatrixFreeHexDCOperator
+    from thermal.matrix_free_mpir_fem import MatrixFreeThermalOperator
+
+    rng = np.random.default_rng(seed)
+    pool: list[PoolBoard] = []
+    kinds = ["thermal"] * thermal_boards + ["dc"] * dc_boards
+    for kind in kinds:
+        rows, cols = (min(max_side, int(round(float(np.exp(v))))) for v in rng.uniform(np.log(32), np.log(384), 2))
+        slabs = int(rng.integers(4, 17))
+        if kind == "thermal":
+            fills = rng.uniform(0.2, 0.7, 2)
+            masks = (copper_mask(rng, rows, cols, fills[0]), copper_mask(rng, rows, cols, fills[1]))
+            problem = layered_board(slabs, rows, cols, masks, random_sources(rng, rows, cols, 1, 0.1, 3.0))
+            op = MatrixFreeThermalOperator(problem)
+            reference = op.default_reference_temperature()
+            base = op.build_rhs(reference) - op.nodal_load()
+            free = op.free_nodes.reshape(-1)
+            rhs = []
+            for _ in range(max_rhs):
+                heat = np.zeros(problem.mesh.element_grid_shape)
+                for s in random_sources(rng, rows, cols, int(rng.integers(1, 9)), 0.1, 3.0):
+                    heat[-1, s.row:s.row + s.side, s.col:s.col + s.side] += s.power_w / (s.side * s.side)
+                rhs.append(np.where(free, base + element_heat_load(problem.mesh, heat), base))
+            pool.append(PoolBoard(kind, op.device_operator, rhs, op))
+        else:
+            board = dc_board(seed=int(rng.integers(1 << 30)), size=cols, rows=rows, slabs=slabs, pairs=max_rhs)
+            op = MatrixFreeHexDCOperator(board.mesh, reference_node=board.reference_node)
+            pool.append(PoolBoard(kind, op.device_operator, [op.build_rhs(t) for t in board.excitations], op))
+    return pool
