@@ -274,6 +274,110 @@ def run_multirhs64(args, torch) -> dict:
     }
 
 
+def run_serve(args, torch) -> dict:
+    """The serving stream: Poisson arrivals of thermal (70 %) and DC (30 %)
+    requests of 1..16 right-hand sides on pool boards of the uneven-batch
+    size distribution, admitted into a 200 GB device budget and iterated by
+    continuous batching; against the same request mix solved one right-hand
+    side at a time through the single-solve path."""
+
+    from electrical.matrix_free_mpir_fem import MPIRConfig
+    from pcbhip import _capi, serving, workloads
+
+    log("building the serving pool")
+    pool = workloads.serving_pool(seed=5, thermal_boards=args.pool_thermal, dc_boards=args.pool_dc)
+    thermal = [b for b in pool if b.kind == "thermal"]
+    dc = [b for b in pool if b.kind == "dc"]
+    config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000).to_c()
+    rng = np.random.default_rng(5)
+
+    def make_requests(rate: float, duration: float) -> list:
+        out = []
+        for i, t in enumerate(serving.poisson_arrivals(rng, rate, duration)):
+            group = thermal if rng.random() < 0.7 else dc
+            board = group[int(rng.integers(len(group)))]
+            m = int(rng.integers(1, 17))
+            out.append(serving.ServeRequest(i, board.operator, board.rhs[:m], config, kind=board.kind,
+                                            arrival_s=float(t)))
+        return out
+
+    # Single-solve path on the same mix: one reusable one-slot batch per board.
+    singles = {}
+    for b in pool:
+        sb = _capi.Batch([b.operator])
+        sb.set_shape(args.rows, 1024)
+        sb.set(0, b.rhs[0], None, config)
+        sb.run()
+        singles[id(b.operator)] = sb
+    sample = make_requests(50.0, 10.0)[: args.single_sample]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n_single = 0
+    for req in sample:
+        sb = singles[id(req.operator)]
+        for rhs in req.rhs:
+            sb.set(0, rhs, None, None)
+            sb.run()
+            sb.result(0)
+            n_single += 1
+    single_rate = n_single / (time.perf_counter() - t0)
+    log(f"single-solve path: {single_rate:.2f} solves/s over {n_single} right-hand sides")
+    lines = []
+    for rate in args.rates:
+        requests = make_requests(rate, args.duration)
+        server = serving.SolveServer(max_slots=args.max_slots, memory_budget_bytes=200e9,
+                                     steps_per_replay=args.replay_steps, max_running_nodes=args.max_running_nodes,
+                                     rows_inner=args.rows)
+        log(f"rate {rate}/s: {len(requests)} requests, {sum(len(r.rhs) for r in requests)} right-hand sides")
+        elapsed = serving.run_arrivals(server, requests, log=log, time_limit_s=args.duration + args.drain_limit)
+        done = server.completed
+        solves = sum(len(r.results) for r in done)
+        unfinished = len(requests) - len(done)
+        summary = serving.latency_summary(done) if unfinished == 0 else {"p50_ms": None, "p99_ms": None}
+        info = server.backend.info()
+        line = {
+            "rate_per_s": rate,
+            "requests": len(requests),
+            "unfinished": unfinished,
+            "offered_solves_per_s": sum(len(r.rhs) for r in requests) / args.duration,
+            "solves_per_s": solves / elapsed,
+            "p50_ms": summary["p50_ms"],
+            "p99_ms": summary["p99_ms"],
+            "converged": int(sum(sum(x["converged"] for x in r.results) for r in done)),
+            "peak_requests_in_flight": server.peak_running_requests,
+            "peak_slots": server.peak_running_slots,
+            "peak_admitted_gb": server.peak_bytes / 1e9,
+            "replays": info["replays"],
+            "graph_builds": info["graph_builds"],
+        }
+        log(json.dumps(line))
+        lines.append(line)
+        server.close()
+    best = max(lines, key=lambda l: l["solves_per_s"])
+    return {
+        "metric": "served MPIR solves/s at 1e-10 rel. residual with p50/p99 latency",
+        "value": best["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 outer",
+        "data": "synthetic (seeded serving pool, seed 5)",
+        "config": {
+            "workload": f"Poisson arrivals, 70 % thermal / 30 % DC, 1..16 right-hand sides, "
+                        f"{len(thermal)} + {len(dc)} pool boards, {args.duration:.0f} s per rate",
+            "single_solve_path_solves_per_s": single_rate,
+            "speedup_vs_single_solve_path": best["solves_per_s"] / single_rate,
+            "rates": lines,
+            "parallelism": "single GPU",
+        },
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -283,7 +387,16 @@ def main() -> int:
     parser.add_argument("--slabs", type=int, default=16)
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--rows", type=int, default=4, help="rows per wave strip of the FP32 inner stencil")
-    parser.add_argument("--workload", default="board512", choices=["board512", "batch200", "multirhs64"])
+    parser.add_argument("--workload", default="board512", choices=["board512", "batch200", "multirhs64", "serve"])
+    parser.add_argument("--rates", type=float, nargs="+", default=[50.0, 100.0, 200.0, 400.0])
+    parser.add_argument("--duration", type=float, default=10.0)
+    parser.add_argument("--drain-limit", type=float, default=60.0)
+    parser.add_argument("--pool-thermal", type=int, default=16)
+    parser.add_argument("--pool-dc", type=int, default=8)
+    parser.add_argument("--single-sample", type=int, default=12)
+    parser.add_argument("--max-slots", type=int, default=1024)
+    parser.add_argument("--replay-steps", type=int, default=16)
+    parser.add_argument("--max-running-nodes", type=int, default=None)
     parser.add_argument("--dc-size", type=int, default=256)
     parser.add_argument("--count", type=int, default=200, help="boards in the batch200 workload")
     args = parser.parse_args()
@@ -304,6 +417,9 @@ def main() -> int:
         return 0
     if args.workload == "multirhs64":
         print(json.dumps(run_multirhs64(args, torch)), flush=True)
+        return 0
+    if args.workload == "serve":
+        print(json.dumps(run_serve(args, torch)), flush=True)
         return 0
     log("building board")
     problem = workloads.board_512x512x16(size=args.size, slabs=args.slabs)

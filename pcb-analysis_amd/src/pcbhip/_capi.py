@@ -415,11 +415,11 @@ class Server:
         x = np.empty(op.size) if solution else None
         res = MpirResultC()
         history = np.zeros(4 * max(1, max_outer))
-        try:
-            check(self._lib.pcbhip_server_collect(self.handle, int(slot), None if x is None else _ptr(x),
-                                                  ctypes.byref(res), _ptr(history)))
-        finally:
+        code = self._lib.pcbhip_server_collect(self.handle, int(slot), None if x is None else _ptr(x),
+                                               ctypes.byref(res), _ptr(history))
+        if code in (OK, RUNTIME):  # collected (a failed solve frees its slot too)
             self._slots.pop(slot, None)
+        check(code)
         return x, result_dict(res, history)
 
     def info(self) -> dict:

@@ -225,16 +225,21 @@ class SolveServer:
             close()
 
 
-def run_arrivals(server: SolveServer, requests: Sequence[ServeRequest], *, log=None) -> float:
+def run_arrivals(server: SolveServer, requests: Sequence[ServeRequest], *, log=None,
+                 time_limit_s: float | None = None) -> float:
     """Feeds ``requests`` (``arrival_s`` relative to the start, ascending)
-    to ``server`` at their arrival times and pumps until all complete.
-    Arrival times are rewritten to the server clock.  Returns the wall time."""
+    to ``server`` at their arrival times and pumps until all complete, or
+    until ``time_limit_s`` has passed (requests still queued or in flight
+    then stay uncompleted).  Arrival times are rewritten to the server clock.
+    Returns the wall time."""
 
     start = server.clock()
     pending = deque(requests)
     last_log = start
     while pending or not server.idle:
         now = server.clock()
+        if time_limit_s is not None and now - start > time_limit_s:
+            break
         while pending and pending[0].arrival_s <= now - start:
             r = pending.popleft()
             r.arrival_s = start + r.arrival_s

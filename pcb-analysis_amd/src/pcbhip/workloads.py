@@ -229,7 +229,7 @@ def serving_pool(seed: int = 5, thermal_boards: int = 16, dc_boards: int = 8, ma
             fills = rng.uniform(0.2, 0.7, 2)
             masks = (copper_mask(rng, rows, cols, fills[0]), copper_mask(rng, rows, cols, fills[1]))
             problem = layered_board(slabs, rows, cols, masks, random_sources(rng, rows, cols, 1, 0.1, 3.0))
-            op = MatrixFreeThermalOperator(problem)
+            op = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
             reference = op.default_reference_temperature()
             base = op.build_rhs(reference) - op.nodal_load()
             free = op.free_nodes.reshape(-1)

@@ -30,7 +30,7 @@ def _device():
 def _thermal(rng, slabs, rows, cols, m):
     masks = (workloads.copper_mask(rng, rows, cols, 0.4), workloads.copper_mask(rng, rows, cols, 0.5))
     problem = workloads.layered_board(slabs, rows, cols, masks, workloads.random_sources(rng, rows, cols, 2, 0.1, 3.0))
-    op = MatrixFreeThermalOperator(problem)
+    op = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
     base = op.build_rhs(op.default_reference_temperature()) - op.nodal_load()
     free = op.free_nodes.reshape(-1)
     rhs = []
