@@ -215,8 +215,10 @@ def run_batch200(args, torch) -> dict:
 
 def run_multirhs64(args, torch) -> dict:
     """64 terminal-pair excitations of the 256 x 256 x 12 copper/dielectric
-    DC board on one device-resident operator, solved together, against the
-    same 64 solved one at a time."""
+    DC board on one device-resident operator, solved together with the block
+    apply (groups of right-hand sides sharing the operator's reads, matrix-core
+    element action), against the same 64 as plain slots of one batch and
+    solved one at a time."""
 
     from electrical.matrix_free_mpir_fem import MPIRConfig
     from electrical.matrix_free_mpir_fem.hex_dc import MatrixFreeHexDCOperator
@@ -226,18 +228,30 @@ def run_multirhs64(args, torch) -> dict:
     operator = MatrixFreeHexDCOperator(board.mesh, reference_node=board.reference_node)
     rhss = [operator.build_rhs(t) for t in board.excitations]
     config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000)
-    batch = _capi.Batch([operator.device_operator] * len(rhss))
-    batch.set_shape(args.rows, 1024)
-    for i, b in enumerate(rhss):
-        batch.set(i, b, None, config.to_c())
-    for _ in range(args.warmup):
-        batch.run()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.run()
-    batched_s = (time.perf_counter() - t0) / args.steps
-    results = [batch.result(i, solution=False)[1] for i in range(len(rhss))]
+
+    def timed_batch(group: int, mfma: bool):
+        batch = _capi.Batch([operator.device_operator] * len(rhss))
+        batch.set_shape(args.rows if group == 0 else 4, 1024)
+        batch.set_block(group, mfma)
+        for i, b in enumerate(rhss):
+            batch.set(i, b, None, config.to_c())
+        for _ in range(args.warmup):
+            batch.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.run()
+        seconds = (time.perf_counter() - t0) / args.steps
+        results = [batch.result(i, solution=False)[1] for i in range(len(rhss))]
+        return batch, seconds, results
+
+    variants = {}
+    for name, group, mfma in (("block_mfma", args.block, True), ("block_vector", args.block, False),
+                              ("slots", 0, False)):
+        batch, seconds, results = timed_batch(group, mfma)
+        variants[name] = (batch, seconds, results)
+        log(f"64 rhs, {name}: {seconds * 1e3:.1f} ms, {sum(r['converged'] for r in results)} converged, "
+            f"max inner {max(r['inner_iterations'] for r in results)}")
     single = _capi.Batch([operator.device_operator])
     single.set_shape(args.rows, 1024)
     single.set(0, rhss[0], None, config.to_c())
@@ -247,28 +261,40 @@ def run_multirhs64(args, torch) -> dict:
         single.set(0, b, None, None)
         single.run()
     single_s = time.perf_counter() - t0
-    log(f"64 rhs: batched {batched_s * 1e3:.1f} ms, one at a time {single_s * 1e3:.1f} ms")
+    log(f"64 rhs one at a time: {single_s * 1e3:.1f} ms")
+    # Answers of the matrix-core block path against the plain FP32 slots (both
+    # refined to 1e-10 in FP64): relative difference in potential.
+    mf_batch, mf_s, mf_results = variants["block_mfma"]
+    pl_batch = variants["slots"][0]
+    diff = 0.0
+    for i in range(0, len(rhss), 9):
+        a = mf_batch.result(i)[0]
+        b = pl_batch.result(i)[0]
+        diff = max(diff, float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
     return {
         "metric": "MPIR solves/s at 1e-10 rel. residual (64 RHS, 256x256x12 DC board)",
-        "value": len(rhss) / batched_s,
+        "value": len(rhss) / mf_s,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": batched_s * 1e3,
+        "ms_per_step": mf_s * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner / f64 outer",
+        "dtype": "split-bf16 matrix-core apply, f32 inner / f64 outer",
         "data": "synthetic (seeded DC board generator, seed 3)",
         "config": {
             "workload": f"{board.mesh.size:,} nodes, 64 terminal pairs in one copper cluster",
-            "path": "64 right-hand-side slots sharing one operator",
-            "converged": int(sum(r["converged"] for r in results)),
+            "path": f"block apply, groups of {args.block} right-hand sides, matrix-core element action",
+            "converged": int(sum(r["converged"] for r in mf_results)),
+            "block_vector_solves_per_s": len(rhss) / variants["block_vector"][1],
+            "plain_slots_solves_per_s": len(rhss) / variants["slots"][1],
             "single_solve_path_solves_per_s": len(rhss) / single_s,
-            "speedup_vs_single_solve_path": single_s / batched_s,
-            "total_inner_iterations": int(sum(r["inner_iterations"] for r in results)),
-            "max_inner_iterations": int(max(r["inner_iterations"] for r in results)),
+            "speedup_vs_single_solve_path": single_s / mf_s,
+            "max_rel_difference_vs_fp32_slots": diff,
+            "total_inner_iterations": int(sum(r["inner_iterations"] for r in mf_results)),
+            "max_inner_iterations": int(max(r["inner_iterations"] for r in mf_results)),
             "parallelism": "single GPU",
         },
     }
@@ -398,6 +424,7 @@ def main() -> int:
     parser.add_argument("--replay-steps", type=int, default=16)
     parser.add_argument("--max-running-nodes", type=int, default=None)
     parser.add_argument("--dc-size", type=int, default=256)
+    parser.add_argument("--block", type=int, default=16, help="right-hand sides per block-apply group")
     parser.add_argument("--count", type=int, default=200, help="boards in the batch200 workload")
     args = parser.parse_args()
     if args.gpus != 1:

@@ -100,6 +100,7 @@ struct pcbhip_operator {
     double* dinv64 = nullptr;
     double* diag64 = nullptr;
     int rows32 = 4, zb32 = 0, rows64 = 4, zb64 = 0;  // standalone apply tile shapes
+    int mfma32 = 0;  // standalone FP32 apply on the matrix cores
 
     OperatorDev view() const {
         OperatorDev v{};
@@ -153,8 +154,12 @@ struct pcbhip_batch {
     Tile* d_outer = nullptr;
     Tile* d_inner = nullptr;
     Tile* d_flat = nullptr;
-    int* d_counts = nullptr;  // live table lengths: outer, inner, pointwise
-    int n_outer = 0, n_inner = 0, n_flat = 0;
+    Tile* d_block = nullptr;
+    SlotGroup* d_groups = nullptr;
+    int* d_counts = nullptr;  // live table lengths: outer, inner, pointwise, block
+    int n_outer = 0, n_inner = 0, n_flat = 0, n_block = 0;
+    int block_group = 0;  // right-hand sides per block-apply group (0: no block apply)
+    int block_mfma = 1;
     int rows_inner = 4;
     int outer_every = 8;
     int tiled_slots = 0;  // slots whose tiles are in the tables
@@ -174,8 +179,8 @@ struct pcbhip_batch {
             if (s.block) (void)hipFree(s.block);
             if (s.history) (void)hipFree(s.history);
         }
-        for (void* p : {(void*)d_slots, (void*)d_states, (void*)d_configs, (void*)d_outer,
-                        (void*)d_inner, (void*)d_flat, (void*)d_active, (void*)d_counts})
+        for (void* p : {(void*)d_slots, (void*)d_states, (void*)d_configs, (void*)d_outer, (void*)d_inner,
+                        (void*)d_flat, (void*)d_block, (void*)d_groups, (void*)d_active, (void*)d_counts})
             if (p) (void)hipFree(p);
         if (h_active) (void)hipHostFree(h_active);
         if (stream) (void)hipStreamDestroy(stream);
@@ -216,7 +221,8 @@ void apply_dev(const pcbhip_operator* op, const T* x, T* y, int mode, hipStream_
     }
     const int rows = sizeof(T) == 4 ? op->rows32 : op->rows64;
     const int zb = sizeof(T) == 4 ? op->zb32 : op->zb64;
-    check(launch::apply<T>(coef, robin, x, y, op->S, op->R, op->C, mode, rows, zb, stream), "apply launch");
+    check(launch::apply<T>(coef, robin, x, y, op->S, op->R, op->C, mode, rows, zb, op->mfma32 != 0, stream),
+          "apply launch");
 }
 
 template <typename T>
@@ -303,21 +309,33 @@ int add_tiles(std::vector<Tile>& out, int slot, int nsx, int nsy, int S, int zb)
 void upload_tiles(pcbhip_batch* b, const std::vector<Tile>& t, Tile** d, int* n) {
     if (*d) check(hipFree(*d), "hipFree");
     *n = (int)t.size();
-    *d = static_cast<Tile*>(device_alloc(sizeof(Tile) * t.size()));
-    check(hipMemcpyAsync(*d, t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, b->stream), "upload");
+    *d = static_cast<Tile*>(device_alloc(sizeof(Tile) * std::max<size_t>(1, t.size())));
+    if (!t.empty())
+        check(hipMemcpyAsync(*d, t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, b->stream), "upload");
 }
 
 // Appends the tiles of the slots that ``slots`` holds (nullptr entries are
-// skipped; the index in the vector is the slot number).  The z split is chosen
-// over all listed slots together: many slots fill the device without
-// splitting any of them.
-void collect_tiles(const std::vector<const SlotHost*>& slots, int rows_inner, long long target_groups,
-                   std::vector<Tile>& outer, std::vector<Tile>& inner, std::vector<Tile>& flat) {
+// skipped; the index in the vector is the slot number).  Slots that belong to
+// one of ``groups`` get no inner tiles of their own: each group gets block
+// tiles (4-row strips, the tile's slot field holding the group index).  The
+// z split is chosen over all listed slots and groups together: many slots
+// fill the device without splitting any of them.
+void collect_tiles(const std::vector<const SlotHost*>& slots, const std::vector<SlotGroup>& groups,
+                   int rows_inner, long long target_groups, std::vector<Tile>& outer, std::vector<Tile>& inner,
+                   std::vector<Tile>& flat, std::vector<Tile>& block) {
+    std::vector<char> grouped(slots.size(), 0);
+    for (const SlotGroup& g : groups)
+        for (int k = 0; k < g.count; ++k) grouped[(size_t)g.slot[k]] = 1;
     long long xy_outer = 0, xy_inner = 0;
-    for (const SlotHost* s : slots) {
+    for (size_t i = 0; i < slots.size(); ++i) {
+        const SlotHost* s = slots[i];
         if (!s) continue;
         xy_outer += (long long)strips_x(s->op->C) * groups_y(s->op->R, kRowsOuter);
-        xy_inner += (long long)strips_x(s->op->C) * groups_y(s->op->R, rows_inner);
+        if (!grouped[i]) xy_inner += (long long)strips_x(s->op->C) * groups_y(s->op->R, rows_inner);
+    }
+    for (const SlotGroup& g : groups) {
+        const pcbhip_operator* op = slots[(size_t)g.slot[0]]->op;
+        xy_inner += (long long)strips_x(op->C) * groups_y(op->R, 4);
     }
     for (int i = 0; i < (int)slots.size(); ++i) {
         const SlotHost* s = slots[(size_t)i];
@@ -326,13 +344,57 @@ void collect_tiles(const std::vector<const SlotHost*>& slots, int rows_inner, lo
         const int nsx = strips_x(op->C);
         const int nyo = groups_y(op->R, kRowsOuter), nyi = groups_y(op->R, rows_inner);
         const long long share_o = (long long)nsx * nyo * target_groups / xy_outer;
-        const long long share_i = (long long)nsx * nyi * target_groups / xy_inner;
         const int zbo = planes_per_block(op->S, (long long)nsx * nyo, std::max(share_o, (long long)nsx * nyo));
-        const int zbi = planes_per_block(op->S, (long long)nsx * nyi, std::max(share_i, (long long)nsx * nyi));
         add_tiles(outer, i, nsx, nyo, op->S, zbo);
-        add_tiles(inner, i, nsx, nyi, op->S, zbi);
+        if (!grouped[(size_t)i]) {
+            const long long share_i = (long long)nsx * nyi * target_groups / xy_inner;
+            const int zbi = planes_per_block(op->S, (long long)nsx * nyi, std::max(share_i, (long long)nsx * nyi));
+            add_tiles(inner, i, nsx, nyi, op->S, zbi);
+        }
         for (int k = 0; k < s->tiles_flat; ++k) flat.push_back(Tile{i, k, 0, 0, 0, 0});
     }
+    for (int gi = 0; gi < (int)groups.size(); ++gi) {
+        const pcbhip_operator* op = slots[(size_t)groups[(size_t)gi].slot[0]]->op;
+        const int nsx = strips_x(op->C), ny = groups_y(op->R, 4);
+        const long long share = (long long)nsx * ny * target_groups / xy_inner;
+        const int zb = planes_per_block(op->S, (long long)nsx * ny, std::max(share, (long long)nsx * ny));
+        add_tiles(block, gi, nsx, ny, op->S, zb);
+    }
+}
+
+// Groups of at most ``size`` listed slots that share an operator, in slot
+// order, split evenly (none when size is 0).  A lone slot is a group of one,
+// so a right-hand side gets the same bits alone as in a group.
+std::vector<SlotGroup> group_slots(const std::vector<const SlotHost*>& slots, int size) {
+    std::vector<SlotGroup> out;
+    if (size <= 0) return out;
+    std::vector<const pcbhip_operator*> ops;
+    std::vector<std::vector<int>> members;
+    for (int i = 0; i < (int)slots.size(); ++i) {
+        if (!slots[(size_t)i]) continue;
+        const pcbhip_operator* op = slots[(size_t)i]->op;
+        size_t k = 0;
+        while (k < ops.size() && ops[k] != op) ++k;
+        if (k == ops.size()) {
+            ops.push_back(op);
+            members.emplace_back();
+        }
+        members[k].push_back(i);
+    }
+    for (const std::vector<int>& m : members) {
+        const int n = (int)m.size();
+        const int chunks = (n + size - 1) / size;
+        int start = 0;
+        for (int c = 0; c < chunks; ++c) {
+            const int len = n / chunks + (c < n % chunks ? 1 : 0);
+            SlotGroup g{};
+            g.count = len;
+            for (int k = 0; k < len; ++k) g.slot[k] = m[(size_t)(start + k)];
+            start += len;
+            out.push_back(g);
+        }
+    }
+    return out;
 }
 
 // Partial-sum layout of a slot: one partial per reduction cell of the
@@ -356,19 +418,27 @@ void build_tiles(pcbhip_batch* b, const std::vector<char>* keep = nullptr) {
         listed[i] = &b->slots[i];
         b->tiled_slots += 1;
     }
-    std::vector<Tile> outer, inner, flat;
-    collect_tiles(listed, rows_inner, b->target_groups, outer, inner, flat);
+    std::vector<Tile> outer, inner, flat, block;
+    const std::vector<SlotGroup> groups = group_slots(listed, b->inner64 ? 0 : b->block_group);
+    collect_tiles(listed, groups, rows_inner, b->target_groups, outer, inner, flat, block);
     if (outer.empty()) outer.push_back(Tile{0, 0, 0, 0, 0, 1});
-    if (inner.empty()) inner.push_back(Tile{0, 0, 0, 0, 0, 1});
+    if (inner.empty() && block.empty()) inner.push_back(Tile{0, 0, 0, 0, 0, 1});
     if (flat.empty()) flat.push_back(Tile{0, 0, 0, 0, 0, 0});
     upload_tiles(b, outer, &b->d_outer, &b->n_outer);
     upload_tiles(b, inner, &b->d_inner, &b->n_inner);
     upload_tiles(b, flat, &b->d_flat, &b->n_flat);
+    upload_tiles(b, block, &b->d_block, &b->n_block);
+    if (b->d_groups) check(hipFree(b->d_groups), "hipFree");
+    b->d_groups = static_cast<SlotGroup*>(device_alloc(sizeof(SlotGroup) * std::max<size_t>(1, groups.size())));
+    if (!groups.empty())
+        check(hipMemcpyAsync(b->d_groups, groups.data(), sizeof(SlotGroup) * groups.size(), hipMemcpyHostToDevice,
+                             b->stream),
+              "groups");
     if (!b->d_counts) b->d_counts = static_cast<int*>(device_alloc(4 * sizeof(int)));
     b->h_counts[0] = b->n_outer;
     b->h_counts[1] = b->n_inner;
     b->h_counts[2] = b->n_flat;
-    b->h_counts[3] = 0;
+    b->h_counts[3] = b->n_block;
     check(hipMemcpyAsync(b->d_counts, b->h_counts, 4 * sizeof(int), hipMemcpyHostToDevice, b->stream), "counts");
     // Tables and counts are read through stream order; the sources above are
     // host memory that changes at the next build, so wait here.
@@ -401,7 +471,8 @@ void build_graph(pcbhip_batch* b, int steps) {
                                      b->d_counts, b->n_outer, b->d_active, b->stream);
         if (err == hipSuccess)
             err = launch::inner_step(b->inner64 != 0, b->rows_inner, n_slots, b->d_slots, b->d_states, b->d_configs,
-                                     b->d_inner, b->d_flat, b->d_counts, b->n_inner, b->n_flat, b->d_active,
+                                     b->d_inner, b->d_flat, b->d_counts, std::max(b->n_inner, 1), b->n_flat,
+                                     b->d_block, b->d_groups, b->n_block, b->block_mfma != 0, b->d_active,
                                      b->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
@@ -567,8 +638,8 @@ void server_build_graph(pcbhip_server* sv) {
                                      outer, sv->d_counts, sv->cap_outer, sv->d_active, sv->stream);
         if (err == hipSuccess)
             err = launch::inner_step(sv->inner64 != 0, sv->rows_inner, sv->max_slots, sv->d_slots, sv->d_states,
-                                     sv->d_configs, inner, flat, sv->d_counts, sv->cap_inner, sv->cap_flat,
-                                     sv->d_active, sv->stream);
+                                     sv->d_configs, inner, flat, sv->d_counts, sv->cap_inner, sv->cap_flat, nullptr,
+                                     nullptr, 0, false, sv->d_active, sv->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(sv->stream, &dropped);
@@ -623,7 +694,8 @@ void server_upload(pcbhip_server* sv, const std::vector<int>& admitted) {
         outer.clear();
         inner.clear();
         flat.clear();
-        collect_tiles(listed, rows_inner, sv->target_groups, outer, inner, flat);
+        std::vector<Tile> block;
+        collect_tiles(listed, {}, rows_inner, sv->target_groups, outer, inner, flat, block);
         if (!server_reserve(sv, outer.size(), inner.size(), flat.size())) break;
     }
     Staging& st = sv->stage[sv->stage_next];
@@ -943,6 +1015,25 @@ int pcbhip_batch_set_shape(pcbhip_batch* b, int rows_inner, int target_groups) {
         b->target_groups = target_groups;
         build_tiles(b);
         check(hipStreamSynchronize(b->stream), "set shape");
+    });
+}
+
+int pcbhip_batch_set_block(pcbhip_batch* b, int group_size, int mfma) {
+    return guarded([&] {
+        if (!b) fail(PCBHIP_INVALID_INPUT, "null batch");
+        if (group_size < 0 || group_size > kGroupMax)
+            fail(PCBHIP_INVALID_INPUT, "group_size must be in 0.." + std::to_string(kGroupMax));
+        set_device(b->device);
+        b->block_group = group_size;
+        b->block_mfma = mfma ? 1 : 0;
+        build_tiles(b);
+    });
+}
+
+int pcbhip_operator_set_engine(pcbhip_operator* op, int mfma) {
+    return guarded([&] {
+        if (!op) fail(PCBHIP_INVALID_INPUT, "null operator");
+        op->mfma32 = mfma ? 1 : 0;
     });
 }
 

@@ -80,6 +80,89 @@ __device__ __forceinline__ T masked(const NodeVal<T>& v) {
     return v.robin >= T(0) ? v.u : T(0);
 }
 
+// ------------------------------------------------- MFMA element action
+//
+// The element action as a matrix product on the gfx950 matrix cores:
+// P = [U_x; U_y; U_z] X, with the 24 x 8 stack of 36 U (entries +-1, +-2,
+// +-4: exact in bf16) against 8 corner values of 32 elements, then
+// y = (a/36) P_x + (b/36) P_y + (c/36) P_z per corner.  The FP32 corner
+// values are split exactly into three bf16 terms (hi + mid + lo = v), the
+// products are exact and v_mfma_f32_32x32x16_bf16 accumulates in FP32, so
+// the action keeps FP32 accuracy.  Operand layout (checked on the device):
+// lane l holds A[l % 32][8 (l / 32) + i] and B[8 (l / 32) + i][l % 32],
+// i = 0..7, and D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32] in register r.
+// With K = 16 = [hi | mid] in one product and [lo | 0] in a second, the
+// rows of D are (direction r / 4, corner 4 (l / 32) + r % 4) and its
+// columns the 32 elements.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// 36 U_d[c][k] for direction d (0: x, 1: y, 2: z), corners c, k = 4dz+2dy+dx.
+__device__ __forceinline__ float unit36(const int d, const int c, const int k) {
+    const int sz = ((c >> 2) ^ (k >> 2)) & 1, sy = ((c >> 1) ^ (k >> 1)) & 1, sx = (c ^ k) & 1;
+    const float wz = sz ? 1.f : 2.f, wy = sy ? 1.f : 2.f, wx = sx ? 1.f : 2.f;
+    const float ez = sz ? -1.f : 1.f, ey = sy ? -1.f : 1.f, ex = sx ? -1.f : 1.f;
+    return d == 0 ? wz * wy * ex : (d == 1 ? wz * ey * wx : ez * wy * wx);
+}
+
+__device__ __forceinline__ bf16x8 unit_operand() {
+    const int m = threadIdx.x & 31;
+    bf16x8 a;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = (__bf16)(m < 24 ? unit36(m >> 3, m & 7, i) : 0.f);
+    return a;
+}
+
+__device__ __forceinline__ bf16x8 swap_halves(const bf16x8 v) {
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    i4 w = __builtin_bit_cast(i4, v);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = __shfl_xor(w[i], 32);
+    return __builtin_bit_cast(bf16x8, w);
+}
+
+// Accumulates the action of the elements of strip row j (its corner values
+// x[k], k = 4dz+2dy+dx, in this lane's element column) into the corner
+// accumulators: lower[t], upper[t] for t = 2dy+dx.
+__device__ __forceinline__ void element_row_mfma(const bf16x8 unit, const float (&xk)[8], const float a,
+                                                 const float b, const float c, float (&lower)[4],
+                                                 float (&upper)[4]) {
+    const bool first_half = (threadIdx.x & 63) < 32;
+    bf16x8 hi, mid, lo;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const __bf16 h = (__bf16)xk[k];
+        const float r1 = xk[k] - (float)h;
+        const __bf16 m = (__bf16)r1;
+        const float r2 = r1 - (float)m;
+        hi[k] = h;
+        mid[k] = m;
+        lo[k] = (__bf16)r2;
+    }
+    const bf16x8 zero = {};
+    const bf16x8 hi_s = swap_halves(hi), mid_s = swap_halves(mid), lo_s = swap_halves(lo);
+    // Product A covers the elements of lanes 0..31, product B those of 32..63.
+    floatx16 da = {}, db = {};
+    da = __builtin_amdgcn_mfma_f32_32x32x16_bf16(unit, first_half ? hi : mid_s, da, 0, 0, 0);
+    da = __builtin_amdgcn_mfma_f32_32x32x16_bf16(unit, first_half ? lo : zero, da, 0, 0, 0);
+    db = __builtin_amdgcn_mfma_f32_32x32x16_bf16(unit, first_half ? hi_s : mid, db, 0, 0, 0);
+    db = __builtin_amdgcn_mfma_f32_32x32x16_bf16(unit, first_half ? lo_s : zero, db, 0, 0, 0);
+    const float as = __shfl_xor(a, 32), bs = __shfl_xor(b, 32), cs = __shfl_xor(c, 32);
+    const float aa = first_half ? a : as, ba = first_half ? b : bs, ca = first_half ? c : cs;
+    const float ab = first_half ? as : a, bb = first_half ? bs : b, cb = first_half ? cs : c;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const float ya = fmadd(aa, da[t], fmadd(ba, da[4 + t], ca * da[8 + t]));
+        const float yb = fmadd(ab, db[t], fmadd(bb, db[4 + t], cb * db[8 + t]));
+        // Lanes 0..31 of A hold their own element's lower corners and lanes
+        // 32..63 of B their own element's upper corners; the other halves
+        // cross over.
+        const float w = __shfl_xor(first_half ? yb : ya, 32);
+        lower[t] = first_half ? ya : w;
+        upper[t] = first_half ? w : yb;
+    }
+}
+
 // The sum-factorised Q1 action over one wave strip.
 //
 // ``load(base, off, owned)`` returns the operand at flat node base + off
@@ -90,7 +173,7 @@ __device__ __forceinline__ T masked(const NodeVal<T>& v) {
 // for every owned node (``row`` its row in the strip), the conduction sum
 // K u restricted to free columns (no Robin term); ``plane_done(z)`` follows
 // every output plane, in uniform control flow.
-template <typename T, int ROWS, class Load, class Emit, class PlaneDone>
+template <typename T, int ROWS, bool MFMA = false, class Load, class Emit, class PlaneDone>
 __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const int S, const int R, const int C,
                                             const int x0, const int y0, const int zlo, const int zhi, Load&& load,
                                             Emit&& emit, PlaneDone&& plane_done) {
@@ -120,6 +203,8 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
     T mlo[ROWS + 2], nlo[ROWS + 2];
     NodeVal<T> vlo[ROWS];
     T Llo[ROWS], Rlo[ROWS];
+    [[maybe_unused]] bf16x8 unit;
+    if constexpr (MFMA) unit = unit_operand();
 
     auto load_plane = [&](const int z, T (&m)[ROWS + 2], T (&n)[ROWS + 2], NodeVal<T> (&v)[ROWS]) {
         const long long pb = (long long)z * NR * NC;
@@ -170,23 +255,6 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
             cc[j] = ld(rc, eoff[j], T());
         }
 
-        // Per node row: x differences in both planes, z differences in both
-        // columns and their 2-point x average.  Every rounding is explicit
-        // (the file is built without contraction): 2 a + b is one fused op
-        // and exact in its doubling, and each corner accumulates its three
-        // terms with fused multiply-adds, so the bits do not depend on the
-        // tile shape that computes an element.
-        T dxl[ROWS + 2], dxh[ROWS + 2], zw0[ROWS + 2], zw1[ROWS + 2];
-#pragma unroll
-        for (int i = 0; i < ROWS + 2; ++i) {
-            dxl[i] = nlo[i] - mlo[i];
-            dxh[i] = nhi[i] - mhi[i];
-            const T dz = mhi[i] - mlo[i];
-            const T dzn = nhi[i] - nlo[i];
-            zw0[i] = fmadd(T(2), dz, dzn);
-            zw1[i] = fmadd(T(2), dzn, dz);
-        }
-
         T Lhi[ROWS], Rhi[ROWS];
 #pragma unroll
         for (int i = 0; i < ROWS; ++i) {
@@ -194,49 +262,88 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
             Rhi[i] = T(0);
         }
 
+        if constexpr (MFMA) {
+            static_assert(sizeof(T) == 4, "the matrix-core action is FP32");
 #pragma unroll
-        for (int j = 0; j < ROWS + 1; ++j) {
-            // a U_x: y average of the x differences, then z average: tx[z][y].
-            const T wl0 = fmadd(T(2), dxl[j], dxl[j + 1]);
-            const T wl1 = fmadd(T(2), dxl[j + 1], dxl[j]);
-            const T wh0 = fmadd(T(2), dxh[j], dxh[j + 1]);
-            const T wh1 = fmadd(T(2), dxh[j + 1], dxh[j]);
-            const T tx00 = fmadd(T(2), wl0, wh0);
-            const T tx01 = fmadd(T(2), wl1, wh1);
-            const T tx10 = fmadd(T(2), wh0, wl0);
-            const T tx11 = fmadd(T(2), wh1, wl1);
-            // b U_y: y differences in both columns, x average, z average: ty[z][x].
-            const T dyl0 = mlo[j + 1] - mlo[j];
-            const T dyl1 = nlo[j + 1] - nlo[j];
-            const T dyh0 = mhi[j + 1] - mhi[j];
-            const T dyh1 = nhi[j + 1] - nhi[j];
-            const T vl0 = fmadd(T(2), dyl0, dyl1);
-            const T vl1 = fmadd(T(2), dyl1, dyl0);
-            const T vh0 = fmadd(T(2), dyh0, dyh1);
-            const T vh1 = fmadd(T(2), dyh1, dyh0);
-            const T ty00 = fmadd(T(2), vl0, vh0);
-            const T ty01 = fmadd(T(2), vl1, vh1);
-            const T ty10 = fmadd(T(2), vh0, vl0);
-            const T ty11 = fmadd(T(2), vh1, vl1);
-            // c U_z: z differences, x average (per node row), y average: tz[y][x].
-            const T tz00 = fmadd(T(2), zw0[j], zw0[j + 1]);
-            const T tz01 = fmadd(T(2), zw1[j], zw1[j + 1]);
-            const T tz10 = fmadd(T(2), zw0[j + 1], zw0[j]);
-            const T tz11 = fmadd(T(2), zw1[j + 1], zw1[j]);
-            const T a = ca[j], b = cb[j], c = cc[j];
-            // Corner (z, y, x) receives s(x) a tx[z][y] + s(y) b ty[z][x] + s(z) c tz[y][x],
-            // s(0) = -1, s(1) = +1.  Row j is output row j - 1, row j + 1 is output row j.
-            if (j >= 1) {
-                Llo[j - 1] = fmadd(-a, tx00, fmadd(-b, ty00, fmadd(-c, tz00, Llo[j - 1])));
-                Rlo[j - 1] = fmadd(a, tx00, fmadd(-b, ty01, fmadd(-c, tz01, Rlo[j - 1])));
-                Lhi[j - 1] = fmadd(-a, tx10, fmadd(-b, ty10, fmadd(c, tz00, Lhi[j - 1])));
-                Rhi[j - 1] = fmadd(a, tx10, fmadd(-b, ty11, fmadd(c, tz01, Rhi[j - 1])));
+            for (int j = 0; j < ROWS + 1; ++j) {
+                const float xk[8] = {mlo[j], nlo[j], mlo[j + 1], nlo[j + 1], mhi[j], nhi[j], mhi[j + 1], nhi[j + 1]};
+                float lower[4], upper[4];
+                element_row_mfma(unit, xk, ca[j], cb[j], cc[j], lower, upper);
+                if (j >= 1) {
+                    Llo[j - 1] += lower[0];
+                    Rlo[j - 1] += lower[1];
+                    Lhi[j - 1] += upper[0];
+                    Rhi[j - 1] += upper[1];
+                }
+                if (j < ROWS) {
+                    Llo[j] += lower[2];
+                    Rlo[j] += lower[3];
+                    Lhi[j] += upper[2];
+                    Rhi[j] += upper[3];
+                }
             }
-            if (j < ROWS) {
-                Llo[j] = fmadd(-a, tx01, fmadd(b, ty00, fmadd(-c, tz10, Llo[j])));
-                Rlo[j] = fmadd(a, tx01, fmadd(b, ty01, fmadd(-c, tz11, Rlo[j])));
-                Lhi[j] = fmadd(-a, tx11, fmadd(b, ty10, fmadd(c, tz10, Lhi[j])));
-                Rhi[j] = fmadd(a, tx11, fmadd(b, ty11, fmadd(c, tz11, Rhi[j])));
+        } else {
+            // Per node row: x differences in both planes, z differences in both
+            // columns and their 2-point x average.  Every rounding is explicit
+            // (the file is built without contraction): 2 a + b is one fused op
+            // and exact in its doubling, and each corner accumulates its three
+            // terms with fused multiply-adds, so the bits do not depend on the
+            // tile shape that computes an element.
+            T dxl[ROWS + 2], dxh[ROWS + 2], zw0[ROWS + 2], zw1[ROWS + 2];
+#pragma unroll
+            for (int i = 0; i < ROWS + 2; ++i) {
+                dxl[i] = nlo[i] - mlo[i];
+                dxh[i] = nhi[i] - mhi[i];
+                const T dz = mhi[i] - mlo[i];
+                const T dzn = nhi[i] - nlo[i];
+                zw0[i] = fmadd(T(2), dz, dzn);
+                zw1[i] = fmadd(T(2), dzn, dz);
+            }
+
+#pragma unroll
+            for (int j = 0; j < ROWS + 1; ++j) {
+                // a U_x: y average of the x differences, then z average: tx[z][y].
+                const T wl0 = fmadd(T(2), dxl[j], dxl[j + 1]);
+                const T wl1 = fmadd(T(2), dxl[j + 1], dxl[j]);
+                const T wh0 = fmadd(T(2), dxh[j], dxh[j + 1]);
+                const T wh1 = fmadd(T(2), dxh[j + 1], dxh[j]);
+                const T tx00 = fmadd(T(2), wl0, wh0);
+                const T tx01 = fmadd(T(2), wl1, wh1);
+                const T tx10 = fmadd(T(2), wh0, wl0);
+                const T tx11 = fmadd(T(2), wh1, wl1);
+                // b U_y: y differences in both columns, x average, z average: ty[z][x].
+                const T dyl0 = mlo[j + 1] - mlo[j];
+                const T dyl1 = nlo[j + 1] - nlo[j];
+                const T dyh0 = mhi[j + 1] - mhi[j];
+                const T dyh1 = nhi[j + 1] - nhi[j];
+                const T vl0 = fmadd(T(2), dyl0, dyl1);
+                const T vl1 = fmadd(T(2), dyl1, dyl0);
+                const T vh0 = fmadd(T(2), dyh0, dyh1);
+                const T vh1 = fmadd(T(2), dyh1, dyh0);
+                const T ty00 = fmadd(T(2), vl0, vh0);
+                const T ty01 = fmadd(T(2), vl1, vh1);
+                const T ty10 = fmadd(T(2), vh0, vl0);
+                const T ty11 = fmadd(T(2), vh1, vl1);
+                // c U_z: z differences, x average (per node row), y average: tz[y][x].
+                const T tz00 = fmadd(T(2), zw0[j], zw0[j + 1]);
+                const T tz01 = fmadd(T(2), zw1[j], zw1[j + 1]);
+                const T tz10 = fmadd(T(2), zw0[j + 1], zw0[j]);
+                const T tz11 = fmadd(T(2), zw1[j + 1], zw1[j]);
+                const T a = ca[j], b = cb[j], c = cc[j];
+                // Corner (z, y, x) receives s(x) a tx[z][y] + s(y) b ty[z][x] + s(z) c tz[y][x],
+                // s(0) = -1, s(1) = +1.  Row j is output row j - 1, row j + 1 is output row j.
+                if (j >= 1) {
+                    Llo[j - 1] = fmadd(-a, tx00, fmadd(-b, ty00, fmadd(-c, tz00, Llo[j - 1])));
+                    Rlo[j - 1] = fmadd(a, tx00, fmadd(-b, ty01, fmadd(-c, tz01, Rlo[j - 1])));
+                    Lhi[j - 1] = fmadd(-a, tx10, fmadd(-b, ty10, fmadd(c, tz00, Lhi[j - 1])));
+                    Rhi[j - 1] = fmadd(a, tx10, fmadd(-b, ty11, fmadd(c, tz01, Rhi[j - 1])));
+                }
+                if (j < ROWS) {
+                    Llo[j] = fmadd(-a, tx01, fmadd(b, ty00, fmadd(-c, tz10, Llo[j])));
+                    Rlo[j] = fmadd(a, tx01, fmadd(b, ty01, fmadd(-c, tz11, Rlo[j])));
+                    Lhi[j] = fmadd(-a, tx11, fmadd(b, ty10, fmadd(c, tz10, Lhi[j])));
+                    Rhi[j] = fmadd(a, tx11, fmadd(b, ty11, fmadd(c, tz11, Rhi[j])));
+                }
             }
         }
 
@@ -364,7 +471,7 @@ __device__ void slot_sum(const double* __restrict__ part, const int stride, cons
 
 // y = A x (mode 0: fixed rows identity, Robin on the diagonal) or y = K x
 // (mode 1: unconstrained stiffness, every node free, no Robin).
-template <typename T, int ROWS, int MODE>
+template <typename T, int ROWS, int MODE, bool MFMA>
 __global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ coef, const T* __restrict__ robin,
                                                           const T* __restrict__ x, T* __restrict__ y, const int S,
                                                           const int R, const int C, const int nsx, const int nsy,
@@ -379,7 +486,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ c
     const int zlo = sz * zb;
     const int zhi = zlo + zb < S + 1 ? zlo + zb : S + 1;
     if (y0 > R) return;
-    strip_apply<T, ROWS>(
+    strip_apply<T, ROWS, MFMA>(
         coef, S, R, C, x0, y0, zlo, zhi,
         [&](const long long pb, const unsigned off, const bool) -> NodeVal<T> {
             if (MODE == 1) return {ldt(x + pb, off), T(0)};
@@ -589,22 +696,11 @@ __global__ __launch_bounds__(kGroupThreads) void k_finish_outer(const SlotDev* _
     st.phase = kPhaseInner;
 }
 
-// Inner step, part 1: the direction p = z + beta p (z = r / diag; p = z on
-// the first step) fused into the operator apply q = A p, and p.q.  The last
-// workgroup checks the curvature and forms alpha = rz / p.q
-// (hex_q1.cpp:454-464).
-template <typename TI, int ROWS>
-__global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __restrict__ slots,
-                                                               const SlotState* __restrict__ states,
-                                                               const Tile* __restrict__ tiles,
-                                                               const int* __restrict__ counts) {
-    // The grid may be longer than the table (a server sizes it for growth).
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    if (t >= counts[1]) return;
-    const Tile tile = tiles[t];
-    const SlotState& st = states[tile.slot];
-    if (st.phase != kPhaseInner) return;
-    const SlotDev& sd = slots[tile.slot];
+// One slot's part of the inner apply on one tile: the direction
+// p = z + beta p_old (z = r / diag; beta = 0 on the first step) fused into
+// q = A p, and p.q per reduction cell (hex_q1.cpp:454-464).
+template <typename TI, int ROWS, bool MFMA>
+__device__ __forceinline__ void inner_apply_slot(const SlotDev& sd, const SlotState& st, const Tile& tile) {
     const OperatorDev& op = sd.op;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int x0 = tile.sx * kStripCols;
@@ -624,7 +720,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
     CellSums<1, ROWS> acc;  // p.q per reduction cell
     acc.clear();
     if (y0 <= op.R) {
-        strip_apply<TI, ROWS>(
+        strip_apply<TI, ROWS, MFMA>(
             InnerOf<TI>::coef(op), op.S, op.R, op.C, x0, y0, zlo, zhi,
             [&](const long long pb, const unsigned off, const bool) -> NodeVal<TI> {
                 const TI z = ldt(r + pb, off) * ldt(dinv + pb, off);
@@ -639,6 +735,46 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
                 acc.v[0][row / kCellRows] += (double)v.u * (double)ap;
             },
             [&](const int z) { acc.plane_done(z, op.S, op.R, op.C, x0, y0, sd.part, sd.part_stride); });
+    }
+}
+
+// Inner step, part 1: the direction fused into the operator apply and p.q,
+// one tile of one slot per workgroup.
+template <typename TI, int ROWS>
+__global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __restrict__ slots,
+                                                               const SlotState* __restrict__ states,
+                                                               const Tile* __restrict__ tiles,
+                                                               const int* __restrict__ counts) {
+    // The grid may be longer than the table (a server sizes it for growth).
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= counts[1]) return;
+    const Tile tile = tiles[t];
+    const SlotState& st = states[tile.slot];
+    if (st.phase != kPhaseInner) return;
+    inner_apply_slot<TI, ROWS, false>(slots[tile.slot], st, tile);
+}
+
+// The multi-right-hand-side block apply: one tile of a group of slots that
+// share an operator per workgroup, the slots one after another, so the
+// group's coefficient, Robin and diagonal lines are fetched from HBM once and
+// served from the caches to the rest of the group; MFMA selects the
+// matrix-core element action.  Each slot keeps its own scalars (per-column
+// alpha and beta) and its own reduction cells.
+template <int ROWS, bool MFMA>
+__global__ __launch_bounds__(kGroupThreads) void k_inner_apply_block(const SlotDev* __restrict__ slots,
+                                                                     const SlotState* __restrict__ states,
+                                                                     const Tile* __restrict__ tiles,
+                                                                     const int* __restrict__ counts,
+                                                                     const SlotGroup* __restrict__ groups) {
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= counts[3]) return;
+    const Tile tile = tiles[t];
+    const SlotGroup& group = groups[tile.slot];
+    for (int k = 0; k < group.count; ++k) {
+        const int slot = group.slot[k];
+        const SlotState& st = states[slot];
+        if (st.phase != kPhaseInner) continue;
+        inner_apply_slot<float, ROWS, MFMA>(slots[slot], st, tile);
     }
 }
 
@@ -772,24 +908,30 @@ namespace launch {
 
 template <typename T>
 hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, int C, int mode, int rows, int zb,
-                 hipStream_t stream) {
+                 bool mfma, hipStream_t stream) {
     const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
     const int nsy = (R + 1 + rows * kWavesPerGroup - 1) / (rows * kWavesPerGroup);
     const int nsz = (S + 1 + zb - 1) / zb;
     const dim3 grid(nsx * nsy * nsz), block(kGroupThreads);
-#define PCBHIP_APPLY(ROWS, MODE)                                                                                   \
-    hipLaunchKernelGGL((k_apply<T, ROWS, MODE>), grid, block, 0, stream, coef, robin, x, y, S, R, C, nsx, nsy, zb)
+#define PCBHIP_APPLY(ROWS, MODE, MF)                                                                               \
+    hipLaunchKernelGGL((k_apply<T, ROWS, MODE, MF>), grid, block, 0, stream, coef, robin, x, y, S, R, C, nsx, nsy, zb)
     if constexpr (sizeof(T) == 4) {
-        if (rows == 8) {
-            if (mode == 0) PCBHIP_APPLY(8, 0); else PCBHIP_APPLY(8, 1);
+        if (mfma) {
+            if (rows == 8) {
+                if (mode == 0) PCBHIP_APPLY(8, 0, true); else PCBHIP_APPLY(8, 1, true);
+            } else {
+                if (mode == 0) PCBHIP_APPLY(4, 0, true); else PCBHIP_APPLY(4, 1, true);
+            }
+        } else if (rows == 8) {
+            if (mode == 0) PCBHIP_APPLY(8, 0, false); else PCBHIP_APPLY(8, 1, false);
         } else {
-            if (mode == 0) PCBHIP_APPLY(4, 0); else PCBHIP_APPLY(4, 1);
+            if (mode == 0) PCBHIP_APPLY(4, 0, false); else PCBHIP_APPLY(4, 1, false);
         }
     } else {
         if (rows == 2) {
-            if (mode == 0) PCBHIP_APPLY(2, 0); else PCBHIP_APPLY(2, 1);
+            if (mode == 0) PCBHIP_APPLY(2, 0, false); else PCBHIP_APPLY(2, 1, false);
         } else {
-            if (mode == 0) PCBHIP_APPLY(4, 0); else PCBHIP_APPLY(4, 1);
+            if (mode == 0) PCBHIP_APPLY(4, 0, false); else PCBHIP_APPLY(4, 1, false);
         }
     }
 #undef PCBHIP_APPLY
@@ -797,9 +939,9 @@ hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, 
 }
 
 template hipError_t apply<float>(const float*, const float*, const float*, float*, int, int, int, int, int, int,
-                                 hipStream_t);
+                                 bool, hipStream_t);
 template hipError_t apply<double>(const double*, const double*, const double*, double*, int, int, int, int, int, int,
-                                  hipStream_t);
+                                  bool, hipStream_t);
 
 hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S,
                     int R, int C, int* bad, hipStream_t stream) {
@@ -835,7 +977,8 @@ hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState
 
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
                       const SlotConfig* configs, const Tile* inner_tiles, const Tile* flat_tiles, const int* counts,
-                      int grid_inner, int grid_flat, int* active, hipStream_t stream) {
+                      int grid_inner, int grid_flat, const Tile* block_tiles, const SlotGroup* groups,
+                      int grid_block, bool block_mfma, int* active, hipStream_t stream) {
     const dim3 block(kGroupThreads);
     if (inner64) {
         hipLaunchKernelGGL((k_inner_apply<double, kRowsInner64>), dim3(grid_inner), block, 0, stream, slots, states,
@@ -851,6 +994,15 @@ hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* 
         } else {
             hipLaunchKernelGGL((k_inner_apply<float, 4>), dim3(grid_inner), block, 0, stream, slots, states,
                                inner_tiles, counts);
+        }
+        if (grid_block > 0) {
+            if (block_mfma) {
+                hipLaunchKernelGGL((k_inner_apply_block<4, true>), dim3(grid_block), block, 0, stream, slots, states,
+                                   block_tiles, counts, groups);
+            } else {
+                hipLaunchKernelGGL((k_inner_apply_block<4, false>), dim3(grid_block), block, 0, stream, slots,
+                                   states, block_tiles, counts, groups);
+            }
         }
         hipLaunchKernelGGL(k_finish_apply<float>, dim3(n_slots), block, 0, stream, slots, states, active);
         hipLaunchKernelGGL(k_inner_update<float>, dim3(grid_flat), block, 0, stream, slots, states, flat_tiles,

@@ -50,6 +50,8 @@ EXPORTS = (
     "pcbhip_batch_set",
     "pcbhip_batch_set_shape",
     "pcbhip_operator_set_shape",
+    "pcbhip_operator_set_engine",
+    "pcbhip_batch_set_block",
     "pcbhip_batch_rhs_dev",
     "pcbhip_batch_run",
     "pcbhip_batch_result",
@@ -138,6 +140,8 @@ def load() -> ctypes.CDLL:
     lib.pcbhip_batch_rhs_dev.argtypes = [_vp, _int, ctypes.POINTER(_vp)]
     lib.pcbhip_batch_set_shape.argtypes = [_vp, _int, _int]
     lib.pcbhip_operator_set_shape.argtypes = [_vp, _int, _int, _int, _int]
+    lib.pcbhip_operator_set_engine.argtypes = [_vp, _int]
+    lib.pcbhip_batch_set_block.argtypes = [_vp, _int, _int]
     lib.pcbhip_batch_run.argtypes = [_vp]
     lib.pcbhip_batch_result.argtypes = [_vp, _int, _vp, ctypes.POINTER(MpirResultC), _vp]
     lib.pcbhip_copy_dev.argtypes = [_vp, _vp, ctypes.c_int64, _vp]
@@ -251,6 +255,11 @@ class Operator:
     def set_shape(self, rows32: int, zb32: int, rows64: int = 4, zb64: int = 1 << 20) -> None:
         check(self._lib.pcbhip_operator_set_shape(self.handle, rows32, zb32, rows64, zb64))
 
+    def set_engine(self, mfma: bool) -> None:
+        """FP32 element action on the matrix cores (True) or the vector units."""
+
+        check(self._lib.pcbhip_operator_set_engine(self.handle, int(bool(mfma))))
+
     def apply_f32_dev(self, x_ptr: int, y_ptr: int, stream: int | None = None) -> None:
         check(self._lib.pcbhip_apply_f32_dev(self.handle, x_ptr, y_ptr, stream))
 
@@ -317,6 +326,11 @@ class Batch:
 
     def set_shape(self, rows_inner: int, target_groups: int) -> None:
         check(self._lib.pcbhip_batch_set_shape(self.handle, int(rows_inner), int(target_groups)))
+
+    def set_block(self, group_size: int, mfma: bool = True) -> None:
+        """Apply slots that share an operator in groups of ``group_size``."""
+
+        check(self._lib.pcbhip_batch_set_block(self.handle, int(group_size), int(bool(mfma))))
 
     def rhs_device_pointer(self, slot: int) -> int:
         p = _vp()

@@ -99,3 +99,26 @@ def test_a_free_node_without_conductance_is_refused():
     free = np.ones(8, dtype=np.uint8)
     with pytest.raises(ValueError, match="positive conductance"):
         _capi.Operator(0, (1, 1, 1), coef, robin, free)
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("void", [False, True])
+def test_matrix_core_action_matches_the_assembled_operator(shape, void):
+    print(f"mfma apply parity shape={shape} void={void}", flush=True)
+    problem = random_body(shape, seed=sum(shape) + 1, void=void)
+    dev, ops, a, _ = _device_and_matrix(problem)
+    dev.set_engine(True)
+    x32 = np.random.default_rng(8).standard_normal(problem.mesh.size).astype(np.float32)
+    exact = a @ x32.astype(np.float64)
+    scale = float((abs(a) @ np.abs(x32.astype(np.float64))).max())
+    y = dev.apply_f32(x32)
+    # Three bf16 terms hold every FP32 value exactly and the products are
+    # exact, so the matrix-core action has the FP32 action's error bound.
+    assert np.max(np.abs(y - exact)) <= 32 * SINGLE_EPS * scale
+    np.testing.assert_array_equal(y[ops["fixed"]], x32[ops["fixed"]])
+    assert dev.apply_f32(x32).tobytes() == y.tobytes()
+    # Every element's product is one matrix-core column: the bits do not
+    # depend on the tile shape.
+    for rows, zb in ((8, 2), (4, 2), (8, 1 << 20)):
+        dev.set_shape(rows, zb)
+        assert dev.apply_f32(x32).tobytes() == y.tobytes(), (rows, zb)
