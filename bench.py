@@ -229,10 +229,10 @@ def run_multirhs64(args, torch) -> dict:
     rhss = [operator.build_rhs(t) for t in board.excitations]
     config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000)
 
-    def timed_batch(group: int, mfma: bool):
+    def timed_batch(group: int, engine: int):
         batch = _capi.Batch([operator.device_operator] * len(rhss))
         batch.set_shape(args.rows if group == 0 else 4, 1024)
-        batch.set_block(group, mfma)
+        batch.set_block(group, engine)
         for i, b in enumerate(rhss):
             batch.set(i, b, None, config.to_c())
         for _ in range(args.warmup):
@@ -246,9 +246,9 @@ def run_multirhs64(args, torch) -> dict:
         return batch, seconds, results
 
     variants = {}
-    for name, group, mfma in (("block_mfma", args.block, True), ("block_vector", args.block, False),
-                              ("slots", 0, False)):
-        batch, seconds, results = timed_batch(group, mfma)
+    for name, group, engine in (("block_mfma", args.block, args.engine), ("block_vector", args.block, 0),
+                                ("slots", 0, 0)):
+        batch, seconds, results = timed_batch(group, engine)
         variants[name] = (batch, seconds, results)
         log(f"64 rhs, {name}: {seconds * 1e3:.1f} ms, {sum(r['converged'] for r in results)} converged, "
             f"max inner {max(r['inner_iterations'] for r in results)}")
@@ -264,12 +264,20 @@ def run_multirhs64(args, torch) -> dict:
     log(f"64 rhs one at a time: {single_s * 1e3:.1f} ms")
     # Answers of the matrix-core block path against the plain FP32 slots (both
     # refined to 1e-10 in FP64): relative difference in potential.
+    # (Potential on the terminals' copper cluster, where the 1e-10 residual
+    # determines it; floating islands couple only through the 1e-12 S/m
+    # dielectric.)
     mf_batch, mf_s, mf_results = variants["block_mfma"]
     pl_batch = variants["slots"][0]
+    top = board.mesh.node_shape[0] - 1
+    on = np.zeros(board.mesh.node_shape[1:], dtype=bool)
+    for dy in (0, 1):
+        for dx in (0, 1):
+            on[dy:dy + board.cluster.shape[0], dx:dx + board.cluster.shape[1]] |= board.cluster
     diff = 0.0
     for i in range(0, len(rhss), 9):
-        a = mf_batch.result(i)[0]
-        b = pl_batch.result(i)[0]
+        a = mf_batch.result(i)[0].reshape(board.mesh.node_shape)[top][on]
+        b = pl_batch.result(i)[0].reshape(board.mesh.node_shape)[top][on]
         diff = max(diff, float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
     return {
         "metric": "MPIR solves/s at 1e-10 rel. residual (64 RHS, 256x256x12 DC board)",
@@ -286,13 +294,14 @@ def run_multirhs64(args, torch) -> dict:
         "data": "synthetic (seeded DC board generator, seed 3)",
         "config": {
             "workload": f"{board.mesh.size:,} nodes, 64 terminal pairs in one copper cluster",
-            "path": f"block apply, groups of {args.block} right-hand sides, matrix-core element action",
+            "path": f"block apply, groups of {args.block} right-hand sides, matrix-core element action "
+                    f"({args.engine}-term bf16 split)",
             "converged": int(sum(r["converged"] for r in mf_results)),
             "block_vector_solves_per_s": len(rhss) / variants["block_vector"][1],
             "plain_slots_solves_per_s": len(rhss) / variants["slots"][1],
             "single_solve_path_solves_per_s": len(rhss) / single_s,
             "speedup_vs_single_solve_path": single_s / mf_s,
-            "max_rel_difference_vs_fp32_slots": diff,
+            "max_rel_difference_vs_fp32_slots_on_cluster": diff,
             "total_inner_iterations": int(sum(r["inner_iterations"] for r in mf_results)),
             "max_inner_iterations": int(max(r["inner_iterations"] for r in mf_results)),
             "parallelism": "single GPU",
@@ -424,7 +433,8 @@ def main() -> int:
     parser.add_argument("--replay-steps", type=int, default=16)
     parser.add_argument("--max-running-nodes", type=int, default=None)
     parser.add_argument("--dc-size", type=int, default=256)
-    parser.add_argument("--block", type=int, default=16, help="right-hand sides per block-apply group")
+    parser.add_argument("--block", type=int, default=1, help="right-hand sides per block-apply group")
+    parser.add_argument("--engine", type=int, default=2, choices=[2, 3], help="bf16 terms of the matrix-core action")
     parser.add_argument("--count", type=int, default=200, help="boards in the batch200 workload")
     args = parser.parse_args()
     if args.gpus != 1:

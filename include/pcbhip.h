@@ -129,20 +129,21 @@ int pcbhip_batch_set_shape(pcbhip_batch *batch, int rows_inner, int target_group
 /* Tile shape of the standalone applies: rows per wave strip (FP32: 4 or 8,
  * FP64: 2 or 4) and node planes per z block. */
 int pcbhip_operator_set_shape(pcbhip_operator *op, int rows32, int zb32, int rows64, int zb64);
-/* Element action of the standalone FP32 apply: 0 = FP32 vector arithmetic,
- * 1 = matrix cores (the 24 x 8 stack of unit element matrices against the
- * corner values of 32 elements, split exactly into three bf16 terms, FP32
- * accumulation).  No reference counterpart (the reference's action is the
- * scalar gather, hex_q1.cpp:90-193). */
-int pcbhip_operator_set_engine(pcbhip_operator *op, int mfma);
+/* Element action of the standalone FP32 apply: 0 = FP32 vector arithmetic;
+ * 2 or 3 = matrix cores: the 24 x 8 stack of unit element matrices (exact
+ * in bf16) against the corner values of 32 elements per product, the values
+ * split into two bf16 terms (hi + lo, 16 significant bits) or three (exact
+ * for FP32), FP32 accumulation.  No reference counterpart (the reference's
+ * action is the scalar gather, hex_q1.cpp:90-193). */
+int pcbhip_operator_set_engine(pcbhip_operator *op, int engine);
 /* Multi-right-hand-side block apply: slots of the batch that share an
  * operator are applied in groups of at most group_size (1..64; 0 turns it
  * off), one tile of a group per workgroup, so the group reads the operator's
- * coefficients, Robin terms and diagonal once; mfma selects the matrix-core
- * element action.  Each slot keeps its own scalars and FP64 refinement.  The
+ * coefficients, Robin terms and diagonal once; engine is the element action
+ * as in pcbhip_operator_set_engine.  Each slot keeps its own scalars and FP64 refinement.  The
  * reference solves such right-hand sides one at a time
  * (cpp/src/fem/dc_ports.cpp:26-49).  FP32 inner solves only. */
-int pcbhip_batch_set_block(pcbhip_batch *batch, int group_size, int mfma);
+int pcbhip_batch_set_block(pcbhip_batch *batch, int group_size, int engine);
 /* Device pointers to a slot's FP64 right-hand side (write it before run). */
 int pcbhip_batch_rhs_dev(pcbhip_batch *batch, int slot, double **rhs_dev);
 /* Reset every slot to its initial guess and iterate until every slot has

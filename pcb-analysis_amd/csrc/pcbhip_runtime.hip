@@ -100,7 +100,7 @@ struct pcbhip_operator {
     double* dinv64 = nullptr;
     double* diag64 = nullptr;
     int rows32 = 4, zb32 = 0, rows64 = 4, zb64 = 0;  // standalone apply tile shapes
-    int mfma32 = 0;  // standalone FP32 apply on the matrix cores
+    int mfma32 = 0;  // standalone FP32 element action: 0 vector, 2 or 3 bf16 terms on the matrix cores
 
     OperatorDev view() const {
         OperatorDev v{};
@@ -159,7 +159,7 @@ struct pcbhip_batch {
     int* d_counts = nullptr;  // live table lengths: outer, inner, pointwise, block
     int n_outer = 0, n_inner = 0, n_flat = 0, n_block = 0;
     int block_group = 0;  // right-hand sides per block-apply group (0: no block apply)
-    int block_mfma = 1;
+    int block_mfma = 3;  // element action of the block apply (0, 2, 3)
     int rows_inner = 4;
     int outer_every = 8;
     int tiled_slots = 0;  // slots whose tiles are in the tables
@@ -221,7 +221,7 @@ void apply_dev(const pcbhip_operator* op, const T* x, T* y, int mode, hipStream_
     }
     const int rows = sizeof(T) == 4 ? op->rows32 : op->rows64;
     const int zb = sizeof(T) == 4 ? op->zb32 : op->zb64;
-    check(launch::apply<T>(coef, robin, x, y, op->S, op->R, op->C, mode, rows, zb, op->mfma32 != 0, stream),
+    check(launch::apply<T>(coef, robin, x, y, op->S, op->R, op->C, mode, rows, zb, op->mfma32, stream),
           "apply launch");
 }
 
@@ -353,12 +353,28 @@ void collect_tiles(const std::vector<const SlotHost*>& slots, const std::vector<
         }
         for (int k = 0; k < s->tiles_flat; ++k) flat.push_back(Tile{i, k, 0, 0, 0, 0});
     }
-    for (int gi = 0; gi < (int)groups.size(); ++gi) {
-        const pcbhip_operator* op = slots[(size_t)groups[(size_t)gi].slot[0]]->op;
+    // Block tiles: the groups of one operator interleaved at every spatial
+    // tile, so the workgroups that read one piece of the operator run
+    // together and share it through the XCD's L2.
+    std::vector<char> placed(groups.size(), 0);
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        if (placed[gi]) continue;
+        const pcbhip_operator* op = slots[(size_t)groups[gi].slot[0]]->op;
+        std::vector<int> same;
+        for (size_t gj = gi; gj < groups.size(); ++gj)
+            if (!placed[gj] && slots[(size_t)groups[gj].slot[0]]->op == op) {
+                same.push_back((int)gj);
+                placed[gj] = 1;
+            }
         const int nsx = strips_x(op->C), ny = groups_y(op->R, 4);
         const long long share = (long long)nsx * ny * target_groups / xy_inner;
         const int zb = planes_per_block(op->S, (long long)nsx * ny, std::max(share, (long long)nsx * ny));
-        add_tiles(block, gi, nsx, ny, op->S, zb);
+        const int nsz = (op->S + 1 + zb - 1) / zb;
+        int index = 0;
+        for (int sz = 0; sz < nsz; ++sz)
+            for (int sy = 0; sy < ny; ++sy)
+                for (int sx = 0; sx < nsx; ++sx, ++index)
+                    for (int g : same) block.push_back(Tile{g, index, (short)sx, (short)sy, (short)sz, (short)zb});
     }
 }
 
@@ -472,7 +488,7 @@ void build_graph(pcbhip_batch* b, int steps) {
         if (err == hipSuccess)
             err = launch::inner_step(b->inner64 != 0, b->rows_inner, n_slots, b->d_slots, b->d_states, b->d_configs,
                                      b->d_inner, b->d_flat, b->d_counts, std::max(b->n_inner, 1), b->n_flat,
-                                     b->d_block, b->d_groups, b->n_block, b->block_mfma != 0, b->d_active,
+                                     b->d_block, b->d_groups, b->n_block, b->block_mfma, b->d_active,
                                      b->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
@@ -639,7 +655,7 @@ void server_build_graph(pcbhip_server* sv) {
         if (err == hipSuccess)
             err = launch::inner_step(sv->inner64 != 0, sv->rows_inner, sv->max_slots, sv->d_slots, sv->d_states,
                                      sv->d_configs, inner, flat, sv->d_counts, sv->cap_inner, sv->cap_flat, nullptr,
-                                     nullptr, 0, false, sv->d_active, sv->stream);
+                                     nullptr, 0, 0, sv->d_active, sv->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(sv->stream, &dropped);
@@ -1023,9 +1039,10 @@ int pcbhip_batch_set_block(pcbhip_batch* b, int group_size, int mfma) {
         if (!b) fail(PCBHIP_INVALID_INPUT, "null batch");
         if (group_size < 0 || group_size > kGroupMax)
             fail(PCBHIP_INVALID_INPUT, "group_size must be in 0.." + std::to_string(kGroupMax));
+        if (mfma != 0 && mfma != 2 && mfma != 3) fail(PCBHIP_INVALID_INPUT, "engine must be 0, 2 or 3");
         set_device(b->device);
         b->block_group = group_size;
-        b->block_mfma = mfma ? 1 : 0;
+        b->block_mfma = mfma;
         build_tiles(b);
     });
 }
@@ -1033,7 +1050,8 @@ int pcbhip_batch_set_block(pcbhip_batch* b, int group_size, int mfma) {
 int pcbhip_operator_set_engine(pcbhip_operator* op, int mfma) {
     return guarded([&] {
         if (!op) fail(PCBHIP_INVALID_INPUT, "null operator");
-        op->mfma32 = mfma ? 1 : 0;
+        if (mfma != 0 && mfma != 2 && mfma != 3) fail(PCBHIP_INVALID_INPUT, "engine must be 0, 2 or 3");
+        op->mfma32 = mfma;
     });
 }
 

@@ -128,10 +128,11 @@ struct SlotGroup {
 
 namespace launch {
 // Host-side launchers of the kernels (q1hex_kernels.hip).
-// mfma selects the matrix-core element action (FP32 only).
+// mfma selects the element action of FP32 applies: 0 vector FMAs, 2 or 3
+// the matrix cores with a two- or three-term bf16 split of the operand.
 template <typename T>
 hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, int C, int mode, int rows, int zb,
-                 bool mfma, hipStream_t stream);
+                 int mfma, hipStream_t stream);
 hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S, int R,
                     int C, int* bad, hipStream_t stream);
 hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream);
@@ -141,13 +142,13 @@ hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t st
 // pointwise and block tile tables; the grids may be longer (workgroups past
 // the count return at once), so a captured graph survives changes of the
 // tables.  Block tiles (a slot group per tile, ``groups``) are launched when
-// grid_block > 0, with the matrix-core action when block_mfma is set.
+// grid_block > 0, with the element action block_mfma (0, 2, 3 as in apply).
 hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
                       const Tile* outer_tiles, const int* counts, int grid_outer, int* active, hipStream_t stream);
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
                       const SlotConfig* configs, const Tile* inner_tiles, const Tile* flat_tiles, const int* counts,
                       int grid_inner, int grid_flat, const Tile* block_tiles, const SlotGroup* groups,
-                      int grid_block, bool block_mfma, int* active, hipStream_t stream);
+                      int grid_block, int block_mfma, int* active, hipStream_t stream);
 // states[index[i]] = fresh[i] for i < count (device arrays).
 hipError_t scatter_states(const int* index, const SlotState* fresh, SlotState* states, int count, hipStream_t stream);
 }  // namespace launch

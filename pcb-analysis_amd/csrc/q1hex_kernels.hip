@@ -113,53 +113,78 @@ __device__ __forceinline__ bf16x8 unit_operand() {
     return a;
 }
 
-__device__ __forceinline__ bf16x8 swap_halves(const bf16x8 v) {
-    typedef int i4 __attribute__((ext_vector_type(4)));
-    i4 w = __builtin_bit_cast(i4, v);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) w[i] = __shfl_xor(w[i], 32);
-    return __builtin_bit_cast(bf16x8, w);
+// v_permlane32_swap of a and b: lo = [a lanes 0..31, b lanes 0..31],
+// hi = [a lanes 32..63, b lanes 32..63] (checked on the device).
+__device__ __forceinline__ void pair_halves(const unsigned a, const unsigned b, unsigned& lo, unsigned& hi) {
+    const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    lo = r[0];
+    hi = r[1];
 }
 
-// Accumulates the action of the elements of strip row j (its corner values
-// x[k], k = 4dz+2dy+dx, in this lane's element column) into the corner
-// accumulators: lower[t], upper[t] for t = 2dy+dx.
-__device__ __forceinline__ void element_row_mfma(const bf16x8 unit, const float (&xk)[8], const float a,
-                                                 const float b, const float c, float (&lower)[4],
-                                                 float (&upper)[4]) {
-    const bool first_half = (threadIdx.x & 63) < 32;
-    bf16x8 hi, mid, lo;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const __bf16 h = (__bf16)xk[k];
-        const float r1 = xk[k] - (float)h;
-        const __bf16 m = (__bf16)r1;
-        const float r2 = r1 - (float)m;
-        hi[k] = h;
-        mid[k] = m;
-        lo[k] = (__bf16)r2;
+__device__ __forceinline__ unsigned bf16_bits(const __bf16 v) { return (unsigned)__builtin_bit_cast(unsigned short, v); }
+
+// Matrix-core operand dwords of one node row of one plane: the exact
+// three-term bf16 split of the lane's node value m and of its x neighbour n
+// (corners dx = 0 and dx = 1 of the lane's element), packed as the bf16 pair
+// of one operand dword, with the lane halves already arranged for the two
+// products: q[0] / q[1] the [hi | mid] dwords of products A / B, q[2] / q[3]
+// the [lo | 0] dwords.  Each node row of a plane is prepared once and serves
+// the two element rows that touch it.
+template <int TERMS>
+__device__ __forceinline__ void plane_operands(const float m, const float n, unsigned (&q)[4]) {
+    const __bf16 hm = (__bf16)m, hn = (__bf16)n;
+    const float rm = m - (float)hm, rn = n - (float)hn;
+    const __bf16 mm = (__bf16)rm, mn = (__bf16)rn;
+    const unsigned ph = bf16_bits(hm) | (bf16_bits(hn) << 16);
+    const unsigned pm = bf16_bits(mm) | (bf16_bits(mn) << 16);
+    pair_halves(ph, pm, q[0], q[1]);
+    if constexpr (TERMS == 3) {
+        const __bf16 lm = (__bf16)(rm - (float)mm), ln = (__bf16)(rn - (float)mn);
+        const unsigned pl = bf16_bits(lm) | (bf16_bits(ln) << 16);
+        pair_halves(pl, 0u, q[2], q[3]);
     }
-    const bf16x8 zero = {};
-    const bf16x8 hi_s = swap_halves(hi), mid_s = swap_halves(mid), lo_s = swap_halves(lo);
-    // Product A covers the elements of lanes 0..31, product B those of 32..63.
+}
+
+// Accumulates the action of the elements of strip row j into the corner
+// accumulators lower[t], upper[t] (t = 2dy+dx).  lo0/lo1 and hi0/hi1 are
+// the operand dwords (plane_operands) of node rows j and j + 1 of the lower
+// and upper plane, i.e. corners (0,1), (2,3), (4,5), (6,7).  Product A
+// covers the elements of lanes 0..31, product B those of lanes 32..63.
+template <int TERMS>
+__device__ __forceinline__ void element_row_mfma(const bf16x8 unit, const unsigned (&lo0)[4],
+                                                 const unsigned (&lo1)[4], const unsigned (&hi0)[4],
+                                                 const unsigned (&hi1)[4], const float a, const float b,
+                                                 const float c, float (&lower)[4], float (&upper)[4]) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    const u4 b1a = {lo0[0], lo1[0], hi0[0], hi1[0]};
+    const u4 b1b = {lo0[1], lo1[1], hi0[1], hi1[1]};
     floatx16 da = {}, db = {};
-    da = __builtin_amdgcn_mfma_f32_32x32x16_bf16(unit, first_half ? hi : mid_s, da, 0, 0, 0);
-    da = __builtin_amdgcn_mfma_f32_32x32x16_bf16(unit, first_half ? lo : zero, da, 0, 0, 0);
-    db = __builtin_amdgcn_mfma_f32_32x32x16_bf16(unit, first_half ? hi_s : mid, db, 0, 0, 0);
-    db = __builtin_amdgcn_mfma_f32_32x32x16_bf16(unit, first_half ? lo_s : zero, db, 0, 0, 0);
-    const float as = __shfl_xor(a, 32), bs = __shfl_xor(b, 32), cs = __shfl_xor(c, 32);
-    const float aa = first_half ? a : as, ba = first_half ? b : bs, ca = first_half ? c : cs;
-    const float ab = first_half ? as : a, bb = first_half ? bs : b, cb = first_half ? cs : c;
+    da = __builtin_amdgcn_mfma_f32_32x32x16_bf16(unit, __builtin_bit_cast(bf16x8, b1a), da, 0, 0, 0);
+    db = __builtin_amdgcn_mfma_f32_32x32x16_bf16(unit, __builtin_bit_cast(bf16x8, b1b), db, 0, 0, 0);
+    if constexpr (TERMS == 3) {
+        const u4 b2a = {lo0[2], lo1[2], hi0[2], hi1[2]};
+        const u4 b2b = {lo0[3], lo1[3], hi0[3], hi1[3]};
+        da = __builtin_amdgcn_mfma_f32_32x32x16_bf16(unit, __builtin_bit_cast(bf16x8, b2a), da, 0, 0, 0);
+        db = __builtin_amdgcn_mfma_f32_32x32x16_bf16(unit, __builtin_bit_cast(bf16x8, b2b), db, 0, 0, 0);
+    }
+    // Coefficients of the elements each product covers, in both lane halves.
+    unsigned ua, ub, va, vb, wa, wb;
+    pair_halves(__float_as_uint(a), __float_as_uint(a), ua, ub);
+    pair_halves(__float_as_uint(b), __float_as_uint(b), va, vb);
+    pair_halves(__float_as_uint(c), __float_as_uint(c), wa, wb);
+    const float aa = __uint_as_float(ua), ab = __uint_as_float(ub);
+    const float ba = __uint_as_float(va), bb = __uint_as_float(vb);
+    const float ca = __uint_as_float(wa), cb = __uint_as_float(wb);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const float ya = fmadd(aa, da[t], fmadd(ba, da[4 + t], ca * da[8 + t]));
         const float yb = fmadd(ab, db[t], fmadd(bb, db[4 + t], cb * db[8 + t]));
-        // Lanes 0..31 of A hold their own element's lower corners and lanes
-        // 32..63 of B their own element's upper corners; the other halves
-        // cross over.
-        const float w = __shfl_xor(first_half ? yb : ya, 32);
-        lower[t] = first_half ? ya : w;
-        upper[t] = first_half ? w : yb;
+        // A holds lower corners in lanes 0..31 and upper corners in 32..63
+        // (elements 0..31), B the same for elements 32..63.
+        unsigned lw, uw;
+        pair_halves(__float_as_uint(ya), __float_as_uint(yb), lw, uw);
+        lower[t] = __uint_as_float(lw);
+        upper[t] = __uint_as_float(uw);
     }
 }
 
@@ -173,7 +198,7 @@ __device__ __forceinline__ void element_row_mfma(const bf16x8 unit, const float 
 // for every owned node (``row`` its row in the strip), the conduction sum
 // K u restricted to free columns (no Robin term); ``plane_done(z)`` follows
 // every output plane, in uniform control flow.
-template <typename T, int ROWS, bool MFMA = false, class Load, class Emit, class PlaneDone>
+template <typename T, int ROWS, int MFMA = 0, class Load, class Emit, class PlaneDone>
 __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const int S, const int R, const int C,
                                             const int x0, const int y0, const int zlo, const int zhi, Load&& load,
                                             Emit&& emit, PlaneDone&& plane_done) {
@@ -204,7 +229,8 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
     NodeVal<T> vlo[ROWS];
     T Llo[ROWS], Rlo[ROWS];
     [[maybe_unused]] bf16x8 unit;
-    if constexpr (MFMA) unit = unit_operand();
+    [[maybe_unused]] unsigned qlo[ROWS + 2][4];
+    if constexpr (MFMA != 0) unit = unit_operand();
 
     auto load_plane = [&](const int z, T (&m)[ROWS + 2], T (&n)[ROWS + 2], NodeVal<T> (&v)[ROWS]) {
         const long long pb = (long long)z * NR * NC;
@@ -234,6 +260,10 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
     const int first_layer = zlo > 0 ? zlo - 1 : 0;
     const int last_layer = zhi - 1 < S - 1 ? zhi - 1 : S - 1;
     load_plane(first_layer, mlo, nlo, vlo);
+    if constexpr (MFMA != 0) {
+#pragma unroll
+        for (int i = 0; i < ROWS + 2; ++i) plane_operands<MFMA>(mlo[i], nlo[i], qlo[i]);
+    }
 #pragma unroll
     for (int i = 0; i < ROWS; ++i) {
         Llo[i] = T(0);
@@ -262,13 +292,15 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
             Rhi[i] = T(0);
         }
 
-        if constexpr (MFMA) {
-            static_assert(sizeof(T) == 4, "the matrix-core action is FP32");
+        [[maybe_unused]] unsigned qhi[ROWS + 2][4];
+        if constexpr (MFMA != 0) {
+            static_assert(sizeof(T) == 4 && (MFMA == 2 || MFMA == 3), "the matrix-core action is FP32, 2 or 3 terms");
+#pragma unroll
+            for (int i = 0; i < ROWS + 2; ++i) plane_operands<MFMA>(mhi[i], nhi[i], qhi[i]);
 #pragma unroll
             for (int j = 0; j < ROWS + 1; ++j) {
-                const float xk[8] = {mlo[j], nlo[j], mlo[j + 1], nlo[j + 1], mhi[j], nhi[j], mhi[j + 1], nhi[j + 1]};
                 float lower[4], upper[4];
-                element_row_mfma(unit, xk, ca[j], cb[j], cc[j], lower, upper);
+                element_row_mfma<MFMA>(unit, qlo[j], qlo[j + 1], qhi[j], qhi[j + 1], ca[j], cb[j], cc[j], lower, upper);
                 if (j >= 1) {
                     Llo[j - 1] += lower[0];
                     Rlo[j - 1] += lower[1];
@@ -353,6 +385,10 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
         for (int i = 0; i < ROWS + 2; ++i) {
             mlo[i] = mhi[i];
             nlo[i] = nhi[i];
+            if constexpr (MFMA != 0) {
+#pragma unroll
+                for (int w = 0; w < 4; ++w) qlo[i][w] = qhi[i][w];
+            }
         }
 #pragma unroll
         for (int i = 0; i < ROWS; ++i) {
@@ -471,7 +507,7 @@ __device__ void slot_sum(const double* __restrict__ part, const int stride, cons
 
 // y = A x (mode 0: fixed rows identity, Robin on the diagonal) or y = K x
 // (mode 1: unconstrained stiffness, every node free, no Robin).
-template <typename T, int ROWS, int MODE, bool MFMA>
+template <typename T, int ROWS, int MODE, int MFMA>
 __global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ coef, const T* __restrict__ robin,
                                                           const T* __restrict__ x, T* __restrict__ y, const int S,
                                                           const int R, const int C, const int nsx, const int nsy,
@@ -699,7 +735,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_finish_outer(const SlotDev* _
 // One slot's part of the inner apply on one tile: the direction
 // p = z + beta p_old (z = r / diag; beta = 0 on the first step) fused into
 // q = A p, and p.q per reduction cell (hex_q1.cpp:454-464).
-template <typename TI, int ROWS, bool MFMA>
+template <typename TI, int ROWS, int MFMA>
 __device__ __forceinline__ void inner_apply_slot(const SlotDev& sd, const SlotState& st, const Tile& tile) {
     const OperatorDev& op = sd.op;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -751,7 +787,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
     const Tile tile = tiles[t];
     const SlotState& st = states[tile.slot];
     if (st.phase != kPhaseInner) return;
-    inner_apply_slot<TI, ROWS, false>(slots[tile.slot], st, tile);
+    inner_apply_slot<TI, ROWS, 0>(slots[tile.slot], st, tile);
 }
 
 // The multi-right-hand-side block apply: one tile of a group of slots that
@@ -760,7 +796,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __
 // served from the caches to the rest of the group; MFMA selects the
 // matrix-core element action.  Each slot keeps its own scalars (per-column
 // alpha and beta) and its own reduction cells.
-template <int ROWS, bool MFMA>
+template <int ROWS, int MFMA>
 __global__ __launch_bounds__(kGroupThreads) void k_inner_apply_block(const SlotDev* __restrict__ slots,
                                                                      const SlotState* __restrict__ states,
                                                                      const Tile* __restrict__ tiles,
@@ -908,40 +944,43 @@ namespace launch {
 
 template <typename T>
 hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, int C, int mode, int rows, int zb,
-                 bool mfma, hipStream_t stream) {
+                 int mfma, hipStream_t stream) {
     const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
     const int nsy = (R + 1 + rows * kWavesPerGroup - 1) / (rows * kWavesPerGroup);
     const int nsz = (S + 1 + zb - 1) / zb;
     const dim3 grid(nsx * nsy * nsz), block(kGroupThreads);
 #define PCBHIP_APPLY(ROWS, MODE, MF)                                                                               \
     hipLaunchKernelGGL((k_apply<T, ROWS, MODE, MF>), grid, block, 0, stream, coef, robin, x, y, S, R, C, nsx, nsy, zb)
+#define PCBHIP_APPLY_ROWS(MF)                                                                                      \
+    if (rows == 8) {                                                                                               \
+        if (mode == 0) PCBHIP_APPLY(8, 0, MF); else PCBHIP_APPLY(8, 1, MF);                                        \
+    } else {                                                                                                       \
+        if (mode == 0) PCBHIP_APPLY(4, 0, MF); else PCBHIP_APPLY(4, 1, MF);                                        \
+    }
     if constexpr (sizeof(T) == 4) {
-        if (mfma) {
-            if (rows == 8) {
-                if (mode == 0) PCBHIP_APPLY(8, 0, true); else PCBHIP_APPLY(8, 1, true);
-            } else {
-                if (mode == 0) PCBHIP_APPLY(4, 0, true); else PCBHIP_APPLY(4, 1, true);
-            }
-        } else if (rows == 8) {
-            if (mode == 0) PCBHIP_APPLY(8, 0, false); else PCBHIP_APPLY(8, 1, false);
+        if (mfma == 3) {
+            PCBHIP_APPLY_ROWS(3)
+        } else if (mfma == 2) {
+            PCBHIP_APPLY_ROWS(2)
         } else {
-            if (mode == 0) PCBHIP_APPLY(4, 0, false); else PCBHIP_APPLY(4, 1, false);
+            PCBHIP_APPLY_ROWS(0)
         }
     } else {
         if (rows == 2) {
-            if (mode == 0) PCBHIP_APPLY(2, 0, false); else PCBHIP_APPLY(2, 1, false);
+            if (mode == 0) PCBHIP_APPLY(2, 0, 0); else PCBHIP_APPLY(2, 1, 0);
         } else {
-            if (mode == 0) PCBHIP_APPLY(4, 0, false); else PCBHIP_APPLY(4, 1, false);
+            if (mode == 0) PCBHIP_APPLY(4, 0, 0); else PCBHIP_APPLY(4, 1, 0);
         }
     }
+#undef PCBHIP_APPLY_ROWS
 #undef PCBHIP_APPLY
     return hipGetLastError();
 }
 
 template hipError_t apply<float>(const float*, const float*, const float*, float*, int, int, int, int, int, int,
-                                 bool, hipStream_t);
+                                 int, hipStream_t);
 template hipError_t apply<double>(const double*, const double*, const double*, double*, int, int, int, int, int, int,
-                                  bool, hipStream_t);
+                                  int, hipStream_t);
 
 hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S,
                     int R, int C, int* bad, hipStream_t stream) {
@@ -978,7 +1017,7 @@ hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
                       const SlotConfig* configs, const Tile* inner_tiles, const Tile* flat_tiles, const int* counts,
                       int grid_inner, int grid_flat, const Tile* block_tiles, const SlotGroup* groups,
-                      int grid_block, bool block_mfma, int* active, hipStream_t stream) {
+                      int grid_block, int block_mfma, int* active, hipStream_t stream) {
     const dim3 block(kGroupThreads);
     if (inner64) {
         hipLaunchKernelGGL((k_inner_apply<double, kRowsInner64>), dim3(grid_inner), block, 0, stream, slots, states,
@@ -996,12 +1035,15 @@ hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* 
                                inner_tiles, counts);
         }
         if (grid_block > 0) {
-            if (block_mfma) {
-                hipLaunchKernelGGL((k_inner_apply_block<4, true>), dim3(grid_block), block, 0, stream, slots, states,
+            if (block_mfma == 3) {
+                hipLaunchKernelGGL((k_inner_apply_block<4, 3>), dim3(grid_block), block, 0, stream, slots, states,
+                                   block_tiles, counts, groups);
+            } else if (block_mfma == 2) {
+                hipLaunchKernelGGL((k_inner_apply_block<4, 2>), dim3(grid_block), block, 0, stream, slots, states,
                                    block_tiles, counts, groups);
             } else {
-                hipLaunchKernelGGL((k_inner_apply_block<4, false>), dim3(grid_block), block, 0, stream, slots,
-                                   states, block_tiles, counts, groups);
+                hipLaunchKernelGGL((k_inner_apply_block<4, 0>), dim3(grid_block), block, 0, stream, slots, states,
+                                   block_tiles, counts, groups);
             }
         }
         hipLaunchKernelGGL(k_finish_apply<float>, dim3(n_slots), block, 0, stream, slots, states, active);

@@ -255,10 +255,11 @@ class Operator:
     def set_shape(self, rows32: int, zb32: int, rows64: int = 4, zb64: int = 1 << 20) -> None:
         check(self._lib.pcbhip_operator_set_shape(self.handle, rows32, zb32, rows64, zb64))
 
-    def set_engine(self, mfma: bool) -> None:
-        """FP32 element action on the matrix cores (True) or the vector units."""
+    def set_engine(self, engine: int) -> None:
+        """FP32 element action: 0 vector units, 2 or 3 matrix cores with a
+        two- or three-term bf16 split."""
 
-        check(self._lib.pcbhip_operator_set_engine(self.handle, int(bool(mfma))))
+        check(self._lib.pcbhip_operator_set_engine(self.handle, int(engine)))
 
     def apply_f32_dev(self, x_ptr: int, y_ptr: int, stream: int | None = None) -> None:
         check(self._lib.pcbhip_apply_f32_dev(self.handle, x_ptr, y_ptr, stream))
@@ -327,10 +328,11 @@ class Batch:
     def set_shape(self, rows_inner: int, target_groups: int) -> None:
         check(self._lib.pcbhip_batch_set_shape(self.handle, int(rows_inner), int(target_groups)))
 
-    def set_block(self, group_size: int, mfma: bool = True) -> None:
-        """Apply slots that share an operator in groups of ``group_size``."""
+    def set_block(self, group_size: int, engine: int = 3) -> None:
+        """Apply slots that share an operator in groups of ``group_size`` with
+        element action ``engine`` (0 vector, 2 or 3 matrix-core bf16 terms)."""
 
-        check(self._lib.pcbhip_batch_set_block(self.handle, int(group_size), int(bool(mfma))))
+        check(self._lib.pcbhip_batch_set_block(self.handle, int(group_size), int(engine)))
 
     def rhs_device_pointer(self, slot: int) -> int:
         p = _vp()

@@ -33,10 +33,10 @@ def dc():
     return board, op, rhs
 
 
-def _run(op, rhss, group, mfma, rows=4):
+def _run(op, rhss, group, engine, rows=4):
     batch = _capi.Batch([op.device_operator] * len(rhss))
     batch.set_shape(rows, 1024)
-    batch.set_block(group, mfma)
+    batch.set_block(group, engine)
     for i, b in enumerate(rhss):
         batch.set(i, b, None, CONFIG)
     batch.run()
@@ -57,10 +57,11 @@ def _dc_oracle(board, rhs):
     return A.solve_direct(a, rhs).reshape(mesh.node_shape)
 
 
-def test_block_mfma_solves_reach_the_oracle_on_the_copper_cluster(dc):
-    print("block mfma: DC excitations vs oracle", flush=True)
+@pytest.mark.parametrize("engine", [2, 3])
+def test_block_mfma_solves_reach_the_oracle_on_the_copper_cluster(dc, engine):
+    print(f"block mfma: DC excitations vs oracle, engine {engine}", flush=True)
     board, op, rhs = dc
-    results = _run(op, rhs, 4, True)
+    results = _run(op, rhs, 4, engine)
     top = board.mesh.node_shape[0] - 1
     on = np.zeros(board.mesh.node_shape[1:], dtype=bool)
     for dy in (0, 1):
@@ -79,13 +80,13 @@ def test_block_mfma_solves_reach_the_oracle_on_the_copper_cluster(dc):
 def test_block_bits_do_not_depend_on_the_group(dc):
     print("block mfma: group independence", flush=True)
     _, op, rhs = dc
-    reference = _run(op, rhs, 6, True)
+    reference = _run(op, rhs, 6, 3)
     for group, rows in ((1, 4), (2, 8), (4, 4)):
-        other = _run(op, rhs, group, True, rows)
+        other = _run(op, rhs, group, 3, rows)
         for (x0, r0), (x1, r1) in zip(reference, other):
             np.testing.assert_array_equal(x1, x0)
             assert r1 == r0
-    alone = _run(op, rhs[2:3], 1, True)
+    alone = _run(op, rhs[2:3], 1, 3)
     np.testing.assert_array_equal(alone[0][0], reference[2][0])
     assert alone[0][1] == reference[2][1]
 
@@ -93,8 +94,8 @@ def test_block_bits_do_not_depend_on_the_group(dc):
 def test_vector_block_apply_is_the_plain_slot_path_bit_for_bit(dc):
     print("block vector: equals plain slots", flush=True)
     _, op, rhs = dc
-    plain = _run(op, rhs, 0, False)
-    block = _run(op, rhs, 3, False)
+    plain = _run(op, rhs, 0, 0)
+    block = _run(op, rhs, 3, 0)
     for (x0, r0), (x1, r1) in zip(plain, block):
         np.testing.assert_array_equal(x1, x0)
         assert r1 == r0
@@ -116,7 +117,7 @@ def test_block_mfma_thermal_right_hand_sides_reach_the_direct_solve():
         rhss.append(np.where(free, base + workloads.element_heat_load(problem.mesh, heat), base))
     ops = P.operator_arrays(P.from_problem(problem))
     a = A.system_matrix(ops["coef"], ops["unit"], ops["robin_total"], ops["free"], problem.mesh.element_grid_shape)
-    for (x, res), b in zip(_run(op, rhss, 5, True), rhss):
+    for (x, res), b in zip(_run(op, rhss, 5, 2), rhss):
         exact = A.solve_direct(a, b)
         err = np.max(np.abs(x - exact)) / np.max(np.abs(exact))
         assert res["converged"] and err < 1e-9, err
