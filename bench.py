@@ -101,7 +101,7 @@ def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
     algorithmic = 13 * n + 12 * e  # x, y, robin (4 B/node each), free mask (1 B/node), coefficients (12 B/element)
     achieved = algorithmic / per / 1e9
     return dict(
-        kernel=f"k_apply<float,{rows},0> (FP32 Q1 apply), {zb} planes per z block",
+        kernel=f"k_apply<float,{rows},0,0> (FP32 Q1 apply), {zb} planes per z block",
         rows=rows,
         zb=zb,
         per_launch_us=per * 1e6,

@@ -483,23 +483,42 @@ __device__ void group_partials(const double (&v)[NV], double* __restrict__ part,
 }
 
 // Fixed-order sum of ``count`` partials of one slot by one workgroup:
-// thread t adds partials t, t + 256, ... in order, then a fixed tree.
+// thread t adds partials t, t + 256, t + 512, ... in that order (four loads
+// in flight before the adds), then a wave butterfly and the four wave sums
+// in order.  The result is for thread 0.
 template <int NV>
 __device__ void slot_sum(const double* __restrict__ part, const int stride, const int count, double (&total)[NV]) {
-    __shared__ double red[kGroupThreads];
+    __shared__ double waves[kWavesPerGroup][NV];
     const int tid = threadIdx.x;
+    double s[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] = 0.0;
+    for (int i = tid; i < count; i += 4 * kGroupThreads) {
+        double v[NV][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = i + u * kGroupThreads;
+#pragma unroll
+            for (int k = 0; k < NV; ++k) v[k][u] = idx < count ? part[k * stride + idx] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) s[k] += v[k][u];
+    }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        double s = 0.0;
-        for (int i = tid; i < count; i += kGroupThreads) s += part[k * stride + i];
-        red[tid] = s;
-        __syncthreads();
-        for (int width = kGroupThreads / 2; width > 0; width >>= 1) {
-            if (tid < width) red[tid] += red[tid + width];
-            __syncthreads();
+        const double w = wave_sum(s[k]);
+        if ((tid & (kWave - 1)) == 0) waves[tid / kWave][k] = w;
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double t = waves[0][k];
+            for (int w = 1; w < kWavesPerGroup; ++w) t += waves[w][k];
+            total[k] = t;
         }
-        total[k] = red[0];
-        __syncthreads();
     }
 }
 
