@@ -309,6 +309,71 @@ def run_multirhs64(args, torch) -> dict:
     }
 
 
+def run_apply_engines(args, torch) -> dict:
+    """The standalone FP32 Q1 apply of the 512 x 512 x 16 board with each
+    element action: FP32 vector FMAs, and the matrix cores with a two- or
+    three-term bf16 split (best tile shape of each, HIP-event timing)."""
+
+    from pcbhip import autotune, workloads
+    from thermal.matrix_free_mpir_fem import MatrixFreeThermalOperator
+
+    problem = workloads.board_512x512x16(size=args.size, slabs=args.slabs)
+    operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    dev = operator.device_operator
+    n = operator.size
+    s, r, c = operator.mesh.element_grid_shape
+    e = s * r * c
+    stream = torch.cuda.Stream()
+    x = torch.randn(n, dtype=torch.float32, device="cuda", generator=torch.Generator("cuda").manual_seed(0))
+    y = torch.empty_like(x)
+    engines = {}
+    for engine in (0, 2, 3):
+        dev.set_engine(engine)
+        best = None
+        for rows, zb in autotune.apply_candidates(operator.mesh.element_grid_shape):
+            dev.set_shape(rows, zb)
+            per = _time_launches(torch, stream,
+                                 lambda: dev.apply_f32_dev(x.data_ptr(), y.data_ptr(), stream.cuda_stream), 100)
+            if best is None or per < best[0]:
+                best = (per, rows, zb)
+        per, rows, zb = best
+        # Matrix-core work of one apply, from the launch geometry: every wave
+        # strip of every z block sweeps its element layers, and each layer
+        # takes (rows + 1) element rows x 2 products (lanes 0..31, 32..63) x
+        # (terms - 1) 32x32x16 bf16 products.
+        entry = {"us": per * 1e6, "rows": rows, "zb": zb, "gdof_s": n / per / 1e9,
+                 "algorithmic_gbs": (13 * n + 12 * e) / per / 1e9}
+        if engine:
+            nsx = -(-(c + 1) // 62)
+            waves = nsx * -(-(r + 1) // rows)
+            nsz = -(-(s + 1) // zb)
+            layers = sum(min(s - 1, (k + 1) * zb - 1) - max(0, k * zb - 1) + 1 for k in range(nsz))
+            products = waves * layers * (rows + 1) * 2 * (engine - 1)
+            flops = products * 32 * 32 * 16 * 2
+            entry["mfma_tflops"] = flops / per / 1e12
+            entry["mfma_frac_of_bf16_dense_peak"] = flops / per / 2.5e15
+        engines[str(engine)] = entry
+        log(f"engine {engine}: {entry}")
+    dev.set_engine(0)
+    best = engines["0"]
+    return {
+        "metric": "Q1 apply GDoF/s (FP32, 512x512x16 board) per element action",
+        "value": best["gdof_s"],
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": 100,
+        "warmup": 3,
+        "ms_per_step": best["us"] / 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded board generator, seed 2), x ~ N(0, 1)",
+        "config": {"workload": f"board {c}x{r}x{s} elements ({n:,} nodes)", "engines": engines,
+                   "parallelism": "single GPU"},
+    }
+
+
 def run_serve(args, torch) -> dict:
     """The serving stream: Poisson arrivals of thermal (70 %) and DC (30 %)
     requests of 1..16 right-hand sides on pool boards of the uneven-batch
@@ -336,27 +401,27 @@ def run_serve(args, torch) -> dict:
                                             arrival_s=float(t)))
         return out
 
-    # Single-solve path on the same mix: one reusable one-slot batch per board.
-    singles = {}
+    # Single-solve path on the same mix: every pool board's first and last
+    # right-hand side solved alone through a reusable one-slot batch; a
+    # served request of m right-hand sides on board b then costs m times
+    # board b's mean single-solve time on that path.
+    single_s = {}
     for b in pool:
         sb = _capi.Batch([b.operator])
         sb.set_shape(args.rows, 1024)
         sb.set(0, b.rhs[0], None, config)
         sb.run()
-        singles[id(b.operator)] = sb
-    sample = make_requests(50.0, 10.0)[: args.single_sample]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    n_single = 0
-    for req in sample:
-        sb = singles[id(req.operator)]
-        for rhs in req.rhs:
+        times = []
+        for rhs in (b.rhs[0], b.rhs[-1]):
             sb.set(0, rhs, None, None)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
             sb.run()
             sb.result(0)
-            n_single += 1
-    single_rate = n_single / (time.perf_counter() - t0)
-    log(f"single-solve path: {single_rate:.2f} solves/s over {n_single} right-hand sides")
+            times.append(time.perf_counter() - t0)
+        single_s[id(b.operator)] = float(np.mean(times))
+        sb.close()
+    log(f"single-solve path: {len(pool)} boards, mean {1e3 * np.mean(list(single_s.values())):.1f} ms per solve")
     lines = []
     for rate in args.rates:
         requests = make_requests(rate, args.duration)
@@ -368,6 +433,9 @@ def run_serve(args, torch) -> dict:
         done = server.completed
         solves = sum(len(r.results) for r in done)
         unfinished = len(requests) - len(done)
+        by_id = {r.request_id: r for r in requests}
+        single_time = sum(len(r.results) * single_s[id(by_id[r.request_id].operator)] for r in done)
+        single_rate = solves / single_time if single_time > 0 else None
         summary = serving.latency_summary(done) if unfinished == 0 else {"p50_ms": None, "p99_ms": None}
         info = server.backend.info()
         line = {
@@ -376,8 +444,12 @@ def run_serve(args, torch) -> dict:
             "unfinished": unfinished,
             "offered_solves_per_s": sum(len(r.rhs) for r in requests) / args.duration,
             "solves_per_s": solves / elapsed,
+            "single_solve_path_solves_per_s": single_rate,
+            "speedup_vs_single_solve_path": (solves / elapsed) / single_rate if single_rate else None,
             "p50_ms": summary["p50_ms"],
             "p99_ms": summary["p99_ms"],
+            "p50_ms_thermal": serving.latency_summary([r for r in done if r.kind == "thermal"])["p50_ms"],
+            "p50_ms_dc": serving.latency_summary([r for r in done if r.kind == "dc"])["p50_ms"],
             "converged": int(sum(sum(x["converged"] for x in r.results) for r in done)),
             "peak_requests_in_flight": server.peak_running_requests,
             "peak_slots": server.peak_running_slots,
@@ -405,8 +477,8 @@ def run_serve(args, torch) -> dict:
         "config": {
             "workload": f"Poisson arrivals, 70 % thermal / 30 % DC, 1..16 right-hand sides, "
                         f"{len(thermal)} + {len(dc)} pool boards, {args.duration:.0f} s per rate",
-            "single_solve_path_solves_per_s": single_rate,
-            "speedup_vs_single_solve_path": best["solves_per_s"] / single_rate,
+            "single_solve_path_solves_per_s": best["single_solve_path_solves_per_s"],
+            "speedup_vs_single_solve_path": best["speedup_vs_single_solve_path"],
             "rates": lines,
             "parallelism": "single GPU",
         },
@@ -422,13 +494,12 @@ def main() -> int:
     parser.add_argument("--slabs", type=int, default=16)
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--rows", type=int, default=4, help="rows per wave strip of the FP32 inner stencil")
-    parser.add_argument("--workload", default="board512", choices=["board512", "batch200", "multirhs64", "serve"])
+    parser.add_argument("--workload", default="board512", choices=["board512", "batch200", "multirhs64", "serve", "apply_engines"])
     parser.add_argument("--rates", type=float, nargs="+", default=[50.0, 100.0, 200.0, 400.0])
     parser.add_argument("--duration", type=float, default=10.0)
     parser.add_argument("--drain-limit", type=float, default=60.0)
     parser.add_argument("--pool-thermal", type=int, default=16)
     parser.add_argument("--pool-dc", type=int, default=8)
-    parser.add_argument("--single-sample", type=int, default=12)
     parser.add_argument("--max-slots", type=int, default=1024)
     parser.add_argument("--replay-steps", type=int, default=16)
     parser.add_argument("--max-running-nodes", type=int, default=None)
@@ -457,6 +528,9 @@ def main() -> int:
         return 0
     if args.workload == "serve":
         print(json.dumps(run_serve(args, torch)), flush=True)
+        return 0
+    if args.workload == "apply_engines":
+        print(json.dumps(run_apply_engines(args, torch)), flush=True)
         return 0
     log("building board")
     problem = workloads.board_512x512x16(size=args.size, slabs=args.slabs)

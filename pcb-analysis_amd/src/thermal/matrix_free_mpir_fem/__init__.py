@@ -1,4 +1,4 @@
-"""Matrix-free hexahedral Q1 steady heat conduction, solved by mixed-precision
+"""Matrix-free hexahedral Q1 steady and transient heat conduction, solved by mixed-precision
 iterative refinement on an MI355X (the reference's
 ``thermal.matrix_free_mpir_fem`` API; see the package modules for the
 reference files each one restates)."""
@@ -16,6 +16,7 @@ from .mesh import (
 from .operator import MatrixFreeThermalOperator
 from .problem import ThermalConductionProblem
 from .solve import ThermalConductionSolution, solve_thermal_conduction
+from .transient import TimeSchedule, TransientStep, TransientThermalSolution, solve_thermal_transient
 
 __all__ = [
     "COPPER_THERMAL_CONDUCTIVITY_W_PER_M_K",
@@ -30,7 +31,11 @@ __all__ = [
     "MatrixFreeThermalOperator",
     "ThermalConductionProblem",
     "ThermalConductionSolution",
+    "TimeSchedule",
+    "TransientStep",
+    "TransientThermalSolution",
     "exposed_element_faces",
     "solve_thermal_conduction",
+    "solve_thermal_transient",
     "unit_hexahedron_matrices",
 ]

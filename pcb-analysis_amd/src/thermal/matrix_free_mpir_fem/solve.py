@@ -122,17 +122,23 @@ def _solve_linear(
     native: bool | None,
     capacity_per_s: np.ndarray | None = None,
     previous_temperature_k: np.ndarray | None = None,
+    operator: MatrixFreeThermalOperator | None = None,
 ) -> ThermalConductionSolution:
-    operator = MatrixFreeThermalOperator(
-        problem,
-        runtime=runtime,
-        backend=backend,
-        device_id=device_id,
-        preconditioner=preconditioner,
-        coarse_block_nodes=coarse_block_nodes,
-        native=native,
-        capacity_per_s=capacity_per_s,
-    )
+    """One linear solve and its heat budget.  ``operator`` (optional) is a
+    prepared operator of ``problem`` with this ``capacity_per_s``, reused by a
+    transient march while its step repeats."""
+
+    if operator is None:
+        operator = MatrixFreeThermalOperator(
+            problem,
+            runtime=runtime,
+            backend=backend,
+            device_id=device_id,
+            preconditioner=preconditioner,
+            coarse_block_nodes=coarse_block_nodes,
+            native=native,
+            capacity_per_s=capacity_per_s,
+        )
     previous = None
     if previous_temperature_k is not None:
         previous = np.asarray(previous_temperature_k, dtype=np.float64).reshape(-1)
