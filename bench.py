@@ -72,7 +72,7 @@ def profiled_traffic(element_shape, rows: int, zb: int):
         return None
     for line in path.read_text().splitlines():
         if line.startswith(f"traffic_bytes rows={rows} zb={zb}:"):
-            return int(line.split(":")[1])
+            return int(line.split(":", 1)[1].split("#")[0].strip())
     return None
 
 
