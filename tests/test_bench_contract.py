@@ -1,0 +1,36 @@
+"""The benchmark's host-side contract on the CPU: the committed profile
+summary holds an HBM traffic figure for every tile shape the autotuner may
+pick on the 512 x 512 x 16 board, and a multi-GPU request is refused with a
+message (one GPU per run)."""
+
+from __future__ import annotations
+
+import importlib.util
+import subprocess
+import sys
+
+from pcbhip import autotune
+from tests.conftest import ROOT
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench_module", ROOT / "bench.py")
+    module = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(module)
+    return module
+
+
+def test_profile_summary_has_traffic_for_every_apply_candidate():
+    bench = _bench()
+    shape = (16, 512, 512)
+    for rows, zb in autotune.apply_candidates(shape):
+        traffic = bench.profiled_traffic(shape, rows, zb)
+        assert isinstance(traffic, int) and 100e6 < traffic < 200e6, (rows, zb, traffic)
+    assert bench.profiled_traffic(shape, 4, 3) is None
+
+
+def test_more_than_one_gpu_is_refused():
+    proc = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                          timeout=300, cwd=ROOT)
+    assert proc.returncode != 0
+    assert "one GPU" in (proc.stdout + proc.stderr)
