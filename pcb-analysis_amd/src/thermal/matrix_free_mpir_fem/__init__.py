@@ -15,6 +15,15 @@ from .mesh import (
 )
 from .operator import MatrixFreeThermalOperator
 from .problem import ThermalConductionProblem
+from .radiation import (
+    STEFAN_BOLTZMANN_W_PER_M2_K4,
+    ExposedFaceRadiation,
+    Radiation,
+    RadiationBoundary,
+    element_temperature_k,
+    face_temperature_k,
+    newton_linearisation,
+)
 from .solve import ThermalConductionSolution, solve_thermal_conduction
 from .transient import TimeSchedule, TransientStep, TransientThermalSolution, solve_thermal_transient
 
@@ -23,18 +32,25 @@ __all__ = [
     "Convection",
     "ConvectionBoundary",
     "ExposedFaceConvection",
+    "ExposedFaceRadiation",
     "FACE_DIRECTIONS",
     "FR4_IN_PLANE_THERMAL_CONDUCTIVITY_W_PER_M_K",
     "FR4_THROUGH_PLANE_THERMAL_CONDUCTIVITY_W_PER_M_K",
     "HeatSource",
     "LayeredThermalMesh",
     "MatrixFreeThermalOperator",
+    "Radiation",
+    "RadiationBoundary",
+    "STEFAN_BOLTZMANN_W_PER_M2_K4",
     "ThermalConductionProblem",
     "ThermalConductionSolution",
     "TimeSchedule",
     "TransientStep",
     "TransientThermalSolution",
+    "element_temperature_k",
     "exposed_element_faces",
+    "face_temperature_k",
+    "newton_linearisation",
     "solve_thermal_conduction",
     "solve_thermal_transient",
     "unit_hexahedron_matrices",

@@ -1,8 +1,4 @@
-"""The steady conduction problem (reference: src/thermal/matrix_free_mpir_fem/problem.py).
-
-Radiation boundaries are not part of this package yet; a problem carrying
-them is refused.
-"""
+"""The steady conduction problem (reference: src/thermal/matrix_free_mpir_fem/problem.py)."""
 
 from __future__ import annotations
 
@@ -12,12 +8,13 @@ import numpy as np
 
 from .boundaries import Convection, ConvectionBoundary, ExposedFaceConvection, HeatSource
 from .mesh import LayeredThermalMesh, flat_index
+from .radiation import ExposedFaceRadiation, RadiationBoundary
 
 
 @dataclass(frozen=True)
 class ThermalConductionProblem:
-    """Steady conduction with convective faces and fixed-temperature nodes
-    (problem.py:15-124).  ``element_heat_w`` is lumped equally onto each
+    """Steady conduction with convective and radiating faces and
+    fixed-temperature nodes (problem.py:15-124).  ``element_heat_w`` is lumped equally onto each
     element's eight corners; ``nodal_heat_w`` is per node; at least one
     positive film coefficient or one fixed node is required."""
 
@@ -38,9 +35,12 @@ class ThermalConductionProblem:
             if not isinstance(boundary, (ConvectionBoundary, ExposedFaceConvection)):
                 raise TypeError("convection entries must be ConvectionBoundary or ExposedFaceConvection")
             boundary.check_shape(self.mesh)
-        if tuple(self.radiation):
-            raise NotImplementedError("radiation boundaries are not supported on the HIP path yet")
-        cools = any(b.cools for b in convection)
+        radiation = tuple(self.radiation)
+        for boundary in radiation:
+            if not isinstance(boundary, (RadiationBoundary, ExposedFaceRadiation)):
+                raise TypeError("radiation entries must be RadiationBoundary or ExposedFaceRadiation")
+            boundary.check_shape(self.mesh)
+        cools = any(b.cools for b in convection) or any(b.radiates for b in radiation)
 
         if self.fixed_temperature_mask is None:
             if self.fixed_temperature_k is not None:
@@ -94,7 +94,7 @@ class ThermalConductionProblem:
                     raise ValueError(f"heat source {source.name!r} touches inactive node {node!r}")
 
         object.__setattr__(self, "convection", convection)
-        object.__setattr__(self, "radiation", ())
+        object.__setattr__(self, "radiation", radiation)
         object.__setattr__(self, "fixed_temperature_mask", mask.copy())
         object.__setattr__(self, "fixed_temperature_k", values.copy())
         object.__setattr__(self, "heat_sources", sources)

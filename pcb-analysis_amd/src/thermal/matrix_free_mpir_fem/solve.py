@@ -5,7 +5,8 @@ cpp/src/thermal/thermal_solve.cpp:49-163).
 The solve works on the rise above a reference temperature (the first
 convective ambient, else the mean fixed temperature), runs MPIR on the HIP
 device, re-references once to the mean rise if the first pass did not
-converge, and reports the heat budget.
+converge, and reports the heat budget; radiating faces are Newton-linearised
+around the last iterate until the field stops moving.
 """
 
 from __future__ import annotations
@@ -59,7 +60,7 @@ def solve_thermal_conduction(
 ) -> ThermalConductionSolution:
     """Solve steady heat conduction in a layered PCB with matrix-free MPIR on the HIP device."""
 
-    return _solve_linear(
+    return _solve(
         problem,
         config=config,
         runtime=runtime,
@@ -70,6 +71,77 @@ def solve_thermal_conduction(
         coarse_block_nodes=coarse_block_nodes,
         reference_temperature_k=reference_temperature_k,
         native=native,
+        radiation_max_iterations=radiation_max_iterations,
+        radiation_tolerance_k=radiation_tolerance_k,
+    )
+
+
+def _solve(
+    problem: ThermalConductionProblem,
+    *,
+    config: MPIRConfig | None = None,
+    runtime=None,
+    backend: str | None = None,
+    device_id: int = 0,
+    initial_temperature_k=None,
+    preconditioner: Preconditioner = "two-level",
+    coarse_block_nodes: int | None = None,
+    reference_temperature_k: float | None = None,
+    native: bool | None = None,
+    radiation_max_iterations: int = 25,
+    radiation_tolerance_k: float = 1.0e-4,
+    capacity_per_s: np.ndarray | None = None,
+    previous_temperature_k: np.ndarray | None = None,
+) -> ThermalConductionSolution:
+    """A steady solve, or one backward-Euler step when ``capacity_per_s`` is
+    given; radiating faces by Newton iteration on their linearisation
+    (solve.py:80-168): each iteration adds the linearised faces as convection
+    faces and solves the linear problem warm-started from the last iterate,
+    until no node moves more than ``radiation_tolerance_k``."""
+
+    linear_options = dict(
+        config=config, runtime=runtime, backend=backend, device_id=device_id, preconditioner=preconditioner,
+        coarse_block_nodes=coarse_block_nodes, reference_temperature_k=reference_temperature_k, native=native,
+        capacity_per_s=capacity_per_s, previous_temperature_k=previous_temperature_k,
+    )
+    if not problem.radiation:
+        return _solve_linear(problem, initial_temperature_k=initial_temperature_k, **linear_options)
+    if radiation_max_iterations < 1:
+        raise ValueError("radiation_max_iterations must be positive")
+    if not radiation_tolerance_k > 0.0:
+        raise ValueError("radiation_tolerance_k must be positive")
+    mesh = problem.mesh
+    fallback = problem.convection[0].mean_ambient_k() if problem.convection else problem.radiation[0].mean_ambient_k()
+    if initial_temperature_k is None:
+        current = np.full(mesh.node_shape, float(fallback))
+    else:
+        guess = np.asarray(initial_temperature_k, dtype=np.float64)
+        current = np.full(mesh.node_shape, float(guess)) if guess.ndim == 0 else guess.reshape(mesh.node_shape).copy()
+        current = np.where(np.isfinite(current), current, problem.radiation[0].mean_ambient_k())
+    n_convection = len(problem.convection)
+    solution = None
+    change = float("inf")
+    converged = False
+    iteration = 0
+    for iteration in range(1, radiation_max_iterations + 1):
+        faces = tuple(boundary.linearize(mesh, current) for boundary in problem.radiation)
+        linear = dataclasses.replace(problem, convection=tuple(problem.convection) + faces, radiation=())
+        solution = _solve_linear(linear, initial_temperature_k=current, **linear_options)
+        proposed = np.where(np.isfinite(solution.temperature_k), solution.temperature_k, current)
+        change = float(np.max(np.abs(proposed - current)))
+        current = proposed
+        if change <= radiation_tolerance_k:
+            converged = True
+            break
+    assert solution is not None
+    heat = solution.convective_heat_w
+    return dataclasses.replace(
+        solution,
+        convective_heat_w=heat[:n_convection],
+        radiative_heat_w=heat[n_convection:],
+        radiation_iterations=iteration,
+        radiation_converged=converged and solution.solve.converged,
+        radiation_change_k=change,
     )
 
 

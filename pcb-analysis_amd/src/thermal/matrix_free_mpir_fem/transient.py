@@ -5,7 +5,8 @@ cpp/src/thermal/thermal_solve.cpp:165-324).
 With the element heat capacity lumped onto the nodes (``C``), one implicit
 step solves ``(K + R + C/dt) T_{n+1} = q + R T_amb + (C/dt) T_n``: the steady
 operator with ``C/dt`` added where the Robin conductance sits, so every step
-is a steady MPIR solve on the HIP device, warm-started from ``T_n``.  The
+is a steady MPIR solve on the HIP device, warm-started from ``T_n`` (with
+radiating faces, a Newton loop of such solves).  Without radiation the
 device operator is prepared once per distinct step length and reused while
 the step repeats (a uniform schedule builds it once).
 """
@@ -21,7 +22,7 @@ from electrical.matrix_free_mpir_fem.solver import MPIRConfig
 from .mesh import Preconditioner
 from .operator import MatrixFreeThermalOperator
 from .problem import ThermalConductionProblem
-from .solve import ThermalConductionSolution, _solve_linear
+from .solve import ThermalConductionSolution, _solve, _solve_linear
 
 
 @dataclass(frozen=True)
@@ -114,13 +115,16 @@ class TransientThermalSolution:
 
 
 def _start_temperature(problem: ThermalConductionProblem, initial_temperature_k) -> np.ndarray:
-    """The initial field: the first convective ambient, else the mean fixed
-    temperature (transient.py:166-180); fixed nodes at their values."""
+    """The initial field: the first convective (else radiative) ambient, else
+    the mean fixed temperature (transient.py:166-180); fixed nodes at their
+    values."""
 
     mesh = problem.mesh
     if initial_temperature_k is None:
         if problem.convection:
             start = float(problem.convection[0].mean_ambient_k())
+        elif problem.radiation:
+            start = float(problem.radiation[0].mean_ambient_k())
         else:
             mask = problem.fixed_temperature_mask
             start = float(np.mean(np.asarray(problem.fixed_temperature_k)[mask]))
@@ -174,18 +178,22 @@ def solve_thermal_transient(
     operator = None
     for index, (time_s, step_s) in enumerate(zip(schedule.times_s[1:], schedule.steps_s), start=1):
         step_s = float(step_s)
-        if operator is None or step_s != cached_step:
-            operator = MatrixFreeThermalOperator(
-                problem, runtime=runtime, backend=backend, device_id=device_id, preconditioner=preconditioner,
-                coarse_block_nodes=coarse_block_nodes, native=native, capacity_per_s=capacity / step_s,
-            )
-            cached_step = step_s
-        final = _solve_linear(
-            problem, config=config, runtime=runtime, backend=backend, device_id=device_id,
-            initial_temperature_k=current, preconditioner=preconditioner, coarse_block_nodes=coarse_block_nodes,
-            reference_temperature_k=None, native=native, capacity_per_s=capacity / step_s,
-            previous_temperature_k=current, operator=operator,
-        )
+        options = dict(config=config, runtime=runtime, backend=backend, device_id=device_id,
+                       initial_temperature_k=current, preconditioner=preconditioner,
+                       coarse_block_nodes=coarse_block_nodes, reference_temperature_k=None, native=native,
+                       capacity_per_s=capacity / step_s, previous_temperature_k=current)
+        if problem.radiation:
+            # The Robin terms change with every Newton iterate: no reuse.
+            final = _solve(problem, radiation_max_iterations=radiation_max_iterations,
+                           radiation_tolerance_k=radiation_tolerance_k, **options)
+        else:
+            if operator is None or step_s != cached_step:
+                operator = MatrixFreeThermalOperator(
+                    problem, runtime=runtime, backend=backend, device_id=device_id, preconditioner=preconditioner,
+                    coarse_block_nodes=coarse_block_nodes, native=native, capacity_per_s=capacity / step_s,
+                )
+                cached_step = step_s
+            final = _solve_linear(problem, operator=operator, **options)
         proposed = np.where(np.isfinite(final.temperature_k), final.temperature_k, current)
         change = float(np.max(np.abs(proposed - current)[active])) if np.any(active) else 0.0
         current = proposed
@@ -203,11 +211,11 @@ def solve_thermal_transient(
             max_change_k=change,
             stored_heat_w=final.stored_heat_w,
             convective_heat_w=float(np.sum(final.convective_heat_w)),
-            radiative_heat_w=0.0,
+            radiative_heat_w=float(np.sum(final.radiative_heat_w)),
             heat_balance_error_w=final.heat_balance_error_w,
             inner_iterations=final.solve.inner_iterations,
-            radiation_iterations=0,
-            converged=final.solve.converged,
+            radiation_iterations=final.radiation_iterations,
+            converged=final.solve.converged and final.radiation_converged,
         ))
         if until_steady and change / step_s <= steady_tolerance_k_per_s:
             reached_steady = True
