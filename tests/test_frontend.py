@@ -83,3 +83,26 @@ def test_board_generators_have_the_stated_shapes_and_power():
     shapes = workloads.uneven_board_shapes()
     assert len(shapes) == 200
     assert all(4 <= s <= 16 and 32 <= r <= 384 and 32 <= c <= 384 for s, r, c in shapes)
+
+
+def test_mpir_dispatch_refuses_complex_gmres_systems_and_validates_inputs():
+    from electrical.matrix_free_mpir_fem import MPIRConfig, solve_mpir
+
+    class ComplexSystem:
+        size = 4
+        inner_solver = "gmres"
+        high_dtype = np.complex128
+
+    class NoDevice:
+        size = 4
+        inner_solver = "pcg"
+        high_dtype = np.float64
+
+    with pytest.raises(NotImplementedError, match="GMRES"):
+        solve_mpir(ComplexSystem(), np.ones(4), config=MPIRConfig())
+    with pytest.raises(TypeError, match="native_solve_mpir"):
+        solve_mpir(NoDevice(), np.ones(4))
+    with pytest.raises(ValueError, match="finite"):
+        solve_mpir(NoDevice(), np.array([1.0, np.nan, 0.0, 0.0]))
+    with pytest.raises(ValueError, match="size"):
+        solve_mpir(NoDevice(), np.ones(3))
