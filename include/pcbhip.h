@@ -81,6 +81,18 @@ int pcbhip_device_count(int *count);
  * conductance is PCBHIP_INVALID_INPUT. */
 int pcbhip_operator_create(int device, int slabs, int rows, int cols, const double *coef, const double *robin,
                            const uint8_t *free_nodes, pcbhip_operator **out);
+/* The layered-sheet DC operator of copper layers joined by resistive vias;
+ * replaces MatrixFreePCBOperator's arrays and LayeredDCSystem
+ * (src/electrical/matrix_free_mpir_fem/pcb.py:297-418, 575-624;
+ * cpp/include/pcbcore/fem/layered_dc_system.hpp:66-172).  Nodes are
+ * (layers, rows + 1, cols + 1) in flat C order; coef is (2, layers, rows,
+ * cols) = (c_x, c_y) with c_x = sigma t h_y / h_x, c_y = sigma t h_x / h_y
+ * (pcb.py:173-186); free_nodes as for the hexahedral operator; via k joins
+ * flat nodes via_a[k] and via_b[k] with conductance via_g[k] (pcb.py:40-64).
+ * The returned handle works with every apply, solve, batch and server call. */
+int pcbhip_sheet_operator_create(int device, int layers, int rows, int cols, const double *coef,
+                                 const uint8_t *free_nodes, int64_t via_count, const int64_t *via_a,
+                                 const int64_t *via_b, const double *via_g, pcbhip_operator **out);
 int pcbhip_operator_destroy(pcbhip_operator *op);
 /* Node count of the operator. */
 int64_t pcbhip_operator_size(const pcbhip_operator *op);

@@ -99,6 +99,11 @@ struct pcbhip_operator {
     float* dinv32 = nullptr;
     double* dinv64 = nullptr;
     double* diag64 = nullptr;
+    int kind = kHex;
+    int* via_ptr = nullptr;  // sheet operators: node-ordered via CSR (null without vias)
+    int* via_nbr = nullptr;
+    float* via_g32 = nullptr;
+    double* via_g64 = nullptr;
     int rows32 = 4, zb32 = 0, rows64 = 4, zb64 = 0;  // standalone apply tile shapes
     int mfma32 = 0;  // standalone FP32 element action: 0 vector, 2 or 3 bf16 terms on the matrix cores
 
@@ -114,10 +119,16 @@ struct pcbhip_operator {
         v.robin64 = robin64;
         v.dinv32 = dinv32;
         v.dinv64 = dinv64;
+        v.kind = kind;
+        v.via_ptr = via_ptr;
+        v.via_nbr = via_nbr;
+        v.via_g32 = via_g32;
+        v.via_g64 = via_g64;
         return v;
     }
     ~pcbhip_operator() {
         if (block) (void)hipFree(block);
+        if (via_ptr) (void)hipFree(via_ptr);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -210,6 +221,10 @@ hipStream_t stream_of(const pcbhip_operator* op, void* stream) {
 template <typename T>
 void apply_dev(const pcbhip_operator* op, const T* x, T* y, int mode, hipStream_t stream) {
     set_device(op->device);
+    if (op->kind == kSheet) {
+        check(launch::sheet_apply<T>(op->view(), x, y, mode, stream), "sheet apply launch");
+        return;
+    }
     const T* coef;
     const T* robin;
     if constexpr (sizeof(T) == 4) {
@@ -306,6 +321,14 @@ int add_tiles(std::vector<Tile>& out, int slot, int nsx, int nsy, int S, int zb)
     return index;
 }
 
+int add_sheet_tiles(std::vector<Tile>& out, int slot, int nsx, int nsy, int layers) {
+    int index = 0;
+    for (int l = 0; l < layers; ++l)
+        for (int sy = 0; sy < nsy; ++sy)
+            for (int sx = 0; sx < nsx; ++sx) out.push_back(Tile{slot, index++, (short)sx, (short)sy, (short)l, 1});
+    return index;
+}
+
 void upload_tiles(pcbhip_batch* b, const std::vector<Tile>& t, Tile** d, int* n) {
     if (*d) check(hipFree(*d), "hipFree");
     *n = (int)t.size();
@@ -330,8 +353,11 @@ void collect_tiles(const std::vector<const SlotHost*>& slots, const std::vector<
     for (size_t i = 0; i < slots.size(); ++i) {
         const SlotHost* s = slots[i];
         if (!s) continue;
-        xy_outer += (long long)strips_x(s->op->C) * groups_y(s->op->R, kRowsOuter);
-        if (!grouped[i]) xy_inner += (long long)strips_x(s->op->C) * groups_y(s->op->R, rows_inner);
+        // Sheet tiles are fixed (one per layer strip); they still count
+        // toward filling the device.
+        const long long layers = s->op->kind == kSheet ? s->op->S : 1;
+        xy_outer += layers * strips_x(s->op->C) * groups_y(s->op->R, kRowsOuter);
+        if (!grouped[i]) xy_inner += layers * strips_x(s->op->C) * groups_y(s->op->R, rows_inner);
     }
     for (const SlotGroup& g : groups) {
         const pcbhip_operator* op = slots[(size_t)g.slot[0]]->op;
@@ -342,6 +368,13 @@ void collect_tiles(const std::vector<const SlotHost*>& slots, const std::vector<
         if (!s) continue;
         const pcbhip_operator* op = s->op;
         const int nsx = strips_x(op->C);
+        if (op->kind == kSheet) {
+            // One layer of 4-row wave strips per tile, for both stencils.
+            add_sheet_tiles(outer, i, nsx, groups_y(op->R, kCellRows), op->S);
+            if (!grouped[(size_t)i]) add_sheet_tiles(inner, i, nsx, groups_y(op->R, kCellRows), op->S);
+            for (int k = 0; k < s->tiles_flat; ++k) flat.push_back(Tile{i, k, 0, 0, 0, 0});
+            continue;
+        }
         const int nyo = groups_y(op->R, kRowsOuter), nyi = groups_y(op->R, rows_inner);
         const long long share_o = (long long)nsx * nyo * target_groups / xy_outer;
         const int zbo = planes_per_block(op->S, (long long)nsx * nyo, std::max(share_o, (long long)nsx * nyo));
@@ -387,7 +420,7 @@ std::vector<SlotGroup> group_slots(const std::vector<const SlotHost*>& slots, in
     std::vector<const pcbhip_operator*> ops;
     std::vector<std::vector<int>> members;
     for (int i = 0; i < (int)slots.size(); ++i) {
-        if (!slots[(size_t)i]) continue;
+        if (!slots[(size_t)i] || slots[(size_t)i]->op->kind != kHex) continue;
         const pcbhip_operator* op = slots[(size_t)i]->op;
         size_t k = 0;
         while (k < ops.size() && ops[k] != op) ++k;
@@ -418,7 +451,7 @@ std::vector<SlotGroup> group_slots(const std::vector<const SlotHost*>& slots, in
 void set_partial_layout(SlotHost& s) {
     const pcbhip_operator* op = s.op;
     s.tiles_flat = (int)((op->N + kFlatChunk - 1) / kFlatChunk);
-    s.cells = cell_count(op->S, op->R, op->C);
+    s.cells = op->kind == kSheet ? sheet_cell_count(op->S, op->R, op->C) : cell_count(op->S, op->R, op->C);
     s.part_stride = std::max(s.cells, s.tiles_flat);
 }
 
@@ -886,6 +919,126 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
         check(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, op->stream), "copy");
         check(hipStreamSynchronize(op->stream), "operator create");
         if (bad) fail(PCBHIP_INVALID_INPUT, "every free node must have positive conductance");
+        *out = op.release();
+    });
+}
+
+int pcbhip_sheet_operator_create(int device, int layers, int rows, int cols, const double* coef,
+                                 const uint8_t* free_nodes, int64_t via_count, const int64_t* via_a,
+                                 const int64_t* via_b, const double* via_g, pcbhip_operator** out) {
+    return guarded([&] {
+        if (!out || !coef || !free_nodes || (via_count > 0 && (!via_a || !via_b || !via_g)))
+            fail(PCBHIP_INVALID_INPUT, "null argument");
+        *out = nullptr;
+        if (layers < 1 || rows < 1 || cols < 1) fail(PCBHIP_INVALID_INPUT, "element grid must be positive");
+        if (via_count < 0) fail(PCBHIP_INVALID_INPUT, "via count must be non-negative");
+        const long long E = (long long)layers * rows * cols;
+        const long long N = (long long)layers * (rows + 1) * (cols + 1);
+        if (N > (1LL << 31) - 1 || 2 * via_count > (1LL << 31) - 1)
+            fail(PCBHIP_INVALID_INPUT, "node or via count exceeds 2^31 - 1");
+        std::vector<double> coef64(2 * (size_t)E), robin64((size_t)N);
+        std::vector<float> coef32(2 * (size_t)E), robin32((size_t)N);
+        for (long long i = 0; i < 2 * E; ++i) {
+            if (!std::isfinite(coef[i]) || coef[i] < 0.0)
+                fail(PCBHIP_INVALID_INPUT, "element coefficients must be finite and non-negative");
+            coef64[i] = coef[i] / 6.0;
+            coef32[i] = (float)coef64[i];
+        }
+        for (long long n = 0; n < N; ++n) {
+            robin64[n] = free_nodes[n] ? 0.0 : -1.0;
+            robin32[n] = (float)robin64[n];
+        }
+        // Node-ordered via lists: every via appears at both ends, in via order.
+        std::vector<int> ptr;
+        std::vector<int> nbr;
+        std::vector<double> g64;
+        std::vector<float> g32;
+        if (via_count > 0) {
+            ptr.assign((size_t)N + 1, 0);
+            for (long long k = 0; k < via_count; ++k) {
+                if (via_a[k] < 0 || via_a[k] >= N || via_b[k] < 0 || via_b[k] >= N || via_a[k] == via_b[k])
+                    fail(PCBHIP_INVALID_INPUT, "via endpoints must be two distinct nodes of the mesh");
+                if (!std::isfinite(via_g[k]) || via_g[k] <= 0.0)
+                    fail(PCBHIP_INVALID_INPUT, "via conductance must be finite and positive");
+                ptr[(size_t)via_a[k] + 1] += 1;
+                ptr[(size_t)via_b[k] + 1] += 1;
+            }
+            for (long long n = 0; n < N; ++n) ptr[(size_t)n + 1] += ptr[(size_t)n];
+            std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+            nbr.resize((size_t)(2 * via_count));
+            g64.resize((size_t)(2 * via_count));
+            for (long long k = 0; k < via_count; ++k) {
+                const int a = (int)via_a[k], b = (int)via_b[k];
+                nbr[(size_t)fill[(size_t)a]] = b;
+                g64[(size_t)fill[(size_t)a]++] = via_g[k];
+                nbr[(size_t)fill[(size_t)b]] = a;
+                g64[(size_t)fill[(size_t)b]++] = via_g[k];
+            }
+            g32.assign(g64.begin(), g64.end());
+        }
+        set_device(device);
+        auto op = std::make_unique<pcbhip_operator>();
+        op->device = device;
+        op->kind = kSheet;
+        op->S = layers;
+        op->R = rows;
+        op->C = cols;
+        op->N = N;
+        op->E = E;
+        check(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking), "stream create");
+        const size_t s_c32 = align256(sizeof(float) * 2 * E), s_c64 = align256(sizeof(double) * 2 * E);
+        const size_t s_n32 = align256(sizeof(float) * N), s_n64 = align256(sizeof(double) * N);
+        char* base = static_cast<char*>(device_alloc(s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + 256));
+        op->block = base;
+        op->coef32 = reinterpret_cast<float*>(base);
+        base += s_c32;
+        op->coef64 = reinterpret_cast<double*>(base);
+        base += s_c64;
+        op->robin32 = reinterpret_cast<float*>(base);
+        base += s_n32;
+        op->dinv32 = reinterpret_cast<float*>(base);
+        base += s_n32;
+        op->robin64 = reinterpret_cast<double*>(base);
+        base += s_n64;
+        op->dinv64 = reinterpret_cast<double*>(base);
+        base += s_n64;
+        op->diag64 = reinterpret_cast<double*>(base);
+        base += s_n64;
+        int* d_bad = reinterpret_cast<int*>(base);
+        if (via_count > 0) {
+            const size_t s_ptr = align256(sizeof(int) * ((size_t)N + 1)), s_nbr = align256(sizeof(int) * nbr.size());
+            const size_t s_g32 = align256(sizeof(float) * g32.size()), s_g64 = align256(sizeof(double) * g64.size());
+            char* vb = static_cast<char*>(device_alloc(s_ptr + s_nbr + s_g32 + s_g64));
+            op->via_ptr = reinterpret_cast<int*>(vb);
+            op->via_nbr = reinterpret_cast<int*>(vb + s_ptr);
+            op->via_g32 = reinterpret_cast<float*>(vb + s_ptr + s_nbr);
+            op->via_g64 = reinterpret_cast<double*>(vb + s_ptr + s_nbr + s_g32);
+            check(hipMemcpyAsync(op->via_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice, op->stream),
+                  "upload");
+            check(hipMemcpyAsync(op->via_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice, op->stream),
+                  "upload");
+            check(hipMemcpyAsync(op->via_g32, g32.data(), sizeof(float) * g32.size(), hipMemcpyHostToDevice,
+                                 op->stream),
+                  "upload");
+            check(hipMemcpyAsync(op->via_g64, g64.data(), sizeof(double) * g64.size(), hipMemcpyHostToDevice,
+                                 op->stream),
+                  "upload");
+        }
+        check(hipMemcpyAsync(op->coef32, coef32.data(), sizeof(float) * 2 * E, hipMemcpyHostToDevice, op->stream),
+              "upload");
+        check(hipMemcpyAsync(op->coef64, coef64.data(), sizeof(double) * 2 * E, hipMemcpyHostToDevice, op->stream),
+              "upload");
+        check(hipMemcpyAsync(op->robin32, robin32.data(), sizeof(float) * N, hipMemcpyHostToDevice, op->stream),
+              "upload");
+        check(hipMemcpyAsync(op->robin64, robin64.data(), sizeof(double) * N, hipMemcpyHostToDevice, op->stream),
+              "upload");
+        check(hipMemsetAsync(d_bad, 0, sizeof(int), op->stream), "memset");
+        check(launch::sheet_diagonal(op->view(), op->diag64, op->dinv64, op->dinv32, d_bad, op->stream),
+              "diagonal launch");
+        int bad = 0;
+        check(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, op->stream), "copy");
+        check(hipStreamSynchronize(op->stream), "operator create");
+        if (bad) fail(PCBHIP_INVALID_INPUT, "every free node must have positive conductivity coupling");
         *out = op.release();
     });
 }

@@ -63,16 +63,35 @@ enum Pending : int { kPendingNone = 0, kPendingCorrection = 1, kPendingZero = 2 
 // action a U_x u is (a / 36) times integer-weighted sums of edge differences.
 // ``robin`` holds the lumped Robin conductance of a free node and -1 on a
 // fixed node (the free mask and the Robin term in one array).
+//
+// A second operator kind, the layered-sheet DC operator of copper layers
+// joined by vias (src/electrical/matrix_free_mpir_fem/pcb.py:575-604): S node
+// layers of (R + 1) x (C + 1) nodes, no coupling between layers but the vias,
+// coefficients (2, S, R, C) = (c_x / 6, c_y / 6) of the 2-D Q1 element
+// c_x U_x + c_y U_y, and per node a CSR list of its vias (other end, g).
+enum OperatorKind : int { kHex = 0, kSheet = 1 };
+
 struct OperatorDev {
-    int S, R, C, pad;
+    int S, R, C, kind;
     long long N;  // nodes
-    const float* coef32;    // (3, S, R, C) a/36, b/36, c/36
+    const float* coef32;    // hex: (3, S, R, C) a/36, b/36, c/36; sheet: (2, S, R, C) c_x/6, c_y/6
     const double* coef64;
     const float* robin32;   // (N)
     const double* robin64;
     const float* dinv32;    // (N) 1 / diagonal, 1 on fixed rows
     const double* dinv64;
+    const int* via_ptr;     // sheet: (N + 1) CSR offsets, or null without vias
+    const int* via_nbr;     // sheet: other end of each via entry
+    const float* via_g32;   // sheet: via conductance of each entry
+    const double* via_g64;
 };
+
+// Reduction cells of the sheet kernels: one per (layer, 4 node rows, strip).
+__host__ __device__ inline int sheet_cell_count(const int layers, const int R, const int C) {
+    const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
+    const int nrg = (R + 1 + kCellRows - 1) / kCellRows;
+    return layers * nrg * nsx;
+}
 
 // One right-hand side.  ``x64`` and ``p`` are ping-pong pairs: a stencil
 // kernel reads one buffer at its halo while neighbouring workgroups write the
@@ -135,6 +154,12 @@ hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, 
                  int mfma, hipStream_t stream);
 hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S, int R,
                     int C, int* bad, hipStream_t stream);
+// The layered-sheet operator: y = A x (mode 0) or y = K x (mode 1), and its
+// Jacobi diagonal (vias included).
+template <typename T>
+hipError_t sheet_apply(const OperatorDev& op, const T* x, T* y, int mode, hipStream_t stream);
+hipError_t sheet_diagonal(const OperatorDev& op, double* diag, double* dinv64, float* dinv32, int* bad,
+                          hipStream_t stream);
 hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream);
 // One outer step (FP64 residual and decision) for the slots in their outer
 // phase, and one inner Jacobi-PCG iteration for the slots in their inner

@@ -400,6 +400,94 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
     if (zhi == S + 1) finish_plane(S, Llo, Rlo, vlo);
 }
 
+// The layered-sheet DC action over one wave strip of one copper layer
+// (src/electrical/matrix_free_mpir_fem/pcb.py:575-604,
+// native/layered_dc_q1.cpp): 2-D Q1 elements c_x U_x + c_y U_y with
+// U_x = M(y) S(x), U_y = S(y) M(x), sum-factorised as in the hexahedral
+// stencil (x differences averaged in y with weights 2, 1 and the 1/6 folded
+// into the coefficients, and likewise for y), four node rows per wave, plus
+// every owned node's vias: sum g (w_n - w_other) over its CSR entries, in
+// their fixed order, with w the masked operand.  ``load`` is the hexahedral
+// stencil's: it is also called at flat index (0, other) for a via's far end.
+template <typename T, class Load, class Emit>
+__device__ __forceinline__ void sheet_apply(const T* __restrict__ coef, const int* __restrict__ via_ptr,
+                                            const int* __restrict__ via_nbr, const T* __restrict__ via_g,
+                                            const int layers, const int R, const int C, const int layer, const int x0,
+                                            const int y0, Load&& load, Emit&& emit) {
+    constexpr int ROWS = kCellRows;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int xc = x0 - 1 + lane;
+    const bool col_in = xc >= 0 && xc <= C;
+    const bool ecol_in = xc >= 0 && xc < C && lane < kWave - 1;
+    const bool out_lane = lane >= 1 && lane <= kWave - 2 && col_in;
+    const long long NC = C + 1;
+    const long long E = (long long)layers * R * C;
+    const long long pb = (long long)layer * (R + 1) * NC;
+    unsigned noff[ROWS + 2], eoff[ROWS + 1];
+#pragma unroll
+    for (int i = 0; i < ROWS + 2; ++i) {
+        const int y = y0 - 1 + i;
+        noff[i] = (col_in && y >= 0 && y <= R) ? (unsigned)(y * (int)NC + xc) : kOutside;
+    }
+#pragma unroll
+    for (int j = 0; j < ROWS + 1; ++j) {
+        const int ye = y0 - 1 + j;
+        eoff[j] = (ecol_in && ye >= 0 && ye < R) ? (unsigned)(ye * C + xc) : kOutside;
+    }
+    T m[ROWS + 2], n[ROWS + 2];
+    NodeVal<T> v[ROWS];
+#pragma unroll
+    for (int i = 0; i < ROWS + 2; ++i) {
+        const NodeVal<T> nv = load(pb, noff[i], false);
+        m[i] = masked(nv);
+        if (i >= 1 && i <= ROWS) v[i - 1] = nv;
+    }
+#pragma unroll
+    for (int i = 0; i < ROWS + 2; ++i) n[i] = __shfl_down(m[i], 1);
+    const T* layer_coef = coef + (long long)layer * R * C;
+    const Rsrc rx = rsrc(layer_coef), ry = rsrc(layer_coef + E);
+    T Lacc[ROWS], Racc[ROWS];
+#pragma unroll
+    for (int i = 0; i < ROWS; ++i) {
+        Lacc[i] = T(0);
+        Racc[i] = T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < ROWS + 1; ++j) {
+        const T cx = ld(rx, eoff[j], T()), cy = ld(ry, eoff[j], T());
+        const T dx0 = n[j] - m[j], dx1 = n[j + 1] - m[j + 1];
+        const T dy0 = m[j + 1] - m[j], dy1 = n[j + 1] - n[j];
+        const T tx0 = fmadd(T(2), dx0, dx1), tx1 = fmadd(T(2), dx1, dx0);
+        const T ty0 = fmadd(T(2), dy0, dy1), ty1 = fmadd(T(2), dy1, dy0);
+        // Corner (y, x) receives s(x) c_x tx[y] + s(y) c_y ty[x], s(0) = -1,
+        // s(1) = +1; node row j is output row j - 1.
+        if (j >= 1) {
+            Lacc[j - 1] = fmadd(-cx, tx0, fmadd(-cy, ty0, Lacc[j - 1]));
+            Racc[j - 1] = fmadd(cx, tx0, fmadd(-cy, ty1, Racc[j - 1]));
+        }
+        if (j < ROWS) {
+            Lacc[j] = fmadd(-cx, tx1, fmadd(cy, ty0, Lacc[j]));
+            Racc[j] = fmadd(cx, tx1, fmadd(cy, ty1, Racc[j]));
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < ROWS; ++i) {
+        T total = Lacc[i] + __shfl_up(Racc[i], 1);
+        if (out_lane && y0 + i <= R) {
+            if (via_ptr) {
+                const long long node = pb + noff[i + 1];
+                const int lo = via_ptr[node], hi = via_ptr[node + 1];
+                const T own = masked(v[i]);
+                for (int k = lo; k < hi; ++k) {
+                    const NodeVal<T> far = load(0LL, (unsigned)via_nbr[k], false);
+                    total = fmadd(via_g[k], own - masked(far), total);
+                }
+            }
+            emit(pb, noff[i + 1], i, total, v[i]);
+        }
+    }
+}
+
 // Workgroups are dealt to the eight XCDs round-robin; give each XCD a
 // contiguous range of tiles so neighbouring strips (which share halo rows)
 // meet in the same L2.
@@ -453,6 +541,19 @@ struct CellSums {
                 const double s = wave_sum(v[k][g]);
                 if (rg < nrg && (threadIdx.x & (kWave - 1)) == 0) part[k * stride + cell] = s;
             }
+        }
+        clear();
+    }
+    // Sheet kernels: the wave's one row cell of (layer, 4 rows, strip).
+    __device__ void flush_sheet(const int layer, const int R, const int C, const int x0, const int y0,
+                                double* __restrict__ part, const int stride) {
+        const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
+        const int nrg = (R + 1 + kCellRows - 1) / kCellRows;
+        const int cell = (layer * nrg + y0 / kCellRows) * nsx + x0 / kStripCols;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const double sum = wave_sum(v[k][0]);
+            if ((threadIdx.x & (kWave - 1)) == 0) part[k * stride + cell] = sum;
         }
         clear();
     }
@@ -553,6 +654,67 @@ __global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ c
         [](const int) {});
 }
 
+// Standalone sheet apply (mode 0: system with fixed rows identity; mode 1:
+// unconstrained K, every node free).  Tile = (strip, 16-row group, layer).
+template <typename T, int MODE>
+__global__ __launch_bounds__(kGroupThreads) void k_sheet_apply(const OperatorDev op, const T* __restrict__ coef,
+                                                               const T* __restrict__ robin,
+                                                               const T* __restrict__ via_g,
+                                                               const T* __restrict__ x, T* __restrict__ y,
+                                                               const int nsx, const int nsy) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int sx = t % nsx;
+    const int sy = (t / nsx) % nsy;
+    const int layer = t / (nsx * nsy);
+    const int x0 = sx * kStripCols;
+    const int y0 = (sy * kWavesPerGroup + wave) * kCellRows;
+    if (y0 > op.R) return;
+    sheet_apply<T>(
+        coef, op.via_ptr, op.via_nbr, via_g, op.S, op.R, op.C, layer, x0, y0,
+        [&](const long long pb, const unsigned off, const bool) -> NodeVal<T> {
+            if (MODE == 1) return {ldt(x + pb, off), T(0)};
+            return {ldt(x + pb, off), ldt(robin + pb, off)};
+        },
+        [&](const long long pb, const unsigned off, const int, const T conduction, const NodeVal<T>& v) {
+            stt(y + pb, off, v.robin >= T(0) ? conduction : v.u);
+        });
+}
+
+// Jacobi diagonal of the sheet operator: 2 (c_x + c_y) / 6 per adjacent
+// element corner (U_x[i][i] = U_y[i][i] = 1 / 3) plus the conductance of
+// every via at the node; 1 on fixed rows (pcb.py:607-624).
+__global__ __launch_bounds__(256) void k_sheet_diagonal(const OperatorDev op, double* __restrict__ diag,
+                                                        double* __restrict__ dinv64, float* __restrict__ dinv32,
+                                                        int* bad) {
+    const long long NC = op.C + 1, NR = op.R + 1;
+    const long long E = (long long)op.S * op.R * op.C;
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= op.N) return;
+    const int x = (int)(n % NC);
+    const int y = (int)((n / NC) % NR);
+    const int l = (int)(n / (NC * NR));
+    double d = 0.0;
+    for (int ey = y - 1; ey <= y; ++ey) {
+        if (ey < 0 || ey >= op.R) continue;
+        for (int ex = x - 1; ex <= x; ++ex) {
+            if (ex < 0 || ex >= op.C) continue;
+            const long long e = ((long long)l * op.R + ey) * op.C + ex;
+            d += 2.0 * (op.coef64[e] + op.coef64[E + e]);
+        }
+    }
+    if (op.via_ptr)
+        for (int k = op.via_ptr[n]; k < op.via_ptr[n + 1]; ++k) d += op.via_g64[k];
+    double value = 1.0;
+    if (op.robin64[n] >= 0.0) {
+        value = d;
+        if (!(value > 0.0)) atomicAdd(bad, 1);
+    }
+    diag[n] = value;
+    dinv64[n] = 1.0 / value;
+    dinv32[n] = (float)(1.0 / value);
+}
+
 // Jacobi diagonal: sum of the element diagonals ((a + b + c) / 9 for every
 // corner of the trilinear unit tensors) over the adjacent elements plus the
 // Robin conductance; 1 on fixed rows (operator.py:289-306).  ``bad`` counts
@@ -620,12 +782,14 @@ struct InnerOf<float> {
     __device__ static const float* dinv(const OperatorDev& op) { return op.dinv32; }
     __device__ static const float* coef(const OperatorDev& op) { return op.coef32; }
     __device__ static const float* robin(const OperatorDev& op) { return op.robin32; }
+    __device__ static const float* via_g(const OperatorDev& op) { return op.via_g32; }
 };
 template <>
 struct InnerOf<double> {
     __device__ static const double* dinv(const OperatorDev& op) { return op.dinv64; }
     __device__ static const double* coef(const OperatorDev& op) { return op.coef64; }
     __device__ static const double* robin(const OperatorDev& op) { return op.robin64; }
+    __device__ static const double* via_g(const OperatorDev& op) { return op.via_g64; }
 };
 
 // Outer step (hex_q1.cpp:870-911 and solver.py:638-686): apply a pending
@@ -666,28 +830,36 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
         // b.b, r.r (FP64), r.r and r.z (inner precision) per reduction cell.
         CellSums<4, kRowsOuter> acc;
         acc.clear();
+        auto load = [&](const long long pb, const unsigned off, const bool) -> NodeVal<double> {
+            const double xv = ldt(xin + pb, off) + cw * (double)ldt(cin + pb, off);
+            return {xv, ldt(robin + pb, off)};
+        };
+        auto emit = [&](const long long pb, const unsigned off, const int row, const double conduction,
+                        const NodeVal<double>& v) {
+            if (pending != kPendingNone) stt(xout + pb, off, v.u);
+            const double ax = v.robin >= 0.0 ? conduction + v.robin * v.u : v.u;
+            const double bv = ldt(b + pb, off);
+            const double r = bv - ax;
+            const TI ri = (TI)r;
+            stt(rout + pb, off, ri);
+            const int g = row / kCellRows;
+            acc.v[0][g] += bv * bv;
+            acc.v[1][g] += r * r;
+            acc.v[2][g] += (double)ri * (double)ri;
+            acc.v[3][g] += (double)ri * (double)(ri * ldt(dinv + pb, off));
+        };
+        if (op.kind == kSheet) {
+            if (y0 <= op.R) {
+                sheet_apply<double>(op.coef64, op.via_ptr, op.via_nbr, op.via_g64, op.S, op.R, op.C, tile.sz, x0, y0,
+                                    load, emit);
+                acc.flush_sheet(tile.sz, op.R, op.C, x0, y0, sd.part, sd.part_stride);
+            }
+            continue;
+        }
         if (y0 <= op.R) {
+            // Stores wait for their plane's emit, so a plane's loads issue back to back.
             strip_apply<double, kRowsOuter>(
-                op.coef64, op.S, op.R, op.C, x0, y0, zlo, zhi,
-                [&](const long long pb, const unsigned off, const bool) -> NodeVal<double> {
-                    const double xv = ldt(xin + pb, off) + cw * (double)ldt(cin + pb, off);
-                    return {xv, ldt(robin + pb, off)};
-                },
-                [&](const long long pb, const unsigned off, const int row, const double conduction,
-                    const NodeVal<double>& v) {
-                    // Stores wait for their plane's emit, so a plane's loads issue back to back.
-                    if (pending != kPendingNone) stt(xout + pb, off, v.u);
-                    const double ax = v.robin >= 0.0 ? conduction + v.robin * v.u : v.u;
-                    const double bv = ldt(b + pb, off);
-                    const double r = bv - ax;
-                    const TI ri = (TI)r;
-                    stt(rout + pb, off, ri);
-                    const int g = row / kCellRows;
-                    acc.v[0][g] += bv * bv;
-                    acc.v[1][g] += r * r;
-                    acc.v[2][g] += (double)ri * (double)ri;
-                    acc.v[3][g] += (double)ri * (double)(ri * ldt(dinv + pb, off));
-                },
+                op.coef64, op.S, op.R, op.C, x0, y0, zlo, zhi, load, emit,
                 [&](const int z) { acc.plane_done(z, op.S, op.R, op.C, x0, y0, sd.part, sd.part_stride); });
         }
     }
@@ -774,21 +946,31 @@ __device__ __forceinline__ void inner_apply_slot(const SlotDev& sd, const SlotSt
     const TI* __restrict__ robin = InnerOf<TI>::robin(op);
     CellSums<1, ROWS> acc;  // p.q per reduction cell
     acc.clear();
+    auto load = [&](const long long pb, const unsigned off, const bool) -> NodeVal<TI> {
+        const TI z = ldt(r + pb, off) * ldt(dinv + pb, off);
+        const TI pv = z + beta * ldt(pin + pb, off);
+        return {pv, ldt(robin + pb, off)};
+    };
+    // p is stored with q at the plane's emit, so a plane's loads issue back to back.
+    auto emit = [&](const long long pb, const unsigned off, const int row, const TI conduction, const NodeVal<TI>& v) {
+        stt(pout + pb, off, v.u);
+        const TI ap = v.robin >= TI(0) ? conduction + v.robin * v.u : v.u;
+        stt(q + pb, off, ap);
+        acc.v[0][row / kCellRows] += (double)v.u * (double)ap;
+    };
+    if (op.kind == kSheet) {
+        // Sheet tiles are one layer of 4-row wave strips.
+        const int ys = (tile.sy * kWavesPerGroup + wave) * kCellRows;
+        if (ys <= op.R) {
+            sheet_apply<TI>(InnerOf<TI>::coef(op), op.via_ptr, op.via_nbr, InnerOf<TI>::via_g(op), op.S, op.R, op.C,
+                            tile.sz, x0, ys, load, emit);
+            acc.flush_sheet(tile.sz, op.R, op.C, x0, ys, sd.part, sd.part_stride);
+        }
+        return;
+    }
     if (y0 <= op.R) {
         strip_apply<TI, ROWS, MFMA>(
-            InnerOf<TI>::coef(op), op.S, op.R, op.C, x0, y0, zlo, zhi,
-            [&](const long long pb, const unsigned off, const bool) -> NodeVal<TI> {
-                const TI z = ldt(r + pb, off) * ldt(dinv + pb, off);
-                const TI pv = z + beta * ldt(pin + pb, off);
-                return {pv, ldt(robin + pb, off)};
-            },
-            [&](const long long pb, const unsigned off, const int row, const TI conduction, const NodeVal<TI>& v) {
-                // p is stored with q at the plane's emit, so a plane's loads issue back to back.
-                stt(pout + pb, off, v.u);
-                const TI ap = v.robin >= TI(0) ? conduction + v.robin * v.u : v.u;
-                stt(q + pb, off, ap);
-                acc.v[0][row / kCellRows] += (double)v.u * (double)ap;
-            },
+            InnerOf<TI>::coef(op), op.S, op.R, op.C, x0, y0, zlo, zhi, load, emit,
             [&](const int z) { acc.plane_done(z, op.S, op.R, op.C, x0, y0, sd.part, sd.part_stride); });
     }
 }
@@ -1000,6 +1182,40 @@ template hipError_t apply<float>(const float*, const float*, const float*, float
                                  int, hipStream_t);
 template hipError_t apply<double>(const double*, const double*, const double*, double*, int, int, int, int, int, int,
                                   int, hipStream_t);
+
+template <typename T>
+hipError_t sheet_apply(const OperatorDev& op, const T* x, T* y, int mode, hipStream_t stream) {
+    const int nsx = (op.C + 1 + kStripCols - 1) / kStripCols;
+    const int nsy = (op.R + 1 + kCellRows * kWavesPerGroup - 1) / (kCellRows * kWavesPerGroup);
+    const dim3 grid(nsx * nsy * op.S), block(kGroupThreads);
+    const T* coef;
+    const T* robin;
+    const T* via_g;
+    if constexpr (sizeof(T) == 4) {
+        coef = op.coef32;
+        robin = op.robin32;
+        via_g = op.via_g32;
+    } else {
+        coef = op.coef64;
+        robin = op.robin64;
+        via_g = op.via_g64;
+    }
+    if (mode == 0)
+        hipLaunchKernelGGL((k_sheet_apply<T, 0>), grid, block, 0, stream, op, coef, robin, via_g, x, y, nsx, nsy);
+    else
+        hipLaunchKernelGGL((k_sheet_apply<T, 1>), grid, block, 0, stream, op, coef, robin, via_g, x, y, nsx, nsy);
+    return hipGetLastError();
+}
+
+template hipError_t sheet_apply<float>(const OperatorDev&, const float*, float*, int, hipStream_t);
+template hipError_t sheet_apply<double>(const OperatorDev&, const double*, double*, int, hipStream_t);
+
+hipError_t sheet_diagonal(const OperatorDev& op, double* diag, double* dinv64, float* dinv32, int* bad,
+                          hipStream_t stream) {
+    const int blocks = (int)((op.N + 255) / 256);
+    hipLaunchKernelGGL(k_sheet_diagonal, dim3(blocks), dim3(256), 0, stream, op, diag, dinv64, dinv32, bad);
+    return hipGetLastError();
+}
 
 hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S,
                     int R, int C, int* bad, hipStream_t stream) {

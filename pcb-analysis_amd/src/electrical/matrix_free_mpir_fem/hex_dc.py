@@ -28,48 +28,11 @@ import numpy as np
 from pcbhip import _capi
 from thermal.matrix_free_mpir_fem.mesh import LayeredThermalMesh, corner_views, flat_index, unit_hexahedron_matrices
 
+from .pcb import CurrentTerminal, VoltageTerminal
 from .runtime import make_float32_runtime
 from .solver import MPIRConfig, MPIRResult, MPIRStep, solve_mpir
 
 Node = tuple[int, int, int]
-
-
-@dataclass(frozen=True)
-class CurrentTerminal:
-    """A total current distributed uniformly over a set of nodes (pcb.py:66-82)."""
-
-    nodes: tuple[Node, ...]
-    current_a: float
-    name: str = "terminal"
-
-    def __post_init__(self) -> None:
-        nodes = tuple(tuple(int(i) for i in node) for node in self.nodes)
-        if not nodes:
-            raise ValueError("a current terminal needs at least one node")
-        if len(set(nodes)) != len(nodes):
-            raise ValueError("terminal nodes must be unique")
-        if not np.isfinite(self.current_a):
-            raise ValueError("terminal current must be finite")
-        object.__setattr__(self, "nodes", nodes)
-
-
-@dataclass(frozen=True)
-class VoltageTerminal:
-    """A set of nodes held at one potential (pcb.py:86-102)."""
-
-    nodes: tuple[Node, ...]
-    voltage_v: float
-    name: str = "terminal"
-
-    def __post_init__(self) -> None:
-        nodes = tuple(tuple(int(i) for i in node) for node in self.nodes)
-        if not nodes:
-            raise ValueError("a voltage terminal needs at least one node")
-        if len(set(nodes)) != len(nodes):
-            raise ValueError("terminal nodes must be unique")
-        if not np.isfinite(self.voltage_v):
-            raise ValueError("terminal voltage must be finite")
-        object.__setattr__(self, "nodes", nodes)
 
 
 def _check_terminals(terminals, voltage_terminals, reference_node) -> None:

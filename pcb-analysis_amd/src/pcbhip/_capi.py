@@ -35,6 +35,7 @@ EXPORTS = (
     "pcbhip_last_error",
     "pcbhip_device_count",
     "pcbhip_operator_create",
+    "pcbhip_sheet_operator_create",
     "pcbhip_operator_destroy",
     "pcbhip_operator_size",
     "pcbhip_operator_diagonal",
@@ -125,6 +126,8 @@ def load() -> ctypes.CDLL:
     lib.pcbhip_last_error.restype = ctypes.c_char_p
     lib.pcbhip_device_count.argtypes = [ctypes.POINTER(_int)]
     lib.pcbhip_operator_create.argtypes = [_int, _int, _int, _int, _vp, _vp, _vp, ctypes.POINTER(_vp)]
+    lib.pcbhip_sheet_operator_create.argtypes = [_int, _int, _int, _int, _vp, _vp, ctypes.c_int64, _vp, _vp, _vp,
+                                                 ctypes.POINTER(_vp)]
     lib.pcbhip_operator_destroy.argtypes = [_vp]
     lib.pcbhip_operator_size.argtypes = [_vp]
     lib.pcbhip_operator_size.restype = ctypes.c_int64
@@ -211,6 +214,34 @@ class Operator:
         h = _vp()
         check(lib.pcbhip_operator_create(self.device, s, r, c, _ptr(coef), _ptr(robin), _ptr(free), ctypes.byref(h)))
         self.handle = h
+
+    @classmethod
+    def sheet(cls, device: int, element_shape, coef, free, via_a=(), via_b=(), via_g=()) -> "Operator":
+        """The layered-sheet DC operator (``pcbhip_sheet_operator_create``):
+        ``element_shape`` = (layers, rows, cols), ``coef`` = (2, layers, rows,
+        cols) sheet coefficients, vias as flat node index pairs."""
+
+        layers, r, c = (int(v) for v in element_shape)
+        n = layers * (r + 1) * (c + 1)
+        coef = np.ascontiguousarray(coef, dtype=np.float64).reshape(-1)
+        free = np.ascontiguousarray(free, dtype=np.uint8).reshape(-1)
+        a = np.ascontiguousarray(via_a, dtype=np.int64).reshape(-1)
+        b = np.ascontiguousarray(via_b, dtype=np.int64).reshape(-1)
+        g = np.ascontiguousarray(via_g, dtype=np.float64).reshape(-1)
+        if coef.size != 2 * layers * r * c or free.size != n or not (a.size == b.size == g.size):
+            raise ValueError("sheet operator arrays have the wrong size")
+        self = cls.__new__(cls)
+        self._lib = load()
+        self.handle = None
+        self.device = int(device)
+        self.element_shape = (layers, r, c)
+        self.size = n
+        h = _vp()
+        check(self._lib.pcbhip_sheet_operator_create(self.device, layers, r, c, _ptr(coef), _ptr(free), int(a.size),
+                                                     _ptr(a) if a.size else None, _ptr(b) if b.size else None,
+                                                     _ptr(g) if g.size else None, ctypes.byref(h)))
+        self.handle = h
+        return self
 
     def close(self) -> None:
         if self.handle is not None:
