@@ -117,3 +117,43 @@ def element_view_apply(coef, element_shape, x, free, via_a=(), via_b=(), via_g=(
         np.add.at(flat, a, delta)
         np.add.at(flat, b, -delta)
     return np.where(free.reshape(-1), flat, np.asarray(x, dtype=float).reshape(-1))
+
+
+def port_conductance(k: sp.csr_matrix, active: np.ndarray, pads) -> tuple[np.ndarray, np.ndarray]:
+    """Conductance matrix of the N-port whose ports are the node sets ``pads``
+    (flat indices), by one sparse LU of the free block (the Schur complement
+    ``K_pp - K_pf K_ff^-1 K_fp`` summed over each pad), and the potential for
+    each port at 1 V with the others at 0 V.  Follows the definition of
+    src/electrical/matrix_free_mpir_fem/ports.py:206-215 (G[j, k] is the
+    current into port j for port k at 1 V); free nodes are the active nodes
+    outside every pad."""
+
+    n = k.shape[0]
+    on_pad = np.zeros(n, dtype=bool)
+    for pad in pads:
+        on_pad[np.asarray(pad, dtype=np.int64)] = True
+    free = np.asarray(active, dtype=bool).reshape(-1) & ~on_pad
+    fi = np.flatnonzero(free)
+    lu = spla.splu(k[fi][:, fi].tocsc())
+    count = len(pads)
+    g = np.zeros((count, count))
+    fields = np.zeros((count, n))
+    for col, pad in enumerate(pads):
+        x = np.zeros(n)
+        x[np.asarray(pad, dtype=np.int64)] = 1.0
+        x[fi] = lu.solve(-(k[fi] @ x))
+        injected = k @ x
+        fields[col] = x
+        for row, other in enumerate(pads):
+            g[row, col] = injected[np.asarray(other, dtype=np.int64)].sum()
+    return g, fields
+
+
+def element_loss(coef, element_shape, x) -> np.ndarray:
+    """Per-element ``c_x v^T U_x v + c_y v^T U_y v`` of the field ``x``."""
+
+    v = np.asarray(x, dtype=float).reshape(-1)[element_nodes(element_shape)]
+    unit = unit_matrices()
+    c = coef.reshape(2, -1)
+    loss = c[0] * np.einsum("ei,ij,ej->e", v, unit[0], v) + c[1] * np.einsum("ei,ij,ej->e", v, unit[1], v)
+    return loss.reshape(element_shape)

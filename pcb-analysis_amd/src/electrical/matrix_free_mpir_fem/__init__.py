@@ -1,6 +1,6 @@
 """Matrix-free MPIR solver contract on the HIP device and the layered-PCB DC
 front end (the reference's ``electrical.matrix_free_mpir_fem`` solver,
-config and result types and ``pcb`` module)."""
+config and result types, ``pcb`` and ``ports`` modules)."""
 
 from .pcb import (
     COPPER_CONDUCTIVITY_S_PER_M,
@@ -13,25 +13,30 @@ from .pcb import (
     VoltageTerminal,
     solve_pcb_dc,
 )
+from .ports import DCPortBasis, PortSet, dc_port_basis, reduce_correlation
 from .runtime import HipFloat32Runtime, RuntimeBackend, make_float32_runtime
 from .solver import MPIRConfig, MPIRResult, MPIRStep, mpir_result_from_core, solve_mpir, solve_mpir_device
 
 __all__ = [
     "COPPER_CONDUCTIVITY_S_PER_M",
     "CurrentTerminal",
+    "DCPortBasis",
     "HipFloat32Runtime",
     "LayeredPCBMesh",
     "MatrixFreePCBOperator",
     "PCBConductionProblem",
     "PCBConductionSolution",
+    "PortSet",
     "ViaConnection",
     "VoltageTerminal",
     "MPIRConfig",
     "MPIRResult",
     "MPIRStep",
     "RuntimeBackend",
+    "dc_port_basis",
     "make_float32_runtime",
     "mpir_result_from_core",
+    "reduce_correlation",
     "solve_mpir",
     "solve_mpir_device",
     "solve_pcb_dc",

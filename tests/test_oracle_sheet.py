@@ -106,3 +106,60 @@ def test_connected_sheets_with_a_gauge_node_have_an_spd_system():
     assert np.min(np.linalg.eigvalsh(a)) > 0.0
     # Without the gauge the free block is singular (a floating conductor).
     assert abs(np.min(np.linalg.eigvalsh(k.toarray()))) < 1e-9 * SIGMA * T
+
+
+def _in_line_ports(l1=4, l2=7, rows=3):
+    """One strip cut by three full-width pads at x = 0, l1, l1 + l2."""
+
+    active = np.ones((1, rows, l1 + l2), dtype=bool)
+    shape = (1, rows + 1, l1 + l2 + 1)
+    pads = [[np.ravel_multi_index((0, y, x), shape) for y in range(rows + 1)] for x in (0, l1, l1 + l2)]
+    return active, shape, pads
+
+
+def test_port_conductance_of_pads_in_line_is_the_series_chain():
+    # A full-width middle pad is an equipotential cut: ports 0 and 2 couple
+    # only through it, so G is the tridiagonal matrix of two strip conductances.
+    l1, l2, rows = 4, 7, 3
+    active, shape, pads = _in_line_ports(l1, l2, rows)
+    coef = O.coefficients(active, [T], 1e-3, 2e-3, SIGMA)
+    k = O.stiffness(coef, active.shape)
+    g, fields = O.port_conductance(k, O.active_nodes(active), pads)
+    g1 = SIGMA * T * rows * 2e-3 / (l1 * 1e-3)
+    g2 = SIGMA * T * rows * 2e-3 / (l2 * 1e-3)
+    expected = np.array([[g1, -g1, 0.0], [-g1, g1 + g2, -g2], [0.0, -g2, g2]])
+    np.testing.assert_allclose(g, expected, rtol=1e-11, atol=1e-11 * g1)
+    # Port 1 at 1 V: a tent, linear down to both outer pads.
+    tent = fields[1].reshape(shape)[0, 0]
+    np.testing.assert_allclose(tent[: l1 + 1], np.arange(l1 + 1) / l1, atol=1e-12)
+    np.testing.assert_allclose(tent[l1:], 1.0 - np.arange(l2 + 1) / l2, atol=1e-12)
+
+
+def test_port_conductance_obeys_reciprocity_kcl_and_the_energy_identity():
+    rng = np.random.default_rng(3)
+    active = rng.random((2, 6, 9)) < 0.9
+    active[:, :, 0] = True
+    active[:, :, -1] = True
+    shape = (2, 7, 10)
+    coef = O.coefficients(active, [35e-6, 18e-6], rng.uniform(0.4e-3, 0.6e-3, 9), rng.uniform(0.4e-3, 0.6e-3, 6),
+                          5.8e7)
+    via_a = [np.ravel_multi_index((0, y, 5), shape) for y in range(7)]
+    via_b = [np.ravel_multi_index((1, y, 5), shape) for y in range(7)]
+    via_g = rng.uniform(500.0, 1500.0, 7)
+    k = O.stiffness(coef, active.shape, via_a, via_b, via_g)
+    act = O.active_nodes(active, via_a, via_b)
+    pads = [[np.ravel_multi_index((0, y, 0), shape) for y in (0, 1, 2)],
+            [np.ravel_multi_index((0, y, 0), shape) for y in (4, 5, 6)],
+            [np.ravel_multi_index((1, y, 9), shape) for y in range(7)]]
+    g, fields = O.port_conductance(k, act, pads)
+    scale = np.abs(g).max()
+    np.testing.assert_allclose(g, g.T, atol=1e-10 * scale)
+    np.testing.assert_allclose(g.sum(axis=0), 0.0, atol=1e-10 * scale)
+    np.testing.assert_allclose(g.sum(axis=1), 0.0, atol=1e-10 * scale)
+    assert np.all(np.linalg.eigvalsh(g[:2, :2]) > 0.0)
+    # V^T G V is the power the pads drive in, which the elements and vias dissipate.
+    v = np.array([2.0e-3, -1.0e-3, 0.5e-3])
+    field = v @ fields
+    element = O.element_loss(coef, active.shape, field).sum()
+    drop = field[via_a] - field[via_b]
+    assert v @ g @ v == pytest.approx(element + np.sum(via_g * drop * drop), rel=1e-10)
