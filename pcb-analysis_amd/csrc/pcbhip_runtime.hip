@@ -76,7 +76,8 @@ int groups_y(int R, int rows) { return (R + 1 + rows * kWavesPerGroup - 1) / (ro
 // (each block re-reads one halo plane on each side, so never fewer than 2
 // planes).
 int planes_per_block(int S, long long groups_xy, long long target) {
-    const int planes = S + 1;
+    // A tile holds at most kMaxTilePairs plane pairs of reduction cells.
+    const int planes = std::min(S + 1, 2 * kMaxTilePairs);
     if (groups_xy >= target) return planes;
     const long long split = (target + groups_xy - 1) / groups_xy;
     int zb = (int)((planes + split - 1) / split);
@@ -141,15 +142,15 @@ struct SlotHost {
     double* b = nullptr;
     double* x[2] = {nullptr, nullptr};
     double* x0 = nullptr;
-    void* r = nullptr;
+    void* r[2] = {nullptr, nullptr};
     void* p[2] = {nullptr, nullptr};
-    void* q = nullptr;
+    void* q[2] = {nullptr, nullptr};
     void* c = nullptr;
     double* part = nullptr;
     double* history = nullptr;
     int history_rows = 0;
     pcbhip_mpir_config config{1e-10, 0.0, 2e-3, 8, 200};
-    int tiles_flat = 0, cells = 0, part_stride = 0;
+    int cells = 0, part_stride = 0;
 };
 
 }  // namespace
@@ -164,11 +165,10 @@ struct pcbhip_batch {
     SlotConfig* d_configs = nullptr;
     Tile* d_outer = nullptr;
     Tile* d_inner = nullptr;
-    Tile* d_flat = nullptr;
     Tile* d_block = nullptr;
     SlotGroup* d_groups = nullptr;
-    int* d_counts = nullptr;  // live table lengths: outer, inner, pointwise, block
-    int n_outer = 0, n_inner = 0, n_flat = 0, n_block = 0;
+    int* d_counts = nullptr;  // live table lengths: outer, inner, (unused), block
+    int n_outer = 0, n_inner = 0, n_block = 0;
     int block_group = 0;  // right-hand sides per block-apply group (0: no block apply)
     int block_mfma = 3;  // element action of the block apply (0, 2, 3)
     int rows_inner = 4;
@@ -191,7 +191,7 @@ struct pcbhip_batch {
             if (s.history) (void)hipFree(s.history);
         }
         for (void* p : {(void*)d_slots, (void*)d_states, (void*)d_configs, (void*)d_outer, (void*)d_inner,
-                        (void*)d_flat, (void*)d_block, (void*)d_groups, (void*)d_active, (void*)d_counts})
+                        (void*)d_block, (void*)d_groups, (void*)d_active, (void*)d_counts})
             if (p) (void)hipFree(p);
         if (h_active) (void)hipHostFree(h_active);
         if (stream) (void)hipStreamDestroy(stream);
@@ -265,15 +265,15 @@ SlotDev slot_dev(const SlotHost& s) {
     d.b64 = s.b;
     d.x64[0] = s.x[0];
     d.x64[1] = s.x[1];
-    d.r = s.r;
-    d.p[0] = s.p[0];
-    d.p[1] = s.p[1];
-    d.q = s.q;
+    for (int k = 0; k < 2; ++k) {
+        d.r[k] = s.r[k];
+        d.p[k] = s.p[k];
+        d.q[k] = s.q[k];
+    }
     d.c = s.c;
     d.part = s.part;
     d.history = s.history;
     d.cells = s.cells;
-    d.tiles_flat = s.tiles_flat;
     d.part_stride = s.part_stride;
     return d;
 }
@@ -283,12 +283,12 @@ SlotConfig slot_config(const pcbhip_mpir_config& c) {
                       c.max_inner_iterations};
 }
 
-// Device bytes of one slot's vectors: b and the x pair in FP64, r, the p pair,
-// q and c in the inner precision, and the partial sums.
+// Device bytes of one slot's vectors: b and the x pair in FP64, the r, p and
+// q pairs and c in the inner precision, and the partial sums.
 size_t slot_vector_bytes(const SlotHost& s, int inner64) {
     const size_t ti = inner64 ? sizeof(double) : sizeof(float);
-    return 3 * align256(sizeof(double) * s.op->N) + 5 * align256(ti * s.op->N) +
-           align256(sizeof(double) * 4 * (size_t)s.part_stride);
+    return 3 * align256(sizeof(double) * s.op->N) + 7 * align256(ti * s.op->N) +
+           align256(sizeof(double) * kInnerSums * (size_t)s.part_stride);
 }
 
 // Points the slot's vectors into one block laid out as slot_vector_bytes
@@ -301,12 +301,13 @@ void carve_slot(SlotHost& s, char* base, int inner64) {
     s.x[0] = reinterpret_cast<double*>(base + n64);
     s.x[1] = reinterpret_cast<double*>(base + 2 * n64);
     base += 3 * n64;
-    s.r = base;
-    s.p[0] = base + nti;
-    s.p[1] = base + 2 * nti;
-    s.q = base + 3 * nti;
-    s.c = base + 4 * nti;
-    s.part = reinterpret_cast<double*>(base + 5 * nti);
+    for (int k = 0; k < 2; ++k) {
+        s.r[k] = base + k * nti;
+        s.p[k] = base + (2 + k) * nti;
+        s.q[k] = base + (4 + k) * nti;
+    }
+    s.c = base + 6 * nti;
+    s.part = reinterpret_cast<double*>(base + 7 * nti);
 }
 
 // Tiles of one slot: stencil workgroups in (z block, y group, x strip)
@@ -345,7 +346,7 @@ void upload_tiles(pcbhip_batch* b, const std::vector<Tile>& t, Tile** d, int* n)
 // fill the device without splitting any of them.
 void collect_tiles(const std::vector<const SlotHost*>& slots, const std::vector<SlotGroup>& groups,
                    int rows_inner, long long target_groups, std::vector<Tile>& outer, std::vector<Tile>& inner,
-                   std::vector<Tile>& flat, std::vector<Tile>& block) {
+                   std::vector<Tile>& block) {
     std::vector<char> grouped(slots.size(), 0);
     for (const SlotGroup& g : groups)
         for (int k = 0; k < g.count; ++k) grouped[(size_t)g.slot[k]] = 1;
@@ -372,7 +373,6 @@ void collect_tiles(const std::vector<const SlotHost*>& slots, const std::vector<
             // One layer of 4-row wave strips per tile, for both stencils.
             add_sheet_tiles(outer, i, nsx, groups_y(op->R, kCellRows), op->S);
             if (!grouped[(size_t)i]) add_sheet_tiles(inner, i, nsx, groups_y(op->R, kCellRows), op->S);
-            for (int k = 0; k < s->tiles_flat; ++k) flat.push_back(Tile{i, k, 0, 0, 0, 0});
             continue;
         }
         const int nyo = groups_y(op->R, kRowsOuter), nyi = groups_y(op->R, rows_inner);
@@ -384,7 +384,6 @@ void collect_tiles(const std::vector<const SlotHost*>& slots, const std::vector<
             const int zbi = planes_per_block(op->S, (long long)nsx * nyi, std::max(share_i, (long long)nsx * nyi));
             add_tiles(inner, i, nsx, nyi, op->S, zbi);
         }
-        for (int k = 0; k < s->tiles_flat; ++k) flat.push_back(Tile{i, k, 0, 0, 0, 0});
     }
     // Block tiles: the groups of one operator interleaved at every spatial
     // tile, so the workgroups that read one piece of the operator run
@@ -447,15 +446,14 @@ std::vector<SlotGroup> group_slots(const std::vector<const SlotHost*>& slots, in
 }
 
 // Partial-sum layout of a slot: one partial per reduction cell of the
-// stencils or per chunk of the pointwise kernel, whichever is more.
+// stencils for each of the sums a kernel keeps.
 void set_partial_layout(SlotHost& s) {
     const pcbhip_operator* op = s.op;
-    s.tiles_flat = (int)((op->N + kFlatChunk - 1) / kFlatChunk);
     s.cells = op->kind == kSheet ? sheet_cell_count(op->S, op->R, op->C) : cell_count(op->S, op->R, op->C);
-    s.part_stride = std::max(s.cells, s.tiles_flat);
+    s.part_stride = s.cells;
 }
 
-// (Re)builds the three tile tables for the batch's current shape; with
+// (Re)builds the tile tables for the batch's current shape; with
 // ``keep``, only the slots it marks (finished slots leave the tables, so their
 // early exits stop costing launch slots; their partial layout does not change).
 void build_tiles(pcbhip_batch* b, const std::vector<char>* keep = nullptr) {
@@ -467,15 +465,13 @@ void build_tiles(pcbhip_batch* b, const std::vector<char>* keep = nullptr) {
         listed[i] = &b->slots[i];
         b->tiled_slots += 1;
     }
-    std::vector<Tile> outer, inner, flat, block;
+    std::vector<Tile> outer, inner, block;
     const std::vector<SlotGroup> groups = group_slots(listed, b->inner64 ? 0 : b->block_group);
-    collect_tiles(listed, groups, rows_inner, b->target_groups, outer, inner, flat, block);
+    collect_tiles(listed, groups, rows_inner, b->target_groups, outer, inner, block);
     if (outer.empty()) outer.push_back(Tile{0, 0, 0, 0, 0, 1});
     if (inner.empty() && block.empty()) inner.push_back(Tile{0, 0, 0, 0, 0, 1});
-    if (flat.empty()) flat.push_back(Tile{0, 0, 0, 0, 0, 0});
     upload_tiles(b, outer, &b->d_outer, &b->n_outer);
     upload_tiles(b, inner, &b->d_inner, &b->n_inner);
-    upload_tiles(b, flat, &b->d_flat, &b->n_flat);
     upload_tiles(b, block, &b->d_block, &b->n_block);
     if (b->d_groups) check(hipFree(b->d_groups), "hipFree");
     b->d_groups = static_cast<SlotGroup*>(device_alloc(sizeof(SlotGroup) * std::max<size_t>(1, groups.size())));
@@ -486,7 +482,7 @@ void build_tiles(pcbhip_batch* b, const std::vector<char>* keep = nullptr) {
     if (!b->d_counts) b->d_counts = static_cast<int*>(device_alloc(4 * sizeof(int)));
     b->h_counts[0] = b->n_outer;
     b->h_counts[1] = b->n_inner;
-    b->h_counts[2] = b->n_flat;
+    b->h_counts[2] = 0;
     b->h_counts[3] = b->n_block;
     check(hipMemcpyAsync(b->d_counts, b->h_counts, 4 * sizeof(int), hipMemcpyHostToDevice, b->stream), "counts");
     // Tables and counts are read through stream order; the sources above are
@@ -520,9 +516,8 @@ void build_graph(pcbhip_batch* b, int steps) {
                                      b->d_counts, b->n_outer, b->d_active, b->stream);
         if (err == hipSuccess)
             err = launch::inner_step(b->inner64 != 0, b->rows_inner, n_slots, b->d_slots, b->d_states, b->d_configs,
-                                     b->d_inner, b->d_flat, b->d_counts, std::max(b->n_inner, 1), b->n_flat,
-                                     b->d_block, b->d_groups, b->n_block, b->block_mfma, b->d_active,
-                                     b->stream);
+                                     b->d_inner, b->d_counts, std::max(b->n_inner, 1), b->d_block, b->d_groups,
+                                     b->n_block, b->block_mfma, b->d_active, b->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(b->stream, &dropped);
@@ -596,8 +591,8 @@ struct pcbhip_server {
     int* d_fresh_index = nullptr;
     int* d_active = nullptr;  // decremented by the finish kernels; the server reads states instead
     int* d_counts = nullptr;
-    Tile* d_tiles = nullptr;  // outer | inner | pointwise tables at their capacities
-    int cap_outer = 0, cap_inner = 0, cap_flat = 0;
+    Tile* d_tiles = nullptr;  // outer | inner tables at their capacities
+    int cap_outer = 0, cap_inner = 0;
     Staging stage[2];
     int stage_next = 0;
     SlotState* h_states = nullptr;  // pinned, two snapshots of max_slots states
@@ -678,7 +673,6 @@ void server_build_graph(pcbhip_server* sv) {
     }
     Tile* outer = sv->d_tiles;
     Tile* inner = outer + sv->cap_outer;
-    Tile* flat = inner + sv->cap_inner;
     check(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     for (int s = 0; s < sv->steps; ++s) {
         hipError_t err = hipSuccess;
@@ -687,8 +681,8 @@ void server_build_graph(pcbhip_server* sv) {
                                      outer, sv->d_counts, sv->cap_outer, sv->d_active, sv->stream);
         if (err == hipSuccess)
             err = launch::inner_step(sv->inner64 != 0, sv->rows_inner, sv->max_slots, sv->d_slots, sv->d_states,
-                                     sv->d_configs, inner, flat, sv->d_counts, sv->cap_inner, sv->cap_flat, nullptr,
-                                     nullptr, 0, 0, sv->d_active, sv->stream);
+                                     sv->d_configs, inner, sv->d_counts, sv->cap_inner, nullptr, nullptr, 0, 0,
+                                     sv->d_active, sv->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(sv->stream, &dropped);
@@ -707,11 +701,10 @@ int round_capacity(size_t n) { return (int)std::max<size_t>(256, ((n + n / 4 + 2
 // they are mostly empty (workgroups past the live count still cost dispatch);
 // either way the replays in flight are drained and the graph is captured
 // again with the new grids.  Returns whether it drained.
-bool server_reserve(pcbhip_server* sv, size_t n_outer, size_t n_inner, size_t n_flat) {
-    const bool fits = n_outer <= (size_t)sv->cap_outer && n_inner <= (size_t)sv->cap_inner &&
-                      n_flat <= (size_t)sv->cap_flat;
-    const size_t live = n_outer + n_inner + n_flat;
-    const size_t cap = (size_t)sv->cap_outer + sv->cap_inner + sv->cap_flat;
+bool server_reserve(pcbhip_server* sv, size_t n_outer, size_t n_inner) {
+    const bool fits = n_outer <= (size_t)sv->cap_outer && n_inner <= (size_t)sv->cap_inner;
+    const size_t live = n_outer + n_inner;
+    const size_t cap = (size_t)sv->cap_outer + sv->cap_inner;
     const bool sparse = cap > 3 * 1024 && live * 3 < cap;
     if (sv->exec && fits && !sparse) return false;
     server_drain(sv);
@@ -721,9 +714,7 @@ bool server_reserve(pcbhip_server* sv, size_t n_outer, size_t n_inner, size_t n_
         sv->d_tiles = nullptr;
         sv->cap_outer = round_capacity(n_outer);
         sv->cap_inner = round_capacity(n_inner);
-        sv->cap_flat = round_capacity(n_flat);
-        sv->d_tiles = static_cast<Tile*>(
-            device_alloc(sizeof(Tile) * ((size_t)sv->cap_outer + sv->cap_inner + sv->cap_flat)));
+        sv->d_tiles = static_cast<Tile*>(device_alloc(sizeof(Tile) * ((size_t)sv->cap_outer + sv->cap_inner)));
     }
     server_build_graph(sv);
     return true;
@@ -734,7 +725,7 @@ bool server_reserve(pcbhip_server* sv, size_t n_outer, size_t n_inner, size_t n_
 // of it runs on the replay stream between two replays.
 void server_upload(pcbhip_server* sv, const std::vector<int>& admitted) {
     const int rows_inner = sv->inner64 ? kRowsInner64 : sv->rows_inner;
-    std::vector<Tile> outer, inner, flat;
+    std::vector<Tile> outer, inner;
     // A drain inside the reserve can finish slots: list again after one.
     for (int pass = 0; pass < 2; ++pass) {
         std::vector<const SlotHost*> listed((size_t)sv->max_slots, nullptr);
@@ -742,15 +733,14 @@ void server_upload(pcbhip_server* sv, const std::vector<int>& admitted) {
             if (sv->slots[(size_t)i].status == kSlotRunning) listed[(size_t)i] = &sv->slots[(size_t)i].h;
         outer.clear();
         inner.clear();
-        flat.clear();
         std::vector<Tile> block;
-        collect_tiles(listed, {}, rows_inner, sv->target_groups, outer, inner, flat, block);
-        if (!server_reserve(sv, outer.size(), inner.size(), flat.size())) break;
+        collect_tiles(listed, {}, rows_inner, sv->target_groups, outer, inner, block);
+        if (!server_reserve(sv, outer.size(), inner.size())) break;
     }
     Staging& st = sv->stage[sv->stage_next];
     sv->stage_next ^= 1;
     if (st.recorded) check(hipEventSynchronize(st.used), "staging");
-    const size_t n_tiles = outer.size() + inner.size() + flat.size();
+    const size_t n_tiles = outer.size() + inner.size();
     if (n_tiles > st.tile_cap) {
         if (st.tiles) check(hipHostFree(st.tiles), "pinned free");
         st.tile_cap = n_tiles + n_tiles / 4;
@@ -758,11 +748,10 @@ void server_upload(pcbhip_server* sv, const std::vector<int>& admitted) {
     }
     st.counts[0] = (int)outer.size();
     st.counts[1] = (int)inner.size();
-    st.counts[2] = (int)flat.size();
+    st.counts[2] = 0;
     st.counts[3] = 0;
     std::copy(outer.begin(), outer.end(), st.tiles);
     std::copy(inner.begin(), inner.end(), st.tiles + outer.size());
-    std::copy(flat.begin(), flat.end(), st.tiles + outer.size() + inner.size());
     for (int i = 0; i < sv->max_slots; ++i) {
         const ServerSlot& s = sv->slots[(size_t)i];
         const bool bound = s.status == kSlotRunning || s.status == kSlotFinished;
@@ -776,16 +765,11 @@ void server_upload(pcbhip_server* sv, const std::vector<int>& admitted) {
     }
     Tile* d_outer = sv->d_tiles;
     Tile* d_inner = d_outer + sv->cap_outer;
-    Tile* d_flat = d_inner + sv->cap_inner;
     const hipMemcpyKind h2d = hipMemcpyHostToDevice;
     if (!outer.empty())
         check(hipMemcpyAsync(d_outer, st.tiles, sizeof(Tile) * outer.size(), h2d, sv->stream), "tiles");
     if (!inner.empty())
         check(hipMemcpyAsync(d_inner, st.tiles + outer.size(), sizeof(Tile) * inner.size(), h2d, sv->stream),
-              "tiles");
-    if (!flat.empty())
-        check(hipMemcpyAsync(d_flat, st.tiles + outer.size() + inner.size(), sizeof(Tile) * flat.size(), h2d,
-                             sv->stream),
               "tiles");
     check(hipMemcpyAsync(sv->d_counts, st.counts, 4 * sizeof(int), h2d, sv->stream), "counts");
     check(hipMemcpyAsync(sv->d_slots, st.devs, sizeof(SlotDev) * sv->max_slots, h2d, sv->stream), "slots");
@@ -1118,7 +1102,7 @@ int pcbhip_batch_create(int count, const pcbhip_operator* const* ops, int inner_
             char* base = static_cast<char*>(device_alloc(vec + align256(sizeof(double) * op->N)));
             carve_slot(s, base, b->inner64);
             s.x0 = reinterpret_cast<double*>(base + vec);
-            // Zeroed vectors: the kernels load p, c and x unconditionally and
+            // Zeroed vectors: the kernels load r, p, q, c and x unconditionally and
             // weight them by 0 where the algorithm starts from zero.
             check(hipMemsetAsync(base, 0, vec, b->stream), "memset");
         }

@@ -25,9 +25,6 @@ constexpr int kStripCols = kWave - 2;
 // Waves per workgroup of the stencil kernels (stacked in y).
 constexpr int kWavesPerGroup = 4;
 constexpr int kGroupThreads = kWave * kWavesPerGroup;
-// Nodes per workgroup of the pointwise kernel.
-constexpr int kFlatPerThread = 8;
-constexpr int kFlatChunk = kGroupThreads * kFlatPerThread;
 // Rows per wave strip of the FP64 outer-residual kernel and of the FP64
 // inner kernel; the FP32 inner kernel and the standalone apply are built for
 // 4 and 8 rows and the tile shape picks one.
@@ -38,18 +35,29 @@ constexpr int kRowsInner64 = 4;
 constexpr int kTargetGroups = 1024;
 // Workgroups of the persistent outer-residual kernel.
 constexpr int kOuterGroups = 512;
+// Threads of the one-workgroup-per-slot finish kernels.
+constexpr int kFinishThreads = 512;
 
 // Reduction cells of the stencil kernels: every inner product is summed per
-// (node-plane pair, 4 node rows, one strip of 62 node columns) and stored as
-// one partial, whatever tile shape computed it.
+// (node-plane pair, 16 node rows, one strip of 62 node columns) and stored as
+// one partial, whatever tile shape computed it.  A wave sums 4-row groups
+// (kCellRows); the workgroup adds the four groups of a cell in order.
 constexpr int kCellRows = 4;
 constexpr int kCellPlanes = 2;
+constexpr int kCellBandRows = kCellRows * kWavesPerGroup;
+// Plane pairs one stencil tile may hold (its z block is at most 32 planes).
+constexpr int kMaxTilePairs = 16;
+// Sums per reduction cell: four of the outer residual kernel and five of
+// the fused PCG pass (p.q, r.z, q.z, q.D^-1 q, r.r).
+constexpr int kOuterSums = 4;
+constexpr int kInnerSums = 5;
+
+__host__ __device__ inline int cell_bands(const int R) { return (R + 1 + kCellBandRows - 1) / kCellBandRows; }
 
 __host__ __device__ inline int cell_count(const int S, const int R, const int C) {
     const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
-    const int nrg = (R + 1 + kCellRows - 1) / kCellRows;
     const int nzp = (S + 1 + kCellPlanes - 1) / kCellPlanes;
-    return nsx * nrg * nzp;
+    return nsx * cell_bands(R) * nzp;
 }
 
 enum Phase : int { kPhaseOuter = 0, kPhaseInner = 1, kPhaseDone = 2 };
@@ -86,32 +94,30 @@ struct OperatorDev {
     const double* via_g64;
 };
 
-// Reduction cells of the sheet kernels: one per (layer, 4 node rows, strip).
+// Reduction cells of the sheet kernels: one per (layer, 16 node rows, strip).
 __host__ __device__ inline int sheet_cell_count(const int layers, const int R, const int C) {
     const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
-    const int nrg = (R + 1 + kCellRows - 1) / kCellRows;
-    return layers * nrg * nsx;
+    return layers * cell_bands(R) * nsx;
 }
 
-// One right-hand side.  ``x64`` and ``p`` are ping-pong pairs: a stencil
-// kernel reads one buffer at its halo while neighbouring workgroups write the
-// other, so no workgroup ever reads a value another rewrites in the same
-// launch.  r/q/c are the inner-precision residual, operator-times-direction
-// and correction.
+// One right-hand side.  ``x64``, ``r``, ``p`` and ``q`` are ping-pong
+// pairs: a stencil kernel reads one buffer at its halo while neighbouring
+// workgroups write the other, so no workgroup ever reads a value another
+// rewrites in the same launch.  r/p/q/c are the inner-precision residual,
+// direction, operator-times-direction and correction; c is read and written
+// only at owned nodes, so one buffer serves.
 struct SlotDev {
     OperatorDev op;
     const double* b64;  // (N) right-hand side
     double* x64[2];     // (N) solution pair
-    void* r;
+    void* r[2];
     void* p[2];
-    void* q;
+    void* q[2];
     void* c;
-    double* part;       // 4 * max(tiles) partial sums
+    double* part;       // kInnerSums * part_stride partial sums
     double* history;    // 4 * max_outer: (outer, fp64 rel before, inner its, inner rel)
     int cells;          // reduction cells of the stencil kernels (see cell_count)
-    int tiles_flat;     // workgroups of the pointwise kernel
     int part_stride;
-    int pad;
 };
 
 struct SlotConfig {
@@ -121,6 +127,7 @@ struct SlotConfig {
 
 // Mutable scalars of one slot.  Written only by the one-workgroup "finish"
 // kernels that follow each compute kernel; read by every later launch.
+// ``pcur`` selects the current buffer of the r, p and q pairs.
 struct SlotState {
     int phase, first, pending, outer;
     int inner_its, total_inner, high_apps, low_apps;
@@ -162,18 +169,18 @@ hipError_t sheet_diagonal(const OperatorDev& op, double* diag, double* dinv64, f
                           hipStream_t stream);
 hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream);
 // One outer step (FP64 residual and decision) for the slots in their outer
-// phase, and one inner Jacobi-PCG iteration for the slots in their inner
-// phase.  ``counts`` (device) holds the live lengths of the outer, inner,
-// pointwise and block tile tables; the grids may be longer (workgroups past
+// phase, and one fused Jacobi-PCG pass for the slots in their inner phase.
+// ``counts`` (device) holds the live lengths of the outer, inner and block
+// tile tables (entries 0, 1 and 3); the grids may be longer (workgroups past
 // the count return at once), so a captured graph survives changes of the
 // tables.  Block tiles (a slot group per tile, ``groups``) are launched when
 // grid_block > 0, with the element action block_mfma (0, 2, 3 as in apply).
 hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
                       const Tile* outer_tiles, const int* counts, int grid_outer, int* active, hipStream_t stream);
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
-                      const SlotConfig* configs, const Tile* inner_tiles, const Tile* flat_tiles, const int* counts,
-                      int grid_inner, int grid_flat, const Tile* block_tiles, const SlotGroup* groups,
-                      int grid_block, int block_mfma, int* active, hipStream_t stream);
+                      const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner,
+                      const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma, int* active,
+                      hipStream_t stream);
 // states[index[i]] = fresh[i] for i < count (device arrays).
 hipError_t scatter_states(const int* index, const SlotState* fresh, SlotState* states, int count, hipStream_t stream);
 }  // namespace launch

@@ -23,6 +23,8 @@
 // composition or launch order.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "q1hex_device.h"
 
 namespace pcbhip {
@@ -75,8 +77,22 @@ struct NodeVal {
     T robin;  // Robin conductance of a free node, negative on a fixed node
 };
 
+// What the fused PCG pass carries from a node's load to its emit: the new
+// direction (the operand), the Robin word, the updated residual, the Jacobi
+// inverse and the updated correction.
 template <typename T>
-__device__ __forceinline__ T masked(const NodeVal<T>& v) {
+struct InnerVal {
+    T u;
+    T robin;
+    T r;
+    T d;
+    T c;  // the updated correction (owned nodes)
+};
+
+// The operand with fixed columns zeroed, for any node-value type with u and robin.
+template <class V>
+__device__ __forceinline__ auto masked(const V& v) -> decltype(v.u) {
+    using T = decltype(v.u);
     return v.robin >= T(0) ? v.u : T(0);
 }
 
@@ -225,19 +241,21 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
         eoff[j] = (ecol_in && ye >= 0 && ye < R) ? (unsigned)(ye * C + xc) : kOutside;
     }
 
+    // The node-value type is whatever ``load`` returns (NodeVal or InnerVal).
+    using V = std::decay_t<decltype(load(0LL, 0u, false))>;
     T mlo[ROWS + 2], nlo[ROWS + 2];
-    NodeVal<T> vlo[ROWS];
+    V vlo[ROWS];
     T Llo[ROWS], Rlo[ROWS];
     [[maybe_unused]] bf16x8 unit;
     [[maybe_unused]] unsigned qlo[ROWS + 2][4];
     if constexpr (MFMA != 0) unit = unit_operand();
 
-    auto load_plane = [&](const int z, T (&m)[ROWS + 2], T (&n)[ROWS + 2], NodeVal<T> (&v)[ROWS]) {
+    auto load_plane = [&](const int z, T (&m)[ROWS + 2], T (&n)[ROWS + 2], V (&v)[ROWS]) {
         const long long pb = (long long)z * NR * NC;
 #pragma unroll
         for (int i = 0; i < ROWS + 2; ++i) {
             const bool owned = out_lane && i >= 1 && i <= ROWS && y0 - 1 + i <= R;
-            const NodeVal<T> nv = load(pb, noff[i], owned);
+            const V nv = load(pb, noff[i], owned);
             m[i] = masked(nv);
             if (i >= 1 && i <= ROWS) v[i - 1] = nv;
         }
@@ -245,7 +263,7 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
         for (int i = 0; i < ROWS + 2; ++i) n[i] = __shfl_down(m[i], 1);
     };
 
-    auto finish_plane = [&](const int z, const T (&L)[ROWS], const T (&Rt)[ROWS], const NodeVal<T> (&v)[ROWS]) {
+    auto finish_plane = [&](const int z, const T (&L)[ROWS], const T (&Rt)[ROWS], const V (&v)[ROWS]) {
         const long long pb = (long long)z * NR * NC;
 #pragma unroll
         for (int i = 0; i < ROWS; ++i) {
@@ -272,7 +290,7 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
 
     for (int ez = first_layer; ez <= last_layer; ++ez) {
         T mhi[ROWS + 2], nhi[ROWS + 2];
-        NodeVal<T> vhi[ROWS];
+        V vhi[ROWS];
         load_plane(ez + 1, mhi, nhi, vhi);
 
         T ca[ROWS + 1], cb[ROWS + 1], cc[ROWS + 1];
@@ -434,11 +452,13 @@ __device__ __forceinline__ void sheet_apply(const T* __restrict__ coef, const in
         const int ye = y0 - 1 + j;
         eoff[j] = (ecol_in && ye >= 0 && ye < R) ? (unsigned)(ye * C + xc) : kOutside;
     }
+    using V = std::decay_t<decltype(load(0LL, 0u, false))>;
     T m[ROWS + 2], n[ROWS + 2];
-    NodeVal<T> v[ROWS];
+    V v[ROWS];
 #pragma unroll
     for (int i = 0; i < ROWS + 2; ++i) {
-        const NodeVal<T> nv = load(pb, noff[i], false);
+        const bool owned = out_lane && i >= 1 && i <= ROWS && y0 - 1 + i <= R;
+        const V nv = load(pb, noff[i], owned);
         m[i] = masked(nv);
         if (i >= 1 && i <= ROWS) v[i - 1] = nv;
     }
@@ -479,7 +499,7 @@ __device__ __forceinline__ void sheet_apply(const T* __restrict__ coef, const in
                 const int lo = via_ptr[node], hi = via_ptr[node + 1];
                 const T own = masked(v[i]);
                 for (int k = lo; k < hi; ++k) {
-                    const NodeVal<T> far = load(0LL, (unsigned)via_nbr[k], false);
+                    const V far = load(0LL, (unsigned)via_nbr[k], false);
                     total = fmadd(via_g[k], own - masked(far), total);
                 }
             }
@@ -506,104 +526,109 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Reduction cells of the stencil kernels.  Every inner product is summed
-// per cell of (node-plane pair, 4 node rows, one 62-column strip) -- lanes in
-// a fixed order, then a wave butterfly -- and one partial per cell is stored.
-// The cells do not depend on the tile shape (rows per wave, planes per z
-// block) or on which other slots share the launch, so a slot's bits depend
-// only on its own data.  Partials become visible to the finish kernel at the
-// launch boundary; no workgroup needs a device-scope fence.
-// Per-lane accumulators of NV sums for the ROWS / 4 row cells of a strip.
+// Reduction cells of the stencil kernels.  Every inner product is summed per
+// cell of (node-plane pair, 16 node rows, one 62-column strip): each wave sums
+// its 4-row groups (lanes in a fixed order, then a wave butterfly) into a
+// table in LDS, and at the end of the tile the workgroup adds the four groups
+// of each cell in row order and stores one partial per cell.  The cells do
+// not depend on the tile shape (rows per wave, planes per z block) or on
+// which other slots share the launch, so a slot's bits depend only on its own
+// data.  Partials become visible to the finish kernel at the launch
+// boundary; no workgroup needs a device-scope fence.
 template <int NV, int ROWS>
 struct CellSums {
-    double v[NV][ROWS / kCellRows];
+    static constexpr int kGroups = ROWS / kCellRows;  // 4-row groups per wave
+    static constexpr int kEntries = kWavesPerGroup * kGroups;
+    double v[NV][kGroups];  // per-lane accumulators
 
+    __device__ static double (&table())[kMaxTilePairs][NV][kEntries] {
+        __shared__ double sums[kMaxTilePairs][NV][kEntries];
+        return sums;
+    }
     __device__ void clear() {
 #pragma unroll
         for (int k = 0; k < NV; ++k)
 #pragma unroll
-            for (int g = 0; g < ROWS / kCellRows; ++g) v[k][g] = 0.0;
+            for (int g = 0; g < kGroups; ++g) v[k][g] = 0.0;
     }
-    // After output plane z: a plane pair (or the last plane) is complete, so
-    // store one partial per row cell and start again.
-    __device__ void plane_done(const int z, const int S, const int R, const int C, const int x0, const int y0,
-                               double* __restrict__ part, const int stride) {
-        if (!((z & 1) || z == S)) return;
-        const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
-        const int nrg = (R + 1 + kCellRows - 1) / kCellRows;
-        const int sx = x0 / kStripCols;
+    __device__ void store_groups(const int pair, const int groups) {
+        const int wave = threadIdx.x / kWave;
 #pragma unroll
-        for (int g = 0; g < ROWS / kCellRows; ++g) {
-            const int rg = y0 / kCellRows + g;
-            const int cell = ((z / kCellPlanes) * nrg + rg) * nsx + sx;
+        for (int g = 0; g < kGroups; ++g) {
+            if (g >= groups) break;
 #pragma unroll
             for (int k = 0; k < NV; ++k) {
                 const double s = wave_sum(v[k][g]);
-                if (rg < nrg && (threadIdx.x & (kWave - 1)) == 0) part[k * stride + cell] = s;
+                if ((threadIdx.x & (kWave - 1)) == 0) table()[pair][k][wave * kGroups + g] = s;
             }
         }
         clear();
     }
-    // Sheet kernels: the wave's one row cell of (layer, 4 rows, strip).
-    __device__ void flush_sheet(const int layer, const int R, const int C, const int x0, const int y0,
-                                double* __restrict__ part, const int stride) {
+    // After output plane z of a tile whose z block starts at plane zlo (even):
+    // a plane pair (or the last plane) is complete.
+    __device__ void plane_done(const int z, const int zlo, const int S) {
+        if ((z & 1) || z == S) store_groups(z / kCellPlanes - zlo / kCellPlanes, kGroups);
+    }
+    // Sheet kernels: the wave's one 4-row group of its layer.
+    __device__ void sheet_done() { store_groups(0, 1); }
+    // End of a tile, reached by every wave of the workgroup: the cells of
+    // ``pairs`` plane pairs from pair pz0 (sheet: one layer, pz0 = layer) and
+    // of the workgroup's rows from y0 on.  Group j of a cell is table entry
+    // (c * 4 + j) * step; groups that start past row R hold no nodes.
+    __device__ void flush(const int pairs, const int pz0, const int y0, const int sx, const int R, const int C,
+                          const int cells, const int step, double* __restrict__ part, const int stride) {
+        __syncthreads();
         const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
-        const int nrg = (R + 1 + kCellRows - 1) / kCellRows;
-        const int cell = (layer * nrg + y0 / kCellRows) * nsx + x0 / kStripCols;
+        const int nb = cell_bands(R);
+        for (int i = threadIdx.x; i < pairs * NV * cells; i += kGroupThreads) {
+            const int c = i % cells, k = (i / cells) % NV, p = i / (cells * NV);
+            const int band = y0 / kCellBandRows + c;
+            if (band >= nb) continue;
+            double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            const double sum = wave_sum(v[k][0]);
-            if ((threadIdx.x & (kWave - 1)) == 0) part[k * stride + cell] = sum;
+            for (int j = 0; j < kWavesPerGroup; ++j) {
+                const int group = c * kWavesPerGroup + j;
+                if (y0 + group * kCellRows <= R) s += table()[p][k][group * step];
+            }
+            part[k * stride + ((pz0 + p) * nb + band) * nsx + sx] = s;
         }
-        clear();
+        __syncthreads();
+    }
+    __device__ void flush_hex(const int zlo, const int zhi, const int y0, const int sx, const int R, const int C,
+                              double* __restrict__ part, const int stride) {
+        flush((zhi - zlo + 1) / kCellPlanes, zlo / kCellPlanes, y0, sx, R, C, kGroups, 1, part, stride);
+    }
+    __device__ void flush_sheet(const int layer, const int y0, const int sx, const int R, const int C,
+                                double* __restrict__ part, const int stride) {
+        flush(1, layer, y0, sx, R, C, 1, kGroups, part, stride);
     }
 };
 
-// Workgroup sum for the pointwise kernel: waves in order, one partial per
-// workgroup (its chunk index).
-template <int NV>
-__device__ void group_partials(const double (&v)[NV], double* __restrict__ part, const int stride, const int index) {
-    __shared__ double waves[kWavesPerGroup][NV];
-    const int tid = threadIdx.x;
-    const int wave = tid / kWave;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const double s = wave_sum(v[k]);
-        if ((tid & (kWave - 1)) == 0) waves[wave][k] = s;
-    }
-    __syncthreads();
-    if (tid == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            double s = 0.0;
-            for (int w = 0; w < kWavesPerGroup; ++w) s += waves[w][k];
-            part[k * stride + index] = s;
-        }
-    }
-    __syncthreads();
-}
-
-// Fixed-order sum of ``count`` partials of one slot by one workgroup:
-// thread t adds partials t, t + 256, t + 512, ... in that order (four loads
-// in flight before the adds), then a wave butterfly and the four wave sums
-// in order.  The result is for thread 0.
-template <int NV>
+// Fixed-order sum of ``count`` partials of one slot by one workgroup of
+// THREADS threads: thread t adds partials t, t + THREADS, t + 2 THREADS, ...
+// in that order (kUnroll loads per sum in flight before the adds), then a
+// wave butterfly, and a butterfly over the wave sums.  The result is for
+// thread 0.
+template <int NV, int THREADS>
 __device__ void slot_sum(const double* __restrict__ part, const int stride, const int count, double (&total)[NV]) {
-    __shared__ double waves[kWavesPerGroup][NV];
+    constexpr int kWaves = THREADS / kWave;
+    constexpr int kUnroll = 4;
+    __shared__ double waves[kWaves][NV];
     const int tid = threadIdx.x;
     double s[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) s[k] = 0.0;
-    for (int i = tid; i < count; i += 4 * kGroupThreads) {
-        double v[NV][4];
+#pragma unroll 1
+    for (int i = tid; i < count; i += kUnroll * THREADS) {
+        double v[NV][kUnroll];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int idx = i + u * kGroupThreads;
+        for (int u = 0; u < kUnroll; ++u) {
+            const int idx = i + u * THREADS;
 #pragma unroll
             for (int k = 0; k < NV; ++k) v[k][u] = idx < count ? part[k * stride + idx] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < kUnroll; ++u)
 #pragma unroll
             for (int k = 0; k < NV; ++k) s[k] += v[k][u];
     }
@@ -613,12 +638,13 @@ __device__ void slot_sum(const double* __restrict__ part, const int stride, cons
         if ((tid & (kWave - 1)) == 0) waves[tid / kWave][k] = w;
     }
     __syncthreads();
-    if (tid == 0) {
+    // The wave sums by a butterfly over the first wave (a fixed order too).
+    static_assert(kWaves <= kWave, "one lane per wave sum");
+    if (tid < kWave) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
-            double t = waves[0][k];
-            for (int w = 1; w < kWavesPerGroup; ++w) t += waves[w][k];
-            total[k] = t;
+            const double t = wave_sum(tid < kWaves ? waves[tid][k] : 0.0);
+            if (tid == 0) total[k] = t;
         }
     }
 }
@@ -823,12 +849,12 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
         const double* xin = sd.x64[xcur];
         double* xout = sd.x64[1 - xcur];
         const TI* __restrict__ cin = static_cast<const TI*>(sd.c);
-        TI* __restrict__ rout = static_cast<TI*>(sd.r);
+        TI* __restrict__ rout = static_cast<TI*>(sd.r[st.pcur]);
         const TI* __restrict__ dinv = InnerOf<TI>::dinv(op);
         const double* __restrict__ robin = op.robin64;
         const double* __restrict__ b = sd.b64;
         // b.b, r.r (FP64), r.r and r.z (inner precision) per reduction cell.
-        CellSums<4, kRowsOuter> acc;
+        CellSums<kOuterSums, kRowsOuter> acc;
         acc.clear();
         auto load = [&](const long long pb, const unsigned off, const bool) -> NodeVal<double> {
             const double xv = ldt(xin + pb, off) + cw * (double)ldt(cin + pb, off);
@@ -848,20 +874,22 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
             acc.v[2][g] += (double)ri * (double)ri;
             acc.v[3][g] += (double)ri * (double)(ri * ldt(dinv + pb, off));
         };
+        const int y_group = tile.sy * kWavesPerGroup * kRowsOuter;
         if (op.kind == kSheet) {
             if (y0 <= op.R) {
                 sheet_apply<double>(op.coef64, op.via_ptr, op.via_nbr, op.via_g64, op.S, op.R, op.C, tile.sz, x0, y0,
                                     load, emit);
-                acc.flush_sheet(tile.sz, op.R, op.C, x0, y0, sd.part, sd.part_stride);
+                acc.sheet_done();
             }
+            acc.flush_sheet(tile.sz, y_group, tile.sx, op.R, op.C, sd.part, sd.part_stride);
             continue;
         }
         if (y0 <= op.R) {
             // Stores wait for their plane's emit, so a plane's loads issue back to back.
-            strip_apply<double, kRowsOuter>(
-                op.coef64, op.S, op.R, op.C, x0, y0, zlo, zhi, load, emit,
-                [&](const int z) { acc.plane_done(z, op.S, op.R, op.C, x0, y0, sd.part, sd.part_stride); });
+            strip_apply<double, kRowsOuter>(op.coef64, op.S, op.R, op.C, x0, y0, zlo, zhi, load, emit,
+                                            [&](const int z) { acc.plane_done(z, zlo, op.S); });
         }
+        acc.flush_hex(zlo, zhi, y_group, tile.sx, op.R, op.C, sd.part, sd.part_stride);
     }
 }
 
@@ -869,16 +897,20 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
 // convergence, and the start of the inner solve (hex_q1.cpp:870-911,
 // solver.py:638-686).
 template <typename TI>
-__global__ __launch_bounds__(kGroupThreads) void k_finish_outer(const SlotDev* __restrict__ slots,
+__global__ __launch_bounds__(kFinishThreads) void k_finish_outer(const SlotDev* __restrict__ slots,
                                                                 SlotState* states,
                                                                 const SlotConfig* __restrict__ configs,
                                                                 int* active) {
+    // The slot's descriptor is read with its phase, before the test: a slot
+    // in another phase reads nothing more (its vectors may be released).
     const int slot = blockIdx.x;
     SlotState& st = states[slot];
-    if (st.phase != kPhaseOuter) return;
     const SlotDev& sd = slots[slot];
-    double total[4];
-    slot_sum<4>(sd.part, sd.part_stride, sd.cells, total);
+    const double* part = sd.part;
+    const int stride = sd.part_stride, cells = sd.cells;
+    if (st.phase != kPhaseOuter) return;
+    double total[kOuterSums];
+    slot_sum<kOuterSums, kFinishThreads>(part, stride, cells, total);
     if (threadIdx.x != 0) return;
     const SlotConfig cfg = configs[slot];
     const int pending = st.pending;
@@ -923,40 +955,71 @@ __global__ __launch_bounds__(kGroupThreads) void k_finish_outer(const SlotDev* _
     st.phase = kPhaseInner;
 }
 
-// One slot's part of the inner apply on one tile: the direction
-// p = z + beta p_old (z = r / diag; beta = 0 on the first step) fused into
-// q = A p, and p.q per reduction cell (hex_q1.cpp:454-464).
+// The fused Jacobi-PCG pass (hex_q1.cpp:448-489 restated so that one pass
+// over the vectors does one iteration).  Pass i of an inner solve, on one
+// tile of one slot:
+//   r_{i-1} = r_{i-2} - alpha_{i-1} q_{i-1}   (every loaded node, halo included)
+//   c_{i-1} = c_{i-2} + alpha_{i-1} p_{i-1}   (owned nodes)
+//   p_i     = D^-1 r_{i-1} + beta_i p_{i-1}
+//   q_i     = A p_i
+// and, per reduction cell, p_i.q_i, r_{i-1}.z_{i-1}, q_i.z_{i-1},
+// q_i.D^-1 q_i and r_{i-1}.r_{i-1}.  The first pass has alpha = beta = 0
+// and no correction (the buffers start zeroed, so the zero weights meet
+// finite values).  r, p and q are ping-pong pairs: a pass reads the current
+// buffers at its halo and writes the others.  The finish kernel then
+// completes iteration i - 1 exactly as the two-kernel form does (tolerance
+// on ||r_{i-1}||, the curvature of p_i, alpha_i = r.z / p.q) and forms
+// beta_{i+1} from r_i.z_i = r.z - 2 alpha q.z + alpha^2 q.D^-1 q, the one
+// value the pass cannot yet have summed.
 template <typename TI, int ROWS, int MFMA>
-__device__ __forceinline__ void inner_apply_slot(const SlotDev& sd, const SlotState& st, const Tile& tile) {
+__device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotState& st, const Tile& tile) {
     const OperatorDev& op = sd.op;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int x0 = tile.sx * kStripCols;
     const int y0 = (tile.sy * kWavesPerGroup + wave) * ROWS;
     const int zlo = tile.sz * tile.zb;
     const int zhi = zlo + tile.zb < op.S + 1 ? zlo + tile.zb : op.S + 1;
-    // beta is zero on the first iteration (p = z); p_old is always loaded
-    // (the buffers start zeroed), so no load sits behind a branch.
-    const TI beta = st.first != 0 ? TI(0) : (TI)st.beta;
-    const int pcur = st.pcur;
-    const TI* __restrict__ pin = static_cast<const TI*>(sd.p[pcur]);
-    TI* __restrict__ pout = static_cast<TI*>(sd.p[1 - pcur]);
-    const TI* __restrict__ r = static_cast<const TI*>(sd.r);
-    TI* __restrict__ q = static_cast<TI*>(sd.q);
+    const bool first = st.first != 0;
+    const TI alpha = first ? TI(0) : (TI)st.alpha;
+    const TI beta = first ? TI(0) : (TI)st.beta;
+    const TI keep = first ? TI(0) : TI(1);
+    const int cur = st.pcur;
+    const TI* __restrict__ rin = static_cast<const TI*>(sd.r[cur]);
+    const TI* __restrict__ qin = static_cast<const TI*>(sd.q[cur]);
+    const TI* __restrict__ pin = static_cast<const TI*>(sd.p[cur]);
+    TI* __restrict__ rout = static_cast<TI*>(sd.r[1 - cur]);
+    TI* __restrict__ qout = static_cast<TI*>(sd.q[1 - cur]);
+    TI* __restrict__ pout = static_cast<TI*>(sd.p[1 - cur]);
+    TI* __restrict__ c = static_cast<TI*>(sd.c);
     const TI* __restrict__ dinv = InnerOf<TI>::dinv(op);
     const TI* __restrict__ robin = InnerOf<TI>::robin(op);
-    CellSums<1, ROWS> acc;  // p.q per reduction cell
+    CellSums<kInnerSums, ROWS> acc;
     acc.clear();
-    auto load = [&](const long long pb, const unsigned off, const bool) -> NodeVal<TI> {
-        const TI z = ldt(r + pb, off) * ldt(dinv + pb, off);
-        const TI pv = z + beta * ldt(pin + pb, off);
-        return {pv, ldt(robin + pb, off)};
+    // The correction is loaded at owned nodes only (elsewhere the offset is
+    // out of range and reads zero).
+    auto load = [&](const long long pb, const unsigned off, const bool owned) -> InnerVal<TI> {
+        const TI rv = ldt(rin + pb, off), qv = ldt(qin + pb, off), pv = ldt(pin + pb, off);
+        const TI d = ldt(dinv + pb, off);
+        const TI cv = ldt(c + pb, owned ? off : kOutside);
+        const TI r = rv - alpha * qv;
+        const TI p = r * d + beta * pv;
+        return {p, ldt(robin + pb, off), r, d, keep * cv + alpha * pv};
     };
-    // p is stored with q at the plane's emit, so a plane's loads issue back to back.
-    auto emit = [&](const long long pb, const unsigned off, const int row, const TI conduction, const NodeVal<TI>& v) {
-        stt(pout + pb, off, v.u);
+    // Stores wait for their plane's emit, so a plane's loads issue back to back.
+    auto emit = [&](const long long pb, const unsigned off, const int row, const TI conduction,
+                    const InnerVal<TI>& v) {
         const TI ap = v.robin >= TI(0) ? conduction + v.robin * v.u : v.u;
-        stt(q + pb, off, ap);
-        acc.v[0][row / kCellRows] += (double)v.u * (double)ap;
+        stt(pout + pb, off, v.u);
+        stt(qout + pb, off, ap);
+        stt(rout + pb, off, v.r);
+        stt(c + pb, off, v.c);
+        const TI z = v.r * v.d;
+        const int g = row / kCellRows;
+        acc.v[0][g] += (double)v.u * (double)ap;
+        acc.v[1][g] += (double)v.r * (double)z;
+        acc.v[2][g] += (double)ap * (double)z;
+        acc.v[3][g] += (double)ap * (double)(ap * v.d);
+        acc.v[4][g] += (double)v.r * (double)v.r;
     };
     if (op.kind == kSheet) {
         // Sheet tiles are one layer of 4-row wave strips.
@@ -964,45 +1027,45 @@ __device__ __forceinline__ void inner_apply_slot(const SlotDev& sd, const SlotSt
         if (ys <= op.R) {
             sheet_apply<TI>(InnerOf<TI>::coef(op), op.via_ptr, op.via_nbr, InnerOf<TI>::via_g(op), op.S, op.R, op.C,
                             tile.sz, x0, ys, load, emit);
-            acc.flush_sheet(tile.sz, op.R, op.C, x0, ys, sd.part, sd.part_stride);
+            acc.sheet_done();
         }
+        acc.flush_sheet(tile.sz, tile.sy * kCellBandRows, tile.sx, op.R, op.C, sd.part, sd.part_stride);
         return;
     }
     if (y0 <= op.R) {
-        strip_apply<TI, ROWS, MFMA>(
-            InnerOf<TI>::coef(op), op.S, op.R, op.C, x0, y0, zlo, zhi, load, emit,
-            [&](const int z) { acc.plane_done(z, op.S, op.R, op.C, x0, y0, sd.part, sd.part_stride); });
+        strip_apply<TI, ROWS, MFMA>(InnerOf<TI>::coef(op), op.S, op.R, op.C, x0, y0, zlo, zhi, load, emit,
+                                    [&](const int z) { acc.plane_done(z, zlo, op.S); });
     }
+    acc.flush_hex(zlo, zhi, tile.sy * kWavesPerGroup * ROWS, tile.sx, op.R, op.C, sd.part, sd.part_stride);
 }
 
-// Inner step, part 1: the direction fused into the operator apply and p.q,
-// one tile of one slot per workgroup.
+// One fused PCG pass, one tile of one slot per workgroup.
 template <typename TI, int ROWS>
-__global__ __launch_bounds__(kGroupThreads) void k_inner_apply(const SlotDev* __restrict__ slots,
-                                                               const SlotState* __restrict__ states,
-                                                               const Tile* __restrict__ tiles,
-                                                               const int* __restrict__ counts) {
+__global__ __launch_bounds__(kGroupThreads) void k_inner_step(const SlotDev* __restrict__ slots,
+                                                              const SlotState* __restrict__ states,
+                                                              const Tile* __restrict__ tiles,
+                                                              const int* __restrict__ counts) {
     // The grid may be longer than the table (a server sizes it for growth).
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= counts[1]) return;
     const Tile tile = tiles[t];
     const SlotState& st = states[tile.slot];
     if (st.phase != kPhaseInner) return;
-    inner_apply_slot<TI, ROWS, 0>(slots[tile.slot], st, tile);
+    inner_step_slot<TI, ROWS, 0>(slots[tile.slot], st, tile);
 }
 
-// The multi-right-hand-side block apply: one tile of a group of slots that
+// The multi-right-hand-side block pass: one tile of a group of slots that
 // share an operator per workgroup, the slots one after another, so the
 // group's coefficient, Robin and diagonal lines are fetched from HBM once and
 // served from the caches to the rest of the group; MFMA selects the
 // matrix-core element action.  Each slot keeps its own scalars (per-column
 // alpha and beta) and its own reduction cells.
 template <int ROWS, int MFMA>
-__global__ __launch_bounds__(kGroupThreads) void k_inner_apply_block(const SlotDev* __restrict__ slots,
-                                                                     const SlotState* __restrict__ states,
-                                                                     const Tile* __restrict__ tiles,
-                                                                     const int* __restrict__ counts,
-                                                                     const SlotGroup* __restrict__ groups) {
+__global__ __launch_bounds__(kGroupThreads) void k_inner_step_block(const SlotDev* __restrict__ slots,
+                                                                    const SlotState* __restrict__ states,
+                                                                    const Tile* __restrict__ tiles,
+                                                                    const int* __restrict__ counts,
+                                                                    const SlotGroup* __restrict__ groups) {
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= counts[3]) return;
     const Tile tile = tiles[t];
@@ -1011,123 +1074,75 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_apply_block(const SlotD
         const int slot = group.slot[k];
         const SlotState& st = states[slot];
         if (st.phase != kPhaseInner) continue;
-        inner_apply_slot<float, ROWS, MFMA>(slots[slot], st, tile);
+        inner_step_slot<float, ROWS, MFMA>(slots[slot], st, tile);
     }
 }
 
-// Curvature check and alpha = rz / p.q, one workgroup per slot.
+// The decisions of one pass, one workgroup per slot: the end test of
+// iteration i - 1 (hex_q1.cpp:469-489: tolerance, iteration limit, a zero or
+// non-finite r.z), then the curvature check and alpha_i of iteration i
+// (hex_q1.cpp:454-466) and beta_{i+1}.  A pass whose previous iteration ends
+// the inner solve was not an iteration: its apply is not counted and its
+// outputs (in the buffers that stay unused) are dropped; c already holds the
+// final correction.
 template <typename TI>
-__global__ __launch_bounds__(kGroupThreads) void k_finish_apply(const SlotDev* __restrict__ slots, SlotState* states,
-                                                                int* active) {
+__global__ __launch_bounds__(kFinishThreads) void k_finish_step(const SlotDev* __restrict__ slots, SlotState* states,
+                                                                const SlotConfig* __restrict__ configs, int* active) {
+    // As in k_finish_outer, the descriptor is read with the phase, and thread
+    // 0's scalars are fetched before the sum, so their latency hides behind it.
     const int slot = blockIdx.x;
     SlotState& st = states[slot];
-    if (st.phase != kPhaseInner) return;
     const SlotDev& sd = slots[slot];
-    double total[1];
-    slot_sum<1>(sd.part, sd.part_stride, sd.cells, total);
+    const double* part = sd.part;
+    const int stride = sd.part_stride, cells = sd.cells;
+    if (st.phase != kPhaseInner) return;
+    int first = 0, inner_its = 0;
+    double rhs_norm = 1.0, rz_prev = 0.0;
+    SlotConfig cfg{};
+    if (threadIdx.x == 0) {
+        first = st.first;
+        inner_its = st.inner_its;
+        rhs_norm = st.rhs_norm;
+        rz_prev = st.rz;
+        cfg = configs[slot];
+    }
+    double total[kInnerSums];
+    slot_sum<kInnerSums, kFinishThreads>(part, stride, cells, total);
     if (threadIdx.x != 0) return;
-    st.pcur = 1 - st.pcur;
-    st.first = 0;
+    const double pq = total[0], rz = total[1], qz = total[2], qdq = total[3], rr = total[4];
+    if (first == 0) {
+        inner_its += 1;
+        st.inner_its = inner_its;
+        const double rel = sqrt(rr) / rhs_norm;
+        st.rel_inner = rel;
+        const bool end = rel <= cfg.inner_tol || inner_its >= cfg.max_inner || !isfinite(rz) || rz_prev == 0.0;
+        if (end) {
+            double* h = sd.history + 4 * st.outer;
+            h[0] = st.outer + 1;
+            h[1] = st.rel_outer;
+            h[2] = inner_its;
+            h[3] = rel;
+            st.total_inner += inner_its;
+            st.outer += 1;
+            st.pending = kPendingCorrection;
+            st.phase = kPhaseOuter;
+            return;
+        }
+    }
     st.low_apps += 1;
-    const double curvature = total[0];
-    if (!isfinite(curvature) || curvature <= 0.0) {
+    if (!isfinite(pq) || pq <= 0.0) {
         st.status = kStatusNotSpd;
         st.phase = kPhaseDone;
         atomicSub(active, 1);
         return;
     }
-    st.alpha = (double)(TI)(st.rz / curvature);
-}
-
-// Inner step, part 2: c += alpha p, r -= alpha q, |r|^2 and r.z
-// (hex_q1.cpp:465-489).  The last workgroup tests the inner tolerance and
-// either ends the inner solve (the next outer kernel applies c) or forms
-// beta = r.z' / r.z.
-template <typename TI>
-__global__ __launch_bounds__(kGroupThreads) void k_inner_update(const SlotDev* __restrict__ slots,
-                                                                const SlotState* __restrict__ states,
-                                                                const Tile* __restrict__ tiles,
-                                                                const int* __restrict__ counts) {
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    if (t >= counts[2]) return;
-    const Tile tile = tiles[t];
-    const SlotState& st = states[tile.slot];
-    if (st.phase != kPhaseInner) return;
-    const SlotDev& sd = slots[tile.slot];
-    const TI alpha = (TI)st.alpha;
-    // The correction restarts from zero with each inner solve (weight 0).
-    const TI keep = st.inner_its == 0 ? TI(0) : TI(1);
-    // The chunk's buffers end at the last node, so loads past it read zero
-    // and stores past it are dropped: no bounds branch.
-    const long long base = (long long)tile.index * kFlatChunk;
-    const long long left = sd.op.N - base;
-    const int bytes = (int)((left < kFlatChunk ? left : kFlatChunk) * (long long)sizeof(TI));
-    const Rsrc rp = rsrc(static_cast<const TI*>(sd.p[st.pcur]) + base, bytes);
-    const Rsrc rq = rsrc(static_cast<const TI*>(sd.q) + base, bytes);
-    const Rsrc rr = rsrc(static_cast<TI*>(sd.r) + base, bytes);
-    const Rsrc rc = rsrc(static_cast<TI*>(sd.c) + base, bytes);
-    const Rsrc rd = rsrc(InnerOf<TI>::dinv(sd.op) + base, bytes);
-    TI pv[kFlatPerThread], qv[kFlatPerThread], rv[kFlatPerThread], cv[kFlatPerThread], dv[kFlatPerThread];
-#pragma unroll
-    for (int k = 0; k < kFlatPerThread; ++k) {
-        const unsigned off = (unsigned)(k * kGroupThreads) + threadIdx.x;
-        pv[k] = ld(rp, off, TI());
-        qv[k] = ld(rq, off, TI());
-        rv[k] = ld(rr, off, TI());
-        cv[k] = ld(rc, off, TI());
-        dv[k] = ld(rd, off, TI());
-    }
-    double acc[2] = {0.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < kFlatPerThread; ++k) {
-        const unsigned off = (unsigned)(k * kGroupThreads) + threadIdx.x;
-        const TI c = keep * cv[k] + alpha * pv[k];
-        const TI r = rv[k] - alpha * qv[k];
-        store(c, rc, off);
-        store(r, rr, off);
-        acc[0] += (double)r * (double)r;
-        acc[1] += (double)r * (double)(r * dv[k]);
-    }
-    group_partials<2>(acc, sd.part, sd.part_stride, tile.index);
-}
-
-// Inner tolerance test and beta, or the end of the inner solve, one
-// workgroup per slot (hex_q1.cpp:469-489).
-template <typename TI>
-__global__ __launch_bounds__(kGroupThreads) void k_finish_update(const SlotDev* __restrict__ slots, SlotState* states,
-                                                                 const SlotConfig* __restrict__ configs) {
-    const int slot = blockIdx.x;
-    SlotState& st = states[slot];
-    if (st.phase != kPhaseInner) return;
-    const SlotDev& sd = slots[slot];
-    double total[2];
-    slot_sum<2>(sd.part, sd.part_stride, sd.tiles_flat, total);
-    if (threadIdx.x != 0) return;
-    const SlotConfig cfg = configs[slot];
-    st.inner_its += 1;
-    const double rel = sqrt(total[0]) / st.rhs_norm;
-    st.rel_inner = rel;
-    bool end = rel <= cfg.inner_tol || st.inner_its >= cfg.max_inner;
-    if (!end) {
-        const double next_rz = total[1];
-        if (!isfinite(next_rz) || st.rz == 0.0) {
-            end = true;
-        } else {
-            st.beta = (double)(TI)(next_rz / st.rz);
-            st.rz = next_rz;
-        }
-    }
-    if (end) {
-        double* h = sd.history + 4 * st.outer;
-        h[0] = st.outer + 1;
-        h[1] = st.rel_outer;
-        h[2] = st.inner_its;
-        h[3] = rel;
-        st.total_inner += st.inner_its;
-        st.outer += 1;
-        st.pending = kPendingCorrection;
-        st.phase = kPhaseOuter;
-    }
+    const double alpha = (double)(TI)(rz / pq);
+    const double next_rz = rz - 2.0 * alpha * qz + alpha * alpha * qdq;
+    st.rz = rz;
+    st.alpha = alpha;
+    st.beta = rz != 0.0 ? (double)(TI)(next_rz / rz) : 0.0;
+    st.first = 0;
+    st.pcur = 1 - st.pcur;
 }
 
 // Writes fresh states of newly admitted slots of a server: one thread per
@@ -1241,51 +1256,49 @@ hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState
     const dim3 grid(grid_outer < kOuterGroups ? grid_outer : kOuterGroups);
     if (inner64) {
         hipLaunchKernelGGL(k_outer<double>, grid, block, 0, stream, slots, states, outer_tiles, counts);
-        hipLaunchKernelGGL(k_finish_outer<double>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
+        hipLaunchKernelGGL(k_finish_outer<double>, dim3(n_slots), dim3(kFinishThreads), 0, stream, slots, states,
+                           configs, active);
     } else {
         hipLaunchKernelGGL(k_outer<float>, grid, block, 0, stream, slots, states, outer_tiles, counts);
-        hipLaunchKernelGGL(k_finish_outer<float>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
+        hipLaunchKernelGGL(k_finish_outer<float>, dim3(n_slots), dim3(kFinishThreads), 0, stream, slots, states,
+                           configs, active);
     }
     return hipGetLastError();
 }
 
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
-                      const SlotConfig* configs, const Tile* inner_tiles, const Tile* flat_tiles, const int* counts,
-                      int grid_inner, int grid_flat, const Tile* block_tiles, const SlotGroup* groups,
-                      int grid_block, int block_mfma, int* active, hipStream_t stream) {
+                      const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner,
+                      const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma, int* active,
+                      hipStream_t stream) {
     const dim3 block(kGroupThreads);
     if (inner64) {
-        hipLaunchKernelGGL((k_inner_apply<double, kRowsInner64>), dim3(grid_inner), block, 0, stream, slots, states,
+        hipLaunchKernelGGL((k_inner_step<double, kRowsInner64>), dim3(grid_inner), block, 0, stream, slots, states,
                            inner_tiles, counts);
-        hipLaunchKernelGGL(k_finish_apply<double>, dim3(n_slots), block, 0, stream, slots, states, active);
-        hipLaunchKernelGGL(k_inner_update<double>, dim3(grid_flat), block, 0, stream, slots, states, flat_tiles,
-                           counts);
-        hipLaunchKernelGGL(k_finish_update<double>, dim3(n_slots), block, 0, stream, slots, states, configs);
-    } else {
-        if (rows_inner == 8) {
-            hipLaunchKernelGGL((k_inner_apply<float, 8>), dim3(grid_inner), block, 0, stream, slots, states,
-                               inner_tiles, counts);
-        } else {
-            hipLaunchKernelGGL((k_inner_apply<float, 4>), dim3(grid_inner), block, 0, stream, slots, states,
-                               inner_tiles, counts);
-        }
-        if (grid_block > 0) {
-            if (block_mfma == 3) {
-                hipLaunchKernelGGL((k_inner_apply_block<4, 3>), dim3(grid_block), block, 0, stream, slots, states,
-                                   block_tiles, counts, groups);
-            } else if (block_mfma == 2) {
-                hipLaunchKernelGGL((k_inner_apply_block<4, 2>), dim3(grid_block), block, 0, stream, slots, states,
-                                   block_tiles, counts, groups);
-            } else {
-                hipLaunchKernelGGL((k_inner_apply_block<4, 0>), dim3(grid_block), block, 0, stream, slots, states,
-                                   block_tiles, counts, groups);
-            }
-        }
-        hipLaunchKernelGGL(k_finish_apply<float>, dim3(n_slots), block, 0, stream, slots, states, active);
-        hipLaunchKernelGGL(k_inner_update<float>, dim3(grid_flat), block, 0, stream, slots, states, flat_tiles,
-                           counts);
-        hipLaunchKernelGGL(k_finish_update<float>, dim3(n_slots), block, 0, stream, slots, states, configs);
+        hipLaunchKernelGGL(k_finish_step<double>, dim3(n_slots), dim3(kFinishThreads), 0, stream, slots, states,
+                           configs, active);
+        return hipGetLastError();
     }
+    if (rows_inner == 8) {
+        hipLaunchKernelGGL((k_inner_step<float, 8>), dim3(grid_inner), block, 0, stream, slots, states, inner_tiles,
+                           counts);
+    } else {
+        hipLaunchKernelGGL((k_inner_step<float, 4>), dim3(grid_inner), block, 0, stream, slots, states, inner_tiles,
+                           counts);
+    }
+    if (grid_block > 0) {
+        if (block_mfma == 3) {
+            hipLaunchKernelGGL((k_inner_step_block<4, 3>), dim3(grid_block), block, 0, stream, slots, states,
+                               block_tiles, counts, groups);
+        } else if (block_mfma == 2) {
+            hipLaunchKernelGGL((k_inner_step_block<4, 2>), dim3(grid_block), block, 0, stream, slots, states,
+                               block_tiles, counts, groups);
+        } else {
+            hipLaunchKernelGGL((k_inner_step_block<4, 0>), dim3(grid_block), block, 0, stream, slots, states,
+                               block_tiles, counts, groups);
+        }
+    }
+    hipLaunchKernelGGL(k_finish_step<float>, dim3(n_slots), dim3(kFinishThreads), 0, stream, slots, states, configs,
+                       active);
     return hipGetLastError();
 }
 
