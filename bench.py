@@ -1,6 +1,7 @@
 """Benchmark: MPIR solves/s at 1e-10 relative residual on the 512 x 512 x 16
-board, Q1 apply GDoF/s, per-solve latency, the roofline of the Q1 apply and
-the reference algorithm timed on the host's cores.
+board, Q1 apply GDoF/s, per-solve latency, the roofline of the solve's
+dominant kernel (the fused Jacobi-PCG pass) and of the Q1 apply, and the
+reference algorithm timed on the host's cores.
 
 One step is one complete MPIR solve (FP64 outer residual, update and
 convergence check; FP32 Jacobi-PCG inner solves) of the seeded 512 x 512 x 16
@@ -74,6 +75,41 @@ def profiled_traffic(element_shape, rows: int, zb: int):
         if line.startswith(f"traffic_bytes rows={rows} zb={zb}:"):
             return int(line.split(":", 1)[1].split("#")[0].strip())
     return None
+
+
+def profiled_pass_traffic(element_shape, rows: int, target: int):
+    """HBM bytes per launch of the fused PCG pass at this tile shape, from the
+    committed rocprofv3 summary of the MPIR solve, or None."""
+
+    s, r, c = element_shape
+    path = ROOT / "profiles" / f"q1hex_pcg_pass_board_{c}x{r}x{s}.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith(f"traffic_bytes pass rows={rows} target={target}:"):
+            return int(line.split(":", 1)[1].split("#")[0].strip())
+    return None
+
+
+def pass_roofline(operator, batch, rows: int, target: int, launches: int = 100) -> dict:
+    """The dominant kernel of the solve, the fused Jacobi-PCG pass, timed with
+    HIP events on the batch's stream (``pcbhip_batch_time_pass``)."""
+
+    n = operator.size
+    s, r, c = operator.mesh.element_grid_shape
+    e = s * r * c
+    per_us = batch.time_pass(launches)
+    # Reads r, q, p, 1/diag, Robin and c (4 B/node each) and the coefficients
+    # (12 B/element); writes p, q, r and c (4 B/node each).
+    algorithmic = 40 * n + 12 * e
+    achieved = algorithmic / (per_us * 1e-6) / 1e9
+    traffic = profiled_pass_traffic(operator.mesh.element_grid_shape, rows, target)
+    return dict(
+        kernel=f"k_inner_step<float,{rows}> (fused FP32 Jacobi-PCG pass), target {target} workgroups",
+        per_launch_us=per_us,
+        roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                      traffic=traffic, algorithmic_bytes=algorithmic),
+    )
 
 
 def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
@@ -494,6 +530,7 @@ def main() -> int:
     parser.add_argument("--slabs", type=int, default=16)
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--rows", type=int, default=4, help="rows per wave strip of the FP32 inner stencil")
+    parser.add_argument("--target", type=int, default=1024, help="stencil workgroups per launch before z blocks split")
     parser.add_argument("--workload", default="board512", choices=["board512", "batch200", "multirhs64", "serve", "apply_engines"])
     parser.add_argument("--rates", type=float, nargs="+", default=[50.0, 100.0, 200.0, 400.0])
     parser.add_argument("--duration", type=float, default=10.0)
@@ -539,7 +576,7 @@ def main() -> int:
     rhs = operator.build_rhs(reference)
     config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000)
     batch = _capi.Batch([operator.device_operator])
-    batch.set_shape(args.rows, 1024)
+    batch.set_shape(args.rows, args.target)
     batch.set(0, rhs, None, config.to_c())
 
     log("warmup")
@@ -558,6 +595,8 @@ def main() -> int:
     log(f"solve: {result['outer_iterations']} outer / {result['inner_iterations']} inner, "
         f"rel {result['relative_residual']:.2e}, {elapsed / args.steps * 1e3:.1f} ms/solve")
 
+    fused = pass_roofline(operator, batch, args.rows, args.target)
+    log(f"pcg pass: {fused['per_launch_us']:.1f} us, {fused['roofline']['achieved']:.0f} GB/s")
     stream = torch.cuda.Stream()
     roof = apply_roofline(operator, torch, stream)
     log(f"apply: {roof['per_launch_us']:.1f} us, {roof['roofline']['achieved']:.0f} GB/s")
@@ -589,10 +628,13 @@ def main() -> int:
             "apply_us": roof["per_launch_us"],
             "apply_kernel": roof["kernel"],
             "apply_sweep": roof["sweep"],
+            "apply_roofline": roof["roofline"],
+            "pass_kernel": fused["kernel"],
+            "pass_us": fused["per_launch_us"],
             "stream_copy_gbs": copy_gbs,
             "parallelism": "single GPU",
         },
-        "roofline": roof["roofline"],
+        "roofline": fused["roofline"],
     }
     if not args.no_cpu:
         log("cpu baseline")

@@ -163,6 +163,12 @@ int pcbhip_batch_rhs_dev(pcbhip_batch *batch, int slot, double **rhs_dev);
 int pcbhip_batch_run(pcbhip_batch *batch);
 /* Solution (host, may be NULL), result and history (NULL or 4 * max_outer). */
 int pcbhip_batch_result(pcbhip_batch *batch, int slot, double *x, pcbhip_mpir_result *result, double *history);
+/* Average device time, in microseconds, of one fused Jacobi-PCG pass over
+ * every slot of the batch at its current tile shape: `launches` passes back
+ * to back between two HIP events on the batch's stream (alpha = beta = 0, so
+ * the vectors stay bounded).  Slot states and results are kept; call it
+ * after pcbhip_batch_run.  A measurement aid with no reference counterpart. */
+int pcbhip_batch_time_pass(pcbhip_batch *batch, int launches, double *avg_us);
 
 /* Device bytes an operator handle holds (coefficients, Robin terms and
  * Jacobi inverses in both precisions): the admission footprint of a served

@@ -1284,6 +1284,42 @@ int pcbhip_batch_run(pcbhip_batch* b) {
     });
 }
 
+int pcbhip_batch_time_pass(pcbhip_batch* b, int launches, double* avg_us) {
+    return guarded([&] {
+        if (!b || !avg_us || launches < 1) fail(PCBHIP_INVALID_INPUT, "time_pass needs a batch and launches >= 1");
+        if (b->configs_dirty) fail(PCBHIP_INVALID_INPUT, "run the batch before timing its pass");
+        set_device(b->device);
+        const int count = (int)b->slots.size();
+        std::vector<SlotState> saved((size_t)count);
+        check(hipMemcpy(saved.data(), b->d_states, sizeof(SlotState) * count, hipMemcpyDeviceToHost), "copy states");
+        std::vector<SlotState> timing = saved;
+        for (SlotState& st : timing) {
+            st.phase = kPhaseInner;
+            st.first = 0;
+            st.alpha = 0.0;
+            st.beta = 0.0;
+        }
+        check(hipMemcpy(b->d_states, timing.data(), sizeof(SlotState) * count, hipMemcpyHostToDevice), "states");
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        float ms = 0.0f;
+        hipError_t err = hipEventCreate(&ev[0]);
+        if (err == hipSuccess) err = hipEventCreate(&ev[1]);
+        if (err == hipSuccess) err = hipEventRecord(ev[0], b->stream);
+        for (int k = 0; k < launches && err == hipSuccess; ++k)
+            err = launch::inner_pass(b->inner64 != 0, b->rows_inner, b->d_slots, b->d_states, b->d_inner, b->d_counts,
+                                     std::max(b->n_inner, 1), b->d_block, b->d_groups, b->n_block, b->block_mfma,
+                                     b->stream);
+        if (err == hipSuccess) err = hipEventRecord(ev[1], b->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(ev[1]);
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        check(err, "timed passes");
+        check(hipMemcpy(b->d_states, saved.data(), sizeof(SlotState) * count, hipMemcpyHostToDevice), "restore states");
+        *avg_us = 1e3 * (double)ms / launches;
+    });
+}
+
 int pcbhip_batch_result(pcbhip_batch* b, int slot, double* x, pcbhip_mpir_result* result, double* history) {
     return guarded([&] {
         if (!b || slot < 0 || slot >= (int)b->slots.size()) fail(PCBHIP_INVALID_INPUT, "slot out of range");

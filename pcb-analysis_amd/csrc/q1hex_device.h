@@ -181,6 +181,10 @@ hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* 
                       const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner,
                       const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma, int* active,
                       hipStream_t stream);
+// The fused PCG pass alone (inner_step without its finish kernel).
+hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const SlotState* states,
+                      const Tile* inner_tiles, const int* counts, int grid_inner, const Tile* block_tiles,
+                      const SlotGroup* groups, int grid_block, int block_mfma, hipStream_t stream);
 // states[index[i]] = fresh[i] for i < count (device arrays).
 hipError_t scatter_states(const int* index, const SlotState* fresh, SlotState* states, int count, hipStream_t stream);
 }  // namespace launch

@@ -250,11 +250,14 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
     [[maybe_unused]] unsigned qlo[ROWS + 2][4];
     if constexpr (MFMA != 0) unit = unit_operand();
 
+    // Planes zlo - 1 and zhi are loaded for the elements between them and the
+    // block's output planes but belong to neighbouring z blocks.
     auto load_plane = [&](const int z, T (&m)[ROWS + 2], T (&n)[ROWS + 2], V (&v)[ROWS]) {
         const long long pb = (long long)z * NR * NC;
+        const bool out_plane = z >= zlo && z < zhi;
 #pragma unroll
         for (int i = 0; i < ROWS + 2; ++i) {
-            const bool owned = out_lane && i >= 1 && i <= ROWS && y0 - 1 + i <= R;
+            const bool owned = out_plane && out_lane && i >= 1 && i <= ROWS && y0 - 1 + i <= R;
             const V nv = load(pb, noff[i], owned);
             m[i] = masked(nv);
             if (i >= 1 && i <= ROWS) v[i - 1] = nv;
@@ -1266,16 +1269,13 @@ hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState
     return hipGetLastError();
 }
 
-hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
-                      const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner,
-                      const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma, int* active,
-                      hipStream_t stream) {
+hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const SlotState* states,
+                      const Tile* inner_tiles, const int* counts, int grid_inner, const Tile* block_tiles,
+                      const SlotGroup* groups, int grid_block, int block_mfma, hipStream_t stream) {
     const dim3 block(kGroupThreads);
     if (inner64) {
         hipLaunchKernelGGL((k_inner_step<double, kRowsInner64>), dim3(grid_inner), block, 0, stream, slots, states,
                            inner_tiles, counts);
-        hipLaunchKernelGGL(k_finish_step<double>, dim3(n_slots), dim3(kFinishThreads), 0, stream, slots, states,
-                           configs, active);
         return hipGetLastError();
     }
     if (rows_inner == 8) {
@@ -1297,8 +1297,22 @@ hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* 
                                block_tiles, counts, groups);
         }
     }
-    hipLaunchKernelGGL(k_finish_step<float>, dim3(n_slots), dim3(kFinishThreads), 0, stream, slots, states, configs,
-                       active);
+    return hipGetLastError();
+}
+
+hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
+                      const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner,
+                      const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma, int* active,
+                      hipStream_t stream) {
+    const hipError_t err = inner_pass(inner64, rows_inner, slots, states, inner_tiles, counts, grid_inner,
+                                      block_tiles, groups, grid_block, block_mfma, stream);
+    if (err != hipSuccess) return err;
+    const dim3 block(kFinishThreads);
+    if (inner64) {
+        hipLaunchKernelGGL(k_finish_step<double>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
+    } else {
+        hipLaunchKernelGGL(k_finish_step<float>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
+    }
     return hipGetLastError();
 }
 

@@ -56,6 +56,7 @@ EXPORTS = (
     "pcbhip_batch_rhs_dev",
     "pcbhip_batch_run",
     "pcbhip_batch_result",
+    "pcbhip_batch_time_pass",
     "pcbhip_operator_bytes",
     "pcbhip_server_create",
     "pcbhip_server_destroy",
@@ -147,6 +148,7 @@ def load() -> ctypes.CDLL:
     lib.pcbhip_batch_set_block.argtypes = [_vp, _int, _int]
     lib.pcbhip_batch_run.argtypes = [_vp]
     lib.pcbhip_batch_result.argtypes = [_vp, _int, _vp, ctypes.POINTER(MpirResultC), _vp]
+    lib.pcbhip_batch_time_pass.argtypes = [_vp, _int, ctypes.POINTER(ctypes.c_double)]
     lib.pcbhip_copy_dev.argtypes = [_vp, _vp, ctypes.c_int64, _vp]
     _i64p = ctypes.POINTER(ctypes.c_int64)
     lib.pcbhip_operator_bytes.argtypes = [_vp, _i64p]
@@ -372,6 +374,14 @@ class Batch:
 
     def run(self) -> None:
         check(self._lib.pcbhip_batch_run(self.handle))
+
+    def time_pass(self, launches: int = 100) -> float:
+        """Average device microseconds of one fused PCG pass over every slot
+        (``pcbhip_batch_time_pass``); call after ``run``."""
+
+        us = ctypes.c_double()
+        check(self._lib.pcbhip_batch_time_pass(self.handle, int(launches), ctypes.byref(us)))
+        return us.value
 
     def result(self, slot: int, *, solution: bool = True):
         op = self.operators[slot]

@@ -34,3 +34,9 @@ def test_more_than_one_gpu_is_refused():
                           timeout=300, cwd=ROOT)
     assert proc.returncode != 0
     assert "one GPU" in (proc.stdout + proc.stderr)
+
+
+def test_profile_summary_has_traffic_for_the_default_pcg_pass():
+    traffic = bench.profiled_pass_traffic((16, 512, 512), 4, 1024)
+    assert isinstance(traffic, int) and 200e6 < traffic < 400e6, traffic
+    assert bench.profiled_pass_traffic((16, 512, 512), 8, 1024) is None
