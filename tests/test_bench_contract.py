@@ -1,7 +1,8 @@
 """The benchmark's host-side contract on the CPU: the committed profile
-summary holds an HBM traffic figure for every tile shape the autotuner may
-pick on the 512 x 512 x 16 board, and a multi-GPU request is refused with a
-message (one GPU per run)."""
+summaries hold an HBM traffic figure for every tile shape the autotuner may
+pick for the apply on the 512 x 512 x 16 board and for the default shape of
+the fused PCG pass, and a multi-GPU request is refused with a message (one
+GPU per run)."""
 
 from __future__ import annotations
 
@@ -37,6 +38,7 @@ def test_more_than_one_gpu_is_refused():
 
 
 def test_profile_summary_has_traffic_for_the_default_pcg_pass():
+    bench = _bench()
     traffic = bench.profiled_pass_traffic((16, 512, 512), 4, 1024)
     assert isinstance(traffic, int) and 200e6 < traffic < 400e6, traffic
     assert bench.profiled_pass_traffic((16, 512, 512), 8, 1024) is None
