@@ -310,14 +310,15 @@ void carve_slot(SlotHost& s, char* base, int inner64) {
     s.part = reinterpret_cast<double*>(base + 7 * nti);
 }
 
-// Tiles of one slot: stencil workgroups in (z block, y group, x strip)
-// order, so a contiguous range of tiles is a compact piece of the board.
+// Tiles of one slot: stencil workgroups in (y group, x strip, z block)
+// order, so a contiguous range of tiles is a compact piece of the board and
+// the z blocks of one column, which share a node plane, run side by side.
 int add_tiles(std::vector<Tile>& out, int slot, int nsx, int nsy, int S, int zb) {
     const int nsz = (S + 1 + zb - 1) / zb;
     int index = 0;
-    for (int sz = 0; sz < nsz; ++sz)
-        for (int sy = 0; sy < nsy; ++sy)
-            for (int sx = 0; sx < nsx; ++sx)
+    for (int sy = 0; sy < nsy; ++sy)
+        for (int sx = 0; sx < nsx; ++sx)
+            for (int sz = 0; sz < nsz; ++sz)
                 out.push_back(Tile{slot, index++, (short)sx, (short)sy, (short)sz, (short)zb});
     return index;
 }
@@ -516,8 +517,8 @@ void build_graph(pcbhip_batch* b, int steps) {
                                      b->d_counts, b->n_outer, b->d_active, b->stream);
         if (err == hipSuccess)
             err = launch::inner_step(b->inner64 != 0, b->rows_inner, n_slots, b->d_slots, b->d_states, b->d_configs,
-                                     b->d_inner, b->d_counts, std::max(b->n_inner, 1), b->d_block, b->d_groups,
-                                     b->n_block, b->block_mfma, b->d_active, b->stream);
+                                     b->d_inner, b->d_counts, std::max(b->n_inner, 1), false, b->d_block,
+                                     b->d_groups, b->n_block, b->block_mfma, b->d_active, b->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(b->stream, &dropped);
@@ -681,8 +682,8 @@ void server_build_graph(pcbhip_server* sv) {
                                      outer, sv->d_counts, sv->cap_outer, sv->d_active, sv->stream);
         if (err == hipSuccess)
             err = launch::inner_step(sv->inner64 != 0, sv->rows_inner, sv->max_slots, sv->d_slots, sv->d_states,
-                                     sv->d_configs, inner, sv->d_counts, sv->cap_inner, nullptr, nullptr, 0, 0,
-                                     sv->d_active, sv->stream);
+                                     sv->d_configs, inner, sv->d_counts, std::min(sv->cap_inner, kPassGroups), true,
+                                     nullptr, nullptr, 0, 0, sv->d_active, sv->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(sv->stream, &dropped);
@@ -1307,8 +1308,8 @@ int pcbhip_batch_time_pass(pcbhip_batch* b, int launches, double* avg_us) {
         if (err == hipSuccess) err = hipEventRecord(ev[0], b->stream);
         for (int k = 0; k < launches && err == hipSuccess; ++k)
             err = launch::inner_pass(b->inner64 != 0, b->rows_inner, b->d_slots, b->d_states, b->d_inner, b->d_counts,
-                                     std::max(b->n_inner, 1), b->d_block, b->d_groups, b->n_block, b->block_mfma,
-                                     b->stream);
+                                     std::max(b->n_inner, 1), false, b->d_block, b->d_groups, b->n_block,
+                                     b->block_mfma, b->stream);
         if (err == hipSuccess) err = hipEventRecord(ev[1], b->stream);
         if (err == hipSuccess) err = hipEventSynchronize(ev[1]);
         if (err == hipSuccess) err = hipEventElapsedTime(&ms, ev[0], ev[1]);

@@ -35,6 +35,9 @@ constexpr int kRowsInner64 = 4;
 constexpr int kTargetGroups = 1024;
 // Workgroups of the persistent outer-residual kernel.
 constexpr int kOuterGroups = 512;
+// Workgroups a server launches for the fused PCG pass over its tile table
+// (two per CU; the pass loops over the live entries).
+constexpr int kPassGroups = 512;
 // Threads of the one-workgroup-per-slot finish kernels.
 constexpr int kFinishThreads = 512;
 
@@ -171,19 +174,21 @@ hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t st
 // One outer step (FP64 residual and decision) for the slots in their outer
 // phase, and one fused Jacobi-PCG pass for the slots in their inner phase.
 // ``counts`` (device) holds the live lengths of the outer, inner and block
-// tile tables (entries 0, 1 and 3); the grids may be longer (workgroups past
-// the count return at once), so a captured graph survives changes of the
-// tables.  Block tiles (a slot group per tile, ``groups``) are launched when
-// grid_block > 0, with the element action block_mfma (0, 2, 3 as in apply).
+// tile tables (entries 0, 1 and 3).  The grids may be longer than the live
+// tables (workgroups past the count return at once), so a captured graph
+// survives changes of the tables; with ``loop`` the inner grid may also be
+// shorter and its workgroups loop over the live inner tiles.  Block tiles (a
+// slot group per tile, ``groups``) are launched when grid_block > 0, with
+// the element action block_mfma (0, 2, 3 as in apply).
 hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
                       const Tile* outer_tiles, const int* counts, int grid_outer, int* active, hipStream_t stream);
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
-                      const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner,
+                      const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner, bool loop,
                       const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma, int* active,
                       hipStream_t stream);
 // The fused PCG pass alone (inner_step without its finish kernel).
 hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const SlotState* states,
-                      const Tile* inner_tiles, const int* counts, int grid_inner, const Tile* block_tiles,
+                      const Tile* inner_tiles, const int* counts, int grid_inner, bool loop, const Tile* block_tiles,
                       const SlotGroup* groups, int grid_block, int block_mfma, hipStream_t stream);
 // states[index[i]] = fresh[i] for i < count (device arrays).
 hipError_t scatter_states(const int* index, const SlotState* fresh, SlotState* states, int count, hipStream_t stream);

@@ -1042,19 +1042,31 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
     acc.flush_hex(zlo, zhi, tile.sy * kWavesPerGroup * ROWS, tile.sx, op.R, op.C, sd.part, sd.part_stride);
 }
 
-// One fused PCG pass, one tile of one slot per workgroup.
-template <typename TI, int ROWS>
+// One fused PCG pass, one tile of one slot per workgroup.  LOOP: the grid
+// is shorter than the table and loops over its live entries (a server
+// launches a fixed grid over a table that grows and shrinks, and spends no
+// dispatch on empty entries); otherwise the grid is the table.
+template <typename TI, int ROWS, bool LOOP>
 __global__ __launch_bounds__(kGroupThreads) void k_inner_step(const SlotDev* __restrict__ slots,
                                                               const SlotState* __restrict__ states,
                                                               const Tile* __restrict__ tiles,
                                                               const int* __restrict__ counts) {
-    // The grid may be longer than the table (a server sizes it for growth).
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    if (t >= counts[1]) return;
-    const Tile tile = tiles[t];
-    const SlotState& st = states[tile.slot];
-    if (st.phase != kPhaseInner) return;
-    inner_step_slot<TI, ROWS, 0>(slots[tile.slot], st, tile);
+    const int count = counts[1];
+    if constexpr (LOOP) {
+        for (int i = blockIdx.x; i < count; i += gridDim.x) {
+            const Tile tile = tiles[xcd_tile(i, count)];
+            const SlotState& st = states[tile.slot];
+            if (st.phase != kPhaseInner) continue;
+            inner_step_slot<TI, ROWS, 0>(slots[tile.slot], st, tile);
+        }
+    } else {
+        const int t = xcd_tile(blockIdx.x, gridDim.x);
+        if (t >= count) return;
+        const Tile tile = tiles[t];
+        const SlotState& st = states[tile.slot];
+        if (st.phase != kPhaseInner) return;
+        inner_step_slot<TI, ROWS, 0>(slots[tile.slot], st, tile);
+    }
 }
 
 // The multi-right-hand-side block pass: one tile of a group of slots that
@@ -1270,20 +1282,33 @@ hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState
 }
 
 hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const SlotState* states,
-                      const Tile* inner_tiles, const int* counts, int grid_inner, const Tile* block_tiles,
+                      const Tile* inner_tiles, const int* counts, int grid_inner, bool loop, const Tile* block_tiles,
                       const SlotGroup* groups, int grid_block, int block_mfma, hipStream_t stream) {
     const dim3 block(kGroupThreads);
+    const dim3 grid(grid_inner);
     if (inner64) {
-        hipLaunchKernelGGL((k_inner_step<double, kRowsInner64>), dim3(grid_inner), block, 0, stream, slots, states,
-                           inner_tiles, counts);
+        if (loop)
+            hipLaunchKernelGGL((k_inner_step<double, kRowsInner64, true>), grid, block, 0, stream, slots, states,
+                               inner_tiles, counts);
+        else
+            hipLaunchKernelGGL((k_inner_step<double, kRowsInner64, false>), grid, block, 0, stream, slots, states,
+                               inner_tiles, counts);
         return hipGetLastError();
     }
     if (rows_inner == 8) {
-        hipLaunchKernelGGL((k_inner_step<float, 8>), dim3(grid_inner), block, 0, stream, slots, states, inner_tiles,
-                           counts);
+        if (loop)
+            hipLaunchKernelGGL((k_inner_step<float, 8, true>), grid, block, 0, stream, slots, states, inner_tiles,
+                               counts);
+        else
+            hipLaunchKernelGGL((k_inner_step<float, 8, false>), grid, block, 0, stream, slots, states, inner_tiles,
+                               counts);
     } else {
-        hipLaunchKernelGGL((k_inner_step<float, 4>), dim3(grid_inner), block, 0, stream, slots, states, inner_tiles,
-                           counts);
+        if (loop)
+            hipLaunchKernelGGL((k_inner_step<float, 4, true>), grid, block, 0, stream, slots, states, inner_tiles,
+                               counts);
+        else
+            hipLaunchKernelGGL((k_inner_step<float, 4, false>), grid, block, 0, stream, slots, states, inner_tiles,
+                               counts);
     }
     if (grid_block > 0) {
         if (block_mfma == 3) {
@@ -1301,10 +1326,10 @@ hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const 
 }
 
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
-                      const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner,
+                      const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner, bool loop,
                       const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma, int* active,
                       hipStream_t stream) {
-    const hipError_t err = inner_pass(inner64, rows_inner, slots, states, inner_tiles, counts, grid_inner,
+    const hipError_t err = inner_pass(inner64, rows_inner, slots, states, inner_tiles, counts, grid_inner, loop,
                                       block_tiles, groups, grid_block, block_mfma, stream);
     if (err != hipSuccess) return err;
     const dim3 block(kFinishThreads);
