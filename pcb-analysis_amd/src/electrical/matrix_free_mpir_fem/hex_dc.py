@@ -30,7 +30,7 @@ from thermal.matrix_free_mpir_fem.mesh import LayeredThermalMesh, corner_views, 
 
 from .pcb import CurrentTerminal, VoltageTerminal
 from .runtime import make_float32_runtime
-from .solver import MPIRConfig, MPIRResult, MPIRStep, solve_mpir
+from .solver import MPIRConfig, MPIRResult, MPIRStep, jacobi_default_config, solve_mpir
 
 Node = tuple[int, int, int]
 
@@ -193,7 +193,7 @@ def solve_hex_dc(problem: HexConductionProblem, *, config: MPIRConfig | None = N
     if initial_potential_v is not None:
         initial = np.nan_to_num(np.asarray(initial_potential_v, dtype=np.float64).reshape(-1), nan=0.0)
         initial = np.where(operator.free_nodes.reshape(-1), initial, fixed)
-    result = solve_mpir(operator, rhs, config=config or MPIRConfig(max_outer_iterations=16), initial_guess=initial)
+    result = solve_mpir(operator, rhs, config=config or jacobi_default_config(16), initial_guess=initial)
     potential = result.solution.reshape(problem.mesh.node_shape)
     losses = operator.element_joule_loss(potential)
     return HexConductionSolution(
@@ -213,7 +213,7 @@ def solve_hex_dc_many(mesh: LayeredThermalMesh, excitations: Sequence[Sequence[C
 
     for terminals in excitations:
         _check_terminals(tuple(terminals), (), reference_node)
-    config = config or MPIRConfig(max_outer_iterations=16)
+    config = config or jacobi_default_config(16)
     operator = MatrixFreeHexDCOperator(mesh, reference_node=reference_node, backend=backend, device_id=device_id)
     batch = _capi.Batch([operator.device_operator] * len(excitations))
     for i, terminals in enumerate(excitations):

@@ -24,7 +24,7 @@ from pcbhip import _capi
 
 from .grid import check_pitch_axis
 from .runtime import make_float32_runtime
-from .solver import MPIRConfig, MPIRResult, solve_mpir
+from .solver import MPIRConfig, MPIRResult, jacobi_default_config, solve_mpir
 
 Node = tuple[int, int, int]
 Preconditioner = Literal["two-level", "jacobi"]
@@ -444,7 +444,7 @@ def solve_pcb_dc(problem: PCBConductionProblem, *, config: MPIRConfig | None = N
         initial = np.where(operator.free_nodes.reshape(-1), initial, fixed)
     elif problem.voltage_terminals:
         initial = fixed
-    result = solve_mpir(operator, rhs, config=config, initial_guess=initial)
+    result = solve_mpir(operator, rhs, config=config or jacobi_default_config(), initial_guess=initial)
     potential = result.solution.reshape(problem.mesh.node_shape)
     field = operator.element_electric_field(potential)
     density = problem.mesh.conductivity_s_per_m[..., None] * field

@@ -66,6 +66,20 @@ class MPIRConfig:
                               self.max_outer_iterations, self.max_inner_iterations)
 
 
+# The reference's default limits (8 outer, 200 inner iterations) are sized for
+# its two-level preconditioner (two_level.py:54-229).  Jacobi, the device
+# path's preconditioner, needs longer inner solves to reach the same 2e-3
+# inner tolerance, so the front ends give it this inner budget when the
+# caller passes no configuration.
+JACOBI_MAX_INNER_ITERATIONS = 20000
+
+
+def jacobi_default_config(max_outer_iterations: int = 8) -> MPIRConfig:
+    """The default limits of a front end for the Jacobi-preconditioned device solve."""
+
+    return MPIRConfig(max_outer_iterations=max_outer_iterations, max_inner_iterations=JACOBI_MAX_INNER_ITERATIONS)
+
+
 @dataclass(frozen=True)
 class MPIRStep:
     outer_iteration: int

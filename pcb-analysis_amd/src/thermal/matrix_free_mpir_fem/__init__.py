@@ -4,6 +4,7 @@ iterative refinement on an MI355X (the reference's
 reference files each one restates)."""
 
 from .boundaries import Convection, ConvectionBoundary, ExposedFaceConvection, HeatSource
+from .coupling import element_heat_from_losses, element_joule_heat_w, via_heat_sources, via_joule_heat_sources
 from .mesh import (
     COPPER_THERMAL_CONDUCTIVITY_W_PER_M_K,
     FACE_DIRECTIONS,
@@ -47,6 +48,8 @@ __all__ = [
     "TimeSchedule",
     "TransientStep",
     "TransientThermalSolution",
+    "element_heat_from_losses",
+    "element_joule_heat_w",
     "element_temperature_k",
     "exposed_element_faces",
     "face_temperature_k",
@@ -54,4 +57,6 @@ __all__ = [
     "solve_thermal_conduction",
     "solve_thermal_transient",
     "unit_hexahedron_matrices",
+    "via_heat_sources",
+    "via_joule_heat_sources",
 ]
