@@ -25,7 +25,7 @@ from typing import Sequence
 
 import numpy as np
 
-from pcbhip import _capi
+from pcbhip import _capi, serving
 from thermal.matrix_free_mpir_fem.mesh import LayeredThermalMesh, corner_views, flat_index, unit_hexahedron_matrices
 
 from .pcb import CurrentTerminal, VoltageTerminal
@@ -127,7 +127,7 @@ class MatrixFreeHexDCOperator:
         return self._device.diagonal().astype(np.float32)
 
     def native_solve_mpir(self, rhs, config: MPIRConfig, initial_guess=None):
-        x, r = _capi.solve_mpir(self._device, rhs, initial_guess, config.to_c())
+        x, r = serving.solve_mpir_routed(self._device, rhs, initial_guess, config.to_c())
         return (x, r["converged"], r["outer_iterations"], r["inner_iterations"], r["relative_residual"],
                 r["high_operator_applications"], r["low_operator_applications"], r["history"])
 

@@ -20,7 +20,7 @@ from typing import Literal, Sequence
 
 import numpy as np
 
-from pcbhip import _capi
+from pcbhip import _capi, serving
 
 from .grid import check_pitch_axis
 from .runtime import make_float32_runtime
@@ -331,7 +331,7 @@ class MatrixFreePCBOperator:
         return self._device.diagonal().astype(np.float32)
 
     def native_solve_mpir(self, rhs, config: MPIRConfig, initial_guess=None):
-        x, r = _capi.solve_mpir(self._device, rhs, initial_guess, config.to_c())
+        x, r = serving.solve_mpir_routed(self._device, rhs, initial_guess, config.to_c())
         return (x, r["converged"], r["outer_iterations"], r["inner_iterations"], r["relative_residual"],
                 r["high_operator_applications"], r["low_operator_applications"], r["history"])
 

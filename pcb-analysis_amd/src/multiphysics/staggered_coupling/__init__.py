@@ -12,6 +12,7 @@ from .electro_thermal import (
     electrical_layer_temperature_k,
     heated_electrical_problem,
     run_electro_thermal,
+    run_electro_thermal_many,
     slab_element_temperature_k,
     thermal_problem_with_joule_heat,
     via_node_temperature_k,
@@ -29,6 +30,7 @@ __all__ = [
     "electrical_layer_temperature_k",
     "heated_electrical_problem",
     "run_electro_thermal",
+    "run_electro_thermal_many",
     "slab_element_temperature_k",
     "thermal_problem_with_joule_heat",
     "via_node_temperature_k",

@@ -16,13 +16,17 @@ away; such scenarios still run, with a DeprecationWarning, as in the
 reference.
 
 Both solves of every iteration are device MPIR solves; the loop itself is
-host work of a few field reductions per iteration.
+host work of a few field reductions per iteration.  ``run_electro_thermal_many``
+runs many loops at once as a served workload: their DC and thermal solves,
+each waiting on the previous one of its loop, stay in flight together on one
+continuous-batching server.
 """
 
 from __future__ import annotations
 
 import dataclasses
 import warnings
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 from typing import Sequence
 
@@ -35,6 +39,7 @@ from electrical.matrix_free_mpir_fem import (
     ViaConnection,
     solve_pcb_dc,
 )
+from pcbhip import serving
 from thermal.matrix_free_mpir_fem import (
     ConvectionBoundary,
     ExposedFaceRadiation,
@@ -344,3 +349,24 @@ def run_electro_thermal(scenario: ElectroThermalScenario, *, config: CouplingCon
                                                              scenario.layer_slabs),
         converged=converged, iterations=len(history), history=tuple(history), cold_joule_loss_w=cold_loss,
     )
+
+
+def run_electro_thermal_many(scenarios: Sequence[ElectroThermalScenario], *, config: CouplingConfig | None = None,
+                             device_id: int = 0, max_slots: int = 256,
+                             memory_budget_bytes: float = 200e9) -> list[ElectroThermalResult]:
+    """Many electro-thermal loops at once: one host thread per loop, every DC
+    and thermal solve of every loop continuously batched on one
+    :class:`pcbhip.serving.ServedSolves` server.  The reference runs such
+    loops one after another; each result here is bitwise what
+    :func:`run_electro_thermal` gives the scenario alone."""
+
+    scenarios = list(scenarios)
+    if not scenarios:
+        return []
+    with serving.ServedSolves(device_id, max_slots=max_slots, memory_budget_bytes=memory_budget_bytes) as served:
+        def one(scenario: ElectroThermalScenario) -> ElectroThermalResult:
+            with serving.route_solves(served):
+                return run_electro_thermal(scenario, config=config, device_id=device_id)
+
+        with ThreadPoolExecutor(max_workers=len(scenarios), thread_name_prefix="coupling") as pool:
+            return list(pool.map(one, scenarios))

@@ -22,8 +22,10 @@ electro_thermal.py:403-423, cpp/src/fem/dc_ports.cpp:26-49).
 
 from __future__ import annotations
 
+import threading
 import time
 from collections import deque
+from contextlib import contextmanager
 from dataclasses import dataclass, field
 from typing import Sequence
 
@@ -58,6 +60,7 @@ class ServeResult:
     arrival_s: float
     admitted_s: float
     finished_s: float
+    error: BaseException | None = None  # the first failed right-hand side's error (e.g. not SPD)
 
     @property
     def latency_s(self) -> float:
@@ -65,7 +68,7 @@ class ServeResult:
 
     @property
     def converged(self) -> bool:
-        return all(r["converged"] for r in self.results)
+        return self.error is None and all(r["converged"] for r in self.results)
 
 
 @dataclass
@@ -76,6 +79,7 @@ class _InFlight:
     solutions: list = field(default_factory=list)
     results: list = field(default_factory=list)
     left: int = 0
+    error: BaseException | None = None
 
 
 class SolveServer:
@@ -199,7 +203,11 @@ class SolveServer:
         for slot, _tag in self.backend.finished():
             request_id, k = self._slot_of.pop(slot)
             entry = self.inflight[request_id]
-            x, res = self.backend.collect(slot, solution=self.keep_solutions)
+            try:
+                x, res = self.backend.collect(slot, solution=self.keep_solutions)
+            except RuntimeError as exc:  # this right-hand side failed; its slot is freed
+                x, res = None, None
+                entry.error = entry.error or exc
             entry.solutions[k] = x
             entry.results[k] = res
             entry.left -= 1
@@ -208,7 +216,7 @@ class SolveServer:
                 del self.inflight[request_id]
                 self._release(entry)
                 result = ServeResult(request_id, entry.request.kind, entry.solutions, entry.results,
-                                     entry.request.arrival_s, entry.admitted_s, finished)
+                                     entry.request.arrival_s, entry.admitted_s, finished, entry.error)
                 done.append(result)
         self.completed.extend(done)
         return done
@@ -223,6 +231,113 @@ class SolveServer:
         close = getattr(self.backend, "close", None)
         if close is not None:
             close()
+
+
+class ServedSolves:
+    """Blocking MPIR solves from many host threads, continuously batched.
+
+    ``solve`` submits one right-hand side to a shared :class:`SolveServer`
+    and waits for it; a pump thread runs the server's replay boundaries
+    while any solve is queued or in flight.  Host threads that each run a
+    chain of dependent solves (an electro-thermal loop alternating DC and
+    thermal solves, say) thus keep their solves in flight together.  A
+    solve's result is bitwise the result of the single-solve path."""
+
+    def __init__(self, device: int = 0, **server_options) -> None:
+        self.server = SolveServer(device, keep_solutions=True, **server_options)
+        self._lock = threading.Lock()
+        self._wake = threading.Condition(self._lock)
+        self._waiting: dict[int, list] = {}
+        self._next_id = 0
+        self._closed = False
+        self._failure: BaseException | None = None
+        self._pump = threading.Thread(target=self._run, name="served-solves", daemon=True)
+        self._pump.start()
+
+    def solve(self, operator: _capi.Operator, rhs, x0, config: _capi.MpirConfigC, kind: str = "solve"):
+        """(solution, result dict) of one right-hand side, as ``_capi.solve_mpir`` returns them."""
+
+        done = threading.Event()
+        slot: list = [done, None]
+        with self._lock:
+            if self._failure is not None:
+                raise RuntimeError("the served-solve pump failed") from self._failure
+            if self._closed:
+                raise RuntimeError("served solves are closed")
+            request_id = self._next_id
+            self._next_id += 1
+            self._waiting[request_id] = slot
+            self.server.submit(ServeRequest(request_id, operator, [np.asarray(rhs, dtype=np.float64)], config,
+                                            kind=kind, arrival_s=self.server.clock(),
+                                            initial_guess=None if x0 is None else [x0]))
+            self._wake.notify()
+        done.wait()
+        result = slot[1]
+        if isinstance(result, BaseException):
+            raise RuntimeError("the served-solve pump failed") from result
+        if result.error is not None:
+            raise result.error
+        return result.solutions[0], result.results[0]
+
+    def _run(self) -> None:
+        with self._lock:
+            while True:
+                while self.server.idle and not self._closed:
+                    self._wake.wait()
+                if self.server.idle:
+                    return
+                try:
+                    done = self.server.pump()
+                except BaseException as exc:  # fail every waiter rather than hang them
+                    self._failure = exc
+                    for slot in self._waiting.values():
+                        slot[1] = exc
+                        slot[0].set()
+                    self._waiting.clear()
+                    return
+                for result in done:
+                    slot = self._waiting.pop(result.request_id)
+                    slot[1] = result
+                    slot[0].set()
+
+    def close(self) -> None:
+        with self._lock:
+            self._closed = True
+            self._wake.notify()
+        self._pump.join()
+        self.server.close()
+
+    def __enter__(self) -> "ServedSolves":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+
+_route = threading.local()
+
+
+@contextmanager
+def route_solves(executor: ServedSolves | None):
+    """Sends this thread's device MPIR solves (``solve_mpir_routed``) to
+    ``executor`` for the duration of the block."""
+
+    previous = getattr(_route, "executor", None)
+    _route.executor = executor
+    try:
+        yield executor
+    finally:
+        _route.executor = previous
+
+
+def solve_mpir_routed(operator: _capi.Operator, rhs, x0, config: _capi.MpirConfigC):
+    """One device MPIR solve: through this thread's :class:`ServedSolves` when
+    one is routed, else through the single-solve path ``_capi.solve_mpir``."""
+
+    executor = getattr(_route, "executor", None)
+    if executor is None:
+        return _capi.solve_mpir(operator, rhs, x0, config)
+    return executor.solve(operator, rhs, x0, config)
 
 
 def run_arrivals(server: SolveServer, requests: Sequence[ServeRequest], *, log=None,

@@ -20,7 +20,7 @@ import numpy as np
 
 from electrical.matrix_free_mpir_fem.runtime import HipFloat32Runtime, make_float32_runtime
 from electrical.matrix_free_mpir_fem.solver import MPIRConfig
-from pcbhip import _capi
+from pcbhip import _capi, serving
 
 from .mesh import Preconditioner, corner_views, flat_index, unit_hexahedron_matrices
 from .problem import ThermalConductionProblem
@@ -138,9 +138,9 @@ class MatrixFreeThermalOperator:
     def native_solve_mpir(self, rhs_high: np.ndarray, config: MPIRConfig, initial_guess: np.ndarray | None = None):
         """The whole MPIR solve on the device (operator.py:318-347 contract)."""
 
-        x, result = _capi.solve_mpir(self._device, self._flat(rhs_high, "rhs"),
-                                     None if initial_guess is None else self._flat(initial_guess, "initial_guess"),
-                                     config.to_c())
+        x, result = serving.solve_mpir_routed(
+            self._device, self._flat(rhs_high, "rhs"),
+            None if initial_guess is None else self._flat(initial_guess, "initial_guess"), config.to_c())
         return (
             x,
             result["converged"],

@@ -233,3 +233,18 @@ def test_coupled_thermal_solve_removes_exactly_the_electrical_loss():
     assert solution.total_heat_input_w == pytest.approx(electrical.joule_loss_w)
     assert float(np.sum(solution.convective_heat_w)) == pytest.approx(electrical.joule_loss_w, rel=1e-8)
     assert solution.min_temperature_k > AMBIENT
+
+
+def test_many_loops_served_together_equal_each_loop_alone():
+    print("coupling served loops", flush=True)
+    from multiphysics.staggered_coupling import run_electro_thermal_many
+
+    scenarios = [_scenario(current) for current in (1.0, 2.5, 4.0)]
+    alone = [run_electro_thermal(s) for s in scenarios]
+    served = run_electro_thermal_many(scenarios)
+    for a, b in zip(alone, served):
+        assert a.converged and b.converged and a.iterations == b.iterations
+        assert np.array_equal(a.electrical.potential_v, b.electrical.potential_v, equal_nan=True)
+        assert np.array_equal(a.thermal.temperature_k, b.thermal.temperature_k, equal_nan=True)
+        assert [s.thermal_inner_iterations for s in a.history] == [s.thermal_inner_iterations for s in b.history]
+        assert a.electrical.joule_loss_w == b.electrical.joule_loss_w

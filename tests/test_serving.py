@@ -169,3 +169,44 @@ def test_poisson_arrivals_have_the_requested_rate():
     times = poisson_arrivals(rng, 200.0, 30.0)
     assert np.all(np.diff(times) > 0) and times[-1] < 30.0
     assert abs(times.size / 30.0 - 200.0) < 200.0 * 0.05
+
+
+class FailingBackend(FakeBackend):
+    """Fails the collection of right-hand sides whose first entry is negative."""
+
+    def collect(self, slot, solution=True):
+        tag, rhs = self.done[slot]
+        if rhs[0] < 0:
+            self.done.pop(slot)
+            self.free.append(slot)
+            raise RuntimeError("inner PCG requires a finite symmetric positive-definite operator")
+        return super().collect(slot, solution)
+
+
+def test_served_solves_from_many_threads_return_each_result_to_its_caller():
+    from concurrent.futures import ThreadPoolExecutor
+
+    from pcbhip.serving import ServedSolves, route_solves, solve_mpir_routed
+
+    op = FakeOperator(4)
+    with ServedSolves(max_slots=4, backend=FailingBackend(max_slots=4, steps=3)) as served:
+        def chain(k):
+            # A chain of dependent solves, each started from the last answer.
+            x = np.full(op.size, float(k + 1))
+            with route_solves(served):
+                for _ in range(3):
+                    x, res = solve_mpir_routed(op, x, None, CONFIG)
+                    assert res["converged"]
+            return x
+
+        with ThreadPoolExecutor(max_workers=6) as pool:
+            out = list(pool.map(chain, range(6)))
+        for k, x in enumerate(out):
+            np.testing.assert_array_equal(x, 8.0 * (k + 1))
+        # A failed solve raises in its own caller only; the server keeps serving.
+        with route_solves(served), pytest.raises(RuntimeError, match="positive-definite"):
+            solve_mpir_routed(op, np.full(op.size, -1.0), None, CONFIG)
+        x, _ = served.solve(op, np.ones(op.size), None, CONFIG)
+        np.testing.assert_array_equal(x, 2.0)
+    with pytest.raises(RuntimeError, match="closed"):
+        served.solve(op, np.ones(op.size), None, CONFIG)
