@@ -521,6 +521,64 @@ def run_serve(args, torch) -> dict:
     }
 
 
+def run_coupling(args, torch) -> dict:
+    """Electro-thermal loops as a served workload: the seeded scenarios run
+    one after another (each loop's DC and thermal solves through the
+    single-solve path, as the reference runs them) and then all at once, their
+    solves continuously batched on one server (``run_electro_thermal_many``)."""
+
+    from multiphysics.staggered_coupling import run_electro_thermal, run_electro_thermal_many
+    from pcbhip import workloads
+
+    scenarios = workloads.coupling_scenarios(count=args.coupling_count)
+    log(f"coupling: {len(scenarios)} scenarios")
+    for _ in range(args.warmup):
+        run_electro_thermal(scenarios[0])
+        run_electro_thermal_many(scenarios[:2])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    alone = [run_electro_thermal(s) for s in scenarios]
+    sequential_s = time.perf_counter() - t0
+    log(f"sequential: {sequential_s:.2f} s")
+    served_s = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        served = run_electro_thermal_many(scenarios)
+        served_s.append(time.perf_counter() - t0)
+    log(f"served: {np.mean(served_s):.2f} s per run")
+    per_run = float(np.mean(served_s))
+    solves = sum(2 * r.iterations for r in served)
+    bitwise = all(np.array_equal(a.thermal.temperature_k, b.thermal.temperature_k, equal_nan=True)
+                  and np.array_equal(a.electrical.potential_v, b.electrical.potential_v, equal_nan=True)
+                  for a, b in zip(alone, served))
+    return {
+        "metric": "coupled electro-thermal loops/s (16 voltage-driven boards, served together)",
+        "value": len(scenarios) / per_run,
+        "unit": "loops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": per_run * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 outer",
+        "data": "synthetic (seeded coupling scenarios, seed 6)",
+        "config": {
+            "workload": f"{len(scenarios)} current loops, 8..48 x 32..256 elements, 2 copper layers + "
+                        "3-slab thermal stack, sigma(T) with Aitken relaxation",
+            "sequential_loops_per_s": len(scenarios) / sequential_s,
+            "speedup_vs_one_loop_at_a_time": sequential_s / per_run,
+            "device_solves": solves,
+            "served_solves_per_s": solves / per_run,
+            "converged": int(sum(r.converged for r in served)),
+            "coupling_iterations": [r.iterations for r in served],
+            "bitwise_equal_alone_and_served": bool(bitwise),
+            "parallelism": "single GPU",
+        },
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -531,7 +589,9 @@ def main() -> int:
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--rows", type=int, default=4, help="rows per wave strip of the FP32 inner stencil")
     parser.add_argument("--target", type=int, default=1024, help="stencil workgroups per launch before z blocks split")
-    parser.add_argument("--workload", default="board512", choices=["board512", "batch200", "multirhs64", "serve", "apply_engines"])
+    parser.add_argument("--workload", default="board512",
+                        choices=["board512", "batch200", "multirhs64", "serve", "apply_engines", "coupling"])
+    parser.add_argument("--coupling-count", type=int, default=16, help="scenarios of the coupling workload")
     parser.add_argument("--rates", type=float, nargs="+", default=[50.0, 100.0, 200.0, 400.0])
     parser.add_argument("--duration", type=float, default=10.0)
     parser.add_argument("--drain-limit", type=float, default=60.0)
@@ -568,6 +628,9 @@ def main() -> int:
         return 0
     if args.workload == "apply_engines":
         print(json.dumps(run_apply_engines(args, torch)), flush=True)
+        return 0
+    if args.workload == "coupling":
+        print(json.dumps(run_coupling(args, torch)), flush=True)
         return 0
     log("building board")
     problem = workloads.board_512x512x16(size=args.size, slabs=args.slabs)

@@ -13,6 +13,8 @@ mask, then sources), so a seed names a board.
                              0.1-2 W (log-uniform) on 3..8-cell patches (seed 2).
 * ``uneven_boards``       -- 200 boards, sides log-uniform 32..384, depth 4..16,
                              20-70 % copper, 1..8 sources of 0.1-3 W (seed 4).
+* ``dc_board``, ``serving_pool`` -- the DC copper/dielectric board and the serving mix.
+* ``coupling_scenarios``  -- 16 voltage-driven electro-thermal current loops (seed 6).
 """
 
 from __future__ import annotations
@@ -245,3 +247,47 @@ def serving_pool(seed: int = 5, thermal_boards: int = 16, dc_boards: int = 8, ma
             op = MatrixFreeHexDCOperator(board.mesh, reference_node=board.reference_node)
             pool.append(PoolBoard(kind, op.device_operator, [op.build_rhs(t) for t in board.excitations], op))
     return pool
+
+
+def coupling_scenarios(count: int = 16, seed: int = 6):
+    """Electro-thermal scenarios of the served coupling workload (seed 6).
+
+    Each board is a current loop on two 35 um copper layers at 0.5 mm pitch,
+    ``rows`` x ``cols`` elements (rows 8..48, cols 32..256, log-uniform),
+    copper in every row but the outer two, fed by two voltage pads at x = 0
+    (top layer source, bottom layer return) and closed by vias in the last
+    two node columns; the voltage is set for a cold current of 1..4 A
+    (log-uniform) from the strip resistance.  The thermal mesh is copper /
+    1.5 mm laminate / copper with films of 10 W/m^2 K on both faces."""
+
+    from electrical.matrix_free_mpir_fem import LayeredPCBMesh, PCBConductionProblem, ViaConnection, VoltageTerminal
+    from multiphysics.staggered_coupling import ElectroThermalScenario
+
+    rng = np.random.default_rng(seed)
+    pitch, thickness, sigma, via_r = 0.5e-3, 35e-6, 1.0 / 1.724e-8, 1.5e-3
+    out = []
+    for _ in range(count):
+        rows = int(round(np.exp(rng.uniform(np.log(8), np.log(48)))))
+        cols = int(round(np.exp(rng.uniform(np.log(32), np.log(256)))))
+        current = float(np.exp(rng.uniform(np.log(1.0), np.log(4.0))))
+        active = np.zeros((2, rows, cols), dtype=bool)
+        active[:, 1:rows - 1, :] = True
+        mesh = LayeredPCBMesh(active, (thickness, thickness), pitch, pitch, conductivity_s_per_m=sigma)
+        pad = tuple(range(1, rows))
+        vias = tuple(ViaConnection((0, r, c), (1, r, c), via_r) for r in pad[1:-1] for c in (cols - 1, cols))
+        strip = cols / (sigma * thickness * (rows - 2))
+        voltage = current * (2.0 * strip + via_r / len(vias))
+        problem = PCBConductionProblem(mesh, voltage_terminals=(
+            VoltageTerminal(tuple((1, r, 0) for r in pad), voltage, "source"),
+            VoltageTerminal(tuple((0, r, 0) for r in pad), 0.0, "return")), vias=vias)
+        k = np.full((3, rows, cols), 0.8)
+        k[0] = np.where(active[0], 385.0, 0.8)
+        k[2] = np.where(active[1], 385.0, 0.8)
+        kz = k.copy()
+        kz[1] = 0.3
+        thermal = LayeredThermalMesh((thickness, 1.5e-3, thickness), pitch, pitch, k,
+                                     through_plane_conductivity_w_per_m_k=kz)
+        films = (ConvectionBoundary("top", FILM_W_PER_M2_K, AMBIENT_K),
+                 ConvectionBoundary("bottom", FILM_W_PER_M2_K, AMBIENT_K))
+        out.append(ElectroThermalScenario(problem, thermal, (0, 2), convection=films))
+    return out
