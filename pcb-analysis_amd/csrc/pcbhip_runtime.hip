@@ -181,6 +181,7 @@ struct pcbhip_batch {
     hipGraphExec_t exec = nullptr;
     int graph_steps = 0;
     bool configs_dirty = true;
+    bool ran = false;  // pcbhip_batch_run has completed once
     int h_counts[4] = {0, 0, 0, 0};  // source of the last counts upload (synchronised before reuse)
 
     ~pcbhip_batch() {
@@ -1214,31 +1215,41 @@ int pcbhip_batch_rhs_dev(pcbhip_batch* b, int slot, double** rhs_dev) {
     });
 }
 
+namespace {
+
+// Uploads every slot's descriptor and limits when they changed since the
+// last upload (history rows are allocated on first use).
+void upload_slots(pcbhip_batch* b) {
+    if (!b->configs_dirty) return;
+    const int count = (int)b->slots.size();
+    std::vector<SlotDev> devs((size_t)count);
+    std::vector<SlotConfig> cfgs((size_t)count);
+    for (int i = 0; i < count; ++i) {
+        SlotHost& s = b->slots[(size_t)i];
+        if (!s.history) {
+            s.history_rows = s.config.max_outer_iterations;
+            s.history = static_cast<double*>(device_alloc(sizeof(double) * 4 * (size_t)s.history_rows));
+        }
+        devs[(size_t)i] = slot_dev(s);
+        cfgs[(size_t)i] = slot_config(s.config);
+    }
+    check(hipMemcpyAsync(b->d_slots, devs.data(), sizeof(SlotDev) * count, hipMemcpyHostToDevice, b->stream),
+          "upload slots");
+    check(hipMemcpyAsync(b->d_configs, cfgs.data(), sizeof(SlotConfig) * count, hipMemcpyHostToDevice, b->stream),
+          "upload configs");
+    // The host vectors are the copies' sources: wait before they go.
+    check(hipStreamSynchronize(b->stream), "upload");
+    b->configs_dirty = false;
+}
+
+}  // namespace
+
 int pcbhip_batch_run(pcbhip_batch* b) {
     return guarded([&] {
         if (!b) fail(PCBHIP_INVALID_INPUT, "null batch");
         set_device(b->device);
         const int count = (int)b->slots.size();
-        if (b->configs_dirty) {
-            std::vector<SlotDev> devs((size_t)count);
-            std::vector<SlotConfig> cfgs((size_t)count);
-            for (int i = 0; i < count; ++i) {
-                SlotHost& s = b->slots[(size_t)i];
-                if (!s.history) {
-                    s.history_rows = s.config.max_outer_iterations;
-                    s.history = static_cast<double*>(device_alloc(sizeof(double) * 4 * (size_t)s.history_rows));
-                }
-                devs[(size_t)i] = slot_dev(s);
-                cfgs[(size_t)i] = slot_config(s.config);
-            }
-            check(hipMemcpyAsync(b->d_slots, devs.data(), sizeof(SlotDev) * count, hipMemcpyHostToDevice,
-                                 b->stream),
-                  "upload slots");
-            check(hipMemcpyAsync(b->d_configs, cfgs.data(), sizeof(SlotConfig) * count, hipMemcpyHostToDevice,
-                                 b->stream),
-                  "upload configs");
-            b->configs_dirty = false;
-        }
+        upload_slots(b);
         // Reset: x = x0 in buffer 0, fresh scalar states, every slot active.
         std::vector<SlotState> states((size_t)count);
         long long cap = 0;
@@ -1282,14 +1293,16 @@ int pcbhip_batch_run(pcbhip_batch* b) {
             }
         }
         if (b->tiled_slots != count) build_tiles(b);
+        b->ran = true;
     });
 }
 
 int pcbhip_batch_time_pass(pcbhip_batch* b, int launches, double* avg_us) {
     return guarded([&] {
         if (!b || !avg_us || launches < 1) fail(PCBHIP_INVALID_INPUT, "time_pass needs a batch and launches >= 1");
-        if (b->configs_dirty) fail(PCBHIP_INVALID_INPUT, "run the batch before timing its pass");
+        if (!b->ran) fail(PCBHIP_INVALID_INPUT, "run the batch before timing its pass");
         set_device(b->device);
+        upload_slots(b);
         const int count = (int)b->slots.size();
         std::vector<SlotState> saved((size_t)count);
         check(hipMemcpy(saved.data(), b->d_states, sizeof(SlotState) * count, hipMemcpyDeviceToHost), "copy states");

@@ -246,3 +246,24 @@ def test_exact_initial_guess_converges_without_inner_work():
     result = solve_mpir(operator, rhs, config=MPIRConfig(relative_tolerance=1e-8), initial_guess=exact)
     assert result.converged and result.outer_iterations == 0 and result.inner_iterations == 0
     assert result.high_operator_applications == 1
+
+
+def test_pass_timer_keeps_results_and_later_runs():
+    print("pass timer", flush=True)
+    problem = random_body((3, 30, 50), seed=21)
+    operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    rhs = operator.build_rhs(operator.default_reference_temperature())
+    cfg = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=30, max_inner_iterations=4000)
+    batch = _capi.Batch([operator.device_operator])
+    batch.set(0, rhs, None, cfg.to_c())
+    with pytest.raises(ValueError, match="run the batch"):
+        batch.time_pass(5)
+    batch.run()
+    before, res_before = batch.result(0)
+    us = batch.time_pass(20)
+    assert 0.0 < us < 1e5
+    after, res_after = batch.result(0)
+    assert after.tobytes() == before.tobytes() and res_after == res_before
+    batch.run()
+    again, res_again = batch.result(0)
+    assert again.tobytes() == before.tobytes() and res_again["inner_iterations"] == res_before["inner_iterations"]
