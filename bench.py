@@ -148,10 +148,16 @@ def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
     )
 
 
-def cpu_baseline(problem, gpu_result, gpu_solve_s: float) -> dict:
+def cpu_baseline(problem, gpu_result, gpu_solve_s: float, *, full_rhs=None, gpu_solution=None) -> dict:
     """The reference algorithm (oracle C restatement of hex_q1.cpp's FP32
     Jacobi PCG) on the host cores: a bounded sample of inner iterations on the
-    same board, scaled by the GPU solve's own iteration counts."""
+    same board, scaled by the GPU solve's own iteration counts.
+
+    With ``full_rhs`` (``--cpu-full``, minutes of host time) it instead runs
+    the whole reference solve of the GPU's right-hand side, and an FP64
+    oracle solve (FP64 inner PCG, relative residual 1e-12) against which it
+    states the relative error of the GPU solution and of the reference
+    algorithm's."""
 
     from oracle import problem as oracle_problem
     from oracle.native import OracleOperator, threads_default
@@ -159,12 +165,18 @@ def cpu_baseline(problem, gpu_result, gpu_solve_s: float) -> dict:
     ops = oracle_problem.operator_arrays(oracle_problem.from_problem(problem))
     op = OracleOperator(ops)
     threads = threads_default()
+    if full_rhs is not None:
+        return _cpu_full(op, threads, full_rhs, gpu_solution)
     rhs = np.random.default_rng(0).standard_normal(op.size)
-    its = 20
+    # Two sample lengths: their difference drops the per-call set-up (the
+    # oracle allocates and first-touches its vectors on every call).
+    short, its = 10, 40
     op.pcg_f32(rhs, 1e-30, 2, threads=threads)  # warm-up
     t0 = time.perf_counter()
+    _, done_short, _, _ = op.pcg_f32(rhs, 1e-30, short, threads=threads)
+    t1 = time.perf_counter()
     _, done, _, _ = op.pcg_f32(rhs, 1e-30, its, threads=threads)
-    per_iter = (time.perf_counter() - t0) / max(done, 1)
+    per_iter = ((time.perf_counter() - t1) - (t1 - t0)) / max(done - done_short, 1)
     x = rhs.copy()
     t0 = time.perf_counter()
     for _ in range(3):
@@ -174,9 +186,27 @@ def cpu_baseline(problem, gpu_result, gpu_solve_s: float) -> dict:
     outer = gpu_result["high_operator_applications"]
     solve_s = inner * per_iter + outer * per_f64
     return dict(value=1.0 / solve_s, unit="solves/s", cores=threads, kind="port",
-                sample=f"{its} FP32 Jacobi-PCG iterations + 3 FP64 applies of the same board on {threads} host "
-                       f"threads (oracle C restatement of hex_q1.cpp), scaled by the GPU solve's {inner} inner / "
-                       f"{outer} outer steps: {per_iter * 1e3:.2f} ms/iteration, est. {solve_s:.1f} s/solve")
+                sample=f"{its} less {short} FP32 Jacobi-PCG iterations + 3 FP64 applies of the same board on "
+                       f"{threads} host threads (oracle C restatement of hex_q1.cpp), scaled by the GPU solve's "
+                       f"{inner} inner / {outer} outer steps: {per_iter * 1e3:.2f} ms/iteration, "
+                       f"est. {solve_s:.1f} s/solve")
+
+
+def _cpu_full(op, threads: int, rhs, gpu_solution) -> dict:
+    limits = dict(max_outer=60, max_inner=100000, threads=threads)
+    t0 = time.perf_counter()
+    x_port, res = op.solve_mpir(rhs, rtol=1e-10, **limits)
+    port_s = time.perf_counter() - t0
+    log(f"cpu reference algorithm: {port_s:.1f} s, {res['inner_iterations']} inner iterations")
+    x64, res64 = op.solve_mpir(rhs, rtol=1e-12, inner_f64=True, **limits)
+    log(f"cpu fp64 oracle: {res64['outer_iterations']} outer / {res64['inner_iterations']} inner")
+    scale = float(np.max(np.abs(x64)))
+    return dict(value=1.0 / port_s, unit="solves/s", cores=threads, kind="port",
+                sample=f"the whole solve on {threads} host threads (oracle C restatement of hex_q1.cpp): "
+                       f"{res['outer_iterations']} outer / {res['inner_iterations']} inner, {port_s:.1f} s",
+                fp64_oracle_rel_error_gpu=float(np.max(np.abs(gpu_solution - x64))) / scale,
+                fp64_oracle_rel_error_cpu_reference=float(np.max(np.abs(x_port - x64))) / scale,
+                fp64_oracle_converged=bool(res64["converged"]))
 
 
 def run_batch200(args, torch) -> dict:
@@ -587,6 +617,8 @@ def main() -> int:
     parser.add_argument("--size", type=int, default=512)
     parser.add_argument("--slabs", type=int, default=16)
     parser.add_argument("--no-cpu", action="store_true")
+    parser.add_argument("--cpu-full", action="store_true",
+                        help="time the whole reference solve on the host and state the FP64-oracle errors (minutes)")
     parser.add_argument("--rows", type=int, default=4, help="rows per wave strip of the FP32 inner stencil")
     parser.add_argument("--target", type=int, default=1024, help="stencil workgroups per launch before z blocks split")
     parser.add_argument("--workload", default="board512",
@@ -701,7 +733,8 @@ def main() -> int:
     }
     if not args.no_cpu:
         log("cpu baseline")
-        line["cpu_baseline"] = cpu_baseline(problem, result, elapsed / args.steps)
+        full = dict(full_rhs=rhs, gpu_solution=x) if args.cpu_full else {}
+        line["cpu_baseline"] = cpu_baseline(problem, result, elapsed / args.steps, **full)
     print(json.dumps(line), flush=True)
     return 0
 
