@@ -98,6 +98,21 @@ int pcbhip_operator_destroy(pcbhip_operator *op);
 int64_t pcbhip_operator_size(const pcbhip_operator *op);
 /* FP64 Jacobi diagonal (replaces operator.py:289-306 / thermal_system.cpp:329-349). */
 int pcbhip_operator_diagonal(const pcbhip_operator *op, double *diag);
+/* Two-level preconditioner M^-1 = D^-1 + Z C^-1 Z^T (replaces the coarse data
+ * of AggregationCoarseCorrection, src/electrical/matrix_free_mpir_fem/
+ * two_level.py:54-229, which the native solve takes as block and cinv32,
+ * thermal_hex.hpp:35-39 and hex_q1.cpp:388-432).  Z is constant over
+ * block x block node patches of one node layer (node planes of a hexahedral
+ * operator, copper layers of a sheet operator), zero on fixed nodes; cinv is
+ * the nc x nc row-major inverse of Z^T A Z, nc = layers * ceil((R + 1) /
+ * block) * ceil((C + 1) / block) <= 16384, stored in FP32.  nc = 0 removes
+ * it.  Batches made afterwards run two-level PCG on this operator; batches
+ * made before keep what they were made with.  Servers take Jacobi operators
+ * only. */
+int pcbhip_operator_set_coarse(pcbhip_operator *op, int block, int nc, const double *cinv);
+/* z = M^-1 r in FP32 with the operator's preconditioner, Jacobi or two-level
+ * (replaces MatrixFreeThermalOperator.precondition_low, operator.py:366-371). */
+int pcbhip_operator_precondition_f32(const pcbhip_operator *op, const float *r, float *z);
 
 /* y = A x with fixed rows the identity and Robin on the diagonal, FP32
  * (replaces hex_q1.cpp:131-193 apply_lines<float>, pcbcore::fem::thermal_hex::

@@ -107,6 +107,12 @@ struct pcbhip_operator {
     double* via_g64 = nullptr;
     int rows32 = 4, zb32 = 0, rows64 = 4, zb64 = 0;  // standalone apply tile shapes
     int mfma32 = 0;  // standalone FP32 element action: 0 vector, 2 or 3 bf16 terms on the matrix cores
+    // Two-level coarse space (nc = 0: none).  A replaced inverse stays
+    // allocated until the operator goes: slots made before the change keep
+    // the view they were made with.
+    float* cinv = nullptr;
+    int cblock = 0, crows = 0, ccols = 0, nc = 0;
+    std::vector<float*> retired;
 
     OperatorDev view() const {
         OperatorDev v{};
@@ -125,10 +131,20 @@ struct pcbhip_operator {
         v.via_nbr = via_nbr;
         v.via_g32 = via_g32;
         v.via_g64 = via_g64;
+        v.cinv = cinv;
+        v.cblock = cblock;
+        v.crows = crows;
+        v.ccols = ccols;
+        v.nc = nc;
         return v;
     }
+    // Node layers of the coarse space: the node planes of a hexahedral
+    // operator, the copper layers of a sheet operator.
+    int layers() const { return kind == kSheet ? S : S + 1; }
     ~pcbhip_operator() {
         if (block) (void)hipFree(block);
+        if (cinv) (void)hipFree(cinv);
+        for (float* p : retired) (void)hipFree(p);
         if (via_ptr) (void)hipFree(via_ptr);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -138,6 +154,7 @@ namespace {
 
 struct SlotHost {
     const pcbhip_operator* op = nullptr;
+    OperatorDev view{};  // the operator as it was when the slot was made
     void* block = nullptr;
     double* b = nullptr;
     double* x[2] = {nullptr, nullptr};
@@ -147,6 +164,9 @@ struct SlotHost {
     void* q[2] = {nullptr, nullptr};
     void* c = nullptr;
     double* part = nullptr;
+    double* cg = nullptr;
+    double* ch = nullptr;
+    double* cw = nullptr;
     double* history = nullptr;
     int history_rows = 0;
     pcbhip_mpir_config config{1e-10, 0.0, 2e-3, 8, 200};
@@ -166,9 +186,12 @@ struct pcbhip_batch {
     Tile* d_outer = nullptr;
     Tile* d_inner = nullptr;
     Tile* d_block = nullptr;
+    Tile* d_patches = nullptr;  // coarse restriction tiles of the two-level slots
+    Tile* d_rows = nullptr;     // coarse row tiles
     SlotGroup* d_groups = nullptr;
-    int* d_counts = nullptr;  // live table lengths: outer, inner, (unused), block
-    int n_outer = 0, n_inner = 0, n_block = 0;
+    int* d_counts = nullptr;  // live table lengths: outer, inner, (unused), block, patches, coarse rows
+    int n_outer = 0, n_inner = 0, n_block = 0, n_patches = 0, n_rows = 0;
+    bool plain = true;  // some tiled slot is Jacobi-only
     int block_group = 0;  // right-hand sides per block-apply group (0: no block apply)
     int block_mfma = 3;  // element action of the block apply (0, 2, 3)
     int rows_inner = 4;
@@ -182,7 +205,17 @@ struct pcbhip_batch {
     int graph_steps = 0;
     bool configs_dirty = true;
     bool ran = false;  // pcbhip_batch_run has completed once
-    int h_counts[4] = {0, 0, 0, 0};  // source of the last counts upload (synchronised before reuse)
+    int h_counts[6] = {0, 0, 0, 0, 0, 0};  // source of the last counts upload (synchronised before reuse)
+
+    launch::CoarseLaunch coarse() const {
+        launch::CoarseLaunch c;
+        c.patches = d_patches;
+        c.rows = d_rows;
+        c.grid_patches = n_patches;
+        c.grid_rows = n_rows;
+        c.plain = plain;
+        return c;
+    }
 
     ~pcbhip_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -192,7 +225,8 @@ struct pcbhip_batch {
             if (s.history) (void)hipFree(s.history);
         }
         for (void* p : {(void*)d_slots, (void*)d_states, (void*)d_configs, (void*)d_outer, (void*)d_inner,
-                        (void*)d_block, (void*)d_groups, (void*)d_active, (void*)d_counts})
+                        (void*)d_block, (void*)d_patches, (void*)d_rows, (void*)d_groups, (void*)d_active,
+                        (void*)d_counts})
             if (p) (void)hipFree(p);
         if (h_active) (void)hipHostFree(h_active);
         if (stream) (void)hipStreamDestroy(stream);
@@ -262,7 +296,7 @@ void apply_host(const pcbhip_operator* op, const T* x, T* y, int mode) {
 
 SlotDev slot_dev(const SlotHost& s) {
     SlotDev d{};
-    d.op = s.op->view();
+    d.op = s.view;
     d.b64 = s.b;
     d.x64[0] = s.x[0];
     d.x64[1] = s.x[1];
@@ -273,6 +307,9 @@ SlotDev slot_dev(const SlotHost& s) {
     }
     d.c = s.c;
     d.part = s.part;
+    d.cg = s.cg;
+    d.ch = s.ch;
+    d.cw = s.cw;
     d.history = s.history;
     d.cells = s.cells;
     d.part_stride = s.part_stride;
@@ -285,11 +322,13 @@ SlotConfig slot_config(const pcbhip_mpir_config& c) {
 }
 
 // Device bytes of one slot's vectors: b and the x pair in FP64, the r, p and
-// q pairs and c in the inner precision, and the partial sums.
+// q pairs and c in the inner precision, the partial sums and, for a
+// two-level slot, its three coarse vectors.
 size_t slot_vector_bytes(const SlotHost& s, int inner64) {
     const size_t ti = inner64 ? sizeof(double) : sizeof(float);
     return 3 * align256(sizeof(double) * s.op->N) + 7 * align256(ti * s.op->N) +
-           align256(sizeof(double) * kInnerSums * (size_t)s.part_stride);
+           align256(sizeof(double) * kInnerSums * (size_t)s.part_stride) +
+           3 * align256(sizeof(double) * (size_t)s.view.nc);
 }
 
 // Points the slot's vectors into one block laid out as slot_vector_bytes
@@ -309,6 +348,24 @@ void carve_slot(SlotHost& s, char* base, int inner64) {
     }
     s.c = base + 6 * nti;
     s.part = reinterpret_cast<double*>(base + 7 * nti);
+    base += 7 * nti + align256(sizeof(double) * kInnerSums * (size_t)s.part_stride);
+    const size_t ncb = align256(sizeof(double) * (size_t)s.view.nc);
+    s.cg = s.view.nc ? reinterpret_cast<double*>(base) : nullptr;
+    s.ch = s.view.nc ? reinterpret_cast<double*>(base + ncb) : nullptr;
+    s.cw = s.view.nc ? reinterpret_cast<double*>(base + 2 * ncb) : nullptr;
+}
+
+// Coarse tiles of a two-level slot: one restriction tile per patch and one
+// row tile per kCoarseRows rows of its inverse.
+void add_coarse_tiles(std::vector<Tile>& patches, std::vector<Tile>& rows, int slot, const OperatorDev& v) {
+    if (v.nc <= 0) return;
+    const int layers = v.kind == kSheet ? v.S : v.S + 1;
+    int index = 0;
+    for (int l = 0; l < layers; ++l)
+        for (int cy = 0; cy < v.crows; ++cy)
+            for (int cx = 0; cx < v.ccols; ++cx)
+                patches.push_back(Tile{slot, index++, (short)cx, (short)cy, (short)l, 1});
+    for (int row = 0; row < v.nc; row += kCoarseRows) rows.push_back(Tile{slot, row, 0, 0, 0, 1});
 }
 
 // Tiles of one slot: stencil workgroups in (y group, x strip, z block)
@@ -421,7 +478,8 @@ std::vector<SlotGroup> group_slots(const std::vector<const SlotHost*>& slots, in
     std::vector<const pcbhip_operator*> ops;
     std::vector<std::vector<int>> members;
     for (int i = 0; i < (int)slots.size(); ++i) {
-        if (!slots[(size_t)i] || slots[(size_t)i]->op->kind != kHex) continue;
+        // Two-level slots keep tiles of their own (the block pass is Jacobi).
+        if (!slots[(size_t)i] || slots[(size_t)i]->op->kind != kHex || slots[(size_t)i]->view.nc > 0) continue;
         const pcbhip_operator* op = slots[(size_t)i]->op;
         size_t k = 0;
         while (k < ops.size() && ops[k] != op) ++k;
@@ -467,26 +525,36 @@ void build_tiles(pcbhip_batch* b, const std::vector<char>* keep = nullptr) {
         listed[i] = &b->slots[i];
         b->tiled_slots += 1;
     }
-    std::vector<Tile> outer, inner, block;
+    std::vector<Tile> outer, inner, block, patches, rows;
     const std::vector<SlotGroup> groups = group_slots(listed, b->inner64 ? 0 : b->block_group);
     collect_tiles(listed, groups, rows_inner, b->target_groups, outer, inner, block);
+    b->plain = false;
+    for (size_t i = 0; i < listed.size(); ++i) {
+        if (!listed[i]) continue;
+        add_coarse_tiles(patches, rows, (int)i, listed[i]->view);
+        if (listed[i]->view.nc == 0) b->plain = true;
+    }
     if (outer.empty()) outer.push_back(Tile{0, 0, 0, 0, 0, 1});
     if (inner.empty() && block.empty()) inner.push_back(Tile{0, 0, 0, 0, 0, 1});
     upload_tiles(b, outer, &b->d_outer, &b->n_outer);
     upload_tiles(b, inner, &b->d_inner, &b->n_inner);
     upload_tiles(b, block, &b->d_block, &b->n_block);
+    upload_tiles(b, patches, &b->d_patches, &b->n_patches);
+    upload_tiles(b, rows, &b->d_rows, &b->n_rows);
     if (b->d_groups) check(hipFree(b->d_groups), "hipFree");
     b->d_groups = static_cast<SlotGroup*>(device_alloc(sizeof(SlotGroup) * std::max<size_t>(1, groups.size())));
     if (!groups.empty())
         check(hipMemcpyAsync(b->d_groups, groups.data(), sizeof(SlotGroup) * groups.size(), hipMemcpyHostToDevice,
                              b->stream),
               "groups");
-    if (!b->d_counts) b->d_counts = static_cast<int*>(device_alloc(4 * sizeof(int)));
+    if (!b->d_counts) b->d_counts = static_cast<int*>(device_alloc(6 * sizeof(int)));
     b->h_counts[0] = b->n_outer;
     b->h_counts[1] = b->n_inner;
     b->h_counts[2] = 0;
     b->h_counts[3] = b->n_block;
-    check(hipMemcpyAsync(b->d_counts, b->h_counts, 4 * sizeof(int), hipMemcpyHostToDevice, b->stream), "counts");
+    b->h_counts[4] = b->n_patches;
+    b->h_counts[5] = b->n_rows;
+    check(hipMemcpyAsync(b->d_counts, b->h_counts, 6 * sizeof(int), hipMemcpyHostToDevice, b->stream), "counts");
     // Tables and counts are read through stream order; the sources above are
     // host memory that changes at the next build, so wait here.
     check(hipStreamSynchronize(b->stream), "tile upload");
@@ -515,11 +583,11 @@ void build_graph(pcbhip_batch* b, int steps) {
         // kernels stay out of most steps.
         if (s % b->outer_every == 0)
             err = launch::outer_step(b->inner64 != 0, n_slots, b->d_slots, b->d_states, b->d_configs, b->d_outer,
-                                     b->d_counts, b->n_outer, b->d_active, b->stream);
+                                     b->d_counts, b->n_outer, b->d_active, b->coarse(), b->stream);
         if (err == hipSuccess)
             err = launch::inner_step(b->inner64 != 0, b->rows_inner, n_slots, b->d_slots, b->d_states, b->d_configs,
                                      b->d_inner, b->d_counts, std::max(b->n_inner, 1), false, b->d_block,
-                                     b->d_groups, b->n_block, b->block_mfma, b->d_active, b->stream);
+                                     b->d_groups, b->n_block, b->block_mfma, b->d_active, b->coarse(), b->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(b->stream, &dropped);
@@ -680,11 +748,11 @@ void server_build_graph(pcbhip_server* sv) {
         hipError_t err = hipSuccess;
         if (s % sv->outer_every == 0)
             err = launch::outer_step(sv->inner64 != 0, sv->max_slots, sv->d_slots, sv->d_states, sv->d_configs,
-                                     outer, sv->d_counts, sv->cap_outer, sv->d_active, sv->stream);
+                                     outer, sv->d_counts, sv->cap_outer, sv->d_active, launch::CoarseLaunch{}, sv->stream);
         if (err == hipSuccess)
             err = launch::inner_step(sv->inner64 != 0, sv->rows_inner, sv->max_slots, sv->d_slots, sv->d_states,
                                      sv->d_configs, inner, sv->d_counts, std::min(sv->cap_inner, kPassGroups), true,
-                                     nullptr, nullptr, 0, 0, sv->d_active, sv->stream);
+                                     nullptr, nullptr, 0, 0, sv->d_active, launch::CoarseLaunch{}, sv->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(sv->stream, &dropped);
@@ -1096,6 +1164,7 @@ int pcbhip_batch_create(int count, const pcbhip_operator* const* ops, int inner_
             SlotHost& s = b->slots[(size_t)i];
             const pcbhip_operator* op = ops[i];
             s.op = op;
+            s.view = op->view();
             set_partial_layout(s);
             s.history_rows = s.config.max_outer_iterations;
             // Vectors of the slot in one allocation, x0 behind them.  History
@@ -1204,6 +1273,72 @@ int pcbhip_operator_set_shape(pcbhip_operator* op, int rows32, int zb32, int row
         op->zb32 = std::min(zb32 + (zb32 & 1), op->S + 1);
         op->rows64 = rows64;
         op->zb64 = std::min(zb64 + (zb64 & 1), op->S + 1);
+    });
+}
+
+int pcbhip_operator_set_coarse(pcbhip_operator* op, int block, int nc, const double* cinv) {
+    return guarded([&] {
+        if (!op) fail(PCBHIP_INVALID_INPUT, "null operator");
+        if (nc == 0) {
+            if (op->cinv) op->retired.push_back(op->cinv);
+            op->cinv = nullptr;
+            op->cblock = op->crows = op->ccols = op->nc = 0;
+            return;
+        }
+        if (!cinv) fail(PCBHIP_INVALID_INPUT, "null coarse inverse");
+        if (block < 1) fail(PCBHIP_INVALID_INPUT, "block must be a positive node count");
+        const long long crows = (op->R + 1 + block - 1) / block, ccols = (op->C + 1 + block - 1) / block;
+        const long long expected = (long long)op->layers() * crows * ccols;
+        if (nc != expected)
+            fail(PCBHIP_INVALID_INPUT, "coarse size " + std::to_string(nc) + " does not match the patches of block " +
+                                           std::to_string(block) + " (" + std::to_string(expected) + ")");
+        if (nc > kMaxCoarse)
+            fail(PCBHIP_INVALID_INPUT, "coarse size " + std::to_string(nc) + " exceeds the limit of " +
+                                           std::to_string(kMaxCoarse));
+        const size_t count = (size_t)nc * (size_t)nc;
+        std::vector<float> values(count);
+        for (size_t i = 0; i < count; ++i) {
+            if (!std::isfinite(cinv[i])) fail(PCBHIP_INVALID_INPUT, "coarse inverse must be finite");
+            values[i] = (float)cinv[i];
+        }
+        set_device(op->device);
+        float* d = static_cast<float*>(device_alloc(sizeof(float) * count));
+        hipError_t err = hipMemcpyAsync(d, values.data(), sizeof(float) * count, hipMemcpyHostToDevice, op->stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(op->stream);
+        if (err != hipSuccess) {
+            (void)hipFree(d);
+            check(err, "coarse upload");
+        }
+        if (op->cinv) op->retired.push_back(op->cinv);
+        op->cinv = d;
+        op->cblock = block;
+        op->crows = (int)crows;
+        op->ccols = (int)ccols;
+        op->nc = nc;
+    });
+}
+
+int pcbhip_operator_precondition_f32(const pcbhip_operator* op, const float* r, float* z) {
+    return guarded([&] {
+        if (!op || !r || !z) fail(PCBHIP_INVALID_INPUT, "null argument");
+        set_device(op->device);
+        const size_t bytes = sizeof(float) * (size_t)op->N;
+        const size_t coarse = align256(sizeof(double) * (size_t)std::max(op->nc, 1));
+        char* base = static_cast<char*>(device_alloc(2 * align256(bytes) + 2 * coarse));
+        float* dr = reinterpret_cast<float*>(base);
+        float* dz = reinterpret_cast<float*>(base + align256(bytes));
+        double* g = reinterpret_cast<double*>(base + 2 * align256(bytes));
+        double* w = reinterpret_cast<double*>(base + 2 * align256(bytes) + coarse);
+        try {
+            check(hipMemcpyAsync(dr, r, bytes, hipMemcpyHostToDevice, op->stream), "copy r");
+            check(launch::precondition(op->view(), dr, dz, g, w, op->stream), "precondition launch");
+            check(hipMemcpyAsync(z, dz, bytes, hipMemcpyDeviceToHost, op->stream), "copy z");
+            check(hipStreamSynchronize(op->stream), "precondition");
+        } catch (...) {
+            (void)hipFree(base);
+            throw;
+        }
+        check(hipFree(base), "hipFree");
     });
 }
 
@@ -1322,7 +1457,7 @@ int pcbhip_batch_time_pass(pcbhip_batch* b, int launches, double* avg_us) {
         for (int k = 0; k < launches && err == hipSuccess; ++k)
             err = launch::inner_pass(b->inner64 != 0, b->rows_inner, b->d_slots, b->d_states, b->d_inner, b->d_counts,
                                      std::max(b->n_inner, 1), false, b->d_block, b->d_groups, b->n_block,
-                                     b->block_mfma, b->stream);
+                                     b->block_mfma, b->coarse(), b->stream);
         if (err == hipSuccess) err = hipEventRecord(ev[1], b->stream);
         if (err == hipSuccess) err = hipEventSynchronize(ev[1]);
         if (err == hipSuccess) err = hipEventElapsedTime(&ms, ev[0], ev[1]);
@@ -1478,6 +1613,8 @@ int pcbhip_server_slot_bytes(const pcbhip_server* sv, const pcbhip_operator* op,
         if (!sv || !op || !bytes || max_outer < 1) fail(PCBHIP_INVALID_INPUT, "invalid argument");
         SlotHost h;
         h.op = op;
+        h.view = op->view();
+        h.view.nc = 0;  // a server slot is Jacobi (pcbhip_server_submit)
         set_partial_layout(h);
         *bytes = (int64_t)(slot_vector_bytes(h, sv->inner64) + align256(sizeof(double) * 4 * (size_t)max_outer));
     });
@@ -1488,6 +1625,8 @@ int pcbhip_server_submit(pcbhip_server* sv, const pcbhip_operator* op, const dou
     return guarded([&] {
         if (!sv || !op || !rhs || !config || !slot_out) fail(PCBHIP_INVALID_INPUT, "null argument");
         if (op->device != sv->device) fail(PCBHIP_INVALID_INPUT, "operator and server live on different devices");
+        if (op->nc > 0)
+            fail(PCBHIP_INVALID_INPUT, "the server runs Jacobi slots; solve two-level operators in a batch");
         check_config(*config);
         for (long long i = 0; i < op->N; ++i)
             if (!std::isfinite(rhs[i])) fail(PCBHIP_INVALID_INPUT, "rhs must contain only finite values");
@@ -1502,6 +1641,7 @@ int pcbhip_server_submit(pcbhip_server* sv, const pcbhip_operator* op, const dou
         ServerSlot& s = sv->slots[(size_t)index];
         SlotHost h;
         h.op = op;
+        h.view = op->view();
         h.config = *config;
         set_partial_layout(h);
         h.history_rows = config->max_outer_iterations;

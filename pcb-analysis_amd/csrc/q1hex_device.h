@@ -95,6 +95,12 @@ struct OperatorDev {
     const int* via_nbr;     // sheet: other end of each via entry
     const float* via_g32;   // sheet: via conductance of each entry
     const double* via_g64;
+    // Two-level coarse correction (null cinv: Jacobi only).  Patches are
+    // cblock x cblock nodes of one node layer; coarse index
+    // (layer * crows + y / cblock) * ccols + x / cblock; cinv is the dense
+    // FP32 inverse of the coarse Galerkin matrix, nc x nc, row-major.
+    const float* cinv;
+    int cblock, crows, ccols, nc;
 };
 
 // Reduction cells of the sheet kernels: one per (layer, 16 node rows, strip).
@@ -119,6 +125,11 @@ struct SlotDev {
     void* c;
     double* part;       // kInnerSums * part_stride partial sums
     double* history;    // 4 * max_outer: (outer, fp64 rel before, inner its, inner rel)
+    // Coarse vectors of a two-level slot (nc each): g the patch sums of the
+    // vector being restricted, h = cinv g, w = cinv Z^T r for the current r.
+    double* cg;
+    double* ch;
+    double* cw;
     int cells;          // reduction cells of the stencil kernels (see cell_count)
     int part_stride;
 };
@@ -147,6 +158,13 @@ struct Tile {
     short sx, sy, sz, zb;
 };
 
+// Tile tables of the coarse kernels reuse Tile: a restriction tile is one
+// patch (sx, sy, sz = coarse column, row, node layer); a coarse row tile is
+// kCoarseRows rows of cinv starting at ``index``.
+constexpr int kCoarseRows = 4;
+// Largest coarse space a slot may carry (its inverse is nc^2 FP32 values).
+constexpr int kMaxCoarse = 16384;
+
 // Slots that share one operator, applied together by the block apply
 // (at most kGroupMax right-hand sides per group).
 constexpr int kGroupMax = 64;
@@ -174,22 +192,38 @@ hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t st
 // One outer step (FP64 residual and decision) for the slots in their outer
 // phase, and one fused Jacobi-PCG pass for the slots in their inner phase.
 // ``counts`` (device) holds the live lengths of the outer, inner and block
-// tile tables (entries 0, 1 and 3).  The grids may be longer than the live
+// tile tables (entries 0, 1 and 3) and of the coarse restriction and coarse
+// row tables (entries 4 and 5).  The grids may be longer than the live
 // tables (workgroups past the count return at once), so a captured graph
 // survives changes of the tables; with ``loop`` the inner grid may also be
 // shorter and its workgroups loop over the live inner tiles.  Block tiles (a
 // slot group per tile, ``groups``) are launched when grid_block > 0, with
 // the element action block_mfma (0, 2, 3 as in apply).
+// Two-level slots: ``coarse`` describes the coarse kernels' tables (counts
+// entries 4 and 5) and which pass kernels run.  With plain = false no slot
+// is Jacobi-only; with grid_patches = 0 none is two-level.
+struct CoarseLaunch {
+    const Tile* patches = nullptr;  // restriction tiles
+    const Tile* rows = nullptr;     // coarse row tiles
+    int grid_patches = 0, grid_rows = 0;
+    bool plain = true;
+};
 hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState* states, const SlotConfig* configs,
-                      const Tile* outer_tiles, const int* counts, int grid_outer, int* active, hipStream_t stream);
+                      const Tile* outer_tiles, const int* counts, int grid_outer, int* active,
+                      const CoarseLaunch& coarse, hipStream_t stream);
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
                       const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner, bool loop,
                       const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma, int* active,
-                      hipStream_t stream);
-// The fused PCG pass alone (inner_step without its finish kernel).
+                      const CoarseLaunch& coarse, hipStream_t stream);
+// The fused PCG pass alone (inner_step without its coarse kernels and its
+// finish kernel).
 hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const SlotState* states,
                       const Tile* inner_tiles, const int* counts, int grid_inner, bool loop, const Tile* block_tiles,
-                      const SlotGroup* groups, int grid_block, int block_mfma, hipStream_t stream);
+                      const SlotGroup* groups, int grid_block, int block_mfma, const CoarseLaunch& coarse,
+                      hipStream_t stream);
+// z = M^-1 r of one operator in FP32 (Jacobi, plus the coarse correction
+// when the operator has one); g and w are nc-long FP64 scratch vectors.
+hipError_t precondition(const OperatorDev& op, const float* r, float* z, double* g, double* w, hipStream_t stream);
 // states[index[i]] = fresh[i] for i < count (device arrays).
 hipError_t scatter_states(const int* index, const SlotState* fresh, SlotState* states, int count, hipStream_t stream);
 }  // namespace launch

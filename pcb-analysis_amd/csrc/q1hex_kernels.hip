@@ -89,6 +89,30 @@ struct InnerVal {
     T c;  // the updated correction (owned nodes)
 };
 
+// The same with the preconditioned residual z = D^-1 r + Z w of a
+// two-level slot (the coarse term read where the node is loaded).
+template <typename T>
+struct InnerValZ {
+    T u;
+    T robin;
+    T r;
+    T d;
+    T c;
+    T z;
+};
+
+// A stencil's node loader takes (plane base, lane offset, owned) and may
+// take the node's strip row and node layer too (a two-level pass needs its
+// patch); row -1 marks a node reached through a via, found from its index.
+template <class Load>
+__device__ __forceinline__ auto call_load(Load& load, const long long pb, const unsigned off, const bool owned,
+                                          const int row, const int layer) {
+    if constexpr (std::is_invocable_v<Load&, long long, unsigned, bool, int, int>)
+        return load(pb, off, owned, row, layer);
+    else
+        return load(pb, off, owned);
+}
+
 // The operand with fixed columns zeroed, for any node-value type with u and robin.
 template <class V>
 __device__ __forceinline__ auto masked(const V& v) -> decltype(v.u) {
@@ -241,8 +265,8 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
         eoff[j] = (ecol_in && ye >= 0 && ye < R) ? (unsigned)(ye * C + xc) : kOutside;
     }
 
-    // The node-value type is whatever ``load`` returns (NodeVal or InnerVal).
-    using V = std::decay_t<decltype(load(0LL, 0u, false))>;
+    // The node-value type is whatever ``load`` returns (NodeVal, InnerVal, InnerValZ).
+    using V = std::decay_t<decltype(call_load(load, 0LL, 0u, false, 0, 0))>;
     T mlo[ROWS + 2], nlo[ROWS + 2];
     V vlo[ROWS];
     T Llo[ROWS], Rlo[ROWS];
@@ -258,7 +282,7 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
 #pragma unroll
         for (int i = 0; i < ROWS + 2; ++i) {
             const bool owned = out_plane && out_lane && i >= 1 && i <= ROWS && y0 - 1 + i <= R;
-            const V nv = load(pb, noff[i], owned);
+            const V nv = call_load(load, pb, noff[i], owned, i, z);
             m[i] = masked(nv);
             if (i >= 1 && i <= ROWS) v[i - 1] = nv;
         }
@@ -455,13 +479,13 @@ __device__ __forceinline__ void sheet_apply(const T* __restrict__ coef, const in
         const int ye = y0 - 1 + j;
         eoff[j] = (ecol_in && ye >= 0 && ye < R) ? (unsigned)(ye * C + xc) : kOutside;
     }
-    using V = std::decay_t<decltype(load(0LL, 0u, false))>;
+    using V = std::decay_t<decltype(call_load(load, 0LL, 0u, false, 0, 0))>;
     T m[ROWS + 2], n[ROWS + 2];
     V v[ROWS];
 #pragma unroll
     for (int i = 0; i < ROWS + 2; ++i) {
         const bool owned = out_lane && i >= 1 && i <= ROWS && y0 - 1 + i <= R;
-        const V nv = load(pb, noff[i], owned);
+        const V nv = call_load(load, pb, noff[i], owned, i, layer);
         m[i] = masked(nv);
         if (i >= 1 && i <= ROWS) v[i - 1] = nv;
     }
@@ -502,7 +526,7 @@ __device__ __forceinline__ void sheet_apply(const T* __restrict__ coef, const in
                 const int lo = via_ptr[node], hi = via_ptr[node + 1];
                 const T own = masked(v[i]);
                 for (int k = lo; k < hi; ++k) {
-                    const V far = load(0LL, (unsigned)via_nbr[k], false);
+                    const V far = call_load(load, 0LL, (unsigned)via_nbr[k], false, -1, -1);
                     total = fmadd(via_g[k], own - masked(far), total);
                 }
             }
@@ -974,7 +998,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish_outer(const SlotDev* 
 // on ||r_{i-1}||, the curvature of p_i, alpha_i = r.z / p.q) and forms
 // beta_{i+1} from r_i.z_i = r.z - 2 alpha q.z + alpha^2 q.D^-1 q, the one
 // value the pass cannot yet have summed.
-template <typename TI, int ROWS, int MFMA>
+template <typename TI, int ROWS, int MFMA, bool COARSE = false>
 __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotState& st, const Tile& tile) {
     const OperatorDev& op = sd.op;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -998,25 +1022,65 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
     const TI* __restrict__ robin = InnerOf<TI>::robin(op);
     CellSums<kInnerSums, ROWS> acc;
     acc.clear();
+    // Two-level slots: the coarse term of z at a node is w at its patch
+    // (free nodes; nodes outside the grid read none).  The patch column of
+    // the lane and the patch rows of the strip are fixed for the sweep, and
+    // every index is clamped into the grid.
+    [[maybe_unused]] int ccol = 0;
+    [[maybe_unused]] int crow[ROWS + 2];
+    if constexpr (COARSE) {
+        const int cb = op.cblock;
+        const int xc = x0 - 1 + (int)(threadIdx.x & (kWave - 1));
+        ccol = min(max(xc, 0), op.C) / cb;
+        const int ybase = op.kind == kSheet ? (tile.sy * kWavesPerGroup + wave) * kCellRows : y0;
+#pragma unroll
+        for (int i = 0; i < ROWS + 2; ++i) crow[i] = min(max(ybase - 1 + i, 0), op.R) / cb;
+    }
     // The correction is loaded at owned nodes only (elsewhere the offset is
     // out of range and reads zero).
-    auto load = [&](const long long pb, const unsigned off, const bool owned) -> InnerVal<TI> {
+    auto load = [&](const long long pb, const unsigned off, const bool owned, const int row, const int layer) {
         const TI rv = ldt(rin + pb, off), qv = ldt(qin + pb, off), pv = ldt(pin + pb, off);
         const TI d = ldt(dinv + pb, off);
         const TI cv = ldt(c + pb, owned ? off : kOutside);
+        const TI rb = ldt(robin + pb, off);
         const TI r = rv - alpha * qv;
-        const TI p = r * d + beta * pv;
-        return {p, ldt(robin + pb, off), r, d, keep * cv + alpha * pv};
+        if constexpr (COARSE) {
+            TI wv = TI(0);
+            if (off != kOutside && rb >= TI(0)) {
+                int patch;
+                if (row >= 0) {
+                    patch = (layer * op.crows + crow[row]) * op.ccols + ccol;
+                } else {
+                    // A via's far end: its layer, row and column from the index.
+                    const long long plane = (long long)(op.R + 1) * (op.C + 1);
+                    const long long n = pb + off;
+                    const int l = (int)(n / plane);
+                    const int rem = (int)(n - l * plane);
+                    const int y = rem / (op.C + 1), x = rem - y * (op.C + 1);
+                    patch = (l * op.crows + y / op.cblock) * op.ccols + x / op.cblock;
+                }
+                wv = (TI)sd.cw[patch];
+            }
+            const TI z = r * d + wv;
+            const TI p = z + beta * pv;
+            return InnerValZ<TI>{p, rb, r, d, keep * cv + alpha * pv, z};
+        } else {
+            const TI p = r * d + beta * pv;
+            return InnerVal<TI>{p, rb, r, d, keep * cv + alpha * pv};
+        }
     };
     // Stores wait for their plane's emit, so a plane's loads issue back to back.
-    auto emit = [&](const long long pb, const unsigned off, const int row, const TI conduction,
-                    const InnerVal<TI>& v) {
+    auto emit = [&](const long long pb, const unsigned off, const int row, const TI conduction, const auto& v) {
         const TI ap = v.robin >= TI(0) ? conduction + v.robin * v.u : v.u;
         stt(pout + pb, off, v.u);
         stt(qout + pb, off, ap);
         stt(rout + pb, off, v.r);
         stt(c + pb, off, v.c);
-        const TI z = v.r * v.d;
+        TI z;
+        if constexpr (COARSE)
+            z = v.z;
+        else
+            z = v.r * v.d;
         const int g = row / kCellRows;
         acc.v[0][g] += (double)v.u * (double)ap;
         acc.v[1][g] += (double)v.r * (double)z;
@@ -1045,8 +1109,11 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
 // One fused PCG pass, one tile of one slot per workgroup.  LOOP: the grid
 // is shorter than the table and loops over its live entries (a server
 // launches a fixed grid over a table that grows and shrinks, and spends no
-// dispatch on empty entries); otherwise the grid is the table.
-template <typename TI, int ROWS, bool LOOP>
+// dispatch on empty entries); otherwise the grid is the table.  COARSE: the
+// pass of two-level slots (the others' tiles are skipped; the plain pass
+// skips the two-level slots' tiles), a kernel of its own so the plain pass
+// keeps its registers.
+template <typename TI, int ROWS, bool LOOP, bool COARSE>
 __global__ __launch_bounds__(kGroupThreads) void k_inner_step(const SlotDev* __restrict__ slots,
                                                               const SlotState* __restrict__ states,
                                                               const Tile* __restrict__ tiles,
@@ -1057,7 +1124,9 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_step(const SlotDev* __r
             const Tile tile = tiles[xcd_tile(i, count)];
             const SlotState& st = states[tile.slot];
             if (st.phase != kPhaseInner) continue;
-            inner_step_slot<TI, ROWS, 0>(slots[tile.slot], st, tile);
+            const SlotDev& sd = slots[tile.slot];
+            if ((sd.op.nc > 0) != COARSE) continue;
+            inner_step_slot<TI, ROWS, 0, COARSE>(sd, st, tile);
         }
     } else {
         const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -1065,7 +1134,9 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_step(const SlotDev* __r
         const Tile tile = tiles[t];
         const SlotState& st = states[tile.slot];
         if (st.phase != kPhaseInner) return;
-        inner_step_slot<TI, ROWS, 0>(slots[tile.slot], st, tile);
+        const SlotDev& sd = slots[tile.slot];
+        if ((sd.op.nc > 0) != COARSE) return;
+        inner_step_slot<TI, ROWS, 0, COARSE>(sd, st, tile);
     }
 }
 
@@ -1093,6 +1164,24 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_step_block(const SlotDe
     }
 }
 
+// Sum of a[k] b[k] over k < n by one workgroup of THREADS threads: each
+// thread takes k = tid, tid + THREADS, ... in order, then the lanes of each
+// wave by the butterfly and the waves in order.  The result is thread 0's.
+template <int THREADS>
+__device__ double block_dot(const double* __restrict__ a, const double* __restrict__ b, const int n) {
+    __shared__ double waves[THREADS / kWave];
+    double s = 0.0;
+#pragma unroll 4
+    for (int k = threadIdx.x; k < n; k += THREADS) s += a[k] * b[k];
+    s = wave_sum(s);
+    if ((threadIdx.x & (kWave - 1)) == 0) waves[threadIdx.x / kWave] = s;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < THREADS / kWave; ++w) t += waves[w];
+    return t;
+}
+
 // The decisions of one pass, one workgroup per slot: the end test of
 // iteration i - 1 (hex_q1.cpp:469-489: tolerance, iteration limit, a zero or
 // non-finite r.z), then the curvature check and alpha_i of iteration i
@@ -1100,6 +1189,11 @@ __global__ __launch_bounds__(kGroupThreads) void k_inner_step_block(const SlotDe
 // the inner solve was not an iteration: its apply is not counted and its
 // outputs (in the buffers that stay unused) are dropped; c already holds the
 // final correction.
+//
+// Two-level slots (nc > 0): M^-1 = D^-1 + Z cinv Z^T.  The coarse kernels
+// have left g = Z^T q_i and h = cinv g; q_i.M^-1 q_i gains g.h, and the
+// coarse part of the next z follows r: w_i = w_{i-1} - alpha_i h, which is
+// cinv Z^T r_i for r_i = r_{i-1} - alpha_i q_i.
 template <typename TI>
 __global__ __launch_bounds__(kFinishThreads) void k_finish_step(const SlotDev* __restrict__ slots, SlotState* states,
                                                                 const SlotConfig* __restrict__ configs, int* active) {
@@ -1110,6 +1204,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish_step(const SlotDev* _
     const SlotDev& sd = slots[slot];
     const double* part = sd.part;
     const int stride = sd.part_stride, cells = sd.cells;
+    const int nc = sd.op.nc;
     if (st.phase != kPhaseInner) return;
     int first = 0, inner_its = 0;
     double rhs_norm = 1.0, rz_prev = 0.0;
@@ -1123,41 +1218,179 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish_step(const SlotDev* _
     }
     double total[kInnerSums];
     slot_sum<kInnerSums, kFinishThreads>(part, stride, cells, total);
-    if (threadIdx.x != 0) return;
-    const double pq = total[0], rz = total[1], qz = total[2], qdq = total[3], rr = total[4];
-    if (first == 0) {
-        inner_its += 1;
-        st.inner_its = inner_its;
-        const double rel = sqrt(rr) / rhs_norm;
-        st.rel_inner = rel;
-        const bool end = rel <= cfg.inner_tol || inner_its >= cfg.max_inner || !isfinite(rz) || rz_prev == 0.0;
-        if (end) {
-            double* h = sd.history + 4 * st.outer;
-            h[0] = st.outer + 1;
-            h[1] = st.rel_outer;
-            h[2] = inner_its;
-            h[3] = rel;
-            st.total_inner += inner_its;
-            st.outer += 1;
-            st.pending = kPendingCorrection;
-            st.phase = kPhaseOuter;
-            return;
+    double gh = 0.0;
+    if (nc > 0) gh = block_dot<kFinishThreads>(sd.cg, sd.ch, nc);
+    int go = 0;
+    double alpha = 0.0;
+    if (threadIdx.x == 0) {
+        const double pq = total[0], rz = total[1], qz = total[2], qdq = total[3], rr = total[4];
+        bool ended = false;
+        if (first == 0) {
+            inner_its += 1;
+            st.inner_its = inner_its;
+            const double rel = sqrt(rr) / rhs_norm;
+            st.rel_inner = rel;
+            ended = rel <= cfg.inner_tol || inner_its >= cfg.max_inner || !isfinite(rz) || rz_prev == 0.0;
+            if (ended) {
+                double* h = sd.history + 4 * st.outer;
+                h[0] = st.outer + 1;
+                h[1] = st.rel_outer;
+                h[2] = inner_its;
+                h[3] = rel;
+                st.total_inner += inner_its;
+                st.outer += 1;
+                st.pending = kPendingCorrection;
+                st.phase = kPhaseOuter;
+            }
+        }
+        if (!ended) {
+            st.low_apps += 1;
+            if (!isfinite(pq) || pq <= 0.0) {
+                st.status = kStatusNotSpd;
+                st.phase = kPhaseDone;
+                atomicSub(active, 1);
+            } else {
+                alpha = (double)(TI)(rz / pq);
+                const double next_rz = rz - 2.0 * alpha * qz + alpha * alpha * (qdq + gh);
+                st.rz = rz;
+                st.alpha = alpha;
+                st.beta = rz != 0.0 ? (double)(TI)(next_rz / rz) : 0.0;
+                st.first = 0;
+                st.pcur = 1 - st.pcur;
+                go = 1;
+            }
         }
     }
-    st.low_apps += 1;
-    if (!isfinite(pq) || pq <= 0.0) {
-        st.status = kStatusNotSpd;
-        st.phase = kPhaseDone;
-        atomicSub(active, 1);
-        return;
+    if (nc == 0) return;
+    __shared__ double shared_alpha;
+    __shared__ int shared_go;
+    if (threadIdx.x == 0) {
+        shared_alpha = alpha;
+        shared_go = go;
     }
-    const double alpha = (double)(TI)(rz / pq);
-    const double next_rz = rz - 2.0 * alpha * qz + alpha * alpha * qdq;
-    st.rz = rz;
-    st.alpha = alpha;
-    st.beta = rz != 0.0 ? (double)(TI)(next_rz / rz) : 0.0;
-    st.first = 0;
-    st.pcur = 1 - st.pcur;
+    __syncthreads();
+    if (!shared_go) return;
+    const double a = shared_alpha;
+    for (int k = threadIdx.x; k < nc; k += kFinishThreads) sd.cw[k] -= a * sd.ch[k];
+}
+
+// ------------------------------------------------- two-level coarse kernels
+//
+// The coarse space of the reference's AggregationCoarseCorrection
+// (src/electrical/matrix_free_mpir_fem/two_level.py:54-229; native
+// precondition hex_q1.cpp:388-432): functions constant over cblock x cblock
+// node patches of one node layer, zero on fixed nodes.  Z^T v sums v over the
+// free nodes of each patch; Z w spreads w[patch] over the patch's free nodes.
+
+// Z^T v at one patch, one workgroup of four waves: wave k sums patch rows
+// k, k + 4, ... (each lane its columns down those rows, in row order), then
+// the lanes by the butterfly and the four waves in order.  Every thread
+// returns the sum.
+template <typename TI>
+__device__ double patch_sum(const OperatorDev& op, const TI* __restrict__ v, const TI* __restrict__ robin,
+                            const int layer, const int cy, const int cx) {
+    __shared__ double waves[kWavesPerGroup];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int b = op.cblock;
+    const long long NC = op.C + 1, NR = op.R + 1;
+    const int y_hi = min(op.R + 1, (cy + 1) * b), x_lo = cx * b, x_hi = min(op.C + 1, (cx + 1) * b);
+    double s = 0.0;
+#pragma unroll 4
+    for (int y = cy * b + wave; y < y_hi; y += kWavesPerGroup) {
+        const long long row = ((long long)layer * NR + y) * NC;
+        for (int x = x_lo + lane; x < x_hi; x += kWave) {
+            const TI value = v[row + x];
+            if (robin[row + x] >= TI(0)) s += (double)value;
+        }
+    }
+    s = wave_sum(s);
+    if (lane == 0) waves[wave] = s;
+    __syncthreads();
+    return ((waves[0] + waves[1]) + waves[2]) + waves[3];
+}
+
+// One row of cinv g by one wave: lane l takes columns l, l + 64, ... in
+// order, then the butterfly.  Every lane returns the sum.
+__device__ __forceinline__ double coarse_row(const float* __restrict__ cinv, const int nc,
+                                             const double* __restrict__ g, const int row) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const float* a = cinv + (long long)row * nc;
+    double s = 0.0;
+#pragma unroll 4
+    for (int k = lane; k < nc; k += kWave) s += (double)a[k] * g[k];
+    return wave_sum(s);
+}
+
+// The restriction of a two-level slot, one patch per workgroup.  which 0:
+// slots that have just entered an inner solve (r_0, written by the outer
+// step); which 1: after a pass, its new q.
+template <typename TI>
+__global__ __launch_bounds__(kGroupThreads) void k_coarse_restrict(const SlotDev* __restrict__ slots,
+                                                                   const SlotState* __restrict__ states,
+                                                                   const Tile* __restrict__ tiles,
+                                                                   const int* __restrict__ counts, const int which) {
+    const int t = blockIdx.x;
+    if (t >= counts[4]) return;
+    const Tile tile = tiles[t];
+    const SlotState& st = states[tile.slot];
+    if (st.phase != kPhaseInner || (which == 0 && st.first == 0)) return;
+    const SlotDev& sd = slots[tile.slot];
+    const OperatorDev& op = sd.op;
+    const TI* v = static_cast<const TI*>(which == 0 ? sd.r[st.pcur] : sd.q[1 - st.pcur]);
+    const double s = patch_sum<TI>(op, v, InnerOf<TI>::robin(op), tile.sz, tile.sy, tile.sx);
+    if (threadIdx.x == 0) sd.cg[((long long)tile.sz * op.crows + tile.sy) * op.ccols + tile.sx] = s;
+}
+
+// cinv g of a two-level slot, kCoarseRows rows per workgroup: into w (which
+// 0, the start of an inner solve) or h (which 1, after a pass).
+__global__ __launch_bounds__(kWave * kCoarseRows) void k_coarse_rows(const SlotDev* __restrict__ slots,
+                                                                    const SlotState* __restrict__ states,
+                                                                    const Tile* __restrict__ tiles,
+                                                                    const int* __restrict__ counts, const int which) {
+    const int t = blockIdx.x;
+    if (t >= counts[5]) return;
+    const Tile tile = tiles[t];
+    const SlotState& st = states[tile.slot];
+    if (st.phase != kPhaseInner || (which == 0 && st.first == 0)) return;
+    const SlotDev& sd = slots[tile.slot];
+    const int row = tile.index + (int)(threadIdx.x / kWave);
+    if (row >= sd.op.nc) return;
+    const double s = coarse_row(sd.op.cinv, sd.op.nc, sd.cg, row);
+    if ((threadIdx.x & (kWave - 1)) == 0) (which == 0 ? sd.cw : sd.ch)[row] = s;
+}
+
+// The standalone preconditioner z = M^-1 r of one operator in FP32
+// (operator.py:366-371): patch sums, coarse rows, then z = D^-1 r + Z w.
+__global__ __launch_bounds__(kGroupThreads) void k_op_restrict(const OperatorDev op, const float* __restrict__ r,
+                                                               double* __restrict__ g) {
+    const int patch = blockIdx.x;
+    const int per_layer = op.crows * op.ccols;
+    const int layer = patch / per_layer, cy = (patch % per_layer) / op.ccols, cx = patch % op.ccols;
+    const double s = patch_sum<float>(op, r, op.robin32, layer, cy, cx);
+    if (threadIdx.x == 0) g[patch] = s;
+}
+
+__global__ __launch_bounds__(kWave * kCoarseRows) void k_op_rows(const OperatorDev op, const double* __restrict__ g,
+                                                                double* __restrict__ w) {
+    const int row = blockIdx.x * kCoarseRows + (int)(threadIdx.x / kWave);
+    if (row >= op.nc) return;
+    const double s = coarse_row(op.cinv, op.nc, g, row);
+    if ((threadIdx.x & (kWave - 1)) == 0) w[row] = s;
+}
+
+__global__ __launch_bounds__(256) void k_op_prolong(const OperatorDev op, const float* __restrict__ r,
+                                                    float* __restrict__ z, const double* __restrict__ w) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= op.N) return;
+    float value = r[n] * op.dinv32[n];
+    if (op.nc > 0 && op.robin32[n] >= 0.0f) {
+        const long long NC = op.C + 1, plane = (long long)(op.R + 1) * NC;
+        const int l = (int)(n / plane);
+        const int rem = (int)(n - l * plane);
+        const int y = (int)(rem / NC), x = (int)(rem - y * NC);
+        value += (float)w[((long long)l * op.crows + y / op.cblock) * op.ccols + x / op.cblock];
+    }
+    z[n] = value;
 }
 
 // Writes fresh states of newly admitted slots of a server: one thread per
@@ -1264,51 +1497,84 @@ hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t st
     return hipGetLastError();
 }
 
+// The restriction and the coarse rows of the two-level slots (which as in
+// k_coarse_restrict).
+template <typename TI>
+void coarse_kernels(const SlotDev* slots, const SlotState* states, const int* counts, const CoarseLaunch& coarse,
+                    const int which, hipStream_t stream) {
+    if (coarse.grid_patches <= 0) return;
+    hipLaunchKernelGGL(k_coarse_restrict<TI>, dim3(coarse.grid_patches), dim3(kGroupThreads), 0, stream, slots, states,
+                       coarse.patches, counts, which);
+    hipLaunchKernelGGL(k_coarse_rows, dim3(coarse.grid_rows), dim3(kWave * kCoarseRows), 0, stream, slots, states,
+                       coarse.rows, counts, which);
+}
+
 hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState* states,
                       const SlotConfig* configs, const Tile* outer_tiles, const int* counts, int grid_outer,
-                      int* active, hipStream_t stream) {
+                      int* active, const CoarseLaunch& coarse, hipStream_t stream) {
     const dim3 block(kGroupThreads);
     const dim3 grid(grid_outer < kOuterGroups ? grid_outer : kOuterGroups);
     if (inner64) {
         hipLaunchKernelGGL(k_outer<double>, grid, block, 0, stream, slots, states, outer_tiles, counts);
         hipLaunchKernelGGL(k_finish_outer<double>, dim3(n_slots), dim3(kFinishThreads), 0, stream, slots, states,
                            configs, active);
+        coarse_kernels<double>(slots, states, counts, coarse, 0, stream);
     } else {
         hipLaunchKernelGGL(k_outer<float>, grid, block, 0, stream, slots, states, outer_tiles, counts);
         hipLaunchKernelGGL(k_finish_outer<float>, dim3(n_slots), dim3(kFinishThreads), 0, stream, slots, states,
                            configs, active);
+        coarse_kernels<float>(slots, states, counts, coarse, 0, stream);
     }
     return hipGetLastError();
 }
 
 hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const SlotState* states,
                       const Tile* inner_tiles, const int* counts, int grid_inner, bool loop, const Tile* block_tiles,
-                      const SlotGroup* groups, int grid_block, int block_mfma, hipStream_t stream) {
+                      const SlotGroup* groups, int grid_block, int block_mfma, const CoarseLaunch& coarse,
+                      hipStream_t stream) {
     const dim3 block(kGroupThreads);
     const dim3 grid(grid_inner);
+    const bool two_level = coarse.grid_patches > 0;
+    // The two-level pass runs over the batch's table only (one workgroup per tile).
+    if (two_level && loop) return hipErrorInvalidValue;
     if (inner64) {
-        if (loop)
-            hipLaunchKernelGGL((k_inner_step<double, kRowsInner64, true>), grid, block, 0, stream, slots, states,
-                               inner_tiles, counts);
-        else
-            hipLaunchKernelGGL((k_inner_step<double, kRowsInner64, false>), grid, block, 0, stream, slots, states,
-                               inner_tiles, counts);
+        if (coarse.plain) {
+            if (loop)
+                hipLaunchKernelGGL((k_inner_step<double, kRowsInner64, true, false>), grid, block, 0, stream, slots,
+                                   states, inner_tiles, counts);
+            else
+                hipLaunchKernelGGL((k_inner_step<double, kRowsInner64, false, false>), grid, block, 0, stream, slots,
+                                   states, inner_tiles, counts);
+        }
+        if (two_level)
+            hipLaunchKernelGGL((k_inner_step<double, kRowsInner64, false, true>), grid, block, 0, stream, slots,
+                               states, inner_tiles, counts);
         return hipGetLastError();
     }
     if (rows_inner == 8) {
-        if (loop)
-            hipLaunchKernelGGL((k_inner_step<float, 8, true>), grid, block, 0, stream, slots, states, inner_tiles,
-                               counts);
-        else
-            hipLaunchKernelGGL((k_inner_step<float, 8, false>), grid, block, 0, stream, slots, states, inner_tiles,
-                               counts);
+        if (coarse.plain) {
+            if (loop)
+                hipLaunchKernelGGL((k_inner_step<float, 8, true, false>), grid, block, 0, stream, slots, states,
+                                   inner_tiles, counts);
+            else
+                hipLaunchKernelGGL((k_inner_step<float, 8, false, false>), grid, block, 0, stream, slots, states,
+                                   inner_tiles, counts);
+        }
+        if (two_level)
+            hipLaunchKernelGGL((k_inner_step<float, 8, false, true>), grid, block, 0, stream, slots, states,
+                               inner_tiles, counts);
     } else {
-        if (loop)
-            hipLaunchKernelGGL((k_inner_step<float, 4, true>), grid, block, 0, stream, slots, states, inner_tiles,
-                               counts);
-        else
-            hipLaunchKernelGGL((k_inner_step<float, 4, false>), grid, block, 0, stream, slots, states, inner_tiles,
-                               counts);
+        if (coarse.plain) {
+            if (loop)
+                hipLaunchKernelGGL((k_inner_step<float, 4, true, false>), grid, block, 0, stream, slots, states,
+                                   inner_tiles, counts);
+            else
+                hipLaunchKernelGGL((k_inner_step<float, 4, false, false>), grid, block, 0, stream, slots, states,
+                                   inner_tiles, counts);
+        }
+        if (two_level)
+            hipLaunchKernelGGL((k_inner_step<float, 4, false, true>), grid, block, 0, stream, slots, states,
+                               inner_tiles, counts);
     }
     if (grid_block > 0) {
         if (block_mfma == 3) {
@@ -1328,16 +1594,28 @@ hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const 
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
                       const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner, bool loop,
                       const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma, int* active,
-                      hipStream_t stream) {
+                      const CoarseLaunch& coarse, hipStream_t stream) {
     const hipError_t err = inner_pass(inner64, rows_inner, slots, states, inner_tiles, counts, grid_inner, loop,
-                                      block_tiles, groups, grid_block, block_mfma, stream);
+                                      block_tiles, groups, grid_block, block_mfma, coarse, stream);
     if (err != hipSuccess) return err;
     const dim3 block(kFinishThreads);
     if (inner64) {
+        coarse_kernels<double>(slots, states, counts, coarse, 1, stream);
         hipLaunchKernelGGL(k_finish_step<double>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
     } else {
+        coarse_kernels<float>(slots, states, counts, coarse, 1, stream);
         hipLaunchKernelGGL(k_finish_step<float>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
     }
+    return hipGetLastError();
+}
+
+hipError_t precondition(const OperatorDev& op, const float* r, float* z, double* g, double* w, hipStream_t stream) {
+    if (op.nc > 0) {
+        hipLaunchKernelGGL(k_op_restrict, dim3(op.nc), dim3(kGroupThreads), 0, stream, op, r, g);
+        hipLaunchKernelGGL(k_op_rows, dim3((op.nc + kCoarseRows - 1) / kCoarseRows), dim3(kWave * kCoarseRows), 0,
+                           stream, op, g, w);
+    }
+    hipLaunchKernelGGL(k_op_prolong, dim3((unsigned)((op.N + 255) / 256)), dim3(256), 0, stream, op, r, z, w);
     return hipGetLastError();
 }
 

@@ -16,8 +16,10 @@ from .pcb import (
 from .ports import DCPortBasis, PortSet, dc_port_basis, reduce_correlation
 from .runtime import HipFloat32Runtime, RuntimeBackend, make_float32_runtime
 from .solver import MPIRConfig, MPIRResult, MPIRStep, mpir_result_from_core, solve_mpir, solve_mpir_device
+from .two_level import DEFAULT_MAX_COARSE_SIZE, AggregationCoarseCorrection, choose_block_size
 
 __all__ = [
+    "AggregationCoarseCorrection",
     "COPPER_CONDUCTIVITY_S_PER_M",
     "CurrentTerminal",
     "DCPortBasis",
@@ -33,6 +35,8 @@ __all__ = [
     "MPIRResult",
     "MPIRStep",
     "RuntimeBackend",
+    "DEFAULT_MAX_COARSE_SIZE",
+    "choose_block_size",
     "dc_port_basis",
     "make_float32_runtime",
     "mpir_result_from_core",

@@ -14,7 +14,6 @@ server paths as the hexahedral operator.
 from __future__ import annotations
 
 import dataclasses
-import warnings
 from dataclasses import dataclass
 from typing import Literal, Sequence
 
@@ -24,7 +23,8 @@ from pcbhip import _capi, serving
 
 from .grid import check_pitch_axis
 from .runtime import make_float32_runtime
-from .solver import MPIRConfig, MPIRResult, jacobi_default_config, solve_mpir
+from .solver import MPIRConfig, MPIRResult, default_config, solve_mpir
+from .two_level import AggregationCoarseCorrection
 
 Node = tuple[int, int, int]
 Preconditioner = Literal["two-level", "jacobi"]
@@ -236,36 +236,32 @@ def _element_views(grid):
     return grid[:, :-1, :-1], grid[:, :-1, 1:], grid[:, 1:, :-1], grid[:, 1:, 1:]
 
 
-_two_level_warned = False
-
-
 class MatrixFreePCBOperator:
     """The layered-sheet conductance operator on the device (pcb.py:275-418).
 
     Fixed nodes are the inactive nodes (no active element and no via), the
     ``reference_node`` and every ``dirichlet_nodes`` entry; the operator is
     the identity on fixed rows and ignores fixed columns, so it stays SPD.
-    ``preconditioner="two-level"`` is accepted and runs Jacobi with a warning."""
+    ``preconditioner="two-level"`` (default) adds the patch-constant coarse
+    correction of :mod:`.two_level`, each copper layer a patch layer of its
+    own (pcb.py:405-418)."""
 
     def __init__(self, mesh: LayeredPCBMesh, *, reference_node: Node | None = None,
                  dirichlet_nodes: Sequence[Node] = (), vias: Sequence[ViaConnection] = (), runtime=None,
                  backend: str | None = None, device_id: int = 0, preconditioner: Preconditioner = "two-level",
                  coarse_block_nodes: int | None = None, native: bool | None = None) -> None:
-        global _two_level_warned
         if runtime is not None and backend is not None:
             raise ValueError("pass either runtime or backend, not both")
         if preconditioner not in ("two-level", "jacobi"):
             raise ValueError("preconditioner must be 'two-level' or 'jacobi'")
         if native is False:
             raise ValueError("native=False asks for the NumPy reference path, which this package does not ship")
-        if preconditioner == "two-level" and not _two_level_warned:
-            warnings.warn("the two-level coarse correction is not on the HIP path yet; using Jacobi", stacklevel=2)
-            _two_level_warned = True
         if reference_node is None and not dirichlet_nodes:
             raise ValueError("pass a reference_node or dirichlet_nodes to fix the potential")
         self.high_dtype = np.float64
         self.inner_solver = "pcg"
-        self.preconditioner = "jacobi"
+        self.preconditioner = preconditioner
+        self.coarse_correction: AggregationCoarseCorrection | None = None
         self.mesh = mesh
         self.size = mesh.size
         self.runtime = runtime or make_float32_runtime(backend or "hip", device_id=device_id)
@@ -304,6 +300,12 @@ class MatrixFreePCBOperator:
             raise
         self.low_operator_backend = "hip-sheet-q1-fp32"
         self.high_operator_backend = "hip-sheet-q1-fp64"
+        if preconditioner == "two-level":
+            self.coarse_correction = AggregationCoarseCorrection(
+                node_shape=shape, free_nodes=self.free_nodes, diagonal_high=self._device.diagonal(),
+                apply_high=self.apply_high, block=coarse_block_nodes, layer_colours=shape[0],
+            )
+            self._device.set_coarse(self.coarse_correction.block, self.coarse_correction.coarse_inverse)
 
     @property
     def device_operator(self) -> _capi.Operator:
@@ -329,6 +331,11 @@ class MatrixFreePCBOperator:
 
     def diagonal_low(self) -> np.ndarray:
         return self._device.diagonal().astype(np.float32)
+
+    def precondition_low(self, vector) -> np.ndarray:
+        """``M^-1 r`` in FP32 on the device (pcb.py:712-717)."""
+
+        return self._device.precondition_f32(np.asarray(vector, dtype=np.float32).reshape(-1))
 
     def native_solve_mpir(self, rhs, config: MPIRConfig, initial_guess=None):
         x, r = serving.solve_mpir_routed(self._device, rhs, initial_guess, config.to_c())
@@ -444,7 +451,7 @@ def solve_pcb_dc(problem: PCBConductionProblem, *, config: MPIRConfig | None = N
         initial = np.where(operator.free_nodes.reshape(-1), initial, fixed)
     elif problem.voltage_terminals:
         initial = fixed
-    result = solve_mpir(operator, rhs, config=config or jacobi_default_config(), initial_guess=initial)
+    result = solve_mpir(operator, rhs, config=config or default_config(operator.preconditioner), initial_guess=initial)
     potential = result.solution.reshape(problem.mesh.node_shape)
     field = operator.element_electric_field(potential)
     density = problem.mesh.conductivity_s_per_m[..., None] * field

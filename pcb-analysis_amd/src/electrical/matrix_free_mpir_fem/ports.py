@@ -28,7 +28,7 @@ import numpy as np
 from pcbhip import _capi
 
 from .pcb import LayeredPCBMesh, MatrixFreePCBOperator, Node, Preconditioner, ViaConnection, VoltageTerminal
-from .solver import MPIRConfig, MPIRResult, MPIRStep, jacobi_default_config
+from .solver import MPIRConfig, MPIRResult, MPIRStep, default_config
 
 
 def reduce_correlation(ports: "PortSet", correlation_a2: np.ndarray) -> tuple[np.ndarray, float]:
@@ -217,7 +217,7 @@ def dc_port_basis(mesh: LayeredPCBMesh, ports: PortSet, *, vias: Sequence[ViaCon
     )
     if initial is not None and (initial.ports != ports or initial.mesh.node_shape != mesh.node_shape):
         raise ValueError("initial basis must share the ports and the mesh shape")
-    config = config or jacobi_default_config()
+    config = config or default_config(operator.preconditioner)
     n = ports.count
     driven = list(ports.driven)
     free = operator.free_nodes.reshape(-1)

@@ -67,10 +67,10 @@ class MPIRConfig:
 
 
 # The reference's default limits (8 outer, 200 inner iterations) are sized for
-# its two-level preconditioner (two_level.py:54-229).  Jacobi, the device
-# path's preconditioner, needs longer inner solves to reach the same 2e-3
-# inner tolerance, so the front ends give it this inner budget when the
-# caller passes no configuration.
+# its default two-level preconditioner (two_level.py:54-229).  Jacobi needs
+# longer inner solves to reach the same 2e-3 inner tolerance, so the front
+# ends give a Jacobi solve this inner budget when the caller passes no
+# configuration.
 JACOBI_MAX_INNER_ITERATIONS = 20000
 
 
@@ -78,6 +78,15 @@ def jacobi_default_config(max_outer_iterations: int = 8) -> MPIRConfig:
     """The default limits of a front end for the Jacobi-preconditioned device solve."""
 
     return MPIRConfig(max_outer_iterations=max_outer_iterations, max_inner_iterations=JACOBI_MAX_INNER_ITERATIONS)
+
+
+def default_config(preconditioner: str, max_outer_iterations: int = 8) -> MPIRConfig:
+    """A front end's limits when the caller passes none: the reference's own
+    with the two-level preconditioner, the Jacobi budget with Jacobi."""
+
+    if preconditioner == "two-level":
+        return MPIRConfig(max_outer_iterations=max_outer_iterations)
+    return jacobi_default_config(max_outer_iterations)
 
 
 @dataclass(frozen=True)

@@ -39,6 +39,8 @@ EXPORTS = (
     "pcbhip_operator_destroy",
     "pcbhip_operator_size",
     "pcbhip_operator_diagonal",
+    "pcbhip_operator_set_coarse",
+    "pcbhip_operator_precondition_f32",
     "pcbhip_apply_f32",
     "pcbhip_apply_f64",
     "pcbhip_stiffness_f64",
@@ -133,6 +135,8 @@ def load() -> ctypes.CDLL:
     lib.pcbhip_operator_size.argtypes = [_vp]
     lib.pcbhip_operator_size.restype = ctypes.c_int64
     lib.pcbhip_operator_diagonal.argtypes = [_vp, _vp]
+    lib.pcbhip_operator_set_coarse.argtypes = [_vp, _int, _int, _vp]
+    lib.pcbhip_operator_precondition_f32.argtypes = [_vp, _vp, _vp]
     for name in ("pcbhip_apply_f32", "pcbhip_apply_f64", "pcbhip_stiffness_f64"):
         getattr(lib, name).argtypes = [_vp, _vp, _vp]
     for name in ("pcbhip_apply_f32_dev", "pcbhip_apply_f64_dev", "pcbhip_stiffness_f64_dev"):
@@ -213,6 +217,7 @@ class Operator:
         self.device = int(device)
         self.element_shape = (s, r, c)
         self.size = n
+        self.coarse_size = 0
         h = _vp()
         check(lib.pcbhip_operator_create(self.device, s, r, c, _ptr(coef), _ptr(robin), _ptr(free), ctypes.byref(h)))
         self.handle = h
@@ -238,6 +243,7 @@ class Operator:
         self.device = int(device)
         self.element_shape = (layers, r, c)
         self.size = n
+        self.coarse_size = 0
         h = _vp()
         check(self._lib.pcbhip_sheet_operator_create(self.device, layers, r, c, _ptr(coef), _ptr(free), int(a.size),
                                                      _ptr(a) if a.size else None, _ptr(b) if b.size else None,
@@ -283,6 +289,29 @@ class Operator:
         v = self._vec(x, np.float64)
         out = np.empty(self.size)
         check(self._lib.pcbhip_stiffness_f64(self.handle, _ptr(v), _ptr(out)))
+        return out
+
+    def set_coarse(self, block: int, inverse: np.ndarray | None) -> None:
+        """Two-level preconditioning from now on: ``inverse`` is the dense
+        inverse of the coarse Galerkin matrix of ``block``-wide patches
+        (``None`` returns to Jacobi)."""
+
+        if inverse is None:
+            check(self._lib.pcbhip_operator_set_coarse(self.handle, 0, 0, None))
+            self.coarse_size = 0
+            return
+        inv = np.ascontiguousarray(inverse, dtype=np.float64)
+        if inv.ndim != 2 or inv.shape[0] != inv.shape[1]:
+            raise ValueError("the coarse inverse must be a square matrix")
+        check(self._lib.pcbhip_operator_set_coarse(self.handle, int(block), int(inv.shape[0]), _ptr(inv)))
+        self.coarse_size = int(inv.shape[0])
+
+    def precondition_f32(self, r) -> np.ndarray:
+        """``M^-1 r`` in FP32 with the operator's preconditioner."""
+
+        v = self._vec(r, np.float32)
+        out = np.empty(self.size, dtype=np.float32)
+        check(self._lib.pcbhip_operator_precondition_f32(self.handle, _ptr(v), _ptr(out)))
         return out
 
     def set_shape(self, rows32: int, zb32: int, rows64: int = 4, zb64: int = 1 << 20) -> None:

@@ -3,6 +3,8 @@ iterative refinement on an MI355X (the reference's
 ``thermal.matrix_free_mpir_fem`` API; see the package modules for the
 reference files each one restates)."""
 
+from electrical.matrix_free_mpir_fem.two_level import AggregationCoarseCorrection, choose_block_size
+
 from .boundaries import Convection, ConvectionBoundary, ExposedFaceConvection, HeatSource
 from .coupling import element_heat_from_losses, element_joule_heat_w, via_heat_sources, via_joule_heat_sources
 from .mesh import (
@@ -29,6 +31,7 @@ from .solve import ThermalConductionSolution, solve_thermal_conduction
 from .transient import TimeSchedule, TransientStep, TransientThermalSolution, solve_thermal_transient
 
 __all__ = [
+    "AggregationCoarseCorrection",
     "COPPER_THERMAL_CONDUCTIVITY_W_PER_M_K",
     "Convection",
     "ConvectionBoundary",
@@ -48,6 +51,7 @@ __all__ = [
     "TimeSchedule",
     "TransientStep",
     "TransientThermalSolution",
+    "choose_block_size",
     "element_heat_from_losses",
     "element_joule_heat_w",
     "element_temperature_k",

@@ -13,13 +13,13 @@ on the diagonal, so the operator is symmetric positive definite.
 
 from __future__ import annotations
 
-import warnings
 from typing import Any
 
 import numpy as np
 
 from electrical.matrix_free_mpir_fem.runtime import HipFloat32Runtime, make_float32_runtime
 from electrical.matrix_free_mpir_fem.solver import MPIRConfig
+from electrical.matrix_free_mpir_fem.two_level import AggregationCoarseCorrection
 from pcbhip import _capi, serving
 
 from .mesh import Preconditioner, corner_views, flat_index, unit_hexahedron_matrices
@@ -27,7 +27,6 @@ from .problem import ThermalConductionProblem
 
 LOW_KERNEL_NAME = "hip-gfx950-sum-factorised-hex-q1"
 HIGH_KERNEL_NAME = "hip-gfx950-sum-factorised-hex-q1-fp64"
-_two_level_warned = False
 
 
 class MatrixFreeThermalOperator:
@@ -45,18 +44,14 @@ class MatrixFreeThermalOperator:
         native: bool | None = None,
         capacity_per_s: np.ndarray | None = None,
     ) -> None:
-        global _two_level_warned
         if runtime is not None and backend is not None:
             raise ValueError("pass either runtime or backend, not both")
         if preconditioner not in ("two-level", "jacobi"):
             raise ValueError("preconditioner must be 'two-level' or 'jacobi'")
         if native is False:
             raise ValueError("native=False asks for the NumPy reference path, which this package does not ship")
-        if preconditioner == "two-level" and not _two_level_warned:
-            warnings.warn("the two-level coarse correction is not on the HIP path yet; using Jacobi", stacklevel=2)
-            _two_level_warned = True
-        self.preconditioner = "jacobi"
-        self.coarse_correction = None
+        self.preconditioner = preconditioner
+        self.coarse_correction: AggregationCoarseCorrection | None = None
         self.high_dtype = np.float64
         self.inner_solver = "pcg"
         self.problem = problem
@@ -104,6 +99,15 @@ class MatrixFreeThermalOperator:
         )
         self._diagonal_high = self._device.diagonal()
         self._diagonal_low = self._diagonal_high.astype(np.float32)
+        if preconditioner == "two-level":
+            # The coarse space of the reference's default preconditioner
+            # (operator.py:168-178): assembled from device FP64 applies, its
+            # inverse resident on the device for every solve of this operator.
+            self.coarse_correction = AggregationCoarseCorrection(
+                node_shape=mesh.node_shape, free_nodes=self.free_nodes, diagonal_high=self._diagonal_high,
+                apply_high=self.apply_high, block=coarse_block_nodes,
+            )
+            self._device.set_coarse(self.coarse_correction.block, self.coarse_correction.coarse_inverse)
         self.low_operator_backend = LOW_KERNEL_NAME
         self.high_operator_backend = HIGH_KERNEL_NAME
 
@@ -134,6 +138,15 @@ class MatrixFreeThermalOperator:
 
     def diagonal_low(self) -> np.ndarray:
         return self._diagonal_low
+
+    def precondition_low(self, vector: Any) -> np.ndarray:
+        """``M^-1 r`` in FP32 on the device: Jacobi, plus the coarse
+        correction with the two-level preconditioner (operator.py:366-371)."""
+
+        v = np.ascontiguousarray(vector, dtype=np.float32).reshape(-1)
+        if v.size != self.size:
+            raise ValueError(f"vector has size {v.size}, expected {self.size}")
+        return self._device.precondition_f32(v)
 
     def native_solve_mpir(self, rhs_high: np.ndarray, config: MPIRConfig, initial_guess: np.ndarray | None = None):
         """The whole MPIR solve on the device (operator.py:318-347 contract)."""

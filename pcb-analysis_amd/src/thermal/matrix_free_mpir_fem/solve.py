@@ -16,7 +16,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from electrical.matrix_free_mpir_fem.solver import MPIRConfig, MPIRResult, jacobi_default_config, solve_mpir
+from electrical.matrix_free_mpir_fem.solver import MPIRConfig, MPIRResult, default_config, solve_mpir
 
 from .mesh import Preconditioner
 from .operator import MatrixFreeThermalOperator
@@ -224,9 +224,9 @@ def _solve_linear(
     initial = initial_rise(operator, initial_temperature_k, reference)
     if config is None:
         # A stiff copper/laminate stack lets the FP32 inner solve buy only
-        # about one decade per outer step (solve.py:223-226); the inner
-        # budget is the Jacobi one.
-        config = jacobi_default_config(16)
+        # about one decade per outer step (solve.py:223-226): 16 outer steps,
+        # and with Jacobi the Jacobi inner budget.
+        config = default_config(operator.preconditioner, 16)
     result = solve_mpir(operator, rhs, config=config, initial_guess=initial)
     free = operator.free_nodes.reshape(-1)
     if not result.converged and np.any(free):

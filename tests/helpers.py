@@ -56,6 +56,28 @@ def stack(rows: int, cols: int, *, fixed: bool = False, slabs: tuple = (35.0e-6,
     )
 
 
+def copper_stack(rows: int, cols: int) -> ThermalConductionProblem:
+    """The reference's two-level fixture (tests/test_thermal_conduction.py:
+    238-257): copper/FR-4/copper, a heated trace on the top slab, both faces
+    cooled, no fixed node."""
+
+    mesh = LayeredThermalMesh(
+        slab_thickness_m=(35.0e-6, 1.5e-3, 35.0e-6),
+        pitch_x_m=0.2e-3,
+        pitch_y_m=0.2e-3,
+        conductivity_w_per_m_k=(385.0, 0.8, 385.0),
+        through_plane_conductivity_w_per_m_k=(385.0, 0.3, 385.0),
+        element_shape=(rows, cols),
+    )
+    heat = np.zeros(mesh.element_grid_shape)
+    heat[2, rows // 2, cols // 4 : 3 * cols // 4] = 0.5 / (cols // 2)
+    return ThermalConductionProblem(
+        mesh,
+        convection=(ConvectionBoundary("top", 10.0, 298.15), ConvectionBoundary("bottom", 10.0, 298.15)),
+        element_heat_w=heat,
+    )
+
+
 def random_body(shape, seed: int = 0, *, void: bool = False):
     """Per-element anisotropic conductivity over 5 decades, graded pitches,
     optional void elements, both face kinds of convection and fixed nodes."""

@@ -36,7 +36,7 @@ from thermal.matrix_free_mpir_fem import (
     via_joule_heat_sources,
 )
 
-pytestmark = [pytest.mark.gpu, pytest.mark.filterwarnings("ignore:the two-level")]
+pytestmark = pytest.mark.gpu
 
 AMBIENT = 298.15
 PITCH = 0.5e-3

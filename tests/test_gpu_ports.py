@@ -23,7 +23,7 @@ from electrical.matrix_free_mpir_fem import (
 from oracle import sheet as O
 from pcbhip import _capi
 
-pytestmark = [pytest.mark.gpu, pytest.mark.filterwarnings("ignore:the two-level")]
+pytestmark = pytest.mark.gpu
 
 TIGHT = MPIRConfig(relative_tolerance=1.0e-12)
 T = 35.0e-6
