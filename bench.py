@@ -221,7 +221,9 @@ def run_batch200(args, torch) -> dict:
     count = args.count
     log(f"building {count} uneven boards")
     boards = workloads.uneven_boards(count)
-    ops = [MatrixFreeThermalOperator(b, preconditioner="jacobi") for b in boards]
+    t0 = time.perf_counter()
+    ops = [MatrixFreeThermalOperator(b, preconditioner=args.preconditioner) for b in boards]
+    log(f"operators ({args.preconditioner}) built in {time.perf_counter() - t0:.1f} s")
     rhss = [o.build_rhs(o.default_reference_temperature()) for o in ops]
     nodes = sum(o.size for o in ops)
     config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000)
@@ -268,6 +270,7 @@ def run_batch200(args, torch) -> dict:
         "data": "synthetic (seeded uneven-board generator, seed 4)",
         "config": {
             "workload": f"{count} boards, sides log-uniform 32..384, depth 4..16, {nodes:,} nodes in total",
+            "preconditioner": args.preconditioner,
             "converged": converged,
             "single_solve_path_solves_per_s": count / single_s,
             "speedup_vs_single_solve_path": single_s / batched_s,
@@ -571,9 +574,10 @@ def run_coupling(args, torch) -> dict:
     sequential_s = time.perf_counter() - t0
     log(f"sequential: {sequential_s:.2f} s")
     served_s = []
-    for _ in range(args.steps):
+    for step in range(args.steps):
         t0 = time.perf_counter()
-        served = run_electro_thermal_many(scenarios)
+        served = run_electro_thermal_many(
+            scenarios, on_done=lambda i: log(f"served run {step}: loop {i} done at {time.perf_counter() - t0:.2f} s"))
         served_s.append(time.perf_counter() - t0)
     log(f"served: {np.mean(served_s):.2f} s per run")
     per_run = float(np.mean(served_s))
@@ -609,6 +613,105 @@ def run_coupling(args, torch) -> dict:
     }
 
 
+# The reference's published two-level MPIR solve times on its 4-slab fixture
+# (docs/THERMAL_MPIR_FEM.md:472-482, 522-526; 1 W trace, MPIRConfig(
+# max_outer_iterations=16), construction excluded): elements -> ms.
+PUBLISHED_FIXTURE_MS = {
+    50: {"xeon_8581c_16t": 26.6, "axion_c4a_32t": 23.0, "outer_inner": (7, 584)},
+    100: {"xeon_8581c_16t": 94.0, "axion_c4a_32t": 74.0, "outer_inner": (7, 795)},
+    200: {"xeon_8581c_16t": 239.7, "axion_c4a_32t": 218.4, "outer_inner": (6, 891)},
+}
+
+
+def _timed_ms(fn, repeats: int, warmups: int):
+    """Median of ``repeats`` timed calls after ``warmups`` (the reference's
+    _timed, experiments/thermal_native_benchmark.py:64-78)."""
+
+    result = None
+    for _ in range(warmups):
+        result = fn()
+    samples = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        result = fn()
+        samples.append((time.perf_counter() - t0) * 1e3)
+    return result, float(np.median(samples))
+
+
+def run_two_level(args, torch) -> dict:
+    """The two-level preconditioner against Jacobi: the reference's own
+    4-slab fixture at its three published sizes (single solves through the
+    front end's solve_mpir, construction timed apart), then the 512 x 512 x 16
+    board."""
+
+    from electrical.matrix_free_mpir_fem import MPIRConfig, solve_mpir
+    from electrical.matrix_free_mpir_fem.solver import jacobi_default_config
+    from pcbhip import workloads
+    from thermal.matrix_free_mpir_fem import MatrixFreeThermalOperator
+
+    cases = []
+    fixture = {}
+    for elements in (50, 100, 200):
+        problem = workloads.four_slab_fixture(elements)
+        row = {"elements": elements, "nodes": problem.mesh.size}
+        for name, precond, config in (("two_level", "two-level", MPIRConfig(max_outer_iterations=16)),
+                                      ("jacobi", "jacobi", jacobi_default_config(16))):
+            t0 = time.perf_counter()
+            operator = MatrixFreeThermalOperator(problem, preconditioner=precond)
+            construction_ms = (time.perf_counter() - t0) * 1e3
+            rhs = operator.build_rhs(operator.default_reference_temperature())
+            result, ms = _timed_ms(lambda: solve_mpir(operator, rhs, config=config), repeats=5, warmups=1)
+            row[name] = {"solve_ms": ms, "construction_ms": construction_ms, "converged": result.converged,
+                         "outer": result.outer_iterations, "inner": result.inner_iterations,
+                         "relative_residual": result.relative_residual}
+            if precond == "two-level":
+                row[name]["coarse_size"] = operator.coarse_correction.coarse_size
+                row[name]["block"] = operator.coarse_correction.block
+        published = PUBLISHED_FIXTURE_MS[elements]
+        row["published_cpu_ms"] = published
+        row["speedup_vs_xeon_16t"] = published["xeon_8581c_16t"] / row["two_level"]["solve_ms"]
+        log(f"fixture {elements}: two-level {row['two_level']['solve_ms']:.2f} ms "
+            f"({row['two_level']['outer']}/{row['two_level']['inner']}), jacobi {row['jacobi']['solve_ms']:.2f} ms "
+            f"({row['jacobi']['outer']}/{row['jacobi']['inner']})")
+        fixture[elements] = row
+    if not args.no_board:
+        problem = workloads.board_512x512x16(size=args.size, slabs=args.slabs)
+        row = {"elements": f"{args.slabs}x{args.size}x{args.size}", "nodes": problem.mesh.size}
+        for name, precond in (("two_level", "two-level"), ("jacobi", "jacobi")):
+            t0 = time.perf_counter()
+            operator = MatrixFreeThermalOperator(problem, preconditioner=precond)
+            construction_ms = (time.perf_counter() - t0) * 1e3
+            rhs = operator.build_rhs(operator.default_reference_temperature())
+            config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000)
+            result, ms = _timed_ms(lambda: solve_mpir(operator, rhs, config=config), repeats=args.steps,
+                                   warmups=args.warmup)
+            row[name] = {"solve_ms": ms, "construction_ms": construction_ms, "converged": result.converged,
+                         "outer": result.outer_iterations, "inner": result.inner_iterations,
+                         "ms_per_inner_iteration": ms / max(1, result.inner_iterations)}
+            log(f"board {name}: {ms:.1f} ms ({result.outer_iterations}/{result.inner_iterations}), "
+                f"construction {construction_ms:.0f} ms")
+        cases.append(row)
+    top = fixture[200]
+    value = 1e3 / top["two_level"]["solve_ms"]
+    return {
+        "metric": "two-level MPIR solves/s at 1e-10 on the reference's 4-slab fixture (202,005 nodes)",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": 5,
+        "warmup": 1,
+        "ms_per_step": top["two_level"]["solve_ms"],
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": value / (1e3 / PUBLISHED_FIXTURE_MS[200]["xeon_8581c_16t"]),
+        "dtype": "f32 inner / f64 outer",
+        "data": "the reference's deterministic 4-slab fixture",
+        "config": {"workload": "4-slab fixture 200 x 200 elements, two-level, MPIRConfig(max_outer_iterations=16)",
+                   "vs_baseline_against": "published C++ 16-thread Xeon Platinum 8581C solve, 239.7 ms",
+                   "fixture": fixture, "boards": cases, "parallelism": "single GPU"},
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -622,7 +725,11 @@ def main() -> int:
     parser.add_argument("--rows", type=int, default=4, help="rows per wave strip of the FP32 inner stencil")
     parser.add_argument("--target", type=int, default=1024, help="stencil workgroups per launch before z blocks split")
     parser.add_argument("--workload", default="board512",
-                        choices=["board512", "batch200", "multirhs64", "serve", "apply_engines", "coupling"])
+                        choices=["board512", "batch200", "multirhs64", "serve", "apply_engines", "coupling",
+                                 "two_level"])
+    parser.add_argument("--no-board", action="store_true", help="two_level: the fixture sizes only")
+    parser.add_argument("--preconditioner", default="jacobi", choices=["jacobi", "two-level"],
+                        help="batch200: the boards' preconditioner")
     parser.add_argument("--coupling-count", type=int, default=16, help="scenarios of the coupling workload")
     parser.add_argument("--rates", type=float, nargs="+", default=[50.0, 100.0, 200.0, 400.0])
     parser.add_argument("--duration", type=float, default=10.0)
@@ -663,6 +770,9 @@ def main() -> int:
         return 0
     if args.workload == "coupling":
         print(json.dumps(run_coupling(args, torch)), flush=True)
+        return 0
+    if args.workload == "two_level":
+        print(json.dumps(run_two_level(args, torch)), flush=True)
         return 0
     log("building board")
     problem = workloads.board_512x512x16(size=args.size, slabs=args.slabs)

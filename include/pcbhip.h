@@ -2,7 +2,7 @@
  * pcbhip -- C ABI of the MI355X (gfx950) matrix-free hexahedral Q1
  * conduction solver: the thermal steady-conduction and hex DC-conduction
  * operator, its Jacobi diagonal and its mixed-precision iterative-refinement
- * (MPIR) Jacobi-PCG solve.
+ * (MPIR) PCG solve, Jacobi or two-level preconditioned.
  *
  * Everything is extern "C" with plain pointers and sizes.  Handles are
  * opaque.  Every function returns PCBHIP_OK or an error code; the message of
@@ -106,9 +106,8 @@ int pcbhip_operator_diagonal(const pcbhip_operator *op, double *diag);
  * operator, copper layers of a sheet operator), zero on fixed nodes; cinv is
  * the nc x nc row-major inverse of Z^T A Z, nc = layers * ceil((R + 1) /
  * block) * ceil((C + 1) / block) <= 16384, stored in FP32.  nc = 0 removes
- * it.  Batches made afterwards run two-level PCG on this operator; batches
- * made before keep what they were made with.  Servers take Jacobi operators
- * only. */
+ * it.  Batches made and server submissions made afterwards run two-level
+ * PCG on this operator; slots made before keep what they were made with. */
 int pcbhip_operator_set_coarse(pcbhip_operator *op, int block, int nc, const double *cinv);
 /* z = M^-1 r in FP32 with the operator's preconditioner, Jacobi or two-level
  * (replaces MatrixFreeThermalOperator.precondition_low, operator.py:366-371). */

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -58,12 +59,80 @@ int guarded(F&& body) {
     }
 }
 
+// Graph capture and the calls HIP may synchronise in (device and pinned
+// allocation and release, graph teardown, device-wide waits) never overlap
+// inside this library: HIP lists them as unsafe while a stream captures, and
+// a server's pump thread captures its graph while host threads build
+// operators and batches.  Each such call and each capture holds this lock.
+std::mutex& device_mutex() {
+    static std::mutex m;
+    return m;
+}
+using DeviceLock = std::lock_guard<std::mutex>;
+
 void* device_alloc(size_t bytes) {
     void* p = nullptr;
-    const hipError_t err = hipMalloc(&p, bytes == 0 ? 256 : bytes);
+    hipError_t err;
+    {
+        DeviceLock lock(device_mutex());
+        err = hipMalloc(&p, bytes == 0 ? 256 : bytes);
+    }
     if (err == hipErrorOutOfMemory) fail(PCBHIP_NO_MEMORY, "out of device memory");
     check(err, "hipMalloc");
     return p;
+}
+
+hipError_t device_free(void* p) {
+    DeviceLock lock(device_mutex());
+    return hipFree(p);
+}
+
+// Operator memory comes from the stream-ordered allocator on the operator's
+// own stream, so releasing it waits for that stream only, not for the whole
+// device (which a server in another thread keeps busy).  The allocation is
+// complete, and usable from every stream, when this returns.
+void* stream_alloc(hipStream_t stream, size_t bytes) {
+    void* p = nullptr;
+    hipError_t err;
+    {
+        DeviceLock lock(device_mutex());
+        err = hipMallocAsync(&p, bytes == 0 ? 256 : bytes, stream);
+    }
+    if (err == hipErrorOutOfMemory) fail(PCBHIP_NO_MEMORY, "out of device memory");
+    check(err, "hipMallocAsync");
+    check(hipStreamSynchronize(stream), "allocation");
+    return p;
+}
+
+hipError_t stream_free(hipStream_t stream, void* p) {
+    DeviceLock lock(device_mutex());
+    return hipFreeAsync(p, stream);
+}
+
+hipError_t pinned_free(void* p) {
+    DeviceLock lock(device_mutex());
+    return hipHostFree(p);
+}
+
+hipError_t stream_create(hipStream_t* stream) {
+    DeviceLock lock(device_mutex());
+    return hipStreamCreateWithFlags(stream, hipStreamNonBlocking);
+}
+
+hipError_t stream_destroy(hipStream_t stream) {
+    DeviceLock lock(device_mutex());
+    return hipStreamDestroy(stream);
+}
+
+hipError_t graph_release(hipGraphExec_t exec, hipGraph_t graph) {
+    DeviceLock lock(device_mutex());
+    hipError_t err = hipSuccess;
+    if (exec) err = hipGraphExecDestroy(exec);
+    if (graph) {
+        const hipError_t e = hipGraphDestroy(graph);
+        if (err == hipSuccess) err = e;
+    }
+    return err;
 }
 
 size_t align256(size_t bytes) { return (bytes + 255) & ~size_t(255); }
@@ -113,6 +182,12 @@ struct pcbhip_operator {
     float* cinv = nullptr;
     int cblock = 0, crows = 0, ccols = 0, nc = 0;
     std::vector<float*> retired;
+    // Device staging of the host-buffer calls (applies, preconditioner),
+    // kept for the operator's life: freeing device memory waits for the
+    // whole device, which a server in another thread keeps busy.
+    mutable std::mutex staging_mutex;
+    mutable void* staging = nullptr;
+    mutable size_t staging_bytes = 0;
 
     OperatorDev view() const {
         OperatorDev v{};
@@ -142,11 +217,10 @@ struct pcbhip_operator {
     // operator, the copper layers of a sheet operator.
     int layers() const { return kind == kSheet ? S : S + 1; }
     ~pcbhip_operator() {
-        if (block) (void)hipFree(block);
-        if (cinv) (void)hipFree(cinv);
-        for (float* p : retired) (void)hipFree(p);
-        if (via_ptr) (void)hipFree(via_ptr);
-        if (stream) (void)hipStreamDestroy(stream);
+        for (void* p : {block, (void*)cinv, staging, (void*)via_ptr})
+            if (p) (void)stream_free(stream, p);
+        for (float* p : retired) (void)stream_free(stream, p);
+        if (stream) (void)stream_destroy(stream);
     }
 };
 
@@ -218,18 +292,17 @@ struct pcbhip_batch {
     }
 
     ~pcbhip_batch() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        (void)graph_release(exec, graph);
         for (SlotHost& s : slots) {
-            if (s.block) (void)hipFree(s.block);
-            if (s.history) (void)hipFree(s.history);
+            if (s.block) (void)device_free(s.block);
+            if (s.history) (void)device_free(s.history);
         }
         for (void* p : {(void*)d_slots, (void*)d_states, (void*)d_configs, (void*)d_outer, (void*)d_inner,
                         (void*)d_block, (void*)d_patches, (void*)d_rows, (void*)d_groups, (void*)d_active,
                         (void*)d_counts})
-            if (p) (void)hipFree(p);
-        if (h_active) (void)hipHostFree(h_active);
-        if (stream) (void)hipStreamDestroy(stream);
+            if (p) (void)device_free(p);
+        if (h_active) (void)pinned_free(h_active);
+        if (stream) (void)stream_destroy(stream);
     }
 };
 
@@ -275,23 +348,35 @@ void apply_dev(const pcbhip_operator* op, const T* x, T* y, int mode, hipStream_
           "apply launch");
 }
 
+// The operator's staging buffer, at least ``bytes`` long; the caller holds
+// op->staging_mutex.  It only grows (to the largest request so far).
+char* staging_of(const pcbhip_operator* op, size_t bytes) {
+    if (op->staging_bytes < bytes) {
+        if (op->staging) {
+            check(stream_free(op->stream, op->staging), "hipFreeAsync");
+            op->staging = nullptr;
+            op->staging_bytes = 0;
+        }
+        op->staging = stream_alloc(op->stream, bytes);
+        op->staging_bytes = bytes;
+    }
+    return static_cast<char*>(op->staging);
+}
+
 template <typename T>
 void apply_host(const pcbhip_operator* op, const T* x, T* y, int mode) {
     if (!op || !x || !y) fail(PCBHIP_INVALID_INPUT, "null argument");
     set_device(op->device);
     const size_t bytes = sizeof(T) * (size_t)op->N;
-    T* dx = static_cast<T*>(device_alloc(2 * align256(bytes)));
-    T* dy = reinterpret_cast<T*>(reinterpret_cast<char*>(dx) + align256(bytes));
-    try {
-        check(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, op->stream), "copy x");
-        apply_dev<T>(op, dx, dy, mode, op->stream);
-        check(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, op->stream), "copy y");
-        check(hipStreamSynchronize(op->stream), "apply");
-    } catch (...) {
-        (void)hipFree(dx);
-        throw;
-    }
-    check(hipFree(dx), "hipFree");
+    std::lock_guard<std::mutex> guard(op->staging_mutex);
+    // Room for FP64 x and y at once, so FP32 and FP64 calls share one buffer.
+    char* base = staging_of(op, 2 * align256(sizeof(double) * (size_t)op->N));
+    T* dx = reinterpret_cast<T*>(base);
+    T* dy = reinterpret_cast<T*>(base + align256(bytes));
+    check(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, op->stream), "copy x");
+    apply_dev<T>(op, dx, dy, mode, op->stream);
+    check(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, op->stream), "copy y");
+    check(hipStreamSynchronize(op->stream), "apply");
 }
 
 SlotDev slot_dev(const SlotHost& s) {
@@ -390,7 +475,7 @@ int add_sheet_tiles(std::vector<Tile>& out, int slot, int nsx, int nsy, int laye
 }
 
 void upload_tiles(pcbhip_batch* b, const std::vector<Tile>& t, Tile** d, int* n) {
-    if (*d) check(hipFree(*d), "hipFree");
+    if (*d) check(device_free(*d), "hipFree");
     *n = (int)t.size();
     *d = static_cast<Tile*>(device_alloc(sizeof(Tile) * std::max<size_t>(1, t.size())));
     if (!t.empty())
@@ -541,7 +626,7 @@ void build_tiles(pcbhip_batch* b, const std::vector<char>* keep = nullptr) {
     upload_tiles(b, block, &b->d_block, &b->n_block);
     upload_tiles(b, patches, &b->d_patches, &b->n_patches);
     upload_tiles(b, rows, &b->d_rows, &b->n_rows);
-    if (b->d_groups) check(hipFree(b->d_groups), "hipFree");
+    if (b->d_groups) check(device_free(b->d_groups), "hipFree");
     b->d_groups = static_cast<SlotGroup*>(device_alloc(sizeof(SlotGroup) * std::max<size_t>(1, groups.size())));
     if (!groups.empty())
         check(hipMemcpyAsync(b->d_groups, groups.data(), sizeof(SlotGroup) * groups.size(), hipMemcpyHostToDevice,
@@ -560,20 +645,16 @@ void build_tiles(pcbhip_batch* b, const std::vector<char>* keep = nullptr) {
     check(hipStreamSynchronize(b->stream), "tile upload");
     b->configs_dirty = true;
     if (b->exec) {
-        check(hipGraphExecDestroy(b->exec), "graph exec destroy");
+        check(graph_release(b->exec, nullptr), "graph exec destroy");
         b->exec = nullptr;
     }
 }
 
 void build_graph(pcbhip_batch* b, int steps) {
-    if (b->exec) {
-        check(hipGraphExecDestroy(b->exec), "graph exec destroy");
-        b->exec = nullptr;
-    }
-    if (b->graph) {
-        check(hipGraphDestroy(b->graph), "graph destroy");
-        b->graph = nullptr;
-    }
+    check(graph_release(b->exec, b->graph), "graph destroy");
+    b->exec = nullptr;
+    b->graph = nullptr;
+    DeviceLock lock(device_mutex());
     check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     const int n_slots = (int)b->slots.size();
     for (int s = 0; s < steps; ++s) {
@@ -660,9 +741,9 @@ struct pcbhip_server {
     SlotState* d_fresh = nullptr;
     int* d_fresh_index = nullptr;
     int* d_active = nullptr;  // decremented by the finish kernels; the server reads states instead
-    int* d_counts = nullptr;
-    Tile* d_tiles = nullptr;  // outer | inner tables at their capacities
-    int cap_outer = 0, cap_inner = 0;
+    int* d_counts = nullptr;  // outer, inner, (unused), (unused), patches, coarse rows
+    Tile* d_tiles = nullptr;  // outer | inner | patches | coarse rows tables at their capacities
+    int cap_outer = 0, cap_inner = 0, cap_patches = 0, cap_rows = 0;
     Staging stage[2];
     int stage_next = 0;
     SlotState* h_states = nullptr;  // pinned, two snapshots of max_slots states
@@ -677,25 +758,24 @@ struct pcbhip_server {
     ~pcbhip_server() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (copy_stream) (void)hipStreamSynchronize(copy_stream);
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        (void)graph_release(exec, graph);
         for (ServerSlot& s : slots)
-            if (s.h.block) (void)hipFree(s.h.block);
-        for (auto& b : pool) (void)hipFree(b.first);
+            if (s.h.block) (void)device_free(s.h.block);
+        for (auto& b : pool) (void)device_free(b.first);
         for (void* p : {(void*)d_slots, (void*)d_states, (void*)d_configs, (void*)d_fresh, (void*)d_fresh_index,
                         (void*)d_active, (void*)d_counts, (void*)d_tiles})
-            if (p) (void)hipFree(p);
+            if (p) (void)device_free(p);
         for (Staging& st : stage) {
             for (void* p : {(void*)st.counts, (void*)st.tiles, (void*)st.devs, (void*)st.cfgs, (void*)st.fresh,
                             (void*)st.fresh_index})
-                if (p) (void)hipHostFree(p);
+                if (p) (void)pinned_free(p);
             if (st.used) (void)hipEventDestroy(st.used);
         }
-        if (h_states) (void)hipHostFree(h_states);
+        if (h_states) (void)pinned_free(h_states);
         for (hipEvent_t e : replay_done)
             if (e) (void)hipEventDestroy(e);
-        if (copy_stream) (void)hipStreamDestroy(copy_stream);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (copy_stream) (void)stream_destroy(copy_stream);
+        if (stream) (void)stream_destroy(stream);
     }
 };
 
@@ -704,7 +784,12 @@ namespace {
 template <typename T>
 T* pinned_alloc(size_t count) {
     void* p = nullptr;
-    check(hipHostMalloc(&p, sizeof(T) * (count ? count : 1), hipHostMallocDefault), "pinned alloc");
+    hipError_t err;
+    {
+        DeviceLock lock(device_mutex());
+        err = hipHostMalloc(&p, sizeof(T) * (count ? count : 1), hipHostMallocDefault);
+    }
+    check(err, "pinned alloc");
     return static_cast<T*>(p);
 }
 
@@ -733,26 +818,29 @@ void server_drain(pcbhip_server* sv) {
 }
 
 void server_build_graph(pcbhip_server* sv) {
-    if (sv->exec) {
-        check(hipGraphExecDestroy(sv->exec), "graph exec destroy");
-        sv->exec = nullptr;
-    }
-    if (sv->graph) {
-        check(hipGraphDestroy(sv->graph), "graph destroy");
-        sv->graph = nullptr;
-    }
+    check(graph_release(sv->exec, sv->graph), "graph destroy");
+    sv->exec = nullptr;
+    sv->graph = nullptr;
     Tile* outer = sv->d_tiles;
     Tile* inner = outer + sv->cap_outer;
+    // Two-level slots: the coarse kernels over their tables (grids of the
+    // tables' capacities; workgroups past the live counts return at once).
+    launch::CoarseLaunch coarse;
+    coarse.patches = inner + sv->cap_inner;
+    coarse.rows = coarse.patches + sv->cap_patches;
+    coarse.grid_patches = sv->cap_patches;
+    coarse.grid_rows = sv->cap_rows;
+    DeviceLock lock(device_mutex());
     check(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     for (int s = 0; s < sv->steps; ++s) {
         hipError_t err = hipSuccess;
         if (s % sv->outer_every == 0)
             err = launch::outer_step(sv->inner64 != 0, sv->max_slots, sv->d_slots, sv->d_states, sv->d_configs,
-                                     outer, sv->d_counts, sv->cap_outer, sv->d_active, launch::CoarseLaunch{}, sv->stream);
+                                     outer, sv->d_counts, sv->cap_outer, sv->d_active, coarse, sv->stream);
         if (err == hipSuccess)
             err = launch::inner_step(sv->inner64 != 0, sv->rows_inner, sv->max_slots, sv->d_slots, sv->d_states,
                                      sv->d_configs, inner, sv->d_counts, std::min(sv->cap_inner, kPassGroups), true,
-                                     nullptr, nullptr, 0, 0, sv->d_active, launch::CoarseLaunch{}, sv->stream);
+                                     nullptr, nullptr, 0, 0, sv->d_active, coarse, sv->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(sv->stream, &dropped);
@@ -771,20 +859,29 @@ int round_capacity(size_t n) { return (int)std::max<size_t>(256, ((n + n / 4 + 2
 // they are mostly empty (workgroups past the live count still cost dispatch);
 // either way the replays in flight are drained and the graph is captured
 // again with the new grids.  Returns whether it drained.
-bool server_reserve(pcbhip_server* sv, size_t n_outer, size_t n_inner) {
-    const bool fits = n_outer <= (size_t)sv->cap_outer && n_inner <= (size_t)sv->cap_inner;
-    const size_t live = n_outer + n_inner;
-    const size_t cap = (size_t)sv->cap_outer + sv->cap_inner;
+// Capacity of a coarse table: none while no two-level slot runs.
+int coarse_capacity(size_t n) { return n ? round_capacity(n) : 0; }
+
+bool server_reserve(pcbhip_server* sv, size_t n_outer, size_t n_inner, size_t n_patches, size_t n_rows) {
+    const bool fits = n_outer <= (size_t)sv->cap_outer && n_inner <= (size_t)sv->cap_inner &&
+                      n_patches <= (size_t)sv->cap_patches && n_rows <= (size_t)sv->cap_rows;
+    const size_t live = n_outer + n_inner + n_patches + n_rows;
+    const size_t cap = (size_t)sv->cap_outer + sv->cap_inner + sv->cap_patches + sv->cap_rows;
     const bool sparse = cap > 3 * 1024 && live * 3 < cap;
-    if (sv->exec && fits && !sparse) return false;
+    // Coarse kernels stay out of the graph once no two-level slot is left.
+    const bool idle_coarse = sv->cap_patches > 0 && n_patches == 0;
+    if (sv->exec && fits && !sparse && !idle_coarse) return false;
     server_drain(sv);
     check(hipStreamSynchronize(sv->stream), "drain");
-    if (!fits || sparse || !sv->d_tiles) {
-        if (sv->d_tiles) check(hipFree(sv->d_tiles), "hipFree");
+    if (!fits || sparse || idle_coarse || !sv->d_tiles) {
+        if (sv->d_tiles) check(device_free(sv->d_tiles), "hipFree");
         sv->d_tiles = nullptr;
         sv->cap_outer = round_capacity(n_outer);
         sv->cap_inner = round_capacity(n_inner);
-        sv->d_tiles = static_cast<Tile*>(device_alloc(sizeof(Tile) * ((size_t)sv->cap_outer + sv->cap_inner)));
+        sv->cap_patches = coarse_capacity(n_patches);
+        sv->cap_rows = coarse_capacity(n_rows);
+        sv->d_tiles = static_cast<Tile*>(device_alloc(
+            sizeof(Tile) * ((size_t)sv->cap_outer + sv->cap_inner + sv->cap_patches + sv->cap_rows)));
     }
     server_build_graph(sv);
     return true;
@@ -795,7 +892,7 @@ bool server_reserve(pcbhip_server* sv, size_t n_outer, size_t n_inner) {
 // of it runs on the replay stream between two replays.
 void server_upload(pcbhip_server* sv, const std::vector<int>& admitted) {
     const int rows_inner = sv->inner64 ? kRowsInner64 : sv->rows_inner;
-    std::vector<Tile> outer, inner;
+    std::vector<Tile> outer, inner, patches, rows;
     // A drain inside the reserve can finish slots: list again after one.
     for (int pass = 0; pass < 2; ++pass) {
         std::vector<const SlotHost*> listed((size_t)sv->max_slots, nullptr);
@@ -803,16 +900,20 @@ void server_upload(pcbhip_server* sv, const std::vector<int>& admitted) {
             if (sv->slots[(size_t)i].status == kSlotRunning) listed[(size_t)i] = &sv->slots[(size_t)i].h;
         outer.clear();
         inner.clear();
+        patches.clear();
+        rows.clear();
         std::vector<Tile> block;
         collect_tiles(listed, {}, rows_inner, sv->target_groups, outer, inner, block);
-        if (!server_reserve(sv, outer.size(), inner.size())) break;
+        for (int i = 0; i < sv->max_slots; ++i)
+            if (listed[(size_t)i]) add_coarse_tiles(patches, rows, i, listed[(size_t)i]->view);
+        if (!server_reserve(sv, outer.size(), inner.size(), patches.size(), rows.size())) break;
     }
     Staging& st = sv->stage[sv->stage_next];
     sv->stage_next ^= 1;
     if (st.recorded) check(hipEventSynchronize(st.used), "staging");
-    const size_t n_tiles = outer.size() + inner.size();
+    const size_t n_tiles = outer.size() + inner.size() + patches.size() + rows.size();
     if (n_tiles > st.tile_cap) {
-        if (st.tiles) check(hipHostFree(st.tiles), "pinned free");
+        if (st.tiles) check(pinned_free(st.tiles), "pinned free");
         st.tile_cap = n_tiles + n_tiles / 4;
         st.tiles = pinned_alloc<Tile>(st.tile_cap);
     }
@@ -820,8 +921,13 @@ void server_upload(pcbhip_server* sv, const std::vector<int>& admitted) {
     st.counts[1] = (int)inner.size();
     st.counts[2] = 0;
     st.counts[3] = 0;
-    std::copy(outer.begin(), outer.end(), st.tiles);
-    std::copy(inner.begin(), inner.end(), st.tiles + outer.size());
+    st.counts[4] = (int)patches.size();
+    st.counts[5] = (int)rows.size();
+    Tile* staged = st.tiles;
+    for (const std::vector<Tile>* table : {&outer, &inner, &patches, &rows}) {
+        std::copy(table->begin(), table->end(), staged);
+        staged += table->size();
+    }
     for (int i = 0; i < sv->max_slots; ++i) {
         const ServerSlot& s = sv->slots[(size_t)i];
         const bool bound = s.status == kSlotRunning || s.status == kSlotFinished;
@@ -833,15 +939,18 @@ void server_upload(pcbhip_server* sv, const std::vector<int>& admitted) {
         st.fresh[k].phase = kPhaseOuter;
         st.fresh_index[k] = admitted[k];
     }
-    Tile* d_outer = sv->d_tiles;
-    Tile* d_inner = d_outer + sv->cap_outer;
     const hipMemcpyKind h2d = hipMemcpyHostToDevice;
-    if (!outer.empty())
-        check(hipMemcpyAsync(d_outer, st.tiles, sizeof(Tile) * outer.size(), h2d, sv->stream), "tiles");
-    if (!inner.empty())
-        check(hipMemcpyAsync(d_inner, st.tiles + outer.size(), sizeof(Tile) * inner.size(), h2d, sv->stream),
-              "tiles");
-    check(hipMemcpyAsync(sv->d_counts, st.counts, 4 * sizeof(int), h2d, sv->stream), "counts");
+    Tile* dest = sv->d_tiles;
+    staged = st.tiles;
+    const int caps[4] = {sv->cap_outer, sv->cap_inner, sv->cap_patches, sv->cap_rows};
+    int k = 0;
+    for (const std::vector<Tile>* table : {&outer, &inner, &patches, &rows}) {
+        if (!table->empty())
+            check(hipMemcpyAsync(dest, staged, sizeof(Tile) * table->size(), h2d, sv->stream), "tiles");
+        staged += table->size();
+        dest += caps[k++];
+    }
+    check(hipMemcpyAsync(sv->d_counts, st.counts, 6 * sizeof(int), h2d, sv->stream), "counts");
     check(hipMemcpyAsync(sv->d_slots, st.devs, sizeof(SlotDev) * sv->max_slots, h2d, sv->stream), "slots");
     check(hipMemcpyAsync(sv->d_configs, st.cfgs, sizeof(SlotConfig) * sv->max_slots, h2d, sv->stream), "configs");
     if (!admitted.empty()) {
@@ -935,12 +1044,12 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
         op->C = cols;
         op->N = N;
         op->E = E;
-        check(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking), "stream create");
+        check(stream_create(&op->stream), "stream create");
         op->zb32 = planes_per_block(slabs, (long long)strips_x(cols) * groups_y(rows, op->rows32), kTargetGroups);
         op->zb64 = planes_per_block(slabs, (long long)strips_x(cols) * groups_y(rows, op->rows64), kTargetGroups);
         const size_t s_c32 = align256(sizeof(float) * 3 * E), s_c64 = align256(sizeof(double) * 3 * E);
         const size_t s_n32 = align256(sizeof(float) * N), s_n64 = align256(sizeof(double) * N);
-        char* base = static_cast<char*>(device_alloc(s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + 256));
+        char* base = static_cast<char*>(stream_alloc(op->stream, s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + 256));
         op->block = base;
         op->coef32 = reinterpret_cast<float*>(base);
         base += s_c32;
@@ -1039,10 +1148,10 @@ int pcbhip_sheet_operator_create(int device, int layers, int rows, int cols, con
         op->C = cols;
         op->N = N;
         op->E = E;
-        check(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking), "stream create");
+        check(stream_create(&op->stream), "stream create");
         const size_t s_c32 = align256(sizeof(float) * 2 * E), s_c64 = align256(sizeof(double) * 2 * E);
         const size_t s_n32 = align256(sizeof(float) * N), s_n64 = align256(sizeof(double) * N);
-        char* base = static_cast<char*>(device_alloc(s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + 256));
+        char* base = static_cast<char*>(stream_alloc(op->stream, s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + 256));
         op->block = base;
         op->coef32 = reinterpret_cast<float*>(base);
         base += s_c32;
@@ -1062,7 +1171,7 @@ int pcbhip_sheet_operator_create(int device, int layers, int rows, int cols, con
         if (via_count > 0) {
             const size_t s_ptr = align256(sizeof(int) * ((size_t)N + 1)), s_nbr = align256(sizeof(int) * nbr.size());
             const size_t s_g32 = align256(sizeof(float) * g32.size()), s_g64 = align256(sizeof(double) * g64.size());
-            char* vb = static_cast<char*>(device_alloc(s_ptr + s_nbr + s_g32 + s_g64));
+            char* vb = static_cast<char*>(stream_alloc(op->stream, s_ptr + s_nbr + s_g32 + s_g64));
             op->via_ptr = reinterpret_cast<int*>(vb);
             op->via_nbr = reinterpret_cast<int*>(vb + s_ptr);
             op->via_g32 = reinterpret_cast<float*>(vb + s_ptr + s_nbr);
@@ -1158,7 +1267,7 @@ int pcbhip_batch_create(int count, const pcbhip_operator* const* ops, int inner_
         auto b = std::make_unique<pcbhip_batch>();
         b->device = device;
         b->inner64 = inner_fp64 ? 1 : 0;
-        check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream create");
+        check(stream_create(&b->stream), "stream create");
         b->slots.resize((size_t)count);
         for (int i = 0; i < count; ++i) {
             SlotHost& s = b->slots[(size_t)i];
@@ -1182,8 +1291,7 @@ int pcbhip_batch_create(int count, const pcbhip_operator* const* ops, int inner_
         b->d_states = static_cast<SlotState*>(device_alloc(sizeof(SlotState) * count));
         b->d_configs = static_cast<SlotConfig*>(device_alloc(sizeof(SlotConfig) * count));
         b->d_active = static_cast<int*>(device_alloc(sizeof(int)));
-        check(hipHostMalloc(reinterpret_cast<void**>(&b->h_active), sizeof(int), hipHostMallocDefault),
-              "pinned alloc");
+        b->h_active = pinned_alloc<int>(1);
         check(hipStreamSynchronize(b->stream), "batch create");
         *out = b.release();
     });
@@ -1220,7 +1328,7 @@ int pcbhip_batch_set(pcbhip_batch* b, int slot, const double* rhs, const double*
             check(hipMemsetAsync(s.x0, 0, bytes, b->stream), "memset");
         }
         if (s.history_rows < s.config.max_outer_iterations || !s.history) {
-            if (s.history) check(hipFree(s.history), "hipFree");
+            if (s.history) check(device_free(s.history), "hipFree");
             s.history_rows = s.config.max_outer_iterations;
             s.history = static_cast<double*>(device_alloc(sizeof(double) * 4 * (size_t)s.history_rows));
         }
@@ -1302,11 +1410,11 @@ int pcbhip_operator_set_coarse(pcbhip_operator* op, int block, int nc, const dou
             values[i] = (float)cinv[i];
         }
         set_device(op->device);
-        float* d = static_cast<float*>(device_alloc(sizeof(float) * count));
+        float* d = static_cast<float*>(stream_alloc(op->stream, sizeof(float) * count));
         hipError_t err = hipMemcpyAsync(d, values.data(), sizeof(float) * count, hipMemcpyHostToDevice, op->stream);
         if (err == hipSuccess) err = hipStreamSynchronize(op->stream);
         if (err != hipSuccess) {
-            (void)hipFree(d);
+            (void)stream_free(op->stream, d);
             check(err, "coarse upload");
         }
         if (op->cinv) op->retired.push_back(op->cinv);
@@ -1324,21 +1432,17 @@ int pcbhip_operator_precondition_f32(const pcbhip_operator* op, const float* r, 
         set_device(op->device);
         const size_t bytes = sizeof(float) * (size_t)op->N;
         const size_t coarse = align256(sizeof(double) * (size_t)std::max(op->nc, 1));
-        char* base = static_cast<char*>(device_alloc(2 * align256(bytes) + 2 * coarse));
+        std::lock_guard<std::mutex> guard(op->staging_mutex);
+        char* base = staging_of(op, std::max(2 * align256(bytes) + 2 * coarse,
+                                             2 * align256(sizeof(double) * (size_t)op->N)));
         float* dr = reinterpret_cast<float*>(base);
         float* dz = reinterpret_cast<float*>(base + align256(bytes));
         double* g = reinterpret_cast<double*>(base + 2 * align256(bytes));
         double* w = reinterpret_cast<double*>(base + 2 * align256(bytes) + coarse);
-        try {
-            check(hipMemcpyAsync(dr, r, bytes, hipMemcpyHostToDevice, op->stream), "copy r");
-            check(launch::precondition(op->view(), dr, dz, g, w, op->stream), "precondition launch");
-            check(hipMemcpyAsync(z, dz, bytes, hipMemcpyDeviceToHost, op->stream), "copy z");
-            check(hipStreamSynchronize(op->stream), "precondition");
-        } catch (...) {
-            (void)hipFree(base);
-            throw;
-        }
-        check(hipFree(base), "hipFree");
+        check(hipMemcpyAsync(dr, r, bytes, hipMemcpyHostToDevice, op->stream), "copy r");
+        check(launch::precondition(op->view(), dr, dz, g, w, op->stream), "precondition launch");
+        check(hipMemcpyAsync(z, dz, bytes, hipMemcpyDeviceToHost, op->stream), "copy z");
+        check(hipStreamSynchronize(op->stream), "precondition");
     });
 }
 
@@ -1419,8 +1523,10 @@ int pcbhip_batch_run(pcbhip_batch* b) {
             if (count > 1 && active * 2 <= b->tiled_slots) {
                 // Half the tiled slots have finished: drop their tiles.
                 host_states.resize((size_t)count);
-                check(hipMemcpy(host_states.data(), b->d_states, sizeof(SlotState) * count, hipMemcpyDeviceToHost),
+                check(hipMemcpyAsync(host_states.data(), b->d_states, sizeof(SlotState) * count,
+                                     hipMemcpyDeviceToHost, b->stream),
                       "copy states");
+                check(hipStreamSynchronize(b->stream), "copy states");
                 std::vector<char> keep((size_t)count);
                 for (int i = 0; i < count; ++i) keep[(size_t)i] = host_states[(size_t)i].phase != kPhaseDone;
                 build_tiles(b, &keep);
@@ -1440,7 +1546,9 @@ int pcbhip_batch_time_pass(pcbhip_batch* b, int launches, double* avg_us) {
         upload_slots(b);
         const int count = (int)b->slots.size();
         std::vector<SlotState> saved((size_t)count);
-        check(hipMemcpy(saved.data(), b->d_states, sizeof(SlotState) * count, hipMemcpyDeviceToHost), "copy states");
+        check(hipMemcpyAsync(saved.data(), b->d_states, sizeof(SlotState) * count, hipMemcpyDeviceToHost, b->stream),
+              "copy states");
+        check(hipStreamSynchronize(b->stream), "copy states");
         std::vector<SlotState> timing = saved;
         for (SlotState& st : timing) {
             st.phase = kPhaseInner;
@@ -1448,7 +1556,9 @@ int pcbhip_batch_time_pass(pcbhip_batch* b, int launches, double* avg_us) {
             st.alpha = 0.0;
             st.beta = 0.0;
         }
-        check(hipMemcpy(b->d_states, timing.data(), sizeof(SlotState) * count, hipMemcpyHostToDevice), "states");
+        check(hipMemcpyAsync(b->d_states, timing.data(), sizeof(SlotState) * count, hipMemcpyHostToDevice, b->stream),
+              "states");
+        check(hipStreamSynchronize(b->stream), "states");
         hipEvent_t ev[2] = {nullptr, nullptr};
         float ms = 0.0f;
         hipError_t err = hipEventCreate(&ev[0]);
@@ -1464,7 +1574,9 @@ int pcbhip_batch_time_pass(pcbhip_batch* b, int launches, double* avg_us) {
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
         check(err, "timed passes");
-        check(hipMemcpy(b->d_states, saved.data(), sizeof(SlotState) * count, hipMemcpyHostToDevice), "restore states");
+        check(hipMemcpyAsync(b->d_states, saved.data(), sizeof(SlotState) * count, hipMemcpyHostToDevice, b->stream),
+              "restore states");
+        check(hipStreamSynchronize(b->stream), "restore states");
         *avg_us = 1e3 * (double)ms / launches;
     });
 }
@@ -1545,8 +1657,8 @@ int pcbhip_server_create(int device, int max_slots, int inner_fp64, int steps_pe
         sv->max_slots = max_slots;
         sv->steps = steps_per_replay;
         sv->slots.resize((size_t)max_slots);
-        check(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking), "stream create");
-        check(hipStreamCreateWithFlags(&sv->copy_stream, hipStreamNonBlocking), "stream create");
+        check(stream_create(&sv->stream), "stream create");
+        check(stream_create(&sv->copy_stream), "stream create");
         const size_t n = (size_t)max_slots;
         sv->d_slots = static_cast<SlotDev*>(device_alloc(sizeof(SlotDev) * n));
         sv->d_states = static_cast<SlotState*>(device_alloc(sizeof(SlotState) * n));
@@ -1554,9 +1666,9 @@ int pcbhip_server_create(int device, int max_slots, int inner_fp64, int steps_pe
         sv->d_fresh = static_cast<SlotState*>(device_alloc(sizeof(SlotState) * n));
         sv->d_fresh_index = static_cast<int*>(device_alloc(sizeof(int) * n));
         sv->d_active = static_cast<int*>(device_alloc(sizeof(int)));
-        sv->d_counts = static_cast<int*>(device_alloc(4 * sizeof(int)));
+        sv->d_counts = static_cast<int*>(device_alloc(6 * sizeof(int)));
         for (Staging& st : sv->stage) {
-            st.counts = pinned_alloc<int>(4);
+            st.counts = pinned_alloc<int>(6);
             st.devs = pinned_alloc<SlotDev>(n);
             st.cfgs = pinned_alloc<SlotConfig>(n);
             st.fresh = pinned_alloc<SlotState>(n);
@@ -1571,12 +1683,13 @@ int pcbhip_server_create(int device, int max_slots, int inner_fp64, int steps_pe
             std::memset(&st, 0, sizeof(SlotState));
             st.phase = kPhaseDone;
         }
-        check(hipMemcpy(sv->d_states, done.data(), sizeof(SlotState) * n, hipMemcpyHostToDevice), "states");
-        check(hipMemset(sv->d_slots, 0, sizeof(SlotDev) * n), "memset");
-        check(hipMemset(sv->d_configs, 0, sizeof(SlotConfig) * n), "memset");
-        check(hipMemset(sv->d_counts, 0, 4 * sizeof(int)), "memset");
-        check(hipMemset(sv->d_active, 0, sizeof(int)), "memset");
-        check(hipDeviceSynchronize(), "server create");
+        check(hipMemcpyAsync(sv->d_states, done.data(), sizeof(SlotState) * n, hipMemcpyHostToDevice, sv->stream),
+              "states");
+        check(hipMemsetAsync(sv->d_slots, 0, sizeof(SlotDev) * n, sv->stream), "memset");
+        check(hipMemsetAsync(sv->d_configs, 0, sizeof(SlotConfig) * n, sv->stream), "memset");
+        check(hipMemsetAsync(sv->d_counts, 0, 6 * sizeof(int), sv->stream), "memset");
+        check(hipMemsetAsync(sv->d_active, 0, sizeof(int), sv->stream), "memset");
+        check(hipStreamSynchronize(sv->stream), "server create");
         *out = sv.release();
     });
 }
@@ -1601,7 +1714,7 @@ int pcbhip_server_set_shape(pcbhip_server* sv, int rows_inner, int target_groups
         sv->rows_inner = rows_inner;
         sv->target_groups = target_groups;
         if (sv->exec) {
-            check(hipGraphExecDestroy(sv->exec), "graph exec destroy");
+            check(graph_release(sv->exec, nullptr), "graph exec destroy");
             sv->exec = nullptr;
         }
         sv->tables_dirty = true;
@@ -1614,7 +1727,6 @@ int pcbhip_server_slot_bytes(const pcbhip_server* sv, const pcbhip_operator* op,
         SlotHost h;
         h.op = op;
         h.view = op->view();
-        h.view.nc = 0;  // a server slot is Jacobi (pcbhip_server_submit)
         set_partial_layout(h);
         *bytes = (int64_t)(slot_vector_bytes(h, sv->inner64) + align256(sizeof(double) * 4 * (size_t)max_outer));
     });
@@ -1625,8 +1737,6 @@ int pcbhip_server_submit(pcbhip_server* sv, const pcbhip_operator* op, const dou
     return guarded([&] {
         if (!sv || !op || !rhs || !config || !slot_out) fail(PCBHIP_INVALID_INPUT, "null argument");
         if (op->device != sv->device) fail(PCBHIP_INVALID_INPUT, "operator and server live on different devices");
-        if (op->nc > 0)
-            fail(PCBHIP_INVALID_INPUT, "the server runs Jacobi slots; solve two-level operators in a batch");
         check_config(*config);
         for (long long i = 0; i < op->N; ++i)
             if (!std::isfinite(rhs[i])) fail(PCBHIP_INVALID_INPUT, "rhs must contain only finite values");
@@ -1786,7 +1896,7 @@ int pcbhip_server_trim(pcbhip_server* sv) {
     return guarded([&] {
         if (!sv) fail(PCBHIP_INVALID_INPUT, "null server");
         set_device(sv->device);
-        for (auto& b : sv->pool) check(hipFree(b.first), "hipFree");
+        for (auto& b : sv->pool) check(device_free(b.first), "hipFree");
         sv->pool.clear();
         sv->bytes_pooled = 0;
     });

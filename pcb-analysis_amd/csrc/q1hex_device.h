@@ -162,6 +162,8 @@ struct Tile {
 // patch (sx, sy, sz = coarse column, row, node layer); a coarse row tile is
 // kCoarseRows rows of cinv starting at ``index``.
 constexpr int kCoarseRows = 4;
+// Workgroups of the coarse kernels; each loops over its table's entries.
+constexpr int kCoarseGroups = 1024;
 // Largest coarse space a slot may carry (its inverse is nc^2 FP32 values).
 constexpr int kMaxCoarse = 16384;
 

@@ -1045,22 +1045,24 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
         const TI rb = ldt(robin + pb, off);
         const TI r = rv - alpha * qv;
         if constexpr (COARSE) {
-            TI wv = TI(0);
-            if (off != kOutside && rb >= TI(0)) {
-                int patch;
-                if (row >= 0) {
-                    patch = (layer * op.crows + crow[row]) * op.ccols + ccol;
-                } else {
-                    // A via's far end: its layer, row and column from the index.
-                    const long long plane = (long long)(op.R + 1) * (op.C + 1);
-                    const long long n = pb + off;
-                    const int l = (int)(n / plane);
-                    const int rem = (int)(n - l * plane);
-                    const int y = rem / (op.C + 1), x = rem - y * (op.C + 1);
-                    patch = (l * op.crows + y / op.cblock) * op.ccols + x / op.cblock;
-                }
-                wv = (TI)sd.cw[patch];
+            // The patch index does not depend on loaded data and is always in
+            // range, so w is fetched with the node's other loads and the free
+            // mask applied afterwards.
+            int patch;
+            if (row >= 0) {
+                patch = (layer * op.crows + crow[row]) * op.ccols + ccol;
+            } else {
+                // A via's far end (always inside the grid): its layer, row and
+                // column from the index.
+                const long long plane = (long long)(op.R + 1) * (op.C + 1);
+                const long long n = pb + off;
+                const int l = (int)(n / plane);
+                const int rem = (int)(n - l * plane);
+                const int y = rem / (op.C + 1), x = rem - y * (op.C + 1);
+                patch = (l * op.crows + y / op.cblock) * op.ccols + x / op.cblock;
             }
+            const double wraw = sd.cw[patch];
+            const TI wv = (off != kOutside && rb >= TI(0)) ? (TI)wraw : TI(0);
             const TI z = r * d + wv;
             const TI p = z + beta * pv;
             return InnerValZ<TI>{p, rb, r, d, keep * cv + alpha * pv, z};
@@ -1321,42 +1323,49 @@ __device__ __forceinline__ double coarse_row(const float* __restrict__ cinv, con
     return wave_sum(s);
 }
 
-// The restriction of a two-level slot, one patch per workgroup.  which 0:
-// slots that have just entered an inner solve (r_0, written by the outer
-// step); which 1: after a pass, its new q.
+// The restriction of the two-level slots: the grid loops over the patch
+// table, one patch per workgroup at a time (a fixed grid, so many slots in
+// flight cost loop trips, not dispatches).  which 0: slots that have just
+// entered an inner solve (r_0, written by the outer step); which 1: after a
+// pass, its new q.
 template <typename TI>
 __global__ __launch_bounds__(kGroupThreads) void k_coarse_restrict(const SlotDev* __restrict__ slots,
                                                                    const SlotState* __restrict__ states,
                                                                    const Tile* __restrict__ tiles,
                                                                    const int* __restrict__ counts, const int which) {
-    const int t = blockIdx.x;
-    if (t >= counts[4]) return;
-    const Tile tile = tiles[t];
-    const SlotState& st = states[tile.slot];
-    if (st.phase != kPhaseInner || (which == 0 && st.first == 0)) return;
-    const SlotDev& sd = slots[tile.slot];
-    const OperatorDev& op = sd.op;
-    const TI* v = static_cast<const TI*>(which == 0 ? sd.r[st.pcur] : sd.q[1 - st.pcur]);
-    const double s = patch_sum<TI>(op, v, InnerOf<TI>::robin(op), tile.sz, tile.sy, tile.sx);
-    if (threadIdx.x == 0) sd.cg[((long long)tile.sz * op.crows + tile.sy) * op.ccols + tile.sx] = s;
+    const int count = counts[4];
+    for (int t = blockIdx.x; t < count; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const SlotState& st = states[tile.slot];
+        if (st.phase != kPhaseInner || (which == 0 && st.first == 0)) continue;
+        const SlotDev& sd = slots[tile.slot];
+        const OperatorDev& op = sd.op;
+        const TI* v = static_cast<const TI*>(which == 0 ? sd.r[st.pcur] : sd.q[1 - st.pcur]);
+        const double s = patch_sum<TI>(op, v, InnerOf<TI>::robin(op), tile.sz, tile.sy, tile.sx);
+        if (threadIdx.x == 0) sd.cg[((long long)tile.sz * op.crows + tile.sy) * op.ccols + tile.sx] = s;
+        // The next patch reuses the wave sums in LDS.
+        __syncthreads();
+    }
 }
 
-// cinv g of a two-level slot, kCoarseRows rows per workgroup: into w (which
-// 0, the start of an inner solve) or h (which 1, after a pass).
+// cinv g of the two-level slots, kCoarseRows rows per table entry, the grid
+// looping over the table: into w (which 0, the start of an inner solve) or
+// h (which 1, after a pass).
 __global__ __launch_bounds__(kWave * kCoarseRows) void k_coarse_rows(const SlotDev* __restrict__ slots,
                                                                     const SlotState* __restrict__ states,
                                                                     const Tile* __restrict__ tiles,
                                                                     const int* __restrict__ counts, const int which) {
-    const int t = blockIdx.x;
-    if (t >= counts[5]) return;
-    const Tile tile = tiles[t];
-    const SlotState& st = states[tile.slot];
-    if (st.phase != kPhaseInner || (which == 0 && st.first == 0)) return;
-    const SlotDev& sd = slots[tile.slot];
-    const int row = tile.index + (int)(threadIdx.x / kWave);
-    if (row >= sd.op.nc) return;
-    const double s = coarse_row(sd.op.cinv, sd.op.nc, sd.cg, row);
-    if ((threadIdx.x & (kWave - 1)) == 0) (which == 0 ? sd.cw : sd.ch)[row] = s;
+    const int count = counts[5];
+    for (int t = blockIdx.x; t < count; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const SlotState& st = states[tile.slot];
+        if (st.phase != kPhaseInner || (which == 0 && st.first == 0)) continue;
+        const SlotDev& sd = slots[tile.slot];
+        const int row = tile.index + (int)(threadIdx.x / kWave);
+        if (row >= sd.op.nc) continue;
+        const double s = coarse_row(sd.op.cinv, sd.op.nc, sd.cg, row);
+        if ((threadIdx.x & (kWave - 1)) == 0) (which == 0 ? sd.cw : sd.ch)[row] = s;
+    }
 }
 
 // The standalone preconditioner z = M^-1 r of one operator in FP32
@@ -1503,9 +1512,11 @@ template <typename TI>
 void coarse_kernels(const SlotDev* slots, const SlotState* states, const int* counts, const CoarseLaunch& coarse,
                     const int which, hipStream_t stream) {
     if (coarse.grid_patches <= 0) return;
-    hipLaunchKernelGGL(k_coarse_restrict<TI>, dim3(coarse.grid_patches), dim3(kGroupThreads), 0, stream, slots, states,
+    const int patches = coarse.grid_patches < kCoarseGroups ? coarse.grid_patches : kCoarseGroups;
+    const int rows = coarse.grid_rows < kCoarseGroups ? coarse.grid_rows : kCoarseGroups;
+    hipLaunchKernelGGL(k_coarse_restrict<TI>, dim3(patches), dim3(kGroupThreads), 0, stream, slots, states,
                        coarse.patches, counts, which);
-    hipLaunchKernelGGL(k_coarse_rows, dim3(coarse.grid_rows), dim3(kWave * kCoarseRows), 0, stream, slots, states,
+    hipLaunchKernelGGL(k_coarse_rows, dim3(rows > 0 ? rows : 1), dim3(kWave * kCoarseRows), 0, stream, slots, states,
                        coarse.rows, counts, which);
 }
 
@@ -1535,47 +1546,28 @@ hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const 
     const dim3 block(kGroupThreads);
     const dim3 grid(grid_inner);
     const bool two_level = coarse.grid_patches > 0;
-    // The two-level pass runs over the batch's table only (one workgroup per tile).
-    if (two_level && loop) return hipErrorInvalidValue;
+#define PCBHIP_PASS(TI, ROWS, COARSE)                                                                              \
+    do {                                                                                                           \
+        if (loop)                                                                                                  \
+            hipLaunchKernelGGL((k_inner_step<TI, ROWS, true, COARSE>), grid, block, 0, stream, slots, states,        \
+                               inner_tiles, counts);                                                               \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_inner_step<TI, ROWS, false, COARSE>), grid, block, 0, stream, slots, states,       \
+                               inner_tiles, counts);                                                               \
+    } while (0)
     if (inner64) {
-        if (coarse.plain) {
-            if (loop)
-                hipLaunchKernelGGL((k_inner_step<double, kRowsInner64, true, false>), grid, block, 0, stream, slots,
-                                   states, inner_tiles, counts);
-            else
-                hipLaunchKernelGGL((k_inner_step<double, kRowsInner64, false, false>), grid, block, 0, stream, slots,
-                                   states, inner_tiles, counts);
-        }
-        if (two_level)
-            hipLaunchKernelGGL((k_inner_step<double, kRowsInner64, false, true>), grid, block, 0, stream, slots,
-                               states, inner_tiles, counts);
+        if (coarse.plain) PCBHIP_PASS(double, kRowsInner64, false);
+        if (two_level) PCBHIP_PASS(double, kRowsInner64, true);
         return hipGetLastError();
     }
     if (rows_inner == 8) {
-        if (coarse.plain) {
-            if (loop)
-                hipLaunchKernelGGL((k_inner_step<float, 8, true, false>), grid, block, 0, stream, slots, states,
-                                   inner_tiles, counts);
-            else
-                hipLaunchKernelGGL((k_inner_step<float, 8, false, false>), grid, block, 0, stream, slots, states,
-                                   inner_tiles, counts);
-        }
-        if (two_level)
-            hipLaunchKernelGGL((k_inner_step<float, 8, false, true>), grid, block, 0, stream, slots, states,
-                               inner_tiles, counts);
+        if (coarse.plain) PCBHIP_PASS(float, 8, false);
+        if (two_level) PCBHIP_PASS(float, 8, true);
     } else {
-        if (coarse.plain) {
-            if (loop)
-                hipLaunchKernelGGL((k_inner_step<float, 4, true, false>), grid, block, 0, stream, slots, states,
-                                   inner_tiles, counts);
-            else
-                hipLaunchKernelGGL((k_inner_step<float, 4, false, false>), grid, block, 0, stream, slots, states,
-                                   inner_tiles, counts);
-        }
-        if (two_level)
-            hipLaunchKernelGGL((k_inner_step<float, 4, false, true>), grid, block, 0, stream, slots, states,
-                               inner_tiles, counts);
+        if (coarse.plain) PCBHIP_PASS(float, 4, false);
+        if (two_level) PCBHIP_PASS(float, 4, true);
     }
+#undef PCBHIP_PASS
     if (grid_block > 0) {
         if (block_mfma == 3) {
             hipLaunchKernelGGL((k_inner_step_block<4, 3>), dim3(grid_block), block, 0, stream, slots, states,

@@ -9,7 +9,8 @@ operator applications on the device: the patches are coloured so that two
 patches of one colour never touch a common element (nor, on a sheet
 operator, a common via), so one application per colour gives every
 coarse-matrix entry between a patch and its neighbour of that colour.  Its
-Cholesky factor proves it SPD and gives the dense inverse, which moves to
+Cholesky factor proves it SPD and gives the dense inverse (one
+factorisation at a time per process), which moves to
 the device once (``pcbhip_operator_set_coarse``).  Every inner PCG iteration
 then runs the restriction, one dense row per coarse unknown and the
 prolongation on the device, inside the batch's captured iterations.
@@ -17,11 +18,37 @@ prolongation on the device, inside the batch's captured iterations.
 
 from __future__ import annotations
 
+import threading
 from typing import Callable
 
 import numpy as np
 
 DEFAULT_MAX_COARSE_SIZE = 2048
+
+# One dense factorisation at a time in the process: each already uses every
+# BLAS thread, and host threads that build operators together (a served
+# electro-thermal loop per thread) would otherwise multiply the BLAS threads
+# far past the cores.
+_FACTOR_LOCK = threading.Lock()
+
+
+def _spd_inverse(matrix: np.ndarray) -> np.ndarray:
+    """Inverse of an SPD matrix, ``L^-T L^-1`` from its Cholesky factor,
+    symmetrised against rounding; raises ValueError when the factorisation
+    fails.  NumPy's own LAPACK only (a second BLAS library would start a
+    second thread pool beside NumPy's)."""
+
+    with _FACTOR_LOCK:
+        try:
+            factor = np.linalg.cholesky(matrix)
+        except np.linalg.LinAlgError as exc:
+            raise ValueError(
+                "the coarse operator is not positive definite; the fine operator must be SPD for the two-level "
+                "preconditioner"
+            ) from exc
+        lower_inverse = np.linalg.solve(factor, np.eye(matrix.shape[0]))
+        inverse = lower_inverse.T @ lower_inverse
+    return 0.5 * (inverse + inverse.T)
 
 
 def _cells(extent: int, width: int) -> int:
@@ -78,18 +105,8 @@ class AggregationCoarseCorrection:
         matrix[empty, :] = 0.0
         matrix[:, empty] = 0.0
         matrix[empty, empty] = 1.0
-        try:
-            factor = np.linalg.cholesky(matrix)
-        except np.linalg.LinAlgError as exc:
-            raise ValueError(
-                "the coarse operator is not positive definite; the fine operator must be SPD for the two-level "
-                "preconditioner"
-            ) from exc
-        # A^-1 = L^-T L^-1, symmetrised against rounding.
-        lower_inverse = np.linalg.solve(factor, np.eye(self.coarse_size))
-        inverse = lower_inverse.T @ lower_inverse
         self.coarse_matrix = matrix
-        self.coarse_inverse = 0.5 * (inverse + inverse.T)
+        self.coarse_inverse = _spd_inverse(matrix)
 
     # ---------------------------------------------------------- transfers
     def restrict(self, grid: np.ndarray) -> np.ndarray:

@@ -28,7 +28,7 @@ import dataclasses
 import warnings
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
-from typing import Sequence
+from typing import Callable, Sequence
 
 import numpy as np
 
@@ -352,21 +352,26 @@ def run_electro_thermal(scenario: ElectroThermalScenario, *, config: CouplingCon
 
 
 def run_electro_thermal_many(scenarios: Sequence[ElectroThermalScenario], *, config: CouplingConfig | None = None,
-                             device_id: int = 0, max_slots: int = 256,
-                             memory_budget_bytes: float = 200e9) -> list[ElectroThermalResult]:
+                             device_id: int = 0, max_slots: int = 256, memory_budget_bytes: float = 200e9,
+                             on_done: Callable[[int], None] | None = None) -> list[ElectroThermalResult]:
     """Many electro-thermal loops at once: one host thread per loop, every DC
     and thermal solve of every loop continuously batched on one
     :class:`pcbhip.serving.ServedSolves` server.  The reference runs such
     loops one after another; each result here is bitwise what
-    :func:`run_electro_thermal` gives the scenario alone."""
+    :func:`run_electro_thermal` gives the scenario alone.  ``on_done(i)`` is
+    called from loop i's thread when it finishes."""
 
     scenarios = list(scenarios)
     if not scenarios:
         return []
     with serving.ServedSolves(device_id, max_slots=max_slots, memory_budget_bytes=memory_budget_bytes) as served:
-        def one(scenario: ElectroThermalScenario) -> ElectroThermalResult:
+        def one(item) -> ElectroThermalResult:
+            index, scenario = item
             with serving.route_solves(served):
-                return run_electro_thermal(scenario, config=config, device_id=device_id)
+                result = run_electro_thermal(scenario, config=config, device_id=device_id)
+            if on_done is not None:
+                on_done(index)
+            return result
 
         with ThreadPoolExecutor(max_workers=len(scenarios), thread_name_prefix="coupling") as pool:
-            return list(pool.map(one, scenarios))
+            return list(pool.map(one, enumerate(scenarios)))

@@ -332,12 +332,10 @@ def route_solves(executor: ServedSolves | None):
 
 def solve_mpir_routed(operator: _capi.Operator, rhs, x0, config: _capi.MpirConfigC):
     """One device MPIR solve: through this thread's :class:`ServedSolves` when
-    one is routed, else through the single-solve path ``_capi.solve_mpir``.
-    Two-level operators always take the single-solve path (a server runs
-    Jacobi slots)."""
+    one is routed, else through the single-solve path ``_capi.solve_mpir``."""
 
     executor = getattr(_route, "executor", None)
-    if executor is None or getattr(operator, "coarse_size", 0):
+    if executor is None:
         return _capi.solve_mpir(operator, rhs, x0, config)
     return executor.solve(operator, rhs, x0, config)
 

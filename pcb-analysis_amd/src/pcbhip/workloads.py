@@ -15,6 +15,7 @@ mask, then sources), so a seed names a board.
                              20-70 % copper, 1..8 sources of 0.1-3 W (seed 4).
 * ``dc_board``, ``serving_pool`` -- the DC copper/dielectric board and the serving mix.
 * ``coupling_scenarios``  -- 16 voltage-driven electro-thermal current loops (seed 6).
+* ``four_slab_fixture``   -- the reference's own thermal benchmark stack (no randomness).
 """
 
 from __future__ import annotations
@@ -91,6 +92,32 @@ def random_sources(rng: np.random.Generator, rows: int, cols: int, count: int, p
         col = int(rng.integers(0, cols - side + 1))
         out.append(Source(row, col, side, power))
     return out
+
+
+def four_slab_fixture(elements: int) -> ThermalConductionProblem:
+    """The stack of the reference's native thermal benchmark
+    (experiments/thermal_native_benchmark.py:43-61): 35 um Cu / 0.7 mm FR-4
+    (0.8 in-plane, 0.3 through) / 0.7 mm FR-4 / 35 um Cu over 50 x 50 mm in
+    ``elements`` x ``elements`` cells, h = 10 W/m^2 K on both faces, 1 W on a
+    trace across the middle half of the top slab.  50, 100 and 200 elements
+    give the 13,005, 51,005 and 202,005 nodes of its published timings."""
+
+    mesh = LayeredThermalMesh(
+        slab_thickness_m=(35.0e-6, 0.7e-3, 0.7e-3, 35.0e-6),
+        pitch_x_m=50.0e-3 / elements,
+        pitch_y_m=50.0e-3 / elements,
+        conductivity_w_per_m_k=(COPPER_K, 0.8, 0.8, COPPER_K),
+        through_plane_conductivity_w_per_m_k=(COPPER_K, 0.3, 0.3, COPPER_K),
+        element_shape=(elements, elements),
+    )
+    heat = np.zeros(mesh.element_grid_shape)
+    heat[3, elements // 2, elements // 4: 3 * elements // 4] = 1.0 / (elements // 2)
+    return ThermalConductionProblem(
+        mesh,
+        convection=(ConvectionBoundary("top", FILM_W_PER_M2_K, AMBIENT_K),
+                    ConvectionBoundary("bottom", FILM_W_PER_M2_K, AMBIENT_K)),
+        element_heat_w=heat,
+    )
 
 
 def board_512x512x16(seed: int = 2, size: int = 512, slabs: int = 16) -> ThermalConductionProblem:

@@ -4,8 +4,8 @@ Galerkin product and the device action tracks the FP64 action; two-level
 PCG cuts the inner iterations at least five-fold and lands on the Jacobi
 answer; the default configuration converges in under 600 inner iterations.
 Beyond them: FP64-oracle parity, slot independence in mixed batches, the
-FP64 inner solve, the layered-sheet operator and its N-port basis, and the
-server's refusal."""
+FP64 inner solve, the layered-sheet operator and its N-port basis, and
+served solves."""
 
 from __future__ import annotations
 
@@ -204,20 +204,34 @@ def test_port_basis_is_the_same_with_either_preconditioner():
                                atol=1e-9 * np.abs(jacobi.conductance_s).max())
 
 
-def test_server_takes_jacobi_operators_only_and_routing_falls_back():
-    print("two-level server refusal", flush=True)
-    problem = stack(6, 8)
-    operator = MatrixFreeThermalOperator(problem, coarse_block_nodes=3)
-    _, _, rhs = _oracle(problem)
+def test_served_two_level_solves_equal_their_solves_alone():
+    print("two-level served", flush=True)
+    problems = [stack(6, 8), stack(13, 9, fixed=True), random_body((2, 9, 30), seed=7)]
+    two = [MatrixFreeThermalOperator(p, coarse_block_nodes=3) for p in problems]
+    plain = MatrixFreeThermalOperator(problems[1], preconditioner="jacobi")
     config = _capi.config_c(1e-10, 0.0, 2e-3, 16, 4000)
-    server = _capi.Server(0, max_slots=4)
-    try:
-        with pytest.raises(ValueError, match="Jacobi"):
-            server.submit(operator.device_operator, rhs, None, config, tag=1)
-    finally:
-        server.close()
-    alone = _capi.solve_mpir(operator.device_operator, rhs, None, config)
-    with serving.ServedSolves(max_slots=8) as executor, serving.route_solves(executor):
-        x, r = serving.solve_mpir_routed(operator.device_operator, rhs, None, config)
-    assert r["converged"]
-    assert x.tobytes() == alone[0].tobytes()
+    rhs = [_oracle(p)[2] for p in problems]
+    alone = [_capi.solve_mpir(op.device_operator, b, None, config) for op, b in zip(two, rhs)]
+    plain_alone = _capi.solve_mpir(plain.device_operator, rhs[1], None, config)
+    # One server, two-level and Jacobi slots in flight together.
+    with serving.ServedSolves(max_slots=8, steps_per_replay=4) as executor, serving.route_solves(executor):
+        import threading
+
+        got = {}
+
+        def run(key, op, b):
+            got[key] = serving.solve_mpir_routed(op, b, None, config)
+
+        threads = [threading.Thread(target=run, args=(k, op.device_operator, b)) for k, (op, b) in
+                   enumerate(zip(two, rhs))]
+        threads.append(threading.Thread(target=run, args=("plain", plain.device_operator, rhs[1])))
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+    for k in range(len(problems)):
+        x, r = got[k]
+        assert r["converged"]
+        assert x.tobytes() == alone[k][0].tobytes()
+        assert r["inner_iterations"] == alone[k][1]["inner_iterations"]
+    assert got["plain"][0].tobytes() == plain_alone[0].tobytes()
