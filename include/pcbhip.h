@@ -109,6 +109,19 @@ int pcbhip_operator_diagonal(const pcbhip_operator *op, double *diag);
  * it.  Batches made and server submissions made afterwards run two-level
  * PCG on this operator; slots made before keep what they were made with. */
 int pcbhip_operator_set_coarse(pcbhip_operator *op, int block, int nc, const double *cinv);
+/* The same from the coarse matrix itself: Z^T A Z (nc x nc FP64, symmetric)
+ * is factored and inverted on the device (Cholesky and its inverse, the
+ * LLT inverse of hex_q1.cpp:819-834 and two_level.py:108-122); a matrix that
+ * is not positive definite is PCBHIP_INVALID_INPUT.  inverse (NULL or nc x nc
+ * doubles) receives the FP64 inverse. */
+int pcbhip_operator_set_coarse_matrix(pcbhip_operator *op, int block, int nc, const double *matrix,
+                                      double *inverse);
+/* out[k] = Z^T A Z v[k] in FP64 for count coarse vectors of nc values each
+ * (row-major count x nc), Z the free-node patch indicator of block x block
+ * patches as above: the Galerkin assembly's applications on the device
+ * (replaces the coloured apply loop of two_level.py:147-195 and
+ * hex_q1.cpp:703-818, which run the FP64 action on host vectors). */
+int pcbhip_operator_galerkin_f64(const pcbhip_operator *op, int block, int count, const double *v, double *out);
 /* z = M^-1 r in FP32 with the operator's preconditioner, Jacobi or two-level
  * (replaces MatrixFreeThermalOperator.precondition_low, operator.py:366-371). */
 int pcbhip_operator_precondition_f32(const pcbhip_operator *op, const float *r, float *z);

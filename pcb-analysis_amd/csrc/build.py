@@ -34,10 +34,15 @@ FLAGS = [
 ]
 
 
+# rocSOLVER (and its rocBLAS) factor and invert the two-level coarse
+# matrices on the device.
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+
+
 def build(output: Path = OUTPUT) -> Path:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = output.with_suffix(".so.tmp")
-    cmd = [hipcc, *FLAGS, "-o", str(tmp), *[str(HERE / s) for s in SOURCES]]
+    cmd = [hipcc, *FLAGS, "-o", str(tmp), *[str(HERE / s) for s in SOURCES], *LIBS]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     noise = "packed-fp32-ops"
     err = "\n".join(l for l in proc.stderr.splitlines() if noise not in l)

@@ -21,6 +21,8 @@
 #include <string>
 #include <vector>
 
+#include <rocsolver/rocsolver.h>
+
 #include "../../include/pcbhip.h"
 #include "q1hex_device.h"
 
@@ -996,6 +998,50 @@ void* server_take_block(pcbhip_server* sv, size_t bytes, size_t* got) {
 
 }  // namespace
 
+namespace {
+
+// One rocBLAS handle per device for the coarse factorisations, made on first
+// use and kept (its workspace too) for the life of the process; calls take
+// the handle's lock and set its stream.
+struct SolverHandle {
+    std::mutex lock;
+    rocblas_handle handle = nullptr;
+};
+
+SolverHandle& solver_handle(int device) {
+    static std::mutex table_lock;
+    static std::vector<std::unique_ptr<SolverHandle>> table;
+    std::lock_guard<std::mutex> guard(table_lock);
+    if ((size_t)device >= table.size()) table.resize((size_t)device + 1);
+    if (!table[(size_t)device]) {
+        auto h = std::make_unique<SolverHandle>();
+        rocblas_status st;
+        {
+            DeviceLock lock(device_mutex());
+            st = rocblas_create_handle(&h->handle);
+        }
+        if (st != rocblas_status_success) fail(PCBHIP_RUNTIME, "rocblas_create_handle failed");
+        table[(size_t)device] = std::move(h);
+    }
+    return *table[(size_t)device];
+}
+
+// Coarse-size checks shared by the coarse entry points; returns (rows, cols).
+std::pair<int, int> coarse_grid(const pcbhip_operator* op, int block, int nc) {
+    if (block < 1) fail(PCBHIP_INVALID_INPUT, "block must be a positive node count");
+    const long long crows = (op->R + 1 + block - 1) / block, ccols = (op->C + 1 + block - 1) / block;
+    const long long expected = (long long)op->layers() * crows * ccols;
+    if (nc != expected)
+        fail(PCBHIP_INVALID_INPUT, "coarse size " + std::to_string(nc) + " does not match the patches of block " +
+                                       std::to_string(block) + " (" + std::to_string(expected) + ")");
+    if (nc > kMaxCoarse)
+        fail(PCBHIP_INVALID_INPUT, "coarse size " + std::to_string(nc) + " exceeds the limit of " +
+                                       std::to_string(kMaxCoarse));
+    return {(int)crows, (int)ccols};
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* pcbhip_last_error(void) { return g_last_error.c_str(); }
@@ -1394,15 +1440,7 @@ int pcbhip_operator_set_coarse(pcbhip_operator* op, int block, int nc, const dou
             return;
         }
         if (!cinv) fail(PCBHIP_INVALID_INPUT, "null coarse inverse");
-        if (block < 1) fail(PCBHIP_INVALID_INPUT, "block must be a positive node count");
-        const long long crows = (op->R + 1 + block - 1) / block, ccols = (op->C + 1 + block - 1) / block;
-        const long long expected = (long long)op->layers() * crows * ccols;
-        if (nc != expected)
-            fail(PCBHIP_INVALID_INPUT, "coarse size " + std::to_string(nc) + " does not match the patches of block " +
-                                           std::to_string(block) + " (" + std::to_string(expected) + ")");
-        if (nc > kMaxCoarse)
-            fail(PCBHIP_INVALID_INPUT, "coarse size " + std::to_string(nc) + " exceeds the limit of " +
-                                           std::to_string(kMaxCoarse));
+        const std::pair<int, int> grid = coarse_grid(op, block, nc);
         const size_t count = (size_t)nc * (size_t)nc;
         std::vector<float> values(count);
         for (size_t i = 0; i < count; ++i) {
@@ -1420,9 +1458,109 @@ int pcbhip_operator_set_coarse(pcbhip_operator* op, int block, int nc, const dou
         if (op->cinv) op->retired.push_back(op->cinv);
         op->cinv = d;
         op->cblock = block;
-        op->crows = (int)crows;
-        op->ccols = (int)ccols;
+        op->crows = grid.first;
+        op->ccols = grid.second;
         op->nc = nc;
+    });
+}
+
+
+int pcbhip_operator_set_coarse_matrix(pcbhip_operator* op, int block, int nc, const double* matrix,
+                                      double* inverse) {
+    return guarded([&] {
+        if (!op || !matrix) fail(PCBHIP_INVALID_INPUT, "null argument");
+        const std::pair<int, int> grid = coarse_grid(op, block, nc);
+        const size_t count = (size_t)nc * (size_t)nc;
+        for (size_t i = 0; i < count; ++i)
+            if (!std::isfinite(matrix[i])) fail(PCBHIP_INVALID_INPUT, "coarse matrix must be finite");
+        set_device(op->device);
+        const size_t b64 = align256(sizeof(double) * count);
+        // The factor in place, then the FP32 inverse the solves read (kept),
+        // and the FP64 inverse when the caller wants it back.
+        char* work = static_cast<char*>(stream_alloc(op->stream, 2 * b64 + 256));
+        double* packed = reinterpret_cast<double*>(work);
+        double* full = inverse ? reinterpret_cast<double*>(work + b64) : nullptr;
+        rocblas_int* info = reinterpret_cast<rocblas_int*>(work + 2 * b64);
+        float* d = nullptr;
+        try {
+            d = static_cast<float*>(stream_alloc(op->stream, sizeof(float) * count));
+            check(hipMemcpyAsync(packed, matrix, sizeof(double) * count, hipMemcpyHostToDevice, op->stream),
+                  "coarse upload");
+            rocblas_int h_info[2] = {0, 0};
+            {
+                SolverHandle& solver = solver_handle(op->device);
+                std::lock_guard<std::mutex> guard(solver.lock);
+                if (rocblas_set_stream(solver.handle, op->stream) != rocblas_status_success)
+                    fail(PCBHIP_RUNTIME, "rocblas_set_stream failed");
+                if (rocsolver_dpotrf(solver.handle, rocblas_fill_lower, nc, packed, nc, info) !=
+                    rocblas_status_success)
+                    fail(PCBHIP_RUNTIME, "rocsolver_dpotrf failed");
+                check(hipMemcpyAsync(&h_info[0], info, sizeof(rocblas_int), hipMemcpyDeviceToHost, op->stream),
+                      "info");
+                check(hipStreamSynchronize(op->stream), "potrf");
+                if (h_info[0] != 0)
+                    fail(PCBHIP_INVALID_INPUT,
+                         "the coarse operator is not positive definite; the fine operator must be SPD for the "
+                         "two-level preconditioner");
+                if (rocsolver_dpotri(solver.handle, rocblas_fill_lower, nc, packed, nc, info) !=
+                    rocblas_status_success)
+                    fail(PCBHIP_RUNTIME, "rocsolver_dpotri failed");
+                check(hipMemcpyAsync(&h_info[1], info, sizeof(rocblas_int), hipMemcpyDeviceToHost, op->stream),
+                      "info");
+                check(launch::symmetric_inverse(packed, nc, d, full, op->stream), "symmetric inverse launch");
+                if (inverse)
+                    check(hipMemcpyAsync(inverse, full, sizeof(double) * count, hipMemcpyDeviceToHost, op->stream),
+                          "inverse download");
+                check(hipStreamSynchronize(op->stream), "potri");
+            }
+            if (h_info[1] != 0) fail(PCBHIP_INVALID_INPUT, "the coarse operator is singular");
+        } catch (...) {
+            if (d) (void)stream_free(op->stream, d);
+            (void)stream_free(op->stream, work);
+            throw;
+        }
+        (void)stream_free(op->stream, work);
+        if (op->cinv) op->retired.push_back(op->cinv);
+        op->cinv = d;
+        op->cblock = block;
+        op->crows = grid.first;
+        op->ccols = grid.second;
+        op->nc = nc;
+    });
+}
+
+int pcbhip_operator_galerkin_f64(const pcbhip_operator* op, int block, int count, const double* v, double* out) {
+    return guarded([&] {
+        if (!op || !v || !out) fail(PCBHIP_INVALID_INPUT, "null argument");
+        if (block < 1) fail(PCBHIP_INVALID_INPUT, "block must be a positive node count");
+        if (count < 1) fail(PCBHIP_INVALID_INPUT, "count must be positive");
+        OperatorDev view = op->view();
+        view.cblock = block;
+        view.crows = (op->R + 1 + block - 1) / block;
+        view.ccols = (op->C + 1 + block - 1) / block;
+        const long long nc = (long long)op->layers() * view.crows * view.ccols;
+        if (nc > kMaxCoarse)
+            fail(PCBHIP_INVALID_INPUT, "coarse size " + std::to_string(nc) + " exceeds the limit of " +
+                                           std::to_string(kMaxCoarse));
+        view.nc = (int)nc;
+        set_device(op->device);
+        const size_t n64 = align256(sizeof(double) * (size_t)op->N);
+        const size_t c64 = align256(sizeof(double) * (size_t)nc * (size_t)count);
+        std::lock_guard<std::mutex> guard(op->staging_mutex);
+        char* base = staging_of(op, std::max(2 * n64 + 2 * c64, 2 * n64));
+        double* fine = reinterpret_cast<double*>(base);
+        double* y = reinterpret_cast<double*>(base + n64);
+        double* dv = reinterpret_cast<double*>(base + 2 * n64);
+        double* dout = reinterpret_cast<double*>(base + 2 * n64 + c64);
+        const size_t bytes = sizeof(double) * (size_t)nc * (size_t)count;
+        check(hipMemcpyAsync(dv, v, bytes, hipMemcpyHostToDevice, op->stream), "copy v");
+        for (int k = 0; k < count; ++k) {
+            check(launch::coarse_spread(view, dv + (size_t)k * nc, fine, op->stream), "spread launch");
+            apply_dev<double>(op, fine, y, 0, op->stream);
+            check(launch::coarse_restrict(view, y, dout + (size_t)k * nc, op->stream), "restrict launch");
+        }
+        check(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, op->stream), "copy out");
+        check(hipStreamSynchronize(op->stream), "galerkin");
     });
 }
 

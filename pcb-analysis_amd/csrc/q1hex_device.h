@@ -226,6 +226,13 @@ hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const 
 // z = M^-1 r of one operator in FP32 (Jacobi, plus the coarse correction
 // when the operator has one); g and w are nc-long FP64 scratch vectors.
 hipError_t precondition(const OperatorDev& op, const float* r, float* z, double* g, double* w, hipStream_t stream);
+// The Galerkin assembly's transfers in FP64 for the patches op describes:
+// fine = Z v, and g = Z^T y (free nodes only).
+hipError_t coarse_spread(const OperatorDev& op, const double* v, double* fine, hipStream_t stream);
+hipError_t coarse_restrict(const OperatorDev& op, const double* y, double* g, hipStream_t stream);
+// The full symmetric inverse from the lower triangle potri leaves
+// (column-major ``packed``): FP32 row-major, and FP64 too when inverse64 is set.
+hipError_t symmetric_inverse(const double* packed, int n, float* inverse32, double* inverse64, hipStream_t stream);
 // states[index[i]] = fresh[i] for i < count (device arrays).
 hipError_t scatter_states(const int* index, const SlotState* fresh, SlotState* states, int count, hipStream_t stream);
 }  // namespace launch

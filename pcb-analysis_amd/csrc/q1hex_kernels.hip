@@ -1370,13 +1370,31 @@ __global__ __launch_bounds__(kWave * kCoarseRows) void k_coarse_rows(const SlotD
 
 // The standalone preconditioner z = M^-1 r of one operator in FP32
 // (operator.py:366-371): patch sums, coarse rows, then z = D^-1 r + Z w.
-__global__ __launch_bounds__(kGroupThreads) void k_op_restrict(const OperatorDev op, const float* __restrict__ r,
+template <typename T>
+__global__ __launch_bounds__(kGroupThreads) void k_op_restrict(const OperatorDev op, const T* __restrict__ r,
                                                                double* __restrict__ g) {
     const int patch = blockIdx.x;
     const int per_layer = op.crows * op.ccols;
     const int layer = patch / per_layer, cy = (patch % per_layer) / op.ccols, cx = patch % op.ccols;
-    const double s = patch_sum<float>(op, r, op.robin32, layer, cy, cx);
+    const double s = patch_sum<T>(op, r, InnerOf<T>::robin(op), layer, cy, cx);
     if (threadIdx.x == 0) g[patch] = s;
+}
+
+// fine = Z v in FP64: every free node takes its patch's coarse value, fixed
+// nodes 0 (the Galerkin assembly's prolongation).
+__global__ __launch_bounds__(256) void k_op_spread(const OperatorDev op, const double* __restrict__ v,
+                                                   double* __restrict__ fine) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= op.N) return;
+    double value = 0.0;
+    if (op.robin64[n] >= 0.0) {
+        const long long NC = op.C + 1, plane = (long long)(op.R + 1) * NC;
+        const int l = (int)(n / plane);
+        const int rem = (int)(n - l * plane);
+        const int y = (int)(rem / NC), x = (int)(rem - y * NC);
+        value = v[((long long)l * op.crows + y / op.cblock) * op.ccols + x / op.cblock];
+    }
+    fine[n] = value;
 }
 
 __global__ __launch_bounds__(kWave * kCoarseRows) void k_op_rows(const OperatorDev op, const double* __restrict__ g,
@@ -1409,6 +1427,21 @@ __global__ void k_scatter_states(const int* __restrict__ index, const SlotState*
                                  SlotState* states, const int count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) states[index[i]] = fresh[i];
+}
+
+// The inverse from LAPACK-style potri (its lower triangle, column-major
+// n x n in ``packed``) as a full symmetric matrix: FP32 row-major into
+// ``inverse32`` and, when ``inverse64`` is not null, FP64.
+__global__ __launch_bounds__(256) void k_symmetric_inverse(const double* __restrict__ packed, const int n,
+                                                           float* __restrict__ inverse32,
+                                                           double* __restrict__ inverse64) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)n * n) return;
+    const int r = (int)(k / n), c = (int)(k - (long long)r * n);
+    // Element (r, c) of the lower triangle sits at r + c n (column-major).
+    const double v = r >= c ? packed[r + (long long)c * n] : packed[c + (long long)r * n];
+    inverse32[k] = (float)v;
+    if (inverse64) inverse64[k] = v;
 }
 
 // ------------------------------------------------------ host launchers
@@ -1601,9 +1634,26 @@ hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* 
     return hipGetLastError();
 }
 
+hipError_t coarse_spread(const OperatorDev& op, const double* v, double* fine, hipStream_t stream) {
+    hipLaunchKernelGGL(k_op_spread, dim3((unsigned)((op.N + 255) / 256)), dim3(256), 0, stream, op, v, fine);
+    return hipGetLastError();
+}
+
+hipError_t symmetric_inverse(const double* packed, int n, float* inverse32, double* inverse64, hipStream_t stream) {
+    const long long count = (long long)n * n;
+    hipLaunchKernelGGL(k_symmetric_inverse, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, packed, n,
+                       inverse32, inverse64);
+    return hipGetLastError();
+}
+
+hipError_t coarse_restrict(const OperatorDev& op, const double* y, double* g, hipStream_t stream) {
+    hipLaunchKernelGGL(k_op_restrict<double>, dim3(op.nc), dim3(kGroupThreads), 0, stream, op, y, g);
+    return hipGetLastError();
+}
+
 hipError_t precondition(const OperatorDev& op, const float* r, float* z, double* g, double* w, hipStream_t stream) {
     if (op.nc > 0) {
-        hipLaunchKernelGGL(k_op_restrict, dim3(op.nc), dim3(kGroupThreads), 0, stream, op, r, g);
+        hipLaunchKernelGGL(k_op_restrict<float>, dim3(op.nc), dim3(kGroupThreads), 0, stream, op, r, g);
         hipLaunchKernelGGL(k_op_rows, dim3((op.nc + kCoarseRows - 1) / kCoarseRows), dim3(kWave * kCoarseRows), 0,
                            stream, op, g, w);
     }

@@ -304,8 +304,8 @@ class MatrixFreePCBOperator:
             self.coarse_correction = AggregationCoarseCorrection(
                 node_shape=shape, free_nodes=self.free_nodes, diagonal_high=self._device.diagonal(),
                 apply_high=self.apply_high, block=coarse_block_nodes, layer_colours=shape[0],
+                galerkin=self._device.galerkin_f64, invert=self._device.set_coarse_matrix,
             )
-            self._device.set_coarse(self.coarse_correction.block, self.coarse_correction.coarse_inverse)
 
     @property
     def device_operator(self) -> _capi.Operator:

@@ -80,11 +80,20 @@ class AggregationCoarseCorrection:
     (every layer its own colour, so a via between any two layers is seen);
     in-plane, a via's two ends must lie within one patch width of each
     other.  Patches made only of fixed nodes get a unit diagonal, as in the
-    reference, so the matrix stays SPD; their restricted values are zero."""
+    reference, so the matrix stays SPD; their restricted values are zero.
+    ``galerkin(block, vectors)``, when given, returns ``Z^T A Z v`` for each
+    row of ``vectors`` in one call (the device's FP64 transfers and action,
+    ``pcbhip_operator_galerkin_f64``); otherwise each product is formed on
+    the host around ``apply_high``.  ``invert(block, matrix)``, when given,
+    returns the inverse of the finished coarse matrix (the device's
+    factorisation, ``pcbhip_operator_set_coarse_matrix``, which also installs
+    it); otherwise it is formed on the host."""
 
     def __init__(self, *, node_shape: tuple[int, int, int], free_nodes: np.ndarray, diagonal_high: np.ndarray,
                  apply_high: Callable[[np.ndarray], np.ndarray], block: int | None = None,
-                 max_coarse_size: int = DEFAULT_MAX_COARSE_SIZE, layer_colours: int = 3) -> None:
+                 max_coarse_size: int = DEFAULT_MAX_COARSE_SIZE, layer_colours: int = 3,
+                 galerkin: Callable[[int, np.ndarray], np.ndarray] | None = None,
+                 invert: Callable[[int, np.ndarray], np.ndarray] | None = None) -> None:
         layers, rows, cols = (int(v) for v in node_shape)
         self.node_shape = (layers, rows, cols)
         if block is None:
@@ -100,13 +109,15 @@ class AggregationCoarseCorrection:
         if diagonal.size != free.size:
             raise ValueError("diagonal_high must hold one value per node")
         self._diagonal = diagonal
-        matrix = self._galerkin(apply_high, max(1, min(int(layer_colours), layers)))
+        if galerkin is None:
+            galerkin = self._host_galerkin(apply_high)
+        matrix = self._galerkin(galerkin, max(1, min(int(layer_colours), layers)))
         empty = self.restrict(free.astype(np.float64)).reshape(-1) <= 0.0
         matrix[empty, :] = 0.0
         matrix[:, empty] = 0.0
         matrix[empty, empty] = 1.0
         self.coarse_matrix = matrix
-        self.coarse_inverse = _spd_inverse(matrix)
+        self.coarse_inverse = _spd_inverse(matrix) if invert is None else np.asarray(invert(self.block, matrix))
 
     # ---------------------------------------------------------- transfers
     def restrict(self, grid: np.ndarray) -> np.ndarray:
@@ -127,7 +138,22 @@ class AggregationCoarseCorrection:
         return spread[:, :rows, :cols]
 
     # ------------------------------------------------------- construction
-    def _galerkin(self, apply_high: Callable[[np.ndarray], np.ndarray], layer_colours: int) -> np.ndarray:
+    def _host_galerkin(self, apply_high: Callable[[np.ndarray], np.ndarray]):
+        """``Z^T A Z v`` for rows ``v``, around a host FP64 action."""
+
+        free = self._free
+
+        def products(block: int, vectors: np.ndarray) -> np.ndarray:
+            out = []
+            for v in np.asarray(vectors, dtype=np.float64):
+                fine = np.where(free, self.prolong(v), 0.0)
+                action = np.asarray(apply_high(fine.reshape(-1))).reshape(self.node_shape)
+                out.append(self.restrict(np.where(free, action, 0.0)).reshape(-1))
+            return np.asarray(out)
+
+        return products
+
+    def _galerkin(self, galerkin: Callable[[int, np.ndarray], np.ndarray], layer_colours: int) -> np.ndarray:
         """``Z^T A Z`` from one FP64 application per patch colour.
 
         A patch's colour is (layer mod ``layer_colours``, row mod 3, column
@@ -137,20 +163,25 @@ class AggregationCoarseCorrection:
         each colour: the sum over I of A applied to that colour's indicator
         is the single entry (I, J) for that patch J."""
 
-        free = self._free
         index = np.indices(self.coarse_shape)
         flat = np.arange(self.coarse_size).reshape(self.coarse_shape)
         periods = (layer_colours, 3, 3)
-        matrix = np.zeros((self.coarse_size, self.coarse_size))
+        colours, indicators = [], []
         for colour in np.ndindex(*periods):
             chosen = np.ones(self.coarse_shape, dtype=bool)
             for axis in range(3):
                 chosen &= index[axis] % periods[axis] == colour[axis]
-            if not chosen.any():
-                continue
-            fine = np.where(free, self.prolong(chosen.astype(np.float64)), 0.0)
-            action = np.where(free, np.asarray(apply_high(fine.reshape(-1))).reshape(self.node_shape), 0.0)
-            sums = self.restrict(action)
+            if chosen.any():
+                colours.append(colour)
+                indicators.append(chosen.reshape(-1).astype(np.float64))
+        # The last product checks the whole matrix: Z^T A Z v for a fixed
+        # pseudo-random v.  An entry assigned to the wrong patch pair (a
+        # coupling the colouring does not separate) shows up there.
+        v = np.random.default_rng(12345).uniform(0.5, 1.5, self.coarse_size)
+        products = np.asarray(galerkin(self.block, np.stack(indicators + [v])))
+        matrix = np.zeros((self.coarse_size, self.coarse_size))
+        for colour, sums in zip(colours, products[:-1]):
+            sums = sums.reshape(self.coarse_shape)
             # The colour's patch next to (or at) each patch: in-plane, the
             # offset in {-1, 0, 1} that lands on the colour's residue; across
             # layers the same with period 3, or the colour's own layer when
@@ -165,18 +196,9 @@ class AggregationCoarseCorrection:
                 target = index[axis] + np.where(step == 2, -1, step)
                 valid &= (target >= 0) & (target < self.coarse_shape[axis])
                 partner.append(np.clip(target, 0, self.coarse_shape[axis] - 1))
-            rows_ = flat[valid]
-            cols_ = flat[tuple(p[valid] for p in partner)]
-            matrix[rows_, cols_] = sums[valid]
-        # One more application checks the whole matrix: Z^T A Z v for a
-        # fixed pseudo-random v.  An entry assigned to the wrong patch pair
-        # (a coupling the colouring does not separate) shows up here.
-        v = np.random.default_rng(12345).uniform(0.5, 1.5, self.coarse_size)
-        fine = np.where(free, self.prolong(v), 0.0)
-        direct = self.restrict(np.where(free, np.asarray(apply_high(fine.reshape(-1))).reshape(self.node_shape),
-                                        0.0)).reshape(-1)
-        if self.coarse_size and float(np.max(np.abs(matrix @ v - direct))) > 1e-9 * max(
-                float(np.max(np.abs(matrix) @ np.abs(v))), 1e-300):
+            matrix[flat[valid], flat[tuple(p[valid] for p in partner)]] = sums[valid]
+        direct = products[-1]
+        if float(np.max(np.abs(matrix @ v - direct))) > 1e-9 * max(float(np.max(np.abs(matrix) @ np.abs(v))), 1e-300):
             raise ValueError("the coloured coarse assembly does not reproduce Z^T A Z: some coupling spans more "
                              "patches than the colouring separates; use preconditioner='jacobi'")
         return matrix

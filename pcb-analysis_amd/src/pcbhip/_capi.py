@@ -40,7 +40,9 @@ EXPORTS = (
     "pcbhip_operator_size",
     "pcbhip_operator_diagonal",
     "pcbhip_operator_set_coarse",
+    "pcbhip_operator_set_coarse_matrix",
     "pcbhip_operator_precondition_f32",
+    "pcbhip_operator_galerkin_f64",
     "pcbhip_apply_f32",
     "pcbhip_apply_f64",
     "pcbhip_stiffness_f64",
@@ -136,7 +138,9 @@ def load() -> ctypes.CDLL:
     lib.pcbhip_operator_size.restype = ctypes.c_int64
     lib.pcbhip_operator_diagonal.argtypes = [_vp, _vp]
     lib.pcbhip_operator_set_coarse.argtypes = [_vp, _int, _int, _vp]
+    lib.pcbhip_operator_set_coarse_matrix.argtypes = [_vp, _int, _int, _vp, _vp]
     lib.pcbhip_operator_precondition_f32.argtypes = [_vp, _vp, _vp]
+    lib.pcbhip_operator_galerkin_f64.argtypes = [_vp, _int, _int, _vp, _vp]
     for name in ("pcbhip_apply_f32", "pcbhip_apply_f64", "pcbhip_stiffness_f64"):
         getattr(lib, name).argtypes = [_vp, _vp, _vp]
     for name in ("pcbhip_apply_f32_dev", "pcbhip_apply_f64_dev", "pcbhip_stiffness_f64_dev"):
@@ -216,6 +220,7 @@ class Operator:
         self.handle = None
         self.device = int(device)
         self.element_shape = (s, r, c)
+        self.node_layers = s + 1
         self.size = n
         self.coarse_size = 0
         h = _vp()
@@ -242,6 +247,7 @@ class Operator:
         self.handle = None
         self.device = int(device)
         self.element_shape = (layers, r, c)
+        self.node_layers = layers
         self.size = n
         self.coarse_size = 0
         h = _vp()
@@ -305,6 +311,33 @@ class Operator:
             raise ValueError("the coarse inverse must be a square matrix")
         check(self._lib.pcbhip_operator_set_coarse(self.handle, int(block), int(inv.shape[0]), _ptr(inv)))
         self.coarse_size = int(inv.shape[0])
+
+    def galerkin_f64(self, block: int, vectors: np.ndarray) -> np.ndarray:
+        """``Z^T A Z v`` for each row ``v`` of ``vectors`` (count x n_c), FP64 on
+        the device, ``Z`` the free-node indicator of ``block``-wide patches."""
+
+        v = np.ascontiguousarray(vectors, dtype=np.float64)
+        _, rows, cols = self.element_shape
+        nc = self.node_layers * -(-(rows + 1) // int(block)) * -(-(cols + 1) // int(block))
+        if v.ndim != 2 or v.shape[1] != nc or v.shape[0] < 1:
+            raise ValueError(f"vectors must be a (count, {nc}) array for block {block}")
+        out = np.empty_like(v)
+        check(self._lib.pcbhip_operator_galerkin_f64(self.handle, int(block), int(v.shape[0]), _ptr(v), _ptr(out)))
+        return out
+
+    def set_coarse_matrix(self, block: int, matrix: np.ndarray) -> np.ndarray:
+        """Two-level preconditioning from now on, from the coarse Galerkin
+        matrix of ``block``-wide patches: factored and inverted on the
+        device.  Returns the FP64 inverse."""
+
+        m = np.ascontiguousarray(matrix, dtype=np.float64)
+        if m.ndim != 2 or m.shape[0] != m.shape[1]:
+            raise ValueError("the coarse matrix must be square")
+        inverse = np.empty_like(m)
+        check(self._lib.pcbhip_operator_set_coarse_matrix(self.handle, int(block), int(m.shape[0]), _ptr(m),
+                                                          _ptr(inverse)))
+        self.coarse_size = int(m.shape[0])
+        return inverse
 
     def precondition_f32(self, r) -> np.ndarray:
         """``M^-1 r`` in FP32 with the operator's preconditioner."""

@@ -105,9 +105,9 @@ class MatrixFreeThermalOperator:
             # inverse resident on the device for every solve of this operator.
             self.coarse_correction = AggregationCoarseCorrection(
                 node_shape=mesh.node_shape, free_nodes=self.free_nodes, diagonal_high=self._diagonal_high,
-                apply_high=self.apply_high, block=coarse_block_nodes,
+                apply_high=self.apply_high, block=coarse_block_nodes, galerkin=self._device.galerkin_f64,
+                invert=self._device.set_coarse_matrix,
             )
-            self._device.set_coarse(self.coarse_correction.block, self.coarse_correction.coarse_inverse)
         self.low_operator_backend = LOW_KERNEL_NAME
         self.high_operator_backend = HIGH_KERNEL_NAME
 

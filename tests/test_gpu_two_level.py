@@ -75,6 +75,44 @@ def test_coarse_matrix_is_the_galerkin_product_and_the_device_action_tracks_fp64
     np.testing.assert_allclose(jacobi.precondition_low(vector), vector / ops["diag"], rtol=2e-7, atol=0.0)
 
 
+@pytest.mark.parametrize("block", [1, 3, 7])
+def test_device_galerkin_products_match_the_oracle(block):
+    print(f"device galerkin block={block}", flush=True)
+    problem = stack(13, 70, fixed=True, slabs=(35e-6, 0.4e-3, 0.4e-3, 0.4e-3, 35e-6))
+    operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    a, ops, _ = _oracle(problem)
+    shape = problem.mesh.node_shape
+    p = T.prolongation(shape, ops["free"], block)
+    vectors = np.random.default_rng(block).standard_normal((3, p.shape[1]))
+    got = operator.device_operator.galerkin_f64(block, vectors)
+    expected = np.stack([p.T @ (a @ (p @ v)) for v in vectors])
+    np.testing.assert_allclose(got, expected, rtol=0.0, atol=1e-12 * np.abs(expected).max())
+    with pytest.raises(ValueError, match="vectors must be"):
+        operator.device_operator.galerkin_f64(block, vectors[:, :-1])
+
+
+def test_device_factorisation_inverts_the_coarse_matrix_and_refuses_an_indefinite_one():
+    print("device coarse factorisation", flush=True)
+    problem = stack(9, 10)
+    operator = MatrixFreeThermalOperator(problem, coarse_block_nodes=2)
+    correction = operator.coarse_correction
+    n = correction.coarse_size
+    np.testing.assert_allclose(correction.coarse_inverse @ correction.coarse_matrix, np.eye(n), atol=1e-9)
+    np.testing.assert_allclose(correction.coarse_inverse, np.linalg.inv(correction.coarse_matrix), rtol=1e-9,
+                               atol=1e-12 * np.abs(correction.coarse_inverse).max())
+    # The host-inverse entry point installs the same preconditioner.
+    twin = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    twin.device_operator.set_coarse(2, correction.coarse_inverse)
+    r = np.random.default_rng(4).standard_normal(operator.size)
+    assert twin.precondition_low(r).tobytes() == operator.precondition_low(r).tobytes()
+    twin.device_operator.set_coarse(0, None)
+    np.testing.assert_array_equal(twin.precondition_low(r), np.float32(r) * (1.0 / twin._diagonal_high).astype(np.float32))
+    with pytest.raises(ValueError, match="positive definite"):
+        operator.device_operator.set_coarse_matrix(2, -correction.coarse_matrix)
+    with pytest.raises(ValueError, match="does not match"):
+        operator.device_operator.set_coarse_matrix(3, correction.coarse_matrix)
+
+
 def test_two_level_preconditioner_cuts_inner_iterations_on_a_copper_stack():
     print("two-level vs jacobi copper stack 24x24", flush=True)
     problem = copper_stack(24, 24)
