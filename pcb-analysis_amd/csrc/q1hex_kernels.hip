@@ -1323,28 +1323,52 @@ __device__ __forceinline__ double coarse_row(const float* __restrict__ cinv, con
     return wave_sum(s);
 }
 
-// The restriction of the two-level slots: the grid loops over the patch
-// table, one patch per workgroup at a time (a fixed grid, so many slots in
-// flight cost loop trips, not dispatches).  which 0: slots that have just
-// entered an inner solve (r_0, written by the outer step); which 1: after a
-// pass, its new q.
+// Z^T v at one patch by one wave: lane l takes the patch's nodes l, l + 64,
+// ... in row-major order (free nodes only), then the butterfly.  Every lane
+// returns the sum.
+template <typename TI>
+__device__ double patch_sum_wave(const OperatorDev& op, const TI* __restrict__ v, const TI* __restrict__ robin,
+                                 const int layer, const int cy, const int cx) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int b = op.cblock;
+    const long long NC = op.C + 1, NR = op.R + 1;
+    const int y_lo = cy * b, x_lo = cx * b;
+    const int h = min(op.R + 1, y_lo + b) - y_lo, w = min(op.C + 1, x_lo + b) - x_lo;
+    const int count = h * w;
+    double s = 0.0;
+#pragma unroll 4
+    for (int k = lane; k < count; k += kWave) {
+        const int yy = k / w, xx = k - yy * w;
+        const long long idx = ((long long)layer * NR + y_lo + yy) * NC + x_lo + xx;
+        const TI value = v[idx];
+        if (robin[idx] >= TI(0)) s += (double)value;
+    }
+    return wave_sum(s);
+}
+
+// The restriction of the two-level slots, one patch per wave: the waves of
+// the grid loop over the patch table on their own (no barrier), so many
+// slots in flight cost loop trips, not dispatches.  which 0: slots that have
+// just entered an inner solve (r_0, written by the outer step); which 1:
+// after a pass, its new q.
 template <typename TI>
 __global__ __launch_bounds__(kGroupThreads) void k_coarse_restrict(const SlotDev* __restrict__ slots,
                                                                    const SlotState* __restrict__ states,
                                                                    const Tile* __restrict__ tiles,
                                                                    const int* __restrict__ counts, const int which) {
     const int count = counts[4];
-    for (int t = blockIdx.x; t < count; t += gridDim.x) {
+    const int waves = (int)gridDim.x * kWavesPerGroup;
+    const int first = __builtin_amdgcn_readfirstlane((int)blockIdx.x * kWavesPerGroup + (int)(threadIdx.x / kWave));
+    for (int t = first; t < count; t += waves) {
         const Tile tile = tiles[t];
         const SlotState& st = states[tile.slot];
         if (st.phase != kPhaseInner || (which == 0 && st.first == 0)) continue;
         const SlotDev& sd = slots[tile.slot];
         const OperatorDev& op = sd.op;
         const TI* v = static_cast<const TI*>(which == 0 ? sd.r[st.pcur] : sd.q[1 - st.pcur]);
-        const double s = patch_sum<TI>(op, v, InnerOf<TI>::robin(op), tile.sz, tile.sy, tile.sx);
-        if (threadIdx.x == 0) sd.cg[((long long)tile.sz * op.crows + tile.sy) * op.ccols + tile.sx] = s;
-        // The next patch reuses the wave sums in LDS.
-        __syncthreads();
+        const double s = patch_sum_wave<TI>(op, v, InnerOf<TI>::robin(op), tile.sz, tile.sy, tile.sx);
+        if ((threadIdx.x & (kWave - 1)) == 0)
+            sd.cg[((long long)tile.sz * op.crows + tile.sy) * op.ccols + tile.sx] = s;
     }
 }
 
@@ -1545,7 +1569,8 @@ template <typename TI>
 void coarse_kernels(const SlotDev* slots, const SlotState* states, const int* counts, const CoarseLaunch& coarse,
                     const int which, hipStream_t stream) {
     if (coarse.grid_patches <= 0) return;
-    const int patches = coarse.grid_patches < kCoarseGroups ? coarse.grid_patches : kCoarseGroups;
+    const int patch_groups = (coarse.grid_patches + kWavesPerGroup - 1) / kWavesPerGroup;
+    const int patches = patch_groups < kCoarseGroups ? patch_groups : kCoarseGroups;
     const int rows = coarse.grid_rows < kCoarseGroups ? coarse.grid_rows : kCoarseGroups;
     hipLaunchKernelGGL(k_coarse_restrict<TI>, dim3(patches), dim3(kGroupThreads), 0, stream, slots, states,
                        coarse.patches, counts, which);
