@@ -165,6 +165,13 @@ int pcbhip_batch_set(pcbhip_batch *batch, int slot, const double *rhs, const dou
  * times candidates through this call.  No reference counterpart (the
  * reference's CPU team splits node lines statically, hex_q1.cpp:195-205). */
 int pcbhip_batch_set_shape(pcbhip_batch *batch, int rows_inner, int target_groups);
+/* Schedule of pcbhip_batch_run: MPIR steps per captured replay (1..1024,
+ * default 32; the host checks for finished slots between replays), inner
+ * iterations between outer steps (default 8) and the share of the tiled
+ * slots, in percent, that must have ended before their tiles leave the
+ * launches (default 50).  Results do not depend on it.  No reference
+ * counterpart. */
+int pcbhip_batch_set_schedule(pcbhip_batch *batch, int replay_steps, int outer_every, int compact_percent);
 /* Tile shape of the standalone applies: rows per wave strip (FP32: 4 or 8,
  * FP64: 2 or 4) and node planes per z block. */
 int pcbhip_operator_set_shape(pcbhip_operator *op, int rows32, int zb32, int rows64, int zb64);

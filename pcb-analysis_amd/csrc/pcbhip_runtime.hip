@@ -272,6 +272,8 @@ struct pcbhip_batch {
     int block_mfma = 3;  // element action of the block apply (0, 2, 3)
     int rows_inner = 4;
     int outer_every = 8;
+    int replay_steps = 32;   // steps per captured replay
+    int compact_percent = 50;  // drop finished slots' tiles once this share of the tiled slots has ended
     int tiled_slots = 0;  // slots whose tiles are in the tables
     long long target_groups = kTargetGroups;
     int* d_active = nullptr;
@@ -1396,6 +1398,21 @@ int pcbhip_batch_set_shape(pcbhip_batch* b, int rows_inner, int target_groups) {
     });
 }
 
+int pcbhip_batch_set_schedule(pcbhip_batch* b, int replay_steps, int outer_every, int compact_percent) {
+    return guarded([&] {
+        if (!b) fail(PCBHIP_INVALID_INPUT, "null batch");
+        if (replay_steps < 1 || replay_steps > 1024) fail(PCBHIP_INVALID_INPUT, "replay_steps must be in 1..1024");
+        if (outer_every < 1 || outer_every > replay_steps)
+            fail(PCBHIP_INVALID_INPUT, "outer_every must be in 1..replay_steps");
+        if (compact_percent < 1 || compact_percent > 100)
+            fail(PCBHIP_INVALID_INPUT, "compact_percent must be in 1..100");
+        b->replay_steps = replay_steps;
+        b->outer_every = outer_every;
+        b->compact_percent = compact_percent;
+        b->graph_steps = 0;  // capture again at the next run
+    });
+}
+
 int pcbhip_batch_set_block(pcbhip_batch* b, int group_size, int mfma) {
     return guarded([&] {
         if (!b) fail(PCBHIP_INVALID_INPUT, "null batch");
@@ -1647,7 +1664,7 @@ int pcbhip_batch_run(pcbhip_batch* b) {
         // Graph length: enough steps that the wait per replay is a small
         // share of the work, few enough that steps launched after the last
         // slot finished cost little.
-        const int steps = 32;
+        const int steps = b->replay_steps;
         if (!b->exec || b->graph_steps != steps) build_graph(b, steps);
         long long done_steps = 0;
         std::vector<SlotState> host_states;
@@ -1658,8 +1675,8 @@ int pcbhip_batch_run(pcbhip_batch* b) {
             const int active = *b->h_active;
             if (active <= 0) break;
             if (done_steps > cap) fail(PCBHIP_RUNTIME, "MPIR iteration did not terminate within its limits");
-            if (count > 1 && active * 2 <= b->tiled_slots) {
-                // Half the tiled slots have finished: drop their tiles.
+            if (count > 1 && (long long)(b->tiled_slots - active) * 100 >= (long long)b->compact_percent * b->tiled_slots) {
+                // Enough of the tiled slots have finished: drop their tiles.
                 host_states.resize((size_t)count);
                 check(hipMemcpyAsync(host_states.data(), b->d_states, sizeof(SlotState) * count,
                                      hipMemcpyDeviceToHost, b->stream),

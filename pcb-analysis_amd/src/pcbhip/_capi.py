@@ -54,6 +54,7 @@ EXPORTS = (
     "pcbhip_batch_destroy",
     "pcbhip_batch_set",
     "pcbhip_batch_set_shape",
+    "pcbhip_batch_set_schedule",
     "pcbhip_operator_set_shape",
     "pcbhip_operator_set_engine",
     "pcbhip_batch_set_block",
@@ -151,6 +152,7 @@ def load() -> ctypes.CDLL:
     lib.pcbhip_batch_set.argtypes = [_vp, _int, _vp, _vp, ctypes.POINTER(MpirConfigC)]
     lib.pcbhip_batch_rhs_dev.argtypes = [_vp, _int, ctypes.POINTER(_vp)]
     lib.pcbhip_batch_set_shape.argtypes = [_vp, _int, _int]
+    lib.pcbhip_batch_set_schedule.argtypes = [_vp, _int, _int, _int]
     lib.pcbhip_operator_set_shape.argtypes = [_vp, _int, _int, _int, _int]
     lib.pcbhip_operator_set_engine.argtypes = [_vp, _int]
     lib.pcbhip_batch_set_block.argtypes = [_vp, _int, _int]
@@ -422,6 +424,13 @@ class Batch:
 
     def set_shape(self, rows_inner: int, target_groups: int) -> None:
         check(self._lib.pcbhip_batch_set_shape(self.handle, int(rows_inner), int(target_groups)))
+
+    def set_schedule(self, replay_steps: int = 32, outer_every: int = 8, compact_percent: int = 50) -> None:
+        """Steps per captured replay, inner iterations between outer steps and
+        the finished share (percent) that triggers compaction."""
+
+        check(self._lib.pcbhip_batch_set_schedule(self.handle, int(replay_steps), int(outer_every),
+                                                   int(compact_percent)))
 
     def set_block(self, group_size: int, engine: int = 3) -> None:
         """Apply slots that share an operator in groups of ``group_size`` with
