@@ -107,7 +107,8 @@ int pcbhip_operator_diagonal(const pcbhip_operator *op, double *diag);
  * the nc x nc row-major inverse of Z^T A Z, nc = layers * ceil((R + 1) /
  * block) * ceil((C + 1) / block) <= 16384, stored in FP32.  nc = 0 removes
  * it.  Batches made and server submissions made afterwards run two-level
- * PCG on this operator; slots made before keep what they were made with. */
+ * PCG on this operator; slots made before keep what they were made with.
+ * Not to be called while another thread uses the same operator. */
 int pcbhip_operator_set_coarse(pcbhip_operator *op, int block, int nc, const double *cinv);
 /* The same from the coarse matrix itself: Z^T A Z (nc x nc FP64, symmetric)
  * is factored and inverted on the device (Cholesky and its inverse, the

@@ -221,6 +221,26 @@ def test_finished_slots_leave_the_batch_without_changing_the_rest():
     assert all(batch.result(i)[1]["converged"] for i in range(len(ops)))
 
 
+def test_batch_schedule_does_not_change_a_single_bit():
+    print("batch schedule independence", flush=True)
+    problems = [random_body((2, 4 + 3 * i, 5 + 7 * i), seed=40 + i) for i in range(5)]
+    cfg = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=30, max_inner_iterations=4000)
+    ops = [MatrixFreeThermalOperator(p, preconditioner="jacobi") for p in problems]
+    rhss = [o.build_rhs(o.default_reference_temperature()) for o in ops]
+    batch = _capi.Batch([o.device_operator for o in ops])
+    for i, b in enumerate(rhss):
+        batch.set(i, b, None, cfg.to_c())
+    batch.run()
+    first = [batch.result(i)[0].tobytes() for i in range(len(ops))]
+    for steps, outer, compact in ((7, 3, 10), (1, 1, 100), (96, 32, 1)):
+        batch.set_schedule(steps, outer, compact)
+        batch.run()
+        assert [batch.result(i)[0].tobytes() for i in range(len(ops))] == first
+    for bad in ((0, 1, 50), (8, 9, 50), (8, 4, 0), (8, 4, 101)):
+        with pytest.raises(ValueError):
+            batch.set_schedule(*bad)
+
+
 def test_a_capped_first_pass_re_references_and_converges():
     print("re-reference restart", flush=True)
     # tests/test_native_thermal_system.py:188-201.
