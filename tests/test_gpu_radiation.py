@@ -44,6 +44,19 @@ def _solve(problem, **kw):
     return solve_thermal_conduction(problem, preconditioner="jacobi", **kw)
 
 
+def test_two_level_newton_lands_on_the_jacobi_answer():
+    print("radiation two-level vs jacobi", flush=True)
+    mesh, heat = _plate(0.6)
+    problem = ThermalConductionProblem(
+        mesh, radiation=(RadiationBoundary("top", 0.8, AMBIENT),), element_heat_w=heat,
+        convection=(ConvectionBoundary("bottom", 5.0, AMBIENT),))
+    jacobi = _solve(problem)
+    two = solve_thermal_conduction(problem)
+    assert two.radiation_converged and jacobi.radiation_converged
+    np.testing.assert_allclose(two.temperature_k, jacobi.temperature_k, rtol=1e-9, atol=1e-7)
+    assert two.heat_balance_error_w == pytest.approx(0.0, abs=1e-9)
+
+
 @pytest.mark.parametrize("power_w, emissivity", [(0.05, 0.9), (0.6, 0.8), (3.0, 0.8)])
 @pytest.mark.parametrize("kind", ["top", "exposed"])
 def test_heated_plate_reaches_the_closed_form_surface_temperature(kind, power_w, emissivity):

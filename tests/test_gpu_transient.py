@@ -90,8 +90,9 @@ def test_heated_plate_marches_to_the_steady_solution_and_stops():
         assert step.stored_heat_w + step.convective_heat_w == pytest.approx(0.4, rel=1e-9)
 
 
-def test_every_step_matches_the_oracle_direct_solve():
-    print("transient: steps vs oracle", flush=True)
+@pytest.mark.parametrize("preconditioner", ["jacobi", "two-level"])
+def test_every_step_matches_the_oracle_direct_solve(preconditioner):
+    print(f"transient: steps vs oracle ({preconditioner})", flush=True)
     rng = np.random.default_rng(31)
     k = np.where(rng.random((3, 10, 14)) < 0.4, 385.0, 0.3)
     mesh = LayeredThermalMesh((0.5e-3,) * 3, 0.25e-3, 0.25e-3, k,
@@ -104,7 +105,7 @@ def test_every_step_matches_the_oracle_direct_solve():
                                        fixed_temperature_mask=fixed, fixed_temperature_k=AMBIENT,
                                        element_heat_w=heat)
     schedule = TimeSchedule((0.0, 0.05, 0.1, 0.3, 0.7, 1.5))
-    run = solve_thermal_transient(problem, schedule, preconditioner="jacobi")
+    run = solve_thermal_transient(problem, schedule, preconditioner=preconditioner)
     oracle = to_oracle(problem)
     ops = P.operator_arrays(oracle)
     capacity = mesh.nodal_heat_capacity_j_per_k().reshape(-1)
