@@ -148,7 +148,22 @@ def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
     )
 
 
-def cpu_baseline(problem, gpu_result, gpu_solve_s: float, *, full_rhs=None, gpu_solution=None) -> dict:
+def _oracle_error(op, threads: int, rhs, gpu_solution) -> dict:
+    """The FP64 oracle's solve of the GPU's right-hand side (FP64 inner PCG,
+    relative residual 1e-12) and the GPU solution's error against it,
+    relative to the largest value of the oracle's rise."""
+
+    t0 = time.perf_counter()
+    x64, res64 = op.solve_mpir(rhs, rtol=1e-12, inner_f64=True, max_outer=60, max_inner=100000, threads=threads)
+    seconds = time.perf_counter() - t0
+    log(f"cpu fp64 oracle: {res64['outer_iterations']} outer / {res64['inner_iterations']} inner, {seconds:.1f} s")
+    scale = float(np.max(np.abs(x64)))
+    return dict(fp64_oracle_rel_error_gpu=float(np.max(np.abs(gpu_solution - x64))) / scale,
+                fp64_oracle_converged=bool(res64["converged"]), fp64_oracle_solve_s=seconds), x64
+
+
+def cpu_baseline(problem, gpu_result, gpu_solve_s: float, *, full_rhs=None, gpu_solution=None,
+                 oracle_rhs=None) -> dict:
     """The reference algorithm (oracle C restatement of hex_q1.cpp's FP32
     Jacobi PCG) on the host cores: a bounded sample of inner iterations on the
     same board, scaled by the GPU solve's own iteration counts.
@@ -157,7 +172,8 @@ def cpu_baseline(problem, gpu_result, gpu_solve_s: float, *, full_rhs=None, gpu_
     the whole reference solve of the GPU's right-hand side, and an FP64
     oracle solve (FP64 inner PCG, relative residual 1e-12) against which it
     states the relative error of the GPU solution and of the reference
-    algorithm's."""
+    algorithm's.  With ``oracle_rhs`` (and ``gpu_solution``) the sample is
+    followed by that FP64 oracle solve and the GPU solution's error."""
 
     from oracle import problem as oracle_problem
     from oracle.native import OracleOperator, threads_default
@@ -185,11 +201,14 @@ def cpu_baseline(problem, gpu_result, gpu_solve_s: float, *, full_rhs=None, gpu_
     inner = gpu_result["inner_iterations"]
     outer = gpu_result["high_operator_applications"]
     solve_s = inner * per_iter + outer * per_f64
-    return dict(value=1.0 / solve_s, unit="solves/s", cores=threads, kind="port",
-                sample=f"{its} less {short} FP32 Jacobi-PCG iterations + 3 FP64 applies of the same board on "
-                       f"{threads} host threads (oracle C restatement of hex_q1.cpp), scaled by the GPU solve's "
-                       f"{inner} inner / {outer} outer steps: {per_iter * 1e3:.2f} ms/iteration, "
-                       f"est. {solve_s:.1f} s/solve")
+    out = dict(value=1.0 / solve_s, unit="solves/s", cores=threads, kind="port",
+               sample=f"{its} less {short} FP32 Jacobi-PCG iterations + 3 FP64 applies of the same board on "
+                      f"{threads} host threads (oracle C restatement of hex_q1.cpp), scaled by the GPU solve's "
+                      f"{inner} inner / {outer} outer steps: {per_iter * 1e3:.2f} ms/iteration, "
+                      f"est. {solve_s:.1f} s/solve")
+    if oracle_rhs is not None:
+        out.update(_oracle_error(op, threads, oracle_rhs, gpu_solution)[0])
+    return out
 
 
 def _cpu_full(op, threads: int, rhs, gpu_solution) -> dict:
@@ -198,15 +217,12 @@ def _cpu_full(op, threads: int, rhs, gpu_solution) -> dict:
     x_port, res = op.solve_mpir(rhs, rtol=1e-10, **limits)
     port_s = time.perf_counter() - t0
     log(f"cpu reference algorithm: {port_s:.1f} s, {res['inner_iterations']} inner iterations")
-    x64, res64 = op.solve_mpir(rhs, rtol=1e-12, inner_f64=True, **limits)
-    log(f"cpu fp64 oracle: {res64['outer_iterations']} outer / {res64['inner_iterations']} inner")
+    errors, x64 = _oracle_error(op, threads, rhs, gpu_solution)
     scale = float(np.max(np.abs(x64)))
     return dict(value=1.0 / port_s, unit="solves/s", cores=threads, kind="port",
                 sample=f"the whole solve on {threads} host threads (oracle C restatement of hex_q1.cpp): "
                        f"{res['outer_iterations']} outer / {res['inner_iterations']} inner, {port_s:.1f} s",
-                fp64_oracle_rel_error_gpu=float(np.max(np.abs(gpu_solution - x64))) / scale,
-                fp64_oracle_rel_error_cpu_reference=float(np.max(np.abs(x_port - x64))) / scale,
-                fp64_oracle_converged=bool(res64["converged"]))
+                fp64_oracle_rel_error_cpu_reference=float(np.max(np.abs(x_port - x64))) / scale, **errors)
 
 
 def run_batch200(args, torch) -> dict:
@@ -720,6 +736,8 @@ def main() -> int:
     parser.add_argument("--size", type=int, default=512)
     parser.add_argument("--slabs", type=int, default=16)
     parser.add_argument("--no-cpu", action="store_true")
+    parser.add_argument("--no-oracle", action="store_true",
+                        help="skip the FP64 oracle solve that states the GPU solution's error")
     parser.add_argument("--cpu-full", action="store_true",
                         help="time the whole reference solve on the host and state the FP64-oracle errors (minutes)")
     parser.add_argument("--rows", type=int, default=4, help="rows per wave strip of the FP32 inner stencil")
@@ -841,10 +859,24 @@ def main() -> int:
         },
         "roofline": fused["roofline"],
     }
+    # The heat budget of the GPU solution (solve.py:255-280), on the
+    # operator's own host and device actions.
+    temperature = x.reshape(problem.mesh.node_shape) + reference
+    residual = operator.unconstrained_residual(temperature, None)
+    fixed_heat = -float(np.sum(residual[operator.fixed_nodes.reshape(-1)]))
+    heat_input = float(np.sum(operator.nodal_load()))
+    convective = float(np.sum(operator.convective_heat(temperature)))
+    line["config"]["heat_input_w"] = heat_input
+    line["config"]["heat_balance_error_w"] = heat_input - convective - fixed_heat
     if not args.no_cpu:
         log("cpu baseline")
-        full = dict(full_rhs=rhs, gpu_solution=x) if args.cpu_full else {}
-        line["cpu_baseline"] = cpu_baseline(problem, result, elapsed / args.steps, **full)
+        if args.cpu_full:
+            extra = dict(full_rhs=rhs, gpu_solution=x)
+        elif args.no_oracle:
+            extra = {}
+        else:
+            extra = dict(oracle_rhs=rhs, gpu_solution=x)
+        line["cpu_baseline"] = cpu_baseline(problem, result, elapsed / args.steps, **extra)
     print(json.dumps(line), flush=True)
     return 0
 
