@@ -825,6 +825,11 @@ def main() -> int:
     log(f"apply: {roof['per_launch_us']:.1f} us, {roof['roofline']['achieved']:.0f} GB/s")
     copy_gbs = copy_bandwidth(torch, stream)
     log(f"copy: {copy_gbs:.0f} GB/s")
+    # The same copy moving only as many bytes as one apply (or one pass) does:
+    # the ramp-up and tail of a ~20 us launch cost what a 1 GiB copy hides.
+    copy_at_apply = copy_bandwidth(torch, stream, roof["roofline"]["algorithmic_bytes"] // 2 // 16 * 16)
+    copy_at_pass = copy_bandwidth(torch, stream, fused["roofline"]["algorithmic_bytes"] // 2 // 16 * 16)
+    log(f"copy at apply bytes: {copy_at_apply:.0f} GB/s, at pass bytes: {copy_at_pass:.0f} GB/s")
     line = {
         "metric": "MPIR solves/s at 1e-10 rel. residual (512x512x16 board)",
         "value": args.steps / elapsed,
@@ -855,6 +860,8 @@ def main() -> int:
             "pass_kernel": fused["kernel"],
             "pass_us": fused["per_launch_us"],
             "stream_copy_gbs": copy_gbs,
+            "stream_copy_gbs_at_apply_bytes": copy_at_apply,
+            "stream_copy_gbs_at_pass_bytes": copy_at_pass,
             "parallelism": "single GPU",
         },
         "roofline": fused["roofline"],

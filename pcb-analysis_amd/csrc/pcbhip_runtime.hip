@@ -459,7 +459,8 @@ void add_coarse_tiles(std::vector<Tile>& patches, std::vector<Tile>& rows, int s
 
 // Tiles of one slot: stencil workgroups in (y group, x strip, z block)
 // order, so a contiguous range of tiles is a compact piece of the board and
-// the z blocks of one column, which share a node plane, run side by side.
+// the z blocks of one column, which share two node planes and sweep towards
+// or away from each other, run side by side.
 int add_tiles(std::vector<Tile>& out, int slot, int nsx, int nsy, int S, int zb) {
     const int nsz = (S + 1 + zb - 1) / zb;
     int index = 0;
@@ -551,9 +552,9 @@ void collect_tiles(const std::vector<const SlotHost*>& slots, const std::vector<
         const int zb = planes_per_block(op->S, (long long)nsx * ny, std::max(share, (long long)nsx * ny));
         const int nsz = (op->S + 1 + zb - 1) / zb;
         int index = 0;
-        for (int sz = 0; sz < nsz; ++sz)
-            for (int sy = 0; sy < ny; ++sy)
-                for (int sx = 0; sx < nsx; ++sx, ++index)
+        for (int sy = 0; sy < ny; ++sy)
+            for (int sx = 0; sx < nsx; ++sx)
+                for (int sz = 0; sz < nsz; ++sz, ++index)
                     for (int g : same) block.push_back(Tile{g, index, (short)sx, (short)sy, (short)sz, (short)zb});
     }
 }

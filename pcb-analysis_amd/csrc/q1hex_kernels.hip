@@ -234,11 +234,20 @@ __device__ __forceinline__ void element_row_mfma(const bf16x8 unit, const unsign
 // (``base`` the wave-uniform start of the node plane, ``off`` the lane's
 // offset in it, kOutside outside the grid, where loads return zero); it is
 // called for the ROWS + 2 rows of every plane.  ``owned`` marks the nodes
-// this lane writes.  ``emit(base, off, row, conduction, value)`` receives,
-// for every owned node (``row`` its row in the strip), the conduction sum
-// K u restricted to free columns (no Robin term); ``plane_done(z)`` follows
-// every output plane, in uniform control flow.
-template <typename T, int ROWS, int MFMA = 0, class Load, class Emit, class PlaneDone>
+// this lane writes.  ``emit(base, off, row, conduction, value, parity)``
+// receives, for every owned node (``row`` its row in the strip, ``parity``
+// the plane's z & 1 as a std::integral_constant), the conduction sum K u
+// restricted to free columns (no Robin term); ``plane_done(z)`` follows every
+// output plane, in uniform control flow.
+//
+// DOWN sweeps the z block from its top plane down.  Neighbouring z blocks of
+// a column run side by side in opposite directions, so the planes they share
+// are read by both at the same time and the second read is served by the
+// XCD's L2.  Each output node's sum is the same in both directions: the
+// element layer below and the element layer above are each summed from zero
+// (their element rows in order, the corner terms by fused multiply-adds) and
+// the two partials then added, a missing layer contributing +0.
+template <typename T, int ROWS, int MFMA = 0, bool DOWN = false, class Load, class Emit, class PlaneDone>
 __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const int S, const int R, const int C,
                                             const int x0, const int y0, const int zlo, const int zhi, Load&& load,
                                             Emit&& emit, PlaneDone&& plane_done) {
@@ -267,16 +276,19 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
 
     // The node-value type is whatever ``load`` returns (NodeVal, InnerVal, InnerValZ).
     using V = std::decay_t<decltype(call_load(load, 0LL, 0u, false, 0, 0))>;
-    T mlo[ROWS + 2], nlo[ROWS + 2];
-    V vlo[ROWS];
-    T Llo[ROWS], Rlo[ROWS];
+    T mlo[ROWS + 2], nlo[ROWS + 2], mhi[ROWS + 2], nhi[ROWS + 2];
+    V vlo[ROWS], vhi[ROWS];
+    // The partial of the layer swept last that belongs to the next output
+    // plane: the layer below it going up, the layer above it going down.
+    T heldL[ROWS], heldR[ROWS];
     [[maybe_unused]] bf16x8 unit;
-    [[maybe_unused]] unsigned qlo[ROWS + 2][4];
+    [[maybe_unused]] unsigned qlo[ROWS + 2][4], qhi[ROWS + 2][4];
     if constexpr (MFMA != 0) unit = unit_operand();
 
     // Planes zlo - 1 and zhi are loaded for the elements between them and the
     // block's output planes but belong to neighbouring z blocks.
-    auto load_plane = [&](const int z, T (&m)[ROWS + 2], T (&n)[ROWS + 2], V (&v)[ROWS]) {
+    auto load_plane = [&](const int z, T (&m)[ROWS + 2], T (&n)[ROWS + 2], V (&v)[ROWS],
+                          unsigned (&q)[ROWS + 2][4]) {
         const long long pb = (long long)z * NR * NC;
         const bool out_plane = z >= zlo && z < zhi;
 #pragma unroll
@@ -288,75 +300,70 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
         }
 #pragma unroll
         for (int i = 0; i < ROWS + 2; ++i) n[i] = __shfl_down(m[i], 1);
+        if constexpr (MFMA != 0) {
+#pragma unroll
+            for (int i = 0; i < ROWS + 2; ++i) plane_operands<MFMA>(m[i], n[i], q[i]);
+        }
     };
 
-    auto finish_plane = [&](const int z, const T (&L)[ROWS], const T (&Rt)[ROWS], const V (&v)[ROWS]) {
+    // Output plane z: its two layer partials (L for the corners right of the
+    // node's element column, R for those left of it, from the lane below).
+    auto finish_plane = [&](const int z, const T (&La)[ROWS], const T (&Ra)[ROWS], const T (&Lb)[ROWS],
+                            const T (&Rb)[ROWS], const V (&v)[ROWS]) {
         const long long pb = (long long)z * NR * NC;
+        auto rows = [&](auto parity) {
 #pragma unroll
-        for (int i = 0; i < ROWS; ++i) {
-            const T total = L[i] + __shfl_up(Rt[i], 1);
-            if (out_lane && y0 + i <= R) emit(pb, noff[i + 1], i, total, v[i]);
-        }
+            for (int i = 0; i < ROWS; ++i) {
+                const T L = La[i] + Lb[i];
+                const T Rt = Ra[i] + Rb[i];
+                const T total = L + __shfl_up(Rt, 1);
+                if (out_lane && y0 + i <= R) emit(pb, noff[i + 1], i, total, v[i], parity);
+            }
+        };
+        if (z & 1)
+            rows(std::integral_constant<int, 1>{});
+        else
+            rows(std::integral_constant<int, 0>{});
         plane_done(z);
     };
 
-    // Output planes [zlo, zhi) need element layers zlo - 1 .. zhi - 1 (those
-    // that exist); the first layer only completes a plane that is not output.
-    const int first_layer = zlo > 0 ? zlo - 1 : 0;
-    const int last_layer = zhi - 1 < S - 1 ? zhi - 1 : S - 1;
-    load_plane(first_layer, mlo, nlo, vlo);
-    if constexpr (MFMA != 0) {
-#pragma unroll
-        for (int i = 0; i < ROWS + 2; ++i) plane_operands<MFMA>(mlo[i], nlo[i], qlo[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < ROWS; ++i) {
-        Llo[i] = T(0);
-        Rlo[i] = T(0);
-    }
-
-    for (int ez = first_layer; ez <= last_layer; ++ez) {
-        T mhi[ROWS + 2], nhi[ROWS + 2];
-        V vhi[ROWS];
-        load_plane(ez + 1, mhi, nhi, vhi);
-
+    // Element layer ez between the lo (ez) and hi (ez + 1) planes in
+    // registers: its part of the lo plane (aL, aR) and of the hi plane
+    // (bL, bR), each summed from zero.
+    auto layer = [&](const int ez, T (&aL)[ROWS], T (&aR)[ROWS], T (&bL)[ROWS], T (&bR)[ROWS]) {
         T ca[ROWS + 1], cb[ROWS + 1], cc[ROWS + 1];
-        const T* layer = coef + (long long)ez * R * C;
-        const Rsrc ra = rsrc(layer), rb = rsrc(layer + E), rc = rsrc(layer + 2 * E);
+        const T* layer_coef = coef + (long long)ez * R * C;
+        const Rsrc ra = rsrc(layer_coef), rb = rsrc(layer_coef + E), rc = rsrc(layer_coef + 2 * E);
 #pragma unroll
         for (int j = 0; j < ROWS + 1; ++j) {
             ca[j] = ld(ra, eoff[j], T());
             cb[j] = ld(rb, eoff[j], T());
             cc[j] = ld(rc, eoff[j], T());
         }
-
-        T Lhi[ROWS], Rhi[ROWS];
 #pragma unroll
         for (int i = 0; i < ROWS; ++i) {
-            Lhi[i] = T(0);
-            Rhi[i] = T(0);
+            aL[i] = T(0);
+            aR[i] = T(0);
+            bL[i] = T(0);
+            bR[i] = T(0);
         }
-
-        [[maybe_unused]] unsigned qhi[ROWS + 2][4];
         if constexpr (MFMA != 0) {
             static_assert(sizeof(T) == 4 && (MFMA == 2 || MFMA == 3), "the matrix-core action is FP32, 2 or 3 terms");
-#pragma unroll
-            for (int i = 0; i < ROWS + 2; ++i) plane_operands<MFMA>(mhi[i], nhi[i], qhi[i]);
 #pragma unroll
             for (int j = 0; j < ROWS + 1; ++j) {
                 float lower[4], upper[4];
                 element_row_mfma<MFMA>(unit, qlo[j], qlo[j + 1], qhi[j], qhi[j + 1], ca[j], cb[j], cc[j], lower, upper);
                 if (j >= 1) {
-                    Llo[j - 1] += lower[0];
-                    Rlo[j - 1] += lower[1];
-                    Lhi[j - 1] += upper[0];
-                    Rhi[j - 1] += upper[1];
+                    aL[j - 1] += lower[0];
+                    aR[j - 1] += lower[1];
+                    bL[j - 1] += upper[0];
+                    bR[j - 1] += upper[1];
                 }
                 if (j < ROWS) {
-                    Llo[j] += lower[2];
-                    Rlo[j] += lower[3];
-                    Lhi[j] += upper[2];
-                    Rhi[j] += upper[3];
+                    aL[j] += lower[2];
+                    aR[j] += lower[3];
+                    bL[j] += upper[2];
+                    bR[j] += upper[3];
                 }
             }
         } else {
@@ -410,39 +417,85 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
                 // Corner (z, y, x) receives s(x) a tx[z][y] + s(y) b ty[z][x] + s(z) c tz[y][x],
                 // s(0) = -1, s(1) = +1.  Row j is output row j - 1, row j + 1 is output row j.
                 if (j >= 1) {
-                    Llo[j - 1] = fmadd(-a, tx00, fmadd(-b, ty00, fmadd(-c, tz00, Llo[j - 1])));
-                    Rlo[j - 1] = fmadd(a, tx00, fmadd(-b, ty01, fmadd(-c, tz01, Rlo[j - 1])));
-                    Lhi[j - 1] = fmadd(-a, tx10, fmadd(-b, ty10, fmadd(c, tz00, Lhi[j - 1])));
-                    Rhi[j - 1] = fmadd(a, tx10, fmadd(-b, ty11, fmadd(c, tz01, Rhi[j - 1])));
+                    aL[j - 1] = fmadd(-a, tx00, fmadd(-b, ty00, fmadd(-c, tz00, aL[j - 1])));
+                    aR[j - 1] = fmadd(a, tx00, fmadd(-b, ty01, fmadd(-c, tz01, aR[j - 1])));
+                    bL[j - 1] = fmadd(-a, tx10, fmadd(-b, ty10, fmadd(c, tz00, bL[j - 1])));
+                    bR[j - 1] = fmadd(a, tx10, fmadd(-b, ty11, fmadd(c, tz01, bR[j - 1])));
                 }
                 if (j < ROWS) {
-                    Llo[j] = fmadd(-a, tx01, fmadd(b, ty00, fmadd(-c, tz10, Llo[j])));
-                    Rlo[j] = fmadd(a, tx01, fmadd(b, ty01, fmadd(-c, tz11, Rlo[j])));
-                    Lhi[j] = fmadd(-a, tx11, fmadd(b, ty10, fmadd(c, tz10, Lhi[j])));
-                    Rhi[j] = fmadd(a, tx11, fmadd(b, ty11, fmadd(c, tz11, Rhi[j])));
+                    aL[j] = fmadd(-a, tx01, fmadd(b, ty00, fmadd(-c, tz10, aL[j])));
+                    aR[j] = fmadd(a, tx01, fmadd(b, ty01, fmadd(-c, tz11, aR[j])));
+                    bL[j] = fmadd(-a, tx11, fmadd(b, ty10, fmadd(c, tz10, bL[j])));
+                    bR[j] = fmadd(a, tx11, fmadd(b, ty11, fmadd(c, tz11, bR[j])));
                 }
             }
         }
+    };
 
-        if (ez >= zlo) finish_plane(ez, Llo, Rlo, vlo);
-
+    // Output planes [zlo, zhi) need element layers zlo - 1 .. zhi - 1 (those
+    // that exist); the layer outside the block only completes a plane.
+    const int first_layer = zlo > 0 ? zlo - 1 : 0;
+    const int last_layer = zhi - 1 < S - 1 ? zhi - 1 : S - 1;
 #pragma unroll
-        for (int i = 0; i < ROWS + 2; ++i) {
-            mlo[i] = mhi[i];
-            nlo[i] = nhi[i];
-            if constexpr (MFMA != 0) {
+    for (int i = 0; i < ROWS; ++i) {
+        heldL[i] = T(0);
+        heldR[i] = T(0);
+    }
+    if constexpr (!DOWN) {
+        load_plane(first_layer, mlo, nlo, vlo, qlo);
+        for (int ez = first_layer; ez <= last_layer; ++ez) {
+            load_plane(ez + 1, mhi, nhi, vhi, qhi);
+            T aL[ROWS], aR[ROWS], bL[ROWS], bR[ROWS];
+            layer(ez, aL, aR, bL, bR);
+            if (ez >= zlo) finish_plane(ez, heldL, heldR, aL, aR, vlo);
 #pragma unroll
-                for (int w = 0; w < 4; ++w) qlo[i][w] = qhi[i][w];
+            for (int i = 0; i < ROWS + 2; ++i) {
+                mlo[i] = mhi[i];
+                nlo[i] = nhi[i];
+                if constexpr (MFMA != 0) {
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) qlo[i][w] = qhi[i][w];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < ROWS; ++i) {
+                vlo[i] = vhi[i];
+                heldL[i] = bL[i];
+                heldR[i] = bR[i];
             }
         }
+        if (zhi == S + 1) {
+            const T none[ROWS] = {};
+            finish_plane(S, heldL, heldR, none, none, vlo);
+        }
+    } else {
+        load_plane(last_layer + 1, mhi, nhi, vhi, qhi);
+        for (int ez = last_layer; ez >= first_layer; --ez) {
+            load_plane(ez, mlo, nlo, vlo, qlo);
+            T aL[ROWS], aR[ROWS], bL[ROWS], bR[ROWS];
+            layer(ez, aL, aR, bL, bR);
+            if (ez + 1 < zhi) finish_plane(ez + 1, bL, bR, heldL, heldR, vhi);
 #pragma unroll
-        for (int i = 0; i < ROWS; ++i) {
-            vlo[i] = vhi[i];
-            Llo[i] = Lhi[i];
-            Rlo[i] = Rhi[i];
+            for (int i = 0; i < ROWS + 2; ++i) {
+                mhi[i] = mlo[i];
+                nhi[i] = nlo[i];
+                if constexpr (MFMA != 0) {
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) qhi[i][w] = qlo[i][w];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < ROWS; ++i) {
+                vhi[i] = vlo[i];
+                heldL[i] = aL[i];
+                heldR[i] = aR[i];
+            }
+        }
+        if (zlo == 0) {
+            const T none[ROWS] = {};
+            finish_plane(0, none, none, heldL, heldR, vhi);
         }
     }
-    if (zhi == S + 1) finish_plane(S, Llo, Rlo, vlo);
 }
 
 // The layered-sheet DC action over one wave strip of one copper layer
@@ -530,7 +583,7 @@ __device__ __forceinline__ void sheet_apply(const T* __restrict__ coef, const in
                     total = fmadd(via_g[k], own - masked(far), total);
                 }
             }
-            emit(pb, noff[i + 1], i, total, v[i]);
+            emit(pb, noff[i + 1], i, total, v[i], std::integral_constant<int, 0>{});
         }
     }
 }
@@ -545,6 +598,23 @@ __device__ __forceinline__ int xcd_tile(const int block, const int count) {
     return k * q + (k < r ? k : r) + j;
 }
 
+// Sweep direction of z block sz of a column of nsz blocks: the even blocks
+// go down and the odd ones up, so every pair of neighbouring blocks meets at
+// the planes they share at the same time (both at their start or both at
+// their end).  A column of one block goes up.
+__device__ __forceinline__ bool sweep_down(const int sz, const int nsz) { return nsz > 1 && (sz & 1) == 0; }
+
+// strip_apply in the direction of z block sz of nsz.
+template <typename T, int ROWS, int MFMA = 0, class Load, class Emit, class PlaneDone>
+__device__ __forceinline__ void strip_sweep(const int sz, const int nsz, const T* __restrict__ coef, const int S,
+                                            const int R, const int C, const int x0, const int y0, const int zlo,
+                                            const int zhi, Load&& load, Emit&& emit, PlaneDone&& plane_done) {
+    if (sweep_down(sz, nsz))
+        strip_apply<T, ROWS, MFMA, true>(coef, S, R, C, x0, y0, zlo, zhi, load, emit, plane_done);
+    else
+        strip_apply<T, ROWS, MFMA, false>(coef, S, R, C, x0, y0, zlo, zhi, load, emit, plane_done);
+}
+
 // ------------------------------------------------------------ reductions
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -554,10 +624,12 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Reduction cells of the stencil kernels.  Every inner product is summed per
-// cell of (node-plane pair, 16 node rows, one 62-column strip): each wave sums
-// its 4-row groups (lanes in a fixed order, then a wave butterfly) into a
-// table in LDS, and at the end of the tile the workgroup adds the four groups
-// of each cell in row order and stores one partial per cell.  The cells do
+// cell of (node-plane pair, 16 node rows, one 62-column strip): each lane sums
+// the even and the odd plane of the pair apart (in row order) and adds the
+// two, so the z direction of the sweep does not matter; each wave sums its
+// 4-row groups (then a wave butterfly) into a table in LDS, and at the end of
+// the tile the workgroup adds the four groups of each cell in row order and
+// stores one partial per cell.  The cells do
 // not depend on the tile shape (rows per wave, planes per z block) or on
 // which other slots share the launch, so a slot's bits depend only on its own
 // data.  Partials become visible to the finish kernel at the launch
@@ -566,7 +638,8 @@ template <int NV, int ROWS>
 struct CellSums {
     static constexpr int kGroups = ROWS / kCellRows;  // 4-row groups per wave
     static constexpr int kEntries = kWavesPerGroup * kGroups;
-    double v[NV][kGroups];  // per-lane accumulators
+    double v[NV][kGroups][2];  // per-lane accumulators, by plane parity
+    int planes;                // planes of the current pair emitted so far
 
     __device__ static double (&table())[kMaxTilePairs][NV][kEntries] {
         __shared__ double sums[kMaxTilePairs][NV][kEntries];
@@ -576,7 +649,11 @@ struct CellSums {
 #pragma unroll
         for (int k = 0; k < NV; ++k)
 #pragma unroll
-            for (int g = 0; g < kGroups; ++g) v[k][g] = 0.0;
+            for (int g = 0; g < kGroups; ++g) {
+                v[k][g][0] = 0.0;
+                v[k][g][1] = 0.0;
+            }
+        planes = 0;
     }
     __device__ void store_groups(const int pair, const int groups) {
         const int wave = threadIdx.x / kWave;
@@ -585,16 +662,17 @@ struct CellSums {
             if (g >= groups) break;
 #pragma unroll
             for (int k = 0; k < NV; ++k) {
-                const double s = wave_sum(v[k][g]);
+                const double s = wave_sum(v[k][g][0] + v[k][g][1]);
                 if ((threadIdx.x & (kWave - 1)) == 0) table()[pair][k][wave * kGroups + g] = s;
             }
         }
         clear();
     }
-    // After output plane z of a tile whose z block starts at plane zlo (even):
-    // a plane pair (or the last plane) is complete.
+    // After output plane z of a tile whose z block starts at plane zlo (even),
+    // in either direction: the pair is complete after its second plane (the
+    // last plane S alone when S is even).
     __device__ void plane_done(const int z, const int zlo, const int S) {
-        if ((z & 1) || z == S) store_groups(z / kCellPlanes - zlo / kCellPlanes, kGroups);
+        if (++planes == ((z | 1) <= S ? 2 : 1)) store_groups(z / kCellPlanes - zlo / kCellPlanes, kGroups);
     }
     // Sheet kernels: the wave's one 4-row group of its layer.
     __device__ void sheet_done() { store_groups(0, 1); }
@@ -686,22 +764,25 @@ __global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ c
                                                           const int R, const int C, const int nsx, const int nsy,
                                                           const int zb) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    // Tiles in (y group, x strip, z block) order: the z blocks of a column
+    // are dealt to one XCD side by side.
     const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int sx = t % nsx;
-    const int sy = (t / nsx) % nsy;
-    const int sz = t / (nsx * nsy);
+    const int nsz = (S + zb) / zb;
+    const int sz = t % nsz;
+    const int sx = (t / nsz) % nsx;
+    const int sy = t / (nsz * nsx);
     const int x0 = sx * kStripCols;
     const int y0 = (sy * kWavesPerGroup + wave) * ROWS;
     const int zlo = sz * zb;
     const int zhi = zlo + zb < S + 1 ? zlo + zb : S + 1;
     if (y0 > R) return;
-    strip_apply<T, ROWS, MFMA>(
-        coef, S, R, C, x0, y0, zlo, zhi,
+    strip_sweep<T, ROWS, MFMA>(
+        sz, nsz, coef, S, R, C, x0, y0, zlo, zhi,
         [&](const long long pb, const unsigned off, const bool) -> NodeVal<T> {
             if (MODE == 1) return {ldt(x + pb, off), T(0)};
             return {ldt(x + pb, off), ldt(robin + pb, off)};
         },
-        [&](const long long pb, const unsigned off, const int, const T conduction, const NodeVal<T>& v) {
+        [&](const long long pb, const unsigned off, const int, const T conduction, const NodeVal<T>& v, auto) {
             stt(y + pb, off, v.robin >= T(0) ? conduction + v.robin * v.u : v.u);
         },
         [](const int) {});
@@ -729,7 +810,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_sheet_apply(const OperatorDev
             if (MODE == 1) return {ldt(x + pb, off), T(0)};
             return {ldt(x + pb, off), ldt(robin + pb, off)};
         },
-        [&](const long long pb, const unsigned off, const int, const T conduction, const NodeVal<T>& v) {
+        [&](const long long pb, const unsigned off, const int, const T conduction, const NodeVal<T>& v, auto) {
             stt(y + pb, off, v.robin >= T(0) ? conduction : v.u);
         });
 }
@@ -880,7 +961,8 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
             return {xv, ldt(robin + pb, off)};
         };
         auto emit = [&](const long long pb, const unsigned off, const int row, const double conduction,
-                        const NodeVal<double>& v) {
+                        const NodeVal<double>& v, auto parity) {
+            constexpr int P = decltype(parity)::value;
             if (pending != kPendingNone) stt(xout + pb, off, v.u);
             const double ax = v.robin >= 0.0 ? conduction + v.robin * v.u : v.u;
             const double bv = ldt(b + pb, off);
@@ -888,10 +970,10 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
             const TI ri = (TI)r;
             stt(rout + pb, off, ri);
             const int g = row / kCellRows;
-            acc.v[0][g] += bv * bv;
-            acc.v[1][g] += r * r;
-            acc.v[2][g] += (double)ri * (double)ri;
-            acc.v[3][g] += (double)ri * (double)(ri * ldt(dinv + pb, off));
+            acc.v[0][g][P] += bv * bv;
+            acc.v[1][g][P] += r * r;
+            acc.v[2][g][P] += (double)ri * (double)ri;
+            acc.v[3][g][P] += (double)ri * (double)(ri * ldt(dinv + pb, off));
         };
         const int y_group = tile.sy * kWavesPerGroup * kRowsOuter;
         if (op.kind == kSheet) {
@@ -905,7 +987,8 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
         }
         if (y0 <= op.R) {
             // Stores wait for their plane's emit, so a plane's loads issue back to back.
-            strip_apply<double, kRowsOuter>(op.coef64, op.S, op.R, op.C, x0, y0, zlo, zhi, load, emit,
+            strip_sweep<double, kRowsOuter>(tile.sz, (op.S + tile.zb) / tile.zb, op.coef64, op.S, op.R, op.C, x0,
+                                            y0, zlo, zhi, load, emit,
                                             [&](const int z) { acc.plane_done(z, zlo, op.S); });
         }
         acc.flush_hex(zlo, zhi, y_group, tile.sx, op.R, op.C, sd.part, sd.part_stride);
@@ -1064,7 +1147,9 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
         }
     };
     // Stores wait for their plane's emit, so a plane's loads issue back to back.
-    auto emit = [&](const long long pb, const unsigned off, const int row, const TI conduction, const auto& v) {
+    auto emit = [&](const long long pb, const unsigned off, const int row, const TI conduction, const auto& v,
+                    auto parity) {
+        constexpr int P = decltype(parity)::value;
         const TI ap = v.robin >= TI(0) ? conduction + v.robin * v.u : v.u;
         stt(pout + pb, off, v.u);
         stt(qout + pb, off, ap);
@@ -1076,11 +1161,11 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
         else
             z = v.r * v.d;
         const int g = row / kCellRows;
-        acc.v[0][g] += (double)v.u * (double)ap;
-        acc.v[1][g] += (double)v.r * (double)z;
-        acc.v[2][g] += (double)ap * (double)z;
-        acc.v[3][g] += (double)ap * (double)(ap * v.d);
-        acc.v[4][g] += (double)v.r * (double)v.r;
+        acc.v[0][g][P] += (double)v.u * (double)ap;
+        acc.v[1][g][P] += (double)v.r * (double)z;
+        acc.v[2][g][P] += (double)ap * (double)z;
+        acc.v[3][g][P] += (double)ap * (double)(ap * v.d);
+        acc.v[4][g][P] += (double)v.r * (double)v.r;
     };
     if (op.kind == kSheet) {
         // Sheet tiles are one layer of 4-row wave strips.
@@ -1094,8 +1179,8 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
         return;
     }
     if (y0 <= op.R) {
-        strip_apply<TI, ROWS, MFMA>(InnerOf<TI>::coef(op), op.S, op.R, op.C, x0, y0, zlo, zhi, load, emit,
-                                    [&](const int z) { acc.plane_done(z, zlo, op.S); });
+        strip_sweep<TI, ROWS, MFMA>(tile.sz, (op.S + tile.zb) / tile.zb, InnerOf<TI>::coef(op), op.S, op.R, op.C,
+                                    x0, y0, zlo, zhi, load, emit, [&](const int z) { acc.plane_done(z, zlo, op.S); });
     }
     acc.flush_hex(zlo, zhi, tile.sy * kWavesPerGroup * ROWS, tile.sx, op.R, op.C, sd.part, sd.part_stride);
 }
