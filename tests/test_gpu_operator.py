@@ -126,3 +126,26 @@ def test_matrix_core_action_matches_the_assembled_operator(shape, void, terms):
     for rows, zb in ((8, 2), (4, 2), (8, 1 << 20)):
         dev.set_shape(rows, zb)
         assert dev.apply_f32(x32).tobytes() == y.tobytes(), (rows, zb)
+
+
+def test_vector_action_bits_do_not_depend_on_the_tile_shape():
+    print("vector apply tile-shape independence", flush=True)
+    # Neighbouring z blocks sweep in opposite directions, and every node adds
+    # the partial of the element layer below it to that of the layer above in
+    # a fixed order.  So z blocks of 2 planes (both directions), of 4 and 6,
+    # and one block going up give the same bits, in FP32 and FP64, and the
+    # answer is the assembled operator's.
+    problem = random_body((9, 21, 70), seed=31, void=True)
+    dev, ops, a, _ = _device_and_matrix(problem)
+    x = np.random.default_rng(9).standard_normal(problem.mesh.size)
+    x32 = x.astype(np.float32)
+    dev.set_shape(4, 1 << 20, 4, 1 << 20)
+    y32 = dev.apply_f32(x32)
+    y64 = dev.apply_f64(x)
+    scale = float((abs(a) @ np.abs(x)).max())
+    np.testing.assert_allclose(y64, a @ x, rtol=0, atol=64 * DOUBLE_EPS * scale)
+    assert np.max(np.abs(y32 - a @ x32.astype(np.float64))) <= 32 * SINGLE_EPS * scale
+    for rows32, zb, rows64 in ((4, 2, 2), (8, 2, 4), (4, 4, 2), (8, 6, 4), (8, 1 << 20, 2)):
+        dev.set_shape(rows32, zb, rows64, zb)
+        assert dev.apply_f32(x32).tobytes() == y32.tobytes(), (rows32, zb)
+        assert dev.apply_f64(x).tobytes() == y64.tobytes(), (rows64, zb)
