@@ -128,14 +128,17 @@ def test_matrix_core_action_matches_the_assembled_operator(shape, void, terms):
         assert dev.apply_f32(x32).tobytes() == y.tobytes(), (rows, zb)
 
 
-def test_vector_action_bits_do_not_depend_on_the_tile_shape():
-    print("vector apply tile-shape independence", flush=True)
+@pytest.mark.parametrize("shape", [(9, 21, 70), (4, 5, 70), (2, 12, 20), (1, 3, 7)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_vector_action_bits_do_not_depend_on_the_tile_shape(shape):
+    print(f"vector apply tile-shape independence shape={shape}", flush=True)
     # Neighbouring z blocks sweep in opposite directions, and every node adds
     # the partial of the element layer below it to that of the layer above in
-    # a fixed order.  So z blocks of 2 planes (both directions), of 4 and 6,
-    # and one block going up give the same bits, in FP32 and FP64, and the
-    # answer is the assembled operator's.
-    problem = random_body((9, 21, 70), seed=31, void=True)
+    # a fixed order.  So z blocks of 2 planes (both directions; with 4 slabs
+    # the last block is the top plane alone, swept down), of 4 and 6, and one
+    # block going up give the same bits, in FP32 and FP64, and the answer is
+    # the assembled operator's.
+    problem = random_body(shape, seed=31 + sum(shape), void=True)
     dev, ops, a, _ = _device_and_matrix(problem)
     x = np.random.default_rng(9).standard_normal(problem.mesh.size)
     x32 = x.astype(np.float32)
