@@ -156,6 +156,22 @@ int planes_per_block(int S, long long groups_xy, long long target) {
     return zb < 2 ? 2 : (zb > planes ? planes : zb);
 }
 
+// Bit z set for each of the first 64 node planes (sheet: layers) of
+// ``per_plane`` nodes whose robin words are all exactly +0: every node free,
+// no Robin term.  The stencils skip the robin load there.
+unsigned long long plain_planes(const std::vector<double>& robin64, long long planes, long long per_plane) {
+    unsigned long long mask = 0;
+    for (long long z = 0; z < planes && z < 64; ++z) {
+        bool plain = true;
+        for (long long i = 0; i < per_plane && plain; ++i) {
+            const double v = robin64[(size_t)(z * per_plane + i)];
+            plain = v == 0.0 && !std::signbit(v);
+        }
+        if (plain) mask |= 1ull << z;
+    }
+    return mask;
+}
+
 }  // namespace
 
 struct pcbhip_operator {
@@ -168,6 +184,7 @@ struct pcbhip_operator {
     double* coef64 = nullptr;
     float* robin32 = nullptr;
     double* robin64 = nullptr;
+    unsigned long long plain = 0;  // bit z: robin is +0 throughout node plane (sheet: layer) z
     float* dinv32 = nullptr;
     double* dinv64 = nullptr;
     double* diag64 = nullptr;
@@ -201,6 +218,7 @@ struct pcbhip_operator {
         v.coef64 = coef64;
         v.robin32 = robin32;
         v.robin64 = robin64;
+        v.plain = plain;
         v.dinv32 = dinv32;
         v.dinv64 = dinv64;
         v.kind = kind;
@@ -348,7 +366,7 @@ void apply_dev(const pcbhip_operator* op, const T* x, T* y, int mode, hipStream_
     }
     const int rows = sizeof(T) == 4 ? op->rows32 : op->rows64;
     const int zb = sizeof(T) == 4 ? op->zb32 : op->zb64;
-    check(launch::apply<T>(coef, robin, x, y, op->S, op->R, op->C, mode, rows, zb, op->mfma32, stream),
+    check(launch::apply<T>(coef, robin, op->plain, x, y, op->S, op->R, op->C, mode, rows, zb, op->mfma32, stream),
           "apply launch");
 }
 
@@ -1096,6 +1114,7 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
         check(stream_create(&op->stream), "stream create");
         op->zb32 = planes_per_block(slabs, (long long)strips_x(cols) * groups_y(rows, op->rows32), kTargetGroups);
         op->zb64 = planes_per_block(slabs, (long long)strips_x(cols) * groups_y(rows, op->rows64), kTargetGroups);
+        op->plain = plain_planes(robin64, slabs + 1, (long long)(rows + 1) * (cols + 1));
         const size_t s_c32 = align256(sizeof(float) * 3 * E), s_c64 = align256(sizeof(double) * 3 * E);
         const size_t s_n32 = align256(sizeof(float) * N), s_n64 = align256(sizeof(double) * N);
         char* base = static_cast<char*>(stream_alloc(op->stream, s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + 256));
@@ -1198,6 +1217,7 @@ int pcbhip_sheet_operator_create(int device, int layers, int rows, int cols, con
         op->N = N;
         op->E = E;
         check(stream_create(&op->stream), "stream create");
+        op->plain = plain_planes(robin64, layers, (long long)(rows + 1) * (cols + 1));
         const size_t s_c32 = align256(sizeof(float) * 2 * E), s_c64 = align256(sizeof(double) * 2 * E);
         const size_t s_n32 = align256(sizeof(float) * N), s_n64 = align256(sizeof(double) * N);
         char* base = static_cast<char*>(stream_alloc(op->stream, s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + 256));

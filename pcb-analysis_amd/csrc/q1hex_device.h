@@ -82,6 +82,11 @@ enum Pending : int { kPendingNone = 0, kPendingCorrection = 1, kPendingZero = 2 
 // c_x U_x + c_y U_y, and per node a CSR list of its vias (other end, g).
 enum OperatorKind : int { kHex = 0, kSheet = 1 };
 
+// Whether bit z of a plain-plane mask is set (planes from 64 on never are).
+__host__ __device__ inline bool plain_plane(const unsigned long long mask, const int z) {
+    return z >= 0 && z < 64 && ((mask >> z) & 1ull);
+}
+
 struct OperatorDev {
     int S, R, C, kind;
     long long N;  // nodes
@@ -89,6 +94,10 @@ struct OperatorDev {
     const double* coef64;
     const float* robin32;   // (N)
     const double* robin64;
+    // Bit z (z < 64): every node of node plane z (sheet: layer z) is free
+    // with a Robin term of exactly +0.  The stencils read robin there through
+    // an out-of-range offset, which returns that +0 without touching memory.
+    unsigned long long plain;
     const float* dinv32;    // (N) 1 / diagonal, 1 on fixed rows
     const double* dinv64;
     const int* via_ptr;     // sheet: (N + 1) CSR offsets, or null without vias
@@ -180,8 +189,8 @@ namespace launch {
 // mfma selects the element action of FP32 applies: 0 vector FMAs, 2 or 3
 // the matrix cores with a two- or three-term bf16 split of the operand.
 template <typename T>
-hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, int C, int mode, int rows, int zb,
-                 int mfma, hipStream_t stream);
+hipError_t apply(const T* coef, const T* robin, unsigned long long plain, const T* x, T* y, int S, int R, int C,
+                 int mode, int rows, int zb, int mfma, hipStream_t stream);
 hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S, int R,
                     int C, int* bad, hipStream_t stream);
 // The layered-sheet operator: y = A x (mode 0) or y = K x (mode 1), and its

@@ -760,7 +760,8 @@ __device__ void slot_sum(const double* __restrict__ part, const int stride, cons
 // (mode 1: unconstrained stiffness, every node free, no Robin).
 template <typename T, int ROWS, int MODE, int MFMA>
 __global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ coef, const T* __restrict__ robin,
-                                                          const T* __restrict__ x, T* __restrict__ y, const int S,
+                                                          const unsigned long long plain, const T* __restrict__ x,
+                                                          T* __restrict__ y, const int S,
                                                           const int R, const int C, const int nsx, const int nsy,
                                                           const int zb) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -778,9 +779,9 @@ __global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ c
     if (y0 > R) return;
     strip_sweep<T, ROWS, MFMA>(
         sz, nsz, coef, S, R, C, x0, y0, zlo, zhi,
-        [&](const long long pb, const unsigned off, const bool) -> NodeVal<T> {
+        [&](const long long pb, const unsigned off, const bool, const int, const int z) -> NodeVal<T> {
             if (MODE == 1) return {ldt(x + pb, off), T(0)};
-            return {ldt(x + pb, off), ldt(robin + pb, off)};
+            return {ldt(x + pb, off), ldt(robin + pb, plain_plane(plain, z) ? kOutside : off)};
         },
         [&](const long long pb, const unsigned off, const int, const T conduction, const NodeVal<T>& v, auto) {
             stt(y + pb, off, v.robin >= T(0) ? conduction + v.robin * v.u : v.u);
@@ -956,9 +957,10 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
         // b.b, r.r (FP64), r.r and r.z (inner precision) per reduction cell.
         CellSums<kOuterSums, kRowsOuter> acc;
         acc.clear();
-        auto load = [&](const long long pb, const unsigned off, const bool) -> NodeVal<double> {
+        auto load = [&](const long long pb, const unsigned off, const bool, const int,
+                        const int layer) -> NodeVal<double> {
             const double xv = ldt(xin + pb, off) + cw * (double)ldt(cin + pb, off);
-            return {xv, ldt(robin + pb, off)};
+            return {xv, ldt(robin + pb, plain_plane(op.plain, layer) ? kOutside : off)};
         };
         auto emit = [&](const long long pb, const unsigned off, const int row, const double conduction,
                         const NodeVal<double>& v, auto parity) {
@@ -1117,7 +1119,7 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
         const TI rv = ldt(rin + pb, off), qv = ldt(qin + pb, off), pv = ldt(pin + pb, off);
         const TI d = ldt(dinv + pb, off);
         const TI cv = ldt(c + pb, owned ? off : kOutside);
-        const TI rb = ldt(robin + pb, off);
+        const TI rb = ldt(robin + pb, plain_plane(op.plain, layer) ? kOutside : off);
         const TI r = rv - alpha * qv;
         if constexpr (COARSE) {
             // The patch index does not depend on loaded data and is always in
@@ -1550,14 +1552,15 @@ __global__ __launch_bounds__(256) void k_symmetric_inverse(const double* __restr
 namespace launch {
 
 template <typename T>
-hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, int C, int mode, int rows, int zb,
-                 int mfma, hipStream_t stream) {
+hipError_t apply(const T* coef, const T* robin, const unsigned long long plain, const T* x, T* y, int S, int R,
+                 int C, int mode, int rows, int zb, int mfma, hipStream_t stream) {
     const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
     const int nsy = (R + 1 + rows * kWavesPerGroup - 1) / (rows * kWavesPerGroup);
     const int nsz = (S + 1 + zb - 1) / zb;
     const dim3 grid(nsx * nsy * nsz), block(kGroupThreads);
 #define PCBHIP_APPLY(ROWS, MODE, MF)                                                                               \
-    hipLaunchKernelGGL((k_apply<T, ROWS, MODE, MF>), grid, block, 0, stream, coef, robin, x, y, S, R, C, nsx, nsy, zb)
+    hipLaunchKernelGGL((k_apply<T, ROWS, MODE, MF>), grid, block, 0, stream, coef, robin, plain, x, y, S, R, C, nsx, \
+                       nsy, zb)
 #define PCBHIP_APPLY_ROWS(MF)                                                                                      \
     if (rows == 8) {                                                                                               \
         if (mode == 0) PCBHIP_APPLY(8, 0, MF); else PCBHIP_APPLY(8, 1, MF);                                        \
@@ -1584,10 +1587,10 @@ hipError_t apply(const T* coef, const T* robin, const T* x, T* y, int S, int R, 
     return hipGetLastError();
 }
 
-template hipError_t apply<float>(const float*, const float*, const float*, float*, int, int, int, int, int, int,
-                                 int, hipStream_t);
-template hipError_t apply<double>(const double*, const double*, const double*, double*, int, int, int, int, int, int,
-                                  int, hipStream_t);
+template hipError_t apply<float>(const float*, const float*, unsigned long long, const float*, float*, int, int, int,
+                                 int, int, int, int, hipStream_t);
+template hipError_t apply<double>(const double*, const double*, unsigned long long, const double*, double*, int, int,
+                                  int, int, int, int, int, hipStream_t);
 
 template <typename T>
 hipError_t sheet_apply(const OperatorDev& op, const T* x, T* y, int mode, hipStream_t stream) {

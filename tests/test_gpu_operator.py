@@ -152,3 +152,29 @@ def test_vector_action_bits_do_not_depend_on_the_tile_shape(shape):
         dev.set_shape(rows32, zb, rows64, zb)
         assert dev.apply_f32(x32).tobytes() == y32.tobytes(), (rows32, zb)
         assert dev.apply_f64(x).tobytes() == y64.tobytes(), (rows64, zb)
+
+
+def test_planes_without_robin_or_fixed_nodes_match_the_assembled_operator():
+    print("apply on a board with plain planes", flush=True)
+    # The 64 x 64 x 8 board is fixed on its bottom plane and cooled on its top
+    # one; its 7 inner planes are free with no Robin term, and there the
+    # stencils take robin = +0 without loading it.
+    from pcbhip import workloads
+
+    problem = workloads.board_64x64x8()
+    dev, ops, a, k = _device_and_matrix(problem)
+    s, r, c = problem.mesh.element_grid_shape
+    robin = ops["robin_total"].reshape(s + 1, r + 1, c + 1)
+    free = ops["free"].reshape(s + 1, r + 1, c + 1)
+    plain = [z for z in range(s + 1) if np.all(robin[z] == 0) and np.all(free[z] == 1)]
+    assert plain == list(range(1, s))
+    x = np.random.default_rng(10).standard_normal(problem.mesh.size)
+    x32 = x.astype(np.float32)
+    scale = float((abs(a) @ np.abs(x)).max())
+    y64 = dev.apply_f64(x)
+    np.testing.assert_allclose(y64, a @ x, rtol=0, atol=64 * DOUBLE_EPS * scale)
+    y32 = dev.apply_f32(x32)
+    assert np.max(np.abs(y32 - a @ x32.astype(np.float64))) <= 32 * SINGLE_EPS * scale
+    np.testing.assert_array_equal(y32[ops["fixed"]], x32[ops["fixed"]])
+    ks = dev.stiffness_f64(x)
+    np.testing.assert_allclose(ks, k @ x, rtol=0, atol=64 * DOUBLE_EPS * float((abs(k) @ np.abs(x)).max()))
