@@ -172,6 +172,28 @@ unsigned long long plain_planes(const std::vector<double>& robin64, long long pl
     return mask;
 }
 
+// Bit ez set for each of the first kMaxUniformLayers element layers whose
+// elements all have the same (a, b, c); their values into ``values``
+// (layer-major, 3 per layer, zero for the others).
+unsigned long long uniform_layers(const std::vector<double>& coef64, int S, long long per_layer,
+                                  std::vector<double>& values) {
+    const long long E = (long long)S * per_layer;
+    values.assign(3 * kMaxUniformLayers, 0.0);
+    unsigned long long mask = 0;
+    for (int ez = 0; ez < S && ez < kMaxUniformLayers; ++ez) {
+        bool same = true;
+        for (int d = 0; d < 3 && same; ++d) {
+            const double* c = coef64.data() + d * E + ez * per_layer;
+            for (long long i = 1; i < per_layer && same; ++i) same = c[i] == c[0] && !std::signbit(c[i]);
+            same = same && !std::signbit(c[0]);
+        }
+        if (!same) continue;
+        mask |= 1ull << ez;
+        for (int d = 0; d < 3; ++d) values[(size_t)(3 * ez + d)] = coef64[(size_t)(d * E + ez * per_layer)];
+    }
+    return mask;
+}
+
 }  // namespace
 
 struct pcbhip_operator {
@@ -185,6 +207,9 @@ struct pcbhip_operator {
     float* robin32 = nullptr;
     double* robin64 = nullptr;
     unsigned long long plain = 0;  // bit z: robin is +0 throughout node plane (sheet: layer) z
+    unsigned long long uniform = 0;  // hex: bit ez, every element of layer ez has the same coefficients
+    float* ucoef32 = nullptr;        // hex: their values, (kMaxUniformLayers, 3), for the standalone apply
+    double* ucoef64 = nullptr;
     float* dinv32 = nullptr;
     double* dinv64 = nullptr;
     double* diag64 = nullptr;
@@ -219,6 +244,7 @@ struct pcbhip_operator {
         v.robin32 = robin32;
         v.robin64 = robin64;
         v.plain = plain;
+        v.uniform = uniform;
         v.dinv32 = dinv32;
         v.dinv64 = dinv64;
         v.kind = kind;
@@ -355,13 +381,13 @@ void apply_dev(const pcbhip_operator* op, const T* x, T* y, int mode, hipStream_
         check(launch::sheet_apply<T>(op->view(), x, y, mode, stream), "sheet apply launch");
         return;
     }
-    const T* coef;
+    CoefSource<T> coef;
     const T* robin;
     if constexpr (sizeof(T) == 4) {
-        coef = op->coef32;
+        coef = {op->coef32, op->uniform, op->ucoef32};
         robin = op->robin32;
     } else {
-        coef = op->coef64;
+        coef = {op->coef64, op->uniform, op->ucoef64};
         robin = op->robin64;
     }
     const int rows = sizeof(T) == 4 ? op->rows32 : op->rows64;
@@ -1085,6 +1111,10 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
         const long long E = (long long)slabs * rows * cols;
         const long long N = (long long)(slabs + 1) * (rows + 1) * (cols + 1);
         if (N > (1LL << 31) - 1) fail(PCBHIP_INVALID_INPUT, "node count exceeds 2^31 - 1");
+        // Stencil offsets inside a node plane are 32-bit byte offsets of FP64
+        // values below the buffer range (2^30) and the out-of-grid offset.
+        if ((long long)(rows + 1) * (cols + 1) >= (1LL << 27))
+            fail(PCBHIP_INVALID_INPUT, "a node plane must hold fewer than 2^27 nodes");
         std::vector<double> coef64(3 * (size_t)E), robin64((size_t)N);
         std::vector<float> coef32(3 * (size_t)E), robin32((size_t)N);
         for (long long i = 0; i < 3 * E; ++i) {
@@ -1115,9 +1145,15 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
         op->zb32 = planes_per_block(slabs, (long long)strips_x(cols) * groups_y(rows, op->rows32), kTargetGroups);
         op->zb64 = planes_per_block(slabs, (long long)strips_x(cols) * groups_y(rows, op->rows64), kTargetGroups);
         op->plain = plain_planes(robin64, slabs + 1, (long long)(rows + 1) * (cols + 1));
+        std::vector<double> uvalues64;
+        op->uniform = uniform_layers(coef64, slabs, (long long)rows * cols, uvalues64);
+        const std::vector<float> uvalues32(uvalues64.begin(), uvalues64.end());
         const size_t s_c32 = align256(sizeof(float) * 3 * E), s_c64 = align256(sizeof(double) * 3 * E);
         const size_t s_n32 = align256(sizeof(float) * N), s_n64 = align256(sizeof(double) * N);
-        char* base = static_cast<char*>(stream_alloc(op->stream, s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + 256));
+        const size_t s_u32 = align256(sizeof(float) * uvalues32.size());
+        const size_t s_u64 = align256(sizeof(double) * uvalues64.size());
+        char* base = static_cast<char*>(
+            stream_alloc(op->stream, s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + s_u32 + s_u64 + 256));
         op->block = base;
         op->coef32 = reinterpret_cast<float*>(base);
         base += s_c32;
@@ -1133,7 +1169,17 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
         base += s_n64;
         op->diag64 = reinterpret_cast<double*>(base);
         base += s_n64;
+        op->ucoef32 = reinterpret_cast<float*>(base);
+        base += s_u32;
+        op->ucoef64 = reinterpret_cast<double*>(base);
+        base += s_u64;
         int* d_bad = reinterpret_cast<int*>(base);
+        check(hipMemcpyAsync(op->ucoef32, uvalues32.data(), sizeof(float) * uvalues32.size(), hipMemcpyHostToDevice,
+                             op->stream),
+              "upload");
+        check(hipMemcpyAsync(op->ucoef64, uvalues64.data(), sizeof(double) * uvalues64.size(), hipMemcpyHostToDevice,
+                             op->stream),
+              "upload");
         check(hipMemcpyAsync(op->coef32, coef32.data(), sizeof(float) * 3 * E, hipMemcpyHostToDevice, op->stream),
               "upload");
         check(hipMemcpyAsync(op->coef64, coef64.data(), sizeof(double) * 3 * E, hipMemcpyHostToDevice, op->stream),
@@ -1167,6 +1213,9 @@ int pcbhip_sheet_operator_create(int device, int layers, int rows, int cols, con
         const long long N = (long long)layers * (rows + 1) * (cols + 1);
         if (N > (1LL << 31) - 1 || 2 * via_count > (1LL << 31) - 1)
             fail(PCBHIP_INVALID_INPUT, "node or via count exceeds 2^31 - 1");
+        // A via's far end is loaded at its node index as a 32-bit byte offset
+        // of an FP64 value below the buffer range (2^30).
+        if (N >= (1LL << 27)) fail(PCBHIP_INVALID_INPUT, "a sheet operator must hold fewer than 2^27 nodes");
         std::vector<double> coef64(2 * (size_t)E), robin64((size_t)N);
         std::vector<float> coef32(2 * (size_t)E), robin32((size_t)N);
         for (long long i = 0; i < 2 * E; ++i) {
@@ -1814,8 +1863,10 @@ int pcbhip_operator_bytes(const pcbhip_operator* op, int64_t* bytes) {
     return guarded([&] {
         if (!op || !bytes) fail(PCBHIP_INVALID_INPUT, "null argument");
         const size_t e = (size_t)op->E, n = (size_t)op->N;
+        const size_t uniform_values = op->kind == kSheet ? 0 : align256(4 * 3 * kMaxUniformLayers) +
+                                                                   align256(8 * 3 * kMaxUniformLayers);
         *bytes = (int64_t)(align256(12 * e) + align256(24 * e) + align256(4 * n) + 2 * align256(8 * n) +
-                           align256(4 * n) + align256(8 * n));
+                           align256(4 * n) + align256(8 * n) + uniform_values);
     });
 }
 

@@ -82,10 +82,28 @@ enum Pending : int { kPendingNone = 0, kPendingCorrection = 1, kPendingZero = 2 
 // c_x U_x + c_y U_y, and per node a CSR list of its vias (other end, g).
 enum OperatorKind : int { kHex = 0, kSheet = 1 };
 
-// Whether bit z of a plain-plane mask is set (planes from 64 on never are).
-__host__ __device__ inline bool plain_plane(const unsigned long long mask, const int z) {
+// Whether bit z of a 64-bit node-plane or element-layer mask is set (planes
+// and layers from 64 on never are).
+__host__ __device__ inline bool mask_bit(const unsigned long long mask, const int z) {
     return z >= 0 && z < 64 && ((mask >> z) & 1ull);
 }
+
+// Uniform element layers: every element of layer ez < 64 has the same
+// (a, b, c); a dielectric slab of a board is such a layer.  The stencils
+// fetch no rows of coefficients there.  With a table of the layers' values
+// (the standalone apply, whose source is a kernel argument) the loads go to
+// the out-of-range offset and the values come from the table; without one
+// (the MPIR kernels, whose operator view sits in memory) every element in
+// the grid reads the layer's first element, one cache line per wave.
+// Elements outside the grid read zero either way.
+constexpr int kMaxUniformLayers = 64;
+
+template <typename T>
+struct CoefSource {
+    const T* coef;               // (3, S, R, C)
+    unsigned long long uniform;  // bit ez: layer ez is uniform
+    const T* values;             // (kMaxUniformLayers, 3) a, b, c of the uniform layers, or null
+};
 
 struct OperatorDev {
     int S, R, C, kind;
@@ -98,6 +116,7 @@ struct OperatorDev {
     // with a Robin term of exactly +0.  The stencils read robin there through
     // an out-of-range offset, which returns that +0 without touching memory.
     unsigned long long plain;
+    unsigned long long uniform;  // hex: bit ez, every element of layer ez has the same coefficients
     const float* dinv32;    // (N) 1 / diagonal, 1 on fixed rows
     const double* dinv64;
     const int* via_ptr;     // sheet: (N + 1) CSR offsets, or null without vias
@@ -189,8 +208,8 @@ namespace launch {
 // mfma selects the element action of FP32 applies: 0 vector FMAs, 2 or 3
 // the matrix cores with a two- or three-term bf16 split of the operand.
 template <typename T>
-hipError_t apply(const T* coef, const T* robin, unsigned long long plain, const T* x, T* y, int S, int R, int C,
-                 int mode, int rows, int zb, int mfma, hipStream_t stream);
+hipError_t apply(const CoefSource<T>& coef, const T* robin, unsigned long long plain, const T* x, T* y, int S, int R,
+                 int C, int mode, int rows, int zb, int mfma, hipStream_t stream);
 hipError_t diagonal(const double* coef, const double* robin, double* diag, double* dinv64, float* dinv32, int S, int R,
                     int C, int* bad, hipStream_t stream);
 // The layered-sheet operator: y = A x (mode 0) or y = K x (mode 1), and its

@@ -247,8 +247,9 @@ __device__ __forceinline__ void element_row_mfma(const bf16x8 unit, const unsign
 // element layer below and the element layer above are each summed from zero
 // (their element rows in order, the corner terms by fused multiply-adds) and
 // the two partials then added, a missing layer contributing +0.
-template <typename T, int ROWS, int MFMA = 0, bool DOWN = false, class Load, class Emit, class PlaneDone>
-__device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const int S, const int R, const int C,
+template <typename T, int ROWS, int MFMA = 0, bool DOWN = false, bool TABLE = false, class Load, class Emit,
+          class PlaneDone>
+__device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S, const int R, const int C,
                                             const int x0, const int y0, const int zlo, const int zhi, Load&& load,
                                             Emit&& emit, PlaneDone&& plane_done) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -331,14 +332,34 @@ __device__ __forceinline__ void strip_apply(const T* __restrict__ coef, const in
     // registers: its part of the lo plane (aL, aR) and of the hi plane
     // (bL, bR), each summed from zero.
     auto layer = [&](const int ez, T (&aL)[ROWS], T (&aR)[ROWS], T (&bL)[ROWS], T (&bR)[ROWS]) {
+        // The three coefficient rows of the layer (uniform layers as
+        // described at CoefSource).
         T ca[ROWS + 1], cb[ROWS + 1], cc[ROWS + 1];
-        const T* layer_coef = coef + (long long)ez * R * C;
+        const T* layer_coef = cs.coef + (long long)ez * R * C;
         const Rsrc ra = rsrc(layer_coef), rb = rsrc(layer_coef + E), rc = rsrc(layer_coef + 2 * E);
+        const bool uniform = mask_bit(cs.uniform, ez);
+        if constexpr (TABLE) {
+            const T* uv = cs.values + 3 * (ez & (kMaxUniformLayers - 1));
+            const T ua = uv[0], ub = uv[1], uc = uv[2];
 #pragma unroll
-        for (int j = 0; j < ROWS + 1; ++j) {
-            ca[j] = ld(ra, eoff[j], T());
-            cb[j] = ld(rb, eoff[j], T());
-            cc[j] = ld(rc, eoff[j], T());
+            for (int j = 0; j < ROWS + 1; ++j) {
+                const unsigned off = uniform ? kOutside : eoff[j];
+                const T la = ld(ra, off, T()), lb = ld(rb, off, T()), lc = ld(rc, off, T());
+                const bool in = eoff[j] != kOutside;
+                ca[j] = uniform ? (in ? ua : T(0)) : la;
+                cb[j] = uniform ? (in ? ub : T(0)) : lb;
+                cc[j] = uniform ? (in ? uc : T(0)) : lc;
+            }
+        } else {
+            // In-grid offsets are below kOutside (a power of two), so masking
+            // with it maps them to the first element and leaves kOutside.
+            const unsigned keep = uniform ? kOutside : ~0u;
+#pragma unroll
+            for (int j = 0; j < ROWS + 1; ++j) {
+                ca[j] = ld(ra, eoff[j] & keep, T());
+                cb[j] = ld(rb, eoff[j] & keep, T());
+                cc[j] = ld(rc, eoff[j] & keep, T());
+            }
         }
 #pragma unroll
         for (int i = 0; i < ROWS; ++i) {
@@ -605,14 +626,14 @@ __device__ __forceinline__ int xcd_tile(const int block, const int count) {
 __device__ __forceinline__ bool sweep_down(const int sz, const int nsz) { return nsz > 1 && (sz & 1) == 0; }
 
 // strip_apply in the direction of z block sz of nsz.
-template <typename T, int ROWS, int MFMA = 0, class Load, class Emit, class PlaneDone>
-__device__ __forceinline__ void strip_sweep(const int sz, const int nsz, const T* __restrict__ coef, const int S,
+template <typename T, int ROWS, int MFMA = 0, bool TABLE = false, class Load, class Emit, class PlaneDone>
+__device__ __forceinline__ void strip_sweep(const int sz, const int nsz, const CoefSource<T>& coef, const int S,
                                             const int R, const int C, const int x0, const int y0, const int zlo,
                                             const int zhi, Load&& load, Emit&& emit, PlaneDone&& plane_done) {
     if (sweep_down(sz, nsz))
-        strip_apply<T, ROWS, MFMA, true>(coef, S, R, C, x0, y0, zlo, zhi, load, emit, plane_done);
+        strip_apply<T, ROWS, MFMA, true, TABLE>(coef, S, R, C, x0, y0, zlo, zhi, load, emit, plane_done);
     else
-        strip_apply<T, ROWS, MFMA, false>(coef, S, R, C, x0, y0, zlo, zhi, load, emit, plane_done);
+        strip_apply<T, ROWS, MFMA, false, TABLE>(coef, S, R, C, x0, y0, zlo, zhi, load, emit, plane_done);
 }
 
 // ------------------------------------------------------------ reductions
@@ -759,7 +780,7 @@ __device__ void slot_sum(const double* __restrict__ part, const int stride, cons
 // y = A x (mode 0: fixed rows identity, Robin on the diagonal) or y = K x
 // (mode 1: unconstrained stiffness, every node free, no Robin).
 template <typename T, int ROWS, int MODE, int MFMA>
-__global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ coef, const T* __restrict__ robin,
+__global__ __launch_bounds__(kGroupThreads) void k_apply(const CoefSource<T> coef, const T* __restrict__ robin,
                                                           const unsigned long long plain, const T* __restrict__ x,
                                                           T* __restrict__ y, const int S,
                                                           const int R, const int C, const int nsx, const int nsy,
@@ -777,11 +798,11 @@ __global__ __launch_bounds__(kGroupThreads) void k_apply(const T* __restrict__ c
     const int zlo = sz * zb;
     const int zhi = zlo + zb < S + 1 ? zlo + zb : S + 1;
     if (y0 > R) return;
-    strip_sweep<T, ROWS, MFMA>(
+    strip_sweep<T, ROWS, MFMA, true>(
         sz, nsz, coef, S, R, C, x0, y0, zlo, zhi,
         [&](const long long pb, const unsigned off, const bool, const int, const int z) -> NodeVal<T> {
             if (MODE == 1) return {ldt(x + pb, off), T(0)};
-            return {ldt(x + pb, off), ldt(robin + pb, plain_plane(plain, z) ? kOutside : off)};
+            return {ldt(x + pb, off), ldt(robin + pb, mask_bit(plain, z) ? kOutside : off)};
         },
         [&](const long long pb, const unsigned off, const int, const T conduction, const NodeVal<T>& v, auto) {
             stt(y + pb, off, v.robin >= T(0) ? conduction + v.robin * v.u : v.u);
@@ -908,6 +929,7 @@ template <>
 struct InnerOf<float> {
     __device__ static const float* dinv(const OperatorDev& op) { return op.dinv32; }
     __device__ static const float* coef(const OperatorDev& op) { return op.coef32; }
+    __device__ static CoefSource<float> coefs(const OperatorDev& op) { return {op.coef32, op.uniform, nullptr}; }
     __device__ static const float* robin(const OperatorDev& op) { return op.robin32; }
     __device__ static const float* via_g(const OperatorDev& op) { return op.via_g32; }
 };
@@ -915,6 +937,7 @@ template <>
 struct InnerOf<double> {
     __device__ static const double* dinv(const OperatorDev& op) { return op.dinv64; }
     __device__ static const double* coef(const OperatorDev& op) { return op.coef64; }
+    __device__ static CoefSource<double> coefs(const OperatorDev& op) { return {op.coef64, op.uniform, nullptr}; }
     __device__ static const double* robin(const OperatorDev& op) { return op.robin64; }
     __device__ static const double* via_g(const OperatorDev& op) { return op.via_g64; }
 };
@@ -960,7 +983,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
         auto load = [&](const long long pb, const unsigned off, const bool, const int,
                         const int layer) -> NodeVal<double> {
             const double xv = ldt(xin + pb, off) + cw * (double)ldt(cin + pb, off);
-            return {xv, ldt(robin + pb, plain_plane(op.plain, layer) ? kOutside : off)};
+            return {xv, ldt(robin + pb, mask_bit(op.plain, layer) ? kOutside : off)};
         };
         auto emit = [&](const long long pb, const unsigned off, const int row, const double conduction,
                         const NodeVal<double>& v, auto parity) {
@@ -989,8 +1012,8 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
         }
         if (y0 <= op.R) {
             // Stores wait for their plane's emit, so a plane's loads issue back to back.
-            strip_sweep<double, kRowsOuter>(tile.sz, (op.S + tile.zb) / tile.zb, op.coef64, op.S, op.R, op.C, x0,
-                                            y0, zlo, zhi, load, emit,
+            strip_sweep<double, kRowsOuter>(tile.sz, (op.S + tile.zb) / tile.zb, InnerOf<double>::coefs(op), op.S,
+                                            op.R, op.C, x0, y0, zlo, zhi, load, emit,
                                             [&](const int z) { acc.plane_done(z, zlo, op.S); });
         }
         acc.flush_hex(zlo, zhi, y_group, tile.sx, op.R, op.C, sd.part, sd.part_stride);
@@ -1119,7 +1142,7 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
         const TI rv = ldt(rin + pb, off), qv = ldt(qin + pb, off), pv = ldt(pin + pb, off);
         const TI d = ldt(dinv + pb, off);
         const TI cv = ldt(c + pb, owned ? off : kOutside);
-        const TI rb = ldt(robin + pb, plain_plane(op.plain, layer) ? kOutside : off);
+        const TI rb = ldt(robin + pb, mask_bit(op.plain, layer) ? kOutside : off);
         const TI r = rv - alpha * qv;
         if constexpr (COARSE) {
             // The patch index does not depend on loaded data and is always in
@@ -1181,7 +1204,7 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
         return;
     }
     if (y0 <= op.R) {
-        strip_sweep<TI, ROWS, MFMA>(tile.sz, (op.S + tile.zb) / tile.zb, InnerOf<TI>::coef(op), op.S, op.R, op.C,
+        strip_sweep<TI, ROWS, MFMA>(tile.sz, (op.S + tile.zb) / tile.zb, InnerOf<TI>::coefs(op), op.S, op.R, op.C,
                                     x0, y0, zlo, zhi, load, emit, [&](const int z) { acc.plane_done(z, zlo, op.S); });
     }
     acc.flush_hex(zlo, zhi, tile.sy * kWavesPerGroup * ROWS, tile.sx, op.R, op.C, sd.part, sd.part_stride);
@@ -1552,8 +1575,8 @@ __global__ __launch_bounds__(256) void k_symmetric_inverse(const double* __restr
 namespace launch {
 
 template <typename T>
-hipError_t apply(const T* coef, const T* robin, const unsigned long long plain, const T* x, T* y, int S, int R,
-                 int C, int mode, int rows, int zb, int mfma, hipStream_t stream) {
+hipError_t apply(const CoefSource<T>& coef, const T* robin, const unsigned long long plain, const T* x, T* y, int S,
+                 int R, int C, int mode, int rows, int zb, int mfma, hipStream_t stream) {
     const int nsx = (C + 1 + kStripCols - 1) / kStripCols;
     const int nsy = (R + 1 + rows * kWavesPerGroup - 1) / (rows * kWavesPerGroup);
     const int nsz = (S + 1 + zb - 1) / zb;
@@ -1587,10 +1610,10 @@ hipError_t apply(const T* coef, const T* robin, const unsigned long long plain, 
     return hipGetLastError();
 }
 
-template hipError_t apply<float>(const float*, const float*, unsigned long long, const float*, float*, int, int, int,
-                                 int, int, int, int, hipStream_t);
-template hipError_t apply<double>(const double*, const double*, unsigned long long, const double*, double*, int, int,
-                                  int, int, int, int, int, hipStream_t);
+template hipError_t apply<float>(const CoefSource<float>&, const float*, unsigned long long, const float*, float*, int,
+                                 int, int, int, int, int, int, hipStream_t);
+template hipError_t apply<double>(const CoefSource<double>&, const double*, unsigned long long, const double*, double*,
+                                  int, int, int, int, int, int, int, hipStream_t);
 
 template <typename T>
 hipError_t sheet_apply(const OperatorDev& op, const T* x, T* y, int mode, hipStream_t stream) {

@@ -154,16 +154,21 @@ def test_vector_action_bits_do_not_depend_on_the_tile_shape(shape):
         assert dev.apply_f64(x).tobytes() == y64.tobytes(), (rows64, zb)
 
 
-def test_planes_without_robin_or_fixed_nodes_match_the_assembled_operator():
-    print("apply on a board with plain planes", flush=True)
+def test_plain_planes_and_uniform_layers_match_the_assembled_operator():
+    print("apply on a board with plain planes and uniform layers", flush=True)
     # The 64 x 64 x 8 board is fixed on its bottom plane and cooled on its top
     # one; its 7 inner planes are free with no Robin term, and there the
-    # stencils take robin = +0 without loading it.
+    # stencils take robin = +0 without loading it.  Copper is on its outer
+    # slabs only, so the 6 FR-4 slabs between are uniform element layers,
+    # whose coefficients come from a per-layer table.
     from pcbhip import workloads
 
     problem = workloads.board_64x64x8()
     dev, ops, a, k = _device_and_matrix(problem)
     s, r, c = problem.mesh.element_grid_shape
+    coef = np.asarray(ops["coef"]).reshape(3, s, r, c)
+    uniform = [ez for ez in range(s) if all(np.all(coef[d, ez] == coef[d, ez].flat[0]) for d in range(3))]
+    assert uniform == list(range(1, s - 1))
     robin = ops["robin_total"].reshape(s + 1, r + 1, c + 1)
     free = ops["free"].reshape(s + 1, r + 1, c + 1)
     plain = [z for z in range(s + 1) if np.all(robin[z] == 0) and np.all(free[z] == 1)]
