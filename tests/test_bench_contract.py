@@ -21,12 +21,18 @@ def _bench():
     return module
 
 
+# On this board the stencils skip the Robin words of the node planes that are
+# all free with no Robin term and the coefficient rows of the uniform FR-4
+# layers, so the measured bytes sit well below the algorithmic 108.5 MB
+# (apply) and 229.3 MB (pass); the bounds catch a missing or garbled line.
+
+
 def test_profile_summary_has_traffic_for_every_apply_candidate():
     bench = _bench()
     shape = (16, 512, 512)
     for rows, zb in autotune.apply_candidates(shape):
         traffic = bench.profiled_traffic(shape, rows, zb)
-        assert isinstance(traffic, int) and 100e6 < traffic < 200e6, (rows, zb, traffic)
+        assert isinstance(traffic, int) and 30e6 < traffic < 200e6, (rows, zb, traffic)
     assert bench.profiled_traffic(shape, 4, 3) is None
 
 
@@ -40,5 +46,5 @@ def test_more_than_one_gpu_is_refused():
 def test_profile_summary_has_traffic_for_the_default_pcg_pass():
     bench = _bench()
     traffic = bench.profiled_pass_traffic((16, 512, 512), 4, 1024)
-    assert isinstance(traffic, int) and 200e6 < traffic < 400e6, traffic
+    assert isinstance(traffic, int) and 100e6 < traffic < 400e6, traffic
     assert bench.profiled_pass_traffic((16, 512, 512), 8, 1024) is None
