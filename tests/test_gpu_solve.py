@@ -289,3 +289,27 @@ def test_pass_timer_keeps_results_and_later_runs():
     batch.run()
     again, res_again = batch.result(0)
     assert again.tobytes() == before.tobytes() and res_again["inner_iterations"] == res_before["inner_iterations"]
+
+
+def test_a_stack_deeper_than_the_plane_masks_matches_the_direct_solve():
+    print("70-slab stack vs direct", flush=True)
+    # 70 slabs: the FR-4 element layers 1..68 are uniform and the node planes
+    # 1..69 are free with no Robin term, but the operator flags only those
+    # below 64.  The others load like any layer, and the column's z blocks
+    # sweep in both directions.
+    rng = np.random.default_rng(41)
+    masks = (workloads.copper_mask(rng, 6, 9, 0.5), workloads.copper_mask(rng, 6, 9, 0.5))
+    problem = workloads.layered_board(70, 6, 9, masks, [workloads.Source(1, 2, 3, 0.5)])
+    exact, rhs, _ = _oracle_solution(problem)
+    operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    a = A.system_matrix(*_arrays(problem))
+    x = np.random.default_rng(42).standard_normal(problem.mesh.size)
+    scale = float((abs(a) @ np.abs(x)).max())
+    np.testing.assert_allclose(operator.device_operator.apply_f64(x), a @ x, rtol=0, atol=64 * 2.0**-52 * scale)
+    x32 = x.astype(np.float32)
+    y32 = operator.device_operator.apply_f32(x32)
+    assert np.max(np.abs(y32 - a @ x32.astype(np.float64))) <= 32 * 2.0**-23 * scale
+    result = solve_mpir(operator, rhs, config=MPIRConfig(relative_tolerance=1e-11, max_outer_iterations=30,
+                                                         max_inner_iterations=20000))
+    assert result.converged
+    assert _rise_error(result.solution, exact) < 1e-9
