@@ -108,8 +108,16 @@ def pass_roofline(operator, batch, rows: int, target: int, launches: int = 100) 
         kernel=f"k_inner_step<float,{rows}> (fused FP32 Jacobi-PCG pass), target {target} workgroups",
         per_launch_us=per_us,
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=traffic, algorithmic_bytes=algorithmic),
+                      traffic=traffic, algorithmic_bytes=algorithmic, hbm_gbs=_hbm_rate(traffic, per_us * 1e-6)),
     )
+
+
+def _hbm_rate(traffic, seconds: float):
+    """GB/s of the measured HBM bytes per launch (the profile's traffic),
+    beside ``achieved``, which counts the algorithmic bytes: on boards with
+    plain planes and uniform layers the kernels skip many of those."""
+
+    return None if traffic is None else traffic / seconds / 1e9
 
 
 def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
@@ -144,7 +152,7 @@ def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
         gdof_s=n / per / 1e9,
         sweep=[dict(rows=r_, zb=z_, us=p_ * 1e6) for p_, r_, z_ in table],
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=traffic, algorithmic_bytes=algorithmic),
+                      traffic=traffic, algorithmic_bytes=algorithmic, hbm_gbs=_hbm_rate(traffic, per)),
     )
 
 
