@@ -487,7 +487,7 @@ def run_serve(args, torch) -> dict:
     def make_requests(rate: float, duration: float) -> list:
         out = []
         for i, t in enumerate(serving.poisson_arrivals(rng, rate, duration)):
-            group = thermal if rng.random() < 0.7 else dc
+            group = thermal if (rng.random() < 0.7 or not dc) else dc
             board = group[int(rng.integers(len(group)))]
             m = int(rng.integers(1, 17))
             out.append(serving.ServeRequest(i, board.operator, board.rhs[:m], config, kind=board.kind,
