@@ -110,3 +110,38 @@ def test_many_excitations_match_the_oracle_on_the_copper_cluster():
         got = potential[top][on_cluster]
         want = exact[top][on_cluster]
         assert np.max(np.abs(got - want)) <= 1e-9 * np.max(np.abs(want))
+
+
+# Graded, unequal x and y pitches: a bar along x and one along y, each driven
+# by voltage terminals on its end faces, carry sigma dV A / L.  An exchange of
+# the in-plane axes anywhere on the device path changes the current.
+_GX = np.array([0.10, 0.35, 0.20, 0.60]) * 1e-3
+_GY = np.array([0.30, 0.70, 0.45, 0.15]) * 1e-3
+_GT = (20e-6, 35e-6)
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+def test_graded_anisotropic_bar_carries_the_closed_form_current(axis):
+    print(f"dc graded bar {axis}", flush=True)
+    mesh = LayeredThermalMesh(_GT, _GX, _GY, SIGMA)
+    nz, ny, nx = mesh.node_shape
+    if axis == "x":
+        low = tuple((z, y, 0) for z in range(nz) for y in range(ny))
+        high = tuple((z, y, nx - 1) for z in range(nz) for y in range(ny))
+        length, area = _GX.sum(), _GY.sum() * sum(_GT)
+    else:
+        low = tuple((z, 0, x) for z in range(nz) for x in range(nx))
+        high = tuple((z, ny - 1, x) for z in range(nz) for x in range(nx))
+        length, area = _GY.sum(), _GX.sum() * sum(_GT)
+    problem = HexConductionProblem(mesh, voltage_terminals=(VoltageTerminal(low, 2.0e-3), VoltageTerminal(high, 0.0)))
+    solution = solve_hex_dc(problem, config=MPIRConfig(relative_tolerance=1e-12, max_outer_iterations=40,
+                                                       max_inner_iterations=20000))
+    current = SIGMA * 2.0e-3 * area / length
+    assert solution.solve.converged
+    assert solution.voltage_terminal_current_a[0] == pytest.approx(current, rel=1e-9)
+    assert solution.voltage_terminal_current_a[1] == pytest.approx(-current, rel=1e-9)
+    # The potential is linear in the graded coordinate.
+    coord = np.concatenate(([0.0], np.cumsum(_GX if axis == "x" else _GY)))
+    shape = (1, 1, -1) if axis == "x" else (1, -1, 1)
+    expected = 2.0e-3 * (1.0 - coord / coord[-1]).reshape(shape)
+    np.testing.assert_allclose(solution.potential_v, np.broadcast_to(expected, mesh.node_shape), atol=1e-9 * 2.0e-3)

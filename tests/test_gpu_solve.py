@@ -313,3 +313,47 @@ def test_a_stack_deeper_than_the_plane_masks_matches_the_direct_solve():
                                                          max_inner_iterations=20000))
     assert result.converged
     assert _rise_error(result.solution, exact) < 1e-9
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+@pytest.mark.parametrize("preconditioner", ["jacobi", "two-level"])
+def test_graded_anisotropic_bar_has_the_linear_profile_and_exact_heat(axis, preconditioner):
+    # Graded, unequal x and y pitches with distinct in-plane and through-plane
+    # conductivities; the faces normal to ``axis`` are held at 300 K and 350 K.
+    # The Q1 space holds the linear field exactly and the heat through the
+    # hot face is k dT A / L; an exchange of the in-plane axes breaks both.
+    print(f"graded bar {axis} {preconditioner}", flush=True)
+    gx = np.array([0.10, 0.35, 0.20, 0.60, 0.25, 0.40]) * 1e-3
+    gy = np.array([0.30, 0.70, 0.45, 0.15, 0.50, 0.20]) * 1e-3
+    thick = (0.2e-3, 0.5e-3)
+    mesh = LayeredThermalMesh(thick, gx, gy, 2.0, through_plane_conductivity_w_per_m_k=0.7)
+    mask = np.zeros(mesh.node_shape, dtype=bool)
+    values = np.zeros(mesh.node_shape)
+    if axis == "x":
+        mask[..., 0] = mask[..., -1] = True
+        values[..., 0], values[..., -1] = 300.0, 350.0
+        coord, area = np.concatenate(([0.0], np.cumsum(gx))), gy.sum() * sum(thick)
+        shape = (1, 1, -1)
+    else:
+        mask[:, 0, :] = mask[:, -1, :] = True
+        values[:, 0, :], values[:, -1, :] = 300.0, 350.0
+        coord, area = np.concatenate(([0.0], np.cumsum(gy))), gx.sum() * sum(thick)
+        shape = (1, -1, 1)
+    problem = ThermalConductionProblem(mesh, fixed_temperature_mask=mask, fixed_temperature_k=values)
+    # Solved to 1e-12.  The residual is relative to |b|, which the fixed rows
+    # (kelvin) dominate over the free rows (watts), so the free nodes keep an
+    # error of about 1e-9 of the range; the bounds below are 1e-7, and an
+    # exchange of the in-plane axes is an error of order 1e-1.
+    config = MPIRConfig(relative_tolerance=1e-12, max_outer_iterations=40, max_inner_iterations=20000)
+    solution = solve_thermal_conduction(problem, config=config, preconditioner=preconditioner, coarse_block_nodes=4)
+    assert solution.solve.converged
+    expected = 300.0 + 50.0 * (coord / coord[-1]).reshape(shape)
+    np.testing.assert_allclose(solution.temperature_k, np.broadcast_to(expected, mesh.node_shape), rtol=0,
+                               atol=1e-7 * 50.0)
+    # The fixed heat sums both faces (zero); the x (or y) flux at every element
+    # centre is -k dT / L.
+    flux = solution.heat_flux_w_per_m2[..., 0 if axis == "x" else 1]
+    np.testing.assert_allclose(flux, -2.0 * 50.0 / coord[-1], rtol=1e-6)
+    assert abs(solution.fixed_temperature_heat_w) <= 1e-7 * 2.0 * 50.0 * area / coord[-1]
+    other = solution.heat_flux_w_per_m2[..., 1 if axis == "x" else 0]
+    assert np.max(np.abs(other)) <= 1e-6 * 2.0 * 50.0 / coord[-1]

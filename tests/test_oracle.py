@@ -176,3 +176,129 @@ def test_oracle_budget_balances_on_a_random_body():
     op, ops, t, stiff = _solve_oracle(random_body((3, 6, 5), seed=9, void=True))
     budget = P.heat_budget(ops, op, t, stiff)
     assert abs(budget["balance"]) <= 1e-10 * budget["total"]
+
+
+# --- Anisotropic and graded pitch -------------------------------------------
+# The checks above use equal x and y pitches, where a transposition of the
+# in-plane axes is invisible.  These use graded, unequal pitches in x and y,
+# distinct in-plane and through-plane conductivities, and closed forms that
+# any swap of the x and y terms (coefficients, face areas, flux) breaks.
+
+# As many cells in x as in y, so that an exchange of the two pitch arrays
+# still broadcasts and must be caught by the closed forms, not by a shape.
+_PITCH_X = np.array([0.10, 0.35, 0.20, 0.60]) * 1e-3
+_PITCH_Y = np.array([0.30, 0.70, 0.45, 0.15]) * 1e-3
+_THICK = np.array([0.20, 0.50]) * 1e-3
+_K_IN, _K_Z = 2.0, 0.7
+
+
+def _graded_box(fixed_axis, t_low=300.0, t_high=350.0, convection=()):
+    """A box of graded cells with the two faces normal to ``fixed_axis``
+    (2 = x, 1 = y, 0 = z, in node-array axes) held at ``t_low`` / ``t_high``."""
+
+    shape = (len(_THICK), len(_PITCH_Y), len(_PITCH_X))
+    nodes = tuple(n + 1 for n in shape)
+    mask = np.zeros(nodes, dtype=bool)
+    values = np.zeros(nodes)
+    if fixed_axis is not None:
+        low = [slice(None)] * 3
+        high = [slice(None)] * 3
+        low[fixed_axis], high[fixed_axis] = 0, -1
+        mask[tuple(low)] = mask[tuple(high)] = True
+        values[tuple(low)], values[tuple(high)] = t_low, t_high
+    return P.OracleProblem(
+        thickness=_THICK.copy(), pitch_x=_PITCH_X.copy(), pitch_y=_PITCH_Y.copy(),
+        k_in=np.full(shape, _K_IN), k_z=np.full(shape, _K_Z), active=np.ones(shape, bool),
+        fixed_mask=mask, fixed_values=values, convection=list(convection),
+    )
+
+
+def _node_coordinate(axis):
+    edges = {2: _PITCH_X, 1: _PITCH_Y, 0: _THICK}[axis]
+    return np.concatenate(([0.0], np.cumsum(edges)))
+
+
+def _bar_errors(axis):
+    """(max nodal deviation from the linear profile in kelvin, relative
+    error of the face heat, relative error of the element flux) of the bar
+    along ``axis`` solved by the oracle."""
+
+    p = _graded_box(axis)
+    ops = P.operator_arrays(p)
+    a = A.system_matrix(ops["coef"], ops["unit"], ops["robin_total"], ops["free"], p.element_shape)
+    k = A.stiffness_matrix(ops["coef"], ops["unit"], p.element_shape)
+    ref = P.default_reference(ops, p)
+    t = A.solve_direct(a, P.build_rhs(ops, ref, lambda v: k @ v)) + ref
+    coord = _node_coordinate(axis)
+    length = coord[-1]
+    shape = [1, 1, 1]
+    shape[axis] = -1
+    linear = 300.0 + 50.0 * (coord / length).reshape(shape)
+    profile = np.max(np.abs(t.reshape(p.node_shape) - linear))
+    # Heat leaving through the hot face = k dT A / L, A the cross-section.
+    extent = {2: _PITCH_X.sum(), 1: _PITCH_Y.sum(), 0: _THICK.sum()}
+    area = np.prod([v for ax, v in extent.items() if ax != axis])
+    cond = _K_Z if axis == 0 else _K_IN
+    exact = cond * 50.0 * area / length
+    residual = (k @ t).reshape(p.node_shape)
+    hot = [slice(None)] * 3
+    hot[axis] = -1
+    face_heat = float(np.sum(residual[tuple(hot)]))
+    flux = P.element_heat_flux(p, t)
+    want = np.zeros(3)
+    want[{2: 0, 1: 1, 0: 2}[axis]] = -cond * 50.0 / length
+    flux_err = float(np.max(np.abs(flux - want)) / (cond * 50.0 / length))
+    return profile, abs(face_heat - exact) / exact, flux_err
+
+
+@pytest.mark.parametrize("axis", [2, 1, 0], ids=["x", "y", "z"])
+def test_graded_anisotropic_bar_has_the_linear_profile_and_exact_flux(axis):
+    # tests/test_tensor_grid.py:181-199 checks the graded strip the same way:
+    # the Q1 space holds the linear field exactly, whatever the grading.
+    profile, heat, flux = _bar_errors(axis)
+    assert profile <= 1e-10
+    assert heat <= 1e-12
+    assert flux <= 1e-12
+
+
+def test_exposed_side_faces_lump_their_own_areas():
+    # Sum of the lumped conductance of each side = h times that side's area:
+    # y L_z for the x faces, x L_z for the y faces (boundaries.py:170-182).
+    lx, ly, lz = _PITCH_X.sum(), _PITCH_Y.sum(), _THICK.sum()
+    for direction, area in (("-x", ly * lz), ("+x", ly * lz), ("-y", lx * lz), ("+y", lx * lz)):
+        p = _graded_box(None, convection=[P.Convection("exposed", 7.0, 290.0, (direction,))])
+        w, rw = P.convection_weights(p, p.convection[0])
+        assert w.sum() == pytest.approx(7.0 * area, rel=1e-13), direction
+        assert rw.sum() == pytest.approx(7.0 * 290.0 * area, rel=1e-13), direction
+        # Only the nodes of that face carry weight.
+        axis, end = {"x": 2, "y": 1}[direction[1]], (0 if direction[0] == "-" else -1)
+        grid = w.reshape(p.node_shape)
+        face = [slice(None)] * 3
+        face[axis] = end
+        assert grid[tuple(face)].sum() == pytest.approx(w.sum(), rel=1e-14)
+
+
+def test_a_transposed_pitch_fails_the_anisotropic_checks(monkeypatch):
+    # Mutation check, on the CPU: the oracle with its x and y pitch arrays
+    # exchanged in the coefficients, or its x and y face areas exchanged,
+    # must fail the closed forms above.
+    original = P.element_coefficients
+
+    def swapped_coefficients(p):
+        return original(P.OracleProblem(**{**p.__dict__, "pitch_x": p.pitch_y, "pitch_y": p.pitch_x}))
+
+    monkeypatch.setattr(P, "element_coefficients", swapped_coefficients)
+    profile, heat, _ = _bar_errors(2)
+    assert profile > 1e-3 or heat > 1e-3
+    monkeypatch.setattr(P, "element_coefficients", original)
+
+    original_area = P.face_area
+
+    def swapped_area(p, direction):
+        other = {"x": "y", "y": "x", "z": "z"}[direction[1]]
+        return original_area(p, direction[0] + other)
+
+    monkeypatch.setattr(P, "face_area", swapped_area)
+    p = _graded_box(None, convection=[P.Convection("exposed", 7.0, 290.0, ("-x",))])
+    w, _ = P.convection_weights(p, p.convection[0])
+    assert w.sum() != pytest.approx(7.0 * _PITCH_Y.sum() * _THICK.sum(), rel=1e-6)
