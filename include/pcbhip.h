@@ -129,6 +129,19 @@ int pcbhip_operator_galerkin_f64(const pcbhip_operator *op, int block, int count
  * (replaces MatrixFreeThermalOperator.precondition_low, operator.py:366-371). */
 int pcbhip_operator_precondition_f32(const pcbhip_operator *op, const float *r, float *z);
 
+/* The same z = M^-1 r as pcbhip_operator_precondition_f32, on device vectors
+ * r and z of pcbhip_operator_size() floats, in stream order on ``stream``
+ * (NULL: the operator's stream), allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops and harnesses that forbid
+ * allocation).  The two-level coarse vectors live in a static device array
+ * of 2 x 16384 doubles (256 KiB) that each device holds until the process
+ * ends; a coarse space of more than 16384 unknowns is PCBHIP_INVALID_INPUT.
+ * Every call on one device shares that array: calls that overlap on one
+ * device (different streams or host threads) race; calls in one stream's
+ * order do not. */
+int pcbhip_operator_precondition_f32_noalloc(const pcbhip_operator *op, const float *r, float *z, void *stream);
+
 /* y = A x with fixed rows the identity and Robin on the diagonal, FP32
  * (replaces hex_q1.cpp:131-193 apply_lines<float>, pcbcore::fem::thermal_hex::
  * apply_low, thermal_hex.hpp:33) and FP64 (apply_high, thermal_hex.hpp:32). */

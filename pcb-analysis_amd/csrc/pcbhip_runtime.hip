@@ -1047,6 +1047,32 @@ void* server_take_block(pcbhip_server* sv, size_t bytes, size_t* got) {
 
 namespace {
 
+// Scratch of the allocation-free preconditioner entry point
+// (pcbhip_operator_precondition_f32_noalloc): the two FP64 coarse vectors
+// g = Z^T r and w = cinv g of at most kMaxCoarse values each, 256 KiB, which
+// every device the library runs on holds until the process ends.  In host
+// code the array's name is not a device address: the address comes from
+// hipGetSymbolAddress, once per device with that device current, and is kept
+// in the table of noalloc_scratch().
+constexpr size_t kNoallocScratchDoubles = 2 * (size_t)kMaxCoarse;
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
+
+// The device address of ``scratch`` on ``device`` (which must be current),
+// looked up on the first call on that device and kept.
+double* noalloc_scratch(const int device) {
+    static std::mutex table_lock;
+    static std::vector<double*> table;
+    std::lock_guard<std::mutex> guard(table_lock);
+    if ((size_t)device >= table.size()) table.resize((size_t)device + 1, nullptr);
+    if (!table[(size_t)device]) {
+        void* ptr = nullptr;
+        check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress");
+        if (!ptr) fail(PCBHIP_RUNTIME, "no device address for the preconditioner scratch");
+        table[(size_t)device] = static_cast<double*>(ptr);
+    }
+    return table[(size_t)device];
+}
+
 // One rocBLAS handle per device for the coarse factorisations, made on first
 // use and kept (its workspace too) for the life of the process; calls take
 // the handle's lock and set its stream.
@@ -1668,6 +1694,24 @@ int pcbhip_operator_precondition_f32(const pcbhip_operator* op, const float* r, 
         check(launch::precondition(op->view(), dr, dz, g, w, op->stream), "precondition launch");
         check(hipMemcpyAsync(z, dz, bytes, hipMemcpyDeviceToHost, op->stream), "copy z");
         check(hipStreamSynchronize(op->stream), "precondition");
+    });
+}
+
+int pcbhip_operator_precondition_f32_noalloc(const pcbhip_operator* op, const float* r, float* z, void* stream) {
+    return guarded([&] {
+        if (!op || !r || !z) fail(PCBHIP_INVALID_INPUT, "null argument");
+        const size_t need = 2 * (size_t)std::max(op->nc, 0);
+        if (need > kNoallocScratchDoubles)
+            fail(PCBHIP_INVALID_INPUT, "coarse size " + std::to_string(op->nc) +
+                                           " exceeds the allocation-free scratch of " +
+                                           std::to_string(kNoallocScratchDoubles / 2) + " coarse unknowns");
+        set_device(op->device);
+        double* g = noalloc_scratch(op->device);
+        // Both coarse vectors are written in full before they are read (the
+        // restriction writes every patch, the coarse rows every row), so the
+        // scratch needs no clearing between calls.
+        check(launch::precondition(op->view(), r, z, g, g + kMaxCoarse, stream_of(op, stream)),
+              "precondition launch");
     });
 }
 

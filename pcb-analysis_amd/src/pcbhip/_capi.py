@@ -42,6 +42,7 @@ EXPORTS = (
     "pcbhip_operator_set_coarse",
     "pcbhip_operator_set_coarse_matrix",
     "pcbhip_operator_precondition_f32",
+    "pcbhip_operator_precondition_f32_noalloc",
     "pcbhip_operator_galerkin_f64",
     "pcbhip_apply_f32",
     "pcbhip_apply_f64",
@@ -141,6 +142,7 @@ def load() -> ctypes.CDLL:
     lib.pcbhip_operator_set_coarse.argtypes = [_vp, _int, _int, _vp]
     lib.pcbhip_operator_set_coarse_matrix.argtypes = [_vp, _int, _int, _vp, _vp]
     lib.pcbhip_operator_precondition_f32.argtypes = [_vp, _vp, _vp]
+    lib.pcbhip_operator_precondition_f32_noalloc.argtypes = [_vp, _vp, _vp, _vp]
     lib.pcbhip_operator_galerkin_f64.argtypes = [_vp, _int, _int, _vp, _vp]
     for name in ("pcbhip_apply_f32", "pcbhip_apply_f64", "pcbhip_stiffness_f64"):
         getattr(lib, name).argtypes = [_vp, _vp, _vp]
