@@ -467,6 +467,76 @@ def run_apply_engines(args, torch) -> dict:
     }
 
 
+def _board(args):
+    """The benchmark board, or with ``--board general`` the board on which no
+    stencil shortcut applies (every plane has Robin terms, no layer is uniform)."""
+
+    from pcbhip import workloads
+
+    if args.board == "general":
+        return workloads.general_board(size=args.size, slabs=args.slabs)
+    return workloads.board_512x512x16(size=args.size, slabs=args.slabs)
+
+
+def run_pass(args, torch) -> dict:
+    """The fused Jacobi-PCG pass and the FP32 apply of one board over tile
+    shapes: one MPIR solve, then each (rows, target) shape of the pass timed
+    with HIP events on the batch's stream (``pcbhip_batch_time_pass``), the
+    apply's candidate shapes on a torch stream, and the box's copy at the
+    pass's algorithmic bytes."""
+
+    from electrical.matrix_free_mpir_fem import MPIRConfig
+    from pcbhip import _capi
+    from thermal.matrix_free_mpir_fem import MatrixFreeThermalOperator
+
+    problem = _board(args)
+    operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    n = operator.size
+    s, r, c = operator.mesh.element_grid_shape
+    e = s * r * c
+    rhs = operator.build_rhs(operator.default_reference_temperature())
+    batch = _capi.Batch([operator.device_operator])
+    batch.set_shape(args.rows, args.target)
+    batch.set(0, rhs, None, MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60,
+                                       max_inner_iterations=100000).to_c())
+    t0 = time.perf_counter()
+    batch.run()
+    solve_s = time.perf_counter() - t0
+    _, result = batch.result(0)
+    log(f"solve {solve_s * 1e3:.1f} ms: {result['outer_iterations']} outer / {result['inner_iterations']} inner")
+    pass_bytes = 40 * n + 12 * e
+    sweep = []
+    for rows in (4, 8):
+        for target in args.targets:
+            batch.set_shape(rows, target)
+            us = batch.time_pass(200)
+            sweep.append(dict(rows=rows, target=target, us=us, algorithmic_gbs=pass_bytes / us / 1e3))
+            log(f"pass rows={rows} target={target}: {us:.2f} us, {pass_bytes / us / 1e3:.0f} GB/s")
+    stream = torch.cuda.Stream()
+    apply = apply_roofline(operator, torch, stream)
+    copy = copy_bandwidth(torch, stream, pass_bytes // 2 // 16 * 16)
+    best = min(sweep, key=lambda row: row["us"])
+    return {
+        "metric": f"fused Jacobi-PCG pass GB/s (algorithmic 40N+12E), {args.board} board {c}x{r}x{s}",
+        "value": best["algorithmic_gbs"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": 200,
+        "warmup": 3,
+        "ms_per_step": best["us"] / 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 sums)",
+        "data": f"synthetic ({args.board} board generator)",
+        "config": {"workload": f"{args.board} board {c}x{r}x{s} elements ({n:,} nodes)", "pass_sweep": sweep,
+                   "pass_bytes": pass_bytes, "apply_us": apply["per_launch_us"], "apply_sweep": apply["sweep"],
+                   "apply_algorithmic_gbs": apply["roofline"]["achieved"], "copy_gbs_at_pass_bytes": copy,
+                   "solve_ms": solve_s * 1e3, "inner_iterations": result["inner_iterations"],
+                   "parallelism": "single GPU"},
+    }
+
+
 def run_serve(args, torch) -> dict:
     """The serving stream: Poisson arrivals of thermal (70 %) and DC (30 %)
     requests of 1..16 right-hand sides on pool boards of the uneven-batch
@@ -752,7 +822,11 @@ def main() -> int:
     parser.add_argument("--target", type=int, default=1024, help="stencil workgroups per launch before z blocks split")
     parser.add_argument("--workload", default="board512",
                         choices=["board512", "batch200", "multirhs64", "serve", "apply_engines", "coupling",
-                                 "two_level"])
+                                 "two_level", "pass"])
+    parser.add_argument("--board", default="standard", choices=["standard", "general"],
+                        help="pass: the benchmark board or the board with no stencil shortcuts")
+    parser.add_argument("--targets", type=int, nargs="+", default=[512, 1024, 2048, 4096],
+                        help="pass: stencil workgroup targets to sweep")
     parser.add_argument("--no-board", action="store_true", help="two_level: the fixture sizes only")
     parser.add_argument("--preconditioner", default="jacobi", choices=["jacobi", "two-level"],
                         help="batch200: the boards' preconditioner")
@@ -799,6 +873,9 @@ def main() -> int:
         return 0
     if args.workload == "two_level":
         print(json.dumps(run_two_level(args, torch)), flush=True)
+        return 0
+    if args.workload == "pass":
+        print(json.dumps(run_pass(args, torch)), flush=True)
         return 0
     log("building board")
     problem = workloads.board_512x512x16(size=args.size, slabs=args.slabs)

@@ -16,6 +16,9 @@ mask, then sources), so a seed names a board.
 * ``dc_board``, ``serving_pool`` -- the DC copper/dielectric board and the serving mix.
 * ``coupling_scenarios``  -- 16 voltage-driven electro-thermal current loops (seed 6).
 * ``four_slab_fixture``   -- the reference's own thermal benchmark stack (no randomness).
+* ``general_board``       -- the 512 x 512 x 16 geometry with every element's conductivity
+                             jittered and every face cooled (seed 7): no node plane is
+                             free of Robin terms and no element layer is uniform.
 """
 
 from __future__ import annotations
@@ -125,6 +128,44 @@ def board_512x512x16(seed: int = 2, size: int = 512, slabs: int = 16) -> Thermal
     fills = rng.uniform(0.3, 0.6, 2)
     masks = (copper_mask(rng, size, size, fills[0]), copper_mask(rng, size, size, fills[1]))
     return layered_board(slabs, size, size, masks, random_sources(rng, size, size, 8, 0.1, 2.0))
+
+
+def general_board(seed: int = 7, size: int = 512, slabs: int = 16) -> ThermalConductionProblem:
+    """A board on which no stencil shortcut applies: the copper masks and
+    sources of ``board_512x512x16``'s kind, every element's in-plane and
+    through-plane conductivity scaled by its own U(0.8, 1.25) draw (so no
+    element layer is uniform), h = 10 W/m^2 K on the top and bottom faces and
+    on the four sides (so every node plane holds Robin terms), and no fixed
+    node."""
+
+    from thermal.matrix_free_mpir_fem import ExposedFaceConvection
+
+    rng = np.random.default_rng(seed)
+    fills = rng.uniform(0.3, 0.6, 2)
+    masks = (copper_mask(rng, size, size, fills[0]), copper_mask(rng, size, size, fills[1]))
+    sources = random_sources(rng, size, size, 8, 0.1, 2.0)
+    k = np.full((slabs, size, size), FR4_K)
+    k[0] = np.where(masks[0], COPPER_K, FR4_K)
+    k[-1] = np.where(masks[1], COPPER_K, FR4_K)
+    k_in = k * rng.uniform(0.8, 1.25, k.shape)
+    k_z = k * rng.uniform(0.8, 1.25, k.shape)
+    mesh = LayeredThermalMesh(
+        slab_thickness_m=(STACK_M / slabs,) * slabs,
+        pitch_x_m=PITCH_M,
+        pitch_y_m=PITCH_M,
+        conductivity_w_per_m_k=k_in,
+        through_plane_conductivity_w_per_m_k=k_z,
+    )
+    heat = np.zeros(mesh.element_grid_shape)
+    for s in sources:
+        heat[-1, s.row : s.row + s.side, s.col : s.col + s.side] += s.power_w / (s.side * s.side)
+    return ThermalConductionProblem(
+        mesh,
+        convection=(ConvectionBoundary("top", FILM_W_PER_M2_K, AMBIENT_K),
+                    ConvectionBoundary("bottom", FILM_W_PER_M2_K, AMBIENT_K),
+                    ExposedFaceConvection(FILM_W_PER_M2_K, AMBIENT_K, ("-x", "+x", "-y", "+y"))),
+        element_heat_w=heat,
+    )
 
 
 def uneven_board_shapes(count: int = 200, seed: int = 4) -> list[tuple[int, int, int]]:
