@@ -12,6 +12,7 @@ one JSON line.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -502,8 +503,10 @@ def run_pass(args, torch) -> dict:
     t0 = time.perf_counter()
     batch.run()
     solve_s = time.perf_counter() - t0
-    _, result = batch.result(0)
-    log(f"solve {solve_s * 1e3:.1f} ms: {result['outer_iterations']} outer / {result['inner_iterations']} inner")
+    x, result = batch.result(0)
+    digest = hashlib.sha256(np.ascontiguousarray(x).tobytes()).hexdigest()[:16]
+    log(f"solve {solve_s * 1e3:.1f} ms: {result['outer_iterations']} outer / {result['inner_iterations']} inner, "
+        f"solution sha256 {digest}")
     pass_bytes = 40 * n + 12 * e
     sweep = []
     for rows in (4, 8):
@@ -533,6 +536,7 @@ def run_pass(args, torch) -> dict:
                    "pass_bytes": pass_bytes, "apply_us": apply["per_launch_us"], "apply_sweep": apply["sweep"],
                    "apply_algorithmic_gbs": apply["roofline"]["achieved"], "copy_gbs_at_pass_bytes": copy,
                    "solve_ms": solve_s * 1e3, "inner_iterations": result["inner_iterations"],
+                   "solution_sha256_16": digest,
                    "parallelism": "single GPU"},
     }
 

@@ -71,6 +71,45 @@ __device__ __forceinline__ double fmadd(const double a, const double b, const do
     return __builtin_fma(a, b, c);
 }
 
+// Neighbour lanes by a full-wave DPP shift (a VALU move, no LDS crossbar):
+// lane_next gives lane l the value of lane l + 1 (wave_shl:1), lane_prev that
+// of lane l - 1 (wave_shr:1); the lane with no source (63 and 0) keeps its
+// own value.
+__device__ __forceinline__ int lane_next_bits(const int b) {
+    return __builtin_amdgcn_update_dpp(b, b, 0x130, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int lane_prev_bits(const int b) {
+    return __builtin_amdgcn_update_dpp(b, b, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float lane_next(const float v) {
+    return __builtin_bit_cast(float, lane_next_bits(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ float lane_prev(const float v) {
+    return __builtin_bit_cast(float, lane_prev_bits(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double lane_next(const double v) {
+    typedef int i2 __attribute__((ext_vector_type(2)));
+    i2 b = __builtin_bit_cast(i2, v);
+    b.x = lane_next_bits(b.x);
+    b.y = lane_next_bits(b.y);
+    return __builtin_bit_cast(double, b);
+}
+__device__ __forceinline__ double lane_prev(const double v) {
+    typedef int i2 __attribute__((ext_vector_type(2)));
+    i2 b = __builtin_bit_cast(i2, v);
+    b.x = lane_prev_bits(b.x);
+    b.y = lane_prev_bits(b.y);
+    return __builtin_bit_cast(double, b);
+}
+
+// acc + a b accumulated in FP64.  For FP32 operands the FP64 product is
+// exact, so one fused multiply-add rounds exactly as multiply-then-add does;
+// FP64 operands keep the separate multiply and add.
+__device__ __forceinline__ double dot_add(const double acc, const float a, const float b) {
+    return __builtin_fma((double)a, (double)b, acc);
+}
+__device__ __forceinline__ double dot_add(const double acc, const double a, const double b) { return acc + a * b; }
+
 template <typename T>
 struct NodeVal {
     T u;      // operand value at the node (unmasked)
@@ -104,13 +143,40 @@ struct InnerValZ {
 // A stencil's node loader takes (plane base, lane offset, owned) and may
 // take the node's strip row and node layer too (a two-level pass needs its
 // patch); row -1 marks a node reached through a via, found from its index.
+//
+// A loader is either a callable that loads and computes in one step, or a
+// two-phase loader with fetch(...) (the raw loads only) and make(raw) (the
+// arithmetic only): the stencil then issues every load of a plane step
+// before any of them is waited for.
+// std::declval for device code (unevaluated contexts only; never defined).
+template <class T>
+__host__ __device__ T&& any_value() noexcept;
+template <class L, class = void>
+struct HasFetch : std::false_type {};
+template <class L>
+struct HasFetch<L, std::void_t<decltype(any_value<L&>().fetch(0LL, 0u, false, 0, 0))>> : std::true_type {};
+
 template <class Load>
-__device__ __forceinline__ auto call_load(Load& load, const long long pb, const unsigned off, const bool owned,
-                                          const int row, const int layer) {
-    if constexpr (std::is_invocable_v<Load&, long long, unsigned, bool, int, int>)
+__device__ __forceinline__ auto fetch_node(Load& load, const long long pb, const unsigned off, const bool owned,
+                                           const int row, const int layer) {
+    if constexpr (HasFetch<Load>::value)
+        return load.fetch(pb, off, owned, row, layer);
+    else if constexpr (std::is_invocable_v<Load&, long long, unsigned, bool, int, int>)
         return load(pb, off, owned, row, layer);
     else
         return load(pb, off, owned);
+}
+template <class Load, class Raw>
+__device__ __forceinline__ auto make_node(Load& load, const Raw& raw) {
+    if constexpr (HasFetch<Load>::value)
+        return load.make(raw);
+    else
+        return raw;
+}
+template <class Load>
+__device__ __forceinline__ auto call_load(Load& load, const long long pb, const unsigned off, const bool owned,
+                                          const int row, const int layer) {
+    return make_node(load, fetch_node(load, pb, off, owned, row, layer));
 }
 
 // The operand with fixed columns zeroed, for any node-value type with u and robin.
@@ -275,8 +341,10 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
         eoff[j] = (ecol_in && ye >= 0 && ye < R) ? (unsigned)(ye * C + xc) : kOutside;
     }
 
-    // The node-value type is whatever ``load`` returns (NodeVal, InnerVal, InnerValZ).
-    using V = std::decay_t<decltype(call_load(load, 0LL, 0u, false, 0, 0))>;
+    // Raw loads of a node (what fetch returns) and the node value made from
+    // them (NodeVal, InnerVal, InnerValZ).
+    using Raw = std::decay_t<decltype(fetch_node(load, 0LL, 0u, false, 0, 0))>;
+    using V = std::decay_t<decltype(make_node(load, any_value<const Raw&>()))>;
     T mlo[ROWS + 2], nlo[ROWS + 2], mhi[ROWS + 2], nhi[ROWS + 2];
     V vlo[ROWS], vhi[ROWS];
     // The partial of the layer swept last that belongs to the next output
@@ -288,19 +356,25 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
 
     // Planes zlo - 1 and zhi are loaded for the elements between them and the
     // block's output planes but belong to neighbouring z blocks.
-    auto load_plane = [&](const int z, T (&m)[ROWS + 2], T (&n)[ROWS + 2], V (&v)[ROWS],
-                          unsigned (&q)[ROWS + 2][4]) {
+    auto fetch_plane = [&](const int z, Raw (&raw)[ROWS + 2]) {
         const long long pb = (long long)z * NR * NC;
         const bool out_plane = z >= zlo && z < zhi;
 #pragma unroll
         for (int i = 0; i < ROWS + 2; ++i) {
             const bool owned = out_plane && out_lane && i >= 1 && i <= ROWS && y0 - 1 + i <= R;
-            const V nv = call_load(load, pb, noff[i], owned, i, z);
+            raw[i] = fetch_node(load, pb, noff[i], owned, i, z);
+        }
+    };
+    auto make_plane = [&](const Raw (&raw)[ROWS + 2], T (&m)[ROWS + 2], T (&n)[ROWS + 2], V (&v)[ROWS],
+                          unsigned (&q)[ROWS + 2][4]) {
+#pragma unroll
+        for (int i = 0; i < ROWS + 2; ++i) {
+            const V nv = make_node(load, raw[i]);
             m[i] = masked(nv);
             if (i >= 1 && i <= ROWS) v[i - 1] = nv;
         }
 #pragma unroll
-        for (int i = 0; i < ROWS + 2; ++i) n[i] = __shfl_down(m[i], 1);
+        for (int i = 0; i < ROWS + 2; ++i) n[i] = lane_next(m[i]);
         if constexpr (MFMA != 0) {
 #pragma unroll
             for (int i = 0; i < ROWS + 2; ++i) plane_operands<MFMA>(m[i], n[i], q[i]);
@@ -317,7 +391,7 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
             for (int i = 0; i < ROWS; ++i) {
                 const T L = La[i] + Lb[i];
                 const T Rt = Ra[i] + Rb[i];
-                const T total = L + __shfl_up(Rt, 1);
+                const T total = L + lane_prev(Rt);
                 if (out_lane && y0 + i <= R) emit(pb, noff[i + 1], i, total, v[i], parity);
             }
         };
@@ -328,13 +402,9 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
         plane_done(z);
     };
 
-    // Element layer ez between the lo (ez) and hi (ez + 1) planes in
-    // registers: its part of the lo plane (aL, aR) and of the hi plane
-    // (bL, bR), each summed from zero.
-    auto layer = [&](const int ez, T (&aL)[ROWS], T (&aR)[ROWS], T (&bL)[ROWS], T (&bR)[ROWS]) {
-        // The three coefficient rows of the layer (uniform layers as
-        // described at CoefSource).
-        T ca[ROWS + 1], cb[ROWS + 1], cc[ROWS + 1];
+    // The three coefficient rows of element layer ez (loads only).
+    auto fetch_layer = [&](const int ez, T (&ca)[ROWS + 1], T (&cb)[ROWS + 1], T (&cc)[ROWS + 1]) {
+        // Uniform layers as described at CoefSource.
         const T* layer_coef = cs.coef + (long long)ez * R * C;
         const Rsrc ra = rsrc(layer_coef), rb = rsrc(layer_coef + E), rc = rsrc(layer_coef + 2 * E);
         const bool uniform = mask_bit(cs.uniform, ez);
@@ -361,6 +431,13 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
                 cc[j] = ld(rc, eoff[j] & keep, T());
             }
         }
+    };
+
+    // Element layer ez between the lo (ez) and hi (ez + 1) planes in
+    // registers, from its coefficient rows: its part of the lo plane (aL, aR)
+    // and of the hi plane (bL, bR), each summed from zero.
+    auto layer = [&](const T (&ca)[ROWS + 1], const T (&cb)[ROWS + 1], const T (&cc)[ROWS + 1], T (&aL)[ROWS],
+                     T (&aR)[ROWS], T (&bL)[ROWS], T (&bR)[ROWS]) {
 #pragma unroll
         for (int i = 0; i < ROWS; ++i) {
             aL[i] = T(0);
@@ -453,8 +530,13 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
         }
     };
 
+
     // Output planes [zlo, zhi) need element layers zlo - 1 .. zhi - 1 (those
-    // that exist); the layer outside the block only completes a plane.
+    // that exist); the layer outside the block only completes a plane.  Each
+    // step fetches one plane and one layer's coefficients, and every load of
+    // a step is issued before the step's arithmetic: the scheduling barrier
+    // keeps the compiler from sinking loads towards their uses, which left
+    // three dependent round trips to memory per step.
     const int first_layer = zlo > 0 ? zlo - 1 : 0;
     const int last_layer = zhi - 1 < S - 1 ? zhi - 1 : S - 1;
 #pragma unroll
@@ -462,12 +544,29 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
         heldL[i] = T(0);
         heldR[i] = T(0);
     }
-    if constexpr (!DOWN) {
-        load_plane(first_layer, mlo, nlo, vlo, qlo);
-        for (int ez = first_layer; ez <= last_layer; ++ez) {
-            load_plane(ez + 1, mhi, nhi, vhi, qhi);
-            T aL[ROWS], aR[ROWS], bL[ROWS], bR[ROWS];
-            layer(ez, aL, aR, bL, bR);
+    // Step k handles element layer ez(k) and fetches plane pz(k): going up,
+    // ez = first_layer + k and pz = ez + 1; going down, ez = last_layer - k
+    // and pz = ez.
+    const int steps = last_layer - first_layer + 1;
+    {
+        Raw r0[ROWS + 2];
+        fetch_plane(DOWN ? last_layer + 1 : first_layer, r0);
+        if constexpr (DOWN)
+            make_plane(r0, mhi, nhi, vhi, qhi);
+        else
+            make_plane(r0, mlo, nlo, vlo, qlo);
+    }
+    for (int k = 0; k < steps; ++k) {
+        const int ez = DOWN ? last_layer - k : first_layer + k;
+        Raw raw[ROWS + 2];
+        T ca[ROWS + 1], cb[ROWS + 1], cc[ROWS + 1];
+        fetch_layer(ez, ca, cb, cc);
+        fetch_plane(DOWN ? ez : ez + 1, raw);
+        __builtin_amdgcn_sched_barrier(0);
+        T aL[ROWS], aR[ROWS], bL[ROWS], bR[ROWS];
+        if constexpr (!DOWN) {
+            make_plane(raw, mhi, nhi, vhi, qhi);
+            layer(ca, cb, cc, aL, aR, bL, bR);
             if (ez >= zlo) finish_plane(ez, heldL, heldR, aL, aR, vlo);
 #pragma unroll
             for (int i = 0; i < ROWS + 2; ++i) {
@@ -484,17 +583,9 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
                 heldL[i] = bL[i];
                 heldR[i] = bR[i];
             }
-        }
-        if (zhi == S + 1) {
-            const T none[ROWS] = {};
-            finish_plane(S, heldL, heldR, none, none, vlo);
-        }
-    } else {
-        load_plane(last_layer + 1, mhi, nhi, vhi, qhi);
-        for (int ez = last_layer; ez >= first_layer; --ez) {
-            load_plane(ez, mlo, nlo, vlo, qlo);
-            T aL[ROWS], aR[ROWS], bL[ROWS], bR[ROWS];
-            layer(ez, aL, aR, bL, bR);
+        } else {
+            make_plane(raw, mlo, nlo, vlo, qlo);
+            layer(ca, cb, cc, aL, aR, bL, bR);
             if (ez + 1 < zhi) finish_plane(ez + 1, bL, bR, heldL, heldR, vhi);
 #pragma unroll
             for (int i = 0; i < ROWS + 2; ++i) {
@@ -512,10 +603,12 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
                 heldR[i] = aR[i];
             }
         }
-        if (zlo == 0) {
-            const T none[ROWS] = {};
-            finish_plane(0, none, none, heldL, heldR, vhi);
-        }
+    }
+    const T none[ROWS] = {};
+    if constexpr (!DOWN) {
+        if (zhi == S + 1) finish_plane(S, heldL, heldR, none, none, vlo);
+    } else {
+        if (zlo == 0) finish_plane(0, none, none, heldL, heldR, vhi);
     }
 }
 
@@ -564,7 +657,7 @@ __device__ __forceinline__ void sheet_apply(const T* __restrict__ coef, const in
         if (i >= 1 && i <= ROWS) v[i - 1] = nv;
     }
 #pragma unroll
-    for (int i = 0; i < ROWS + 2; ++i) n[i] = __shfl_down(m[i], 1);
+    for (int i = 0; i < ROWS + 2; ++i) n[i] = lane_next(m[i]);
     const T* layer_coef = coef + (long long)layer * R * C;
     const Rsrc rx = rsrc(layer_coef), ry = rsrc(layer_coef + E);
     T Lacc[ROWS], Racc[ROWS];
@@ -593,7 +686,7 @@ __device__ __forceinline__ void sheet_apply(const T* __restrict__ coef, const in
     }
 #pragma unroll
     for (int i = 0; i < ROWS; ++i) {
-        T total = Lacc[i] + __shfl_up(Racc[i], 1);
+        T total = Lacc[i] + lane_prev(Racc[i]);
         if (out_lane && y0 + i <= R) {
             if (via_ptr) {
                 const long long node = pb + noff[i + 1];
@@ -636,6 +729,7 @@ __device__ __forceinline__ void strip_sweep(const int sz, const int nsz, const C
         strip_apply<T, ROWS, MFMA, false, TABLE>(coef, S, R, C, x0, y0, zlo, zhi, load, emit, plane_done);
 }
 
+
 // ------------------------------------------------------------ reductions
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -646,10 +740,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Reduction cells of the stencil kernels.  Every inner product is summed per
 // cell of (node-plane pair, 16 node rows, one 62-column strip): each lane sums
-// the even and the odd plane of the pair apart (in row order) and adds the
-// two, so the z direction of the sweep does not matter; each wave sums its
-// 4-row groups (then a wave butterfly) into a table in LDS, and at the end of
-// the tile the workgroup adds the four groups of each cell in row order and
+// each plane of the pair from zero (its rows in order, into ``t``) and adds
+// the two plane sums (``v``), so the z direction of the sweep does not matter
+// (the one addition of the two planes commutes); each wave sums its 4-row
+// groups (then a wave butterfly) into a table in LDS, and at the end of the
+// tile the workgroup adds the four groups of each cell in row order and
 // stores one partial per cell.  The cells do
 // not depend on the tile shape (rows per wave, planes per z block) or on
 // which other slots share the launch, so a slot's bits depend only on its own
@@ -659,8 +754,9 @@ template <int NV, int ROWS>
 struct CellSums {
     static constexpr int kGroups = ROWS / kCellRows;  // 4-row groups per wave
     static constexpr int kEntries = kWavesPerGroup * kGroups;
-    double v[NV][kGroups][2];  // per-lane accumulators, by plane parity
-    int planes;                // planes of the current pair emitted so far
+    double t[NV][kGroups];  // per-lane sums of the current plane
+    double v[NV][kGroups];  // per-lane sums of the pair's planes folded so far
+    int planes;             // planes of the current pair folded so far
 
     __device__ static double (&table())[kMaxTilePairs][NV][kEntries] {
         __shared__ double sums[kMaxTilePairs][NV][kEntries];
@@ -671,10 +767,22 @@ struct CellSums {
         for (int k = 0; k < NV; ++k)
 #pragma unroll
             for (int g = 0; g < kGroups; ++g) {
-                v[k][g][0] = 0.0;
-                v[k][g][1] = 0.0;
+                t[k][g] = 0.0;
+                v[k][g] = 0.0;
             }
         planes = 0;
+    }
+    // The plane's sums join the pair: the first plane's as they are, the
+    // second's by one addition.
+    __device__ void fold() {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g) {
+                v[k][g] = planes == 0 ? t[k][g] : v[k][g] + t[k][g];
+                t[k][g] = 0.0;
+            }
+        ++planes;
     }
     __device__ void store_groups(const int pair, const int groups) {
         const int wave = threadIdx.x / kWave;
@@ -683,7 +791,7 @@ struct CellSums {
             if (g >= groups) break;
 #pragma unroll
             for (int k = 0; k < NV; ++k) {
-                const double s = wave_sum(v[k][g][0] + v[k][g][1]);
+                const double s = wave_sum(v[k][g]);
                 if ((threadIdx.x & (kWave - 1)) == 0) table()[pair][k][wave * kGroups + g] = s;
             }
         }
@@ -693,10 +801,14 @@ struct CellSums {
     // in either direction: the pair is complete after its second plane (the
     // last plane S alone when S is even).
     __device__ void plane_done(const int z, const int zlo, const int S) {
-        if (++planes == ((z | 1) <= S ? 2 : 1)) store_groups(z / kCellPlanes - zlo / kCellPlanes, kGroups);
+        fold();
+        if (planes == ((z | 1) <= S ? 2 : 1)) store_groups(z / kCellPlanes - zlo / kCellPlanes, kGroups);
     }
     // Sheet kernels: the wave's one 4-row group of its layer.
-    __device__ void sheet_done() { store_groups(0, 1); }
+    __device__ void sheet_done() {
+        fold();
+        store_groups(0, 1);
+    }
     // End of a tile, reached by every wave of the workgroup: the cells of
     // ``pairs`` plane pairs from pair pz0 (sheet: one layer, pz0 = layer) and
     // of the workgroup's rows from y0 on.  Group j of a cell is table entry
@@ -738,7 +850,7 @@ struct CellSums {
 template <int NV, int THREADS>
 __device__ void slot_sum(const double* __restrict__ part, const int stride, const int count, double (&total)[NV]) {
     constexpr int kWaves = THREADS / kWave;
-    constexpr int kUnroll = 4;
+    constexpr int kUnroll = 8;
     __shared__ double waves[kWaves][NV];
     const int tid = threadIdx.x;
     double s[NV];
@@ -987,7 +1099,6 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
         };
         auto emit = [&](const long long pb, const unsigned off, const int row, const double conduction,
                         const NodeVal<double>& v, auto parity) {
-            constexpr int P = decltype(parity)::value;
             if (pending != kPendingNone) stt(xout + pb, off, v.u);
             const double ax = v.robin >= 0.0 ? conduction + v.robin * v.u : v.u;
             const double bv = ldt(b + pb, off);
@@ -995,10 +1106,10 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
             const TI ri = (TI)r;
             stt(rout + pb, off, ri);
             const int g = row / kCellRows;
-            acc.v[0][g][P] += bv * bv;
-            acc.v[1][g][P] += r * r;
-            acc.v[2][g][P] += (double)ri * (double)ri;
-            acc.v[3][g][P] += (double)ri * (double)(ri * ldt(dinv + pb, off));
+            acc.t[0][g] += bv * bv;
+            acc.t[1][g] += r * r;
+            acc.t[2][g] = dot_add(acc.t[2][g], ri, ri);
+            acc.t[3][g] = dot_add(acc.t[3][g], ri, ri * ldt(dinv + pb, off));
         };
         const int y_group = tile.sy * kWavesPerGroup * kRowsOuter;
         if (op.kind == kSheet) {
@@ -1098,6 +1209,79 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish_outer(const SlotDev* 
 // on ||r_{i-1}||, the curvature of p_i, alpha_i = r.z / p.q) and forms
 // beta_{i+1} from r_i.z_i = r.z - 2 alpha q.z + alpha^2 q.D^-1 q, the one
 // value the pass cannot yet have summed.
+// The fused pass's node loader, in two phases (see fetch_node): fetch issues
+// the node's loads of r, q, p, 1/diag, the correction (owned nodes only;
+// elsewhere the offset is out of range and reads zero) and Robin (through
+// the out-of-range offset on a plain plane), and for a two-level slot the
+// coarse term w at its patch; make forms
+//   r = r - alpha q,  z = r D^-1 (+ w),  p = z + beta p,  c = keep c + alpha p.
+template <typename TI>
+struct PassRaw {
+    TI rv, qv, pv, d, cv, rb;
+};
+template <typename TI>
+struct PassRawZ {
+    TI rv, qv, pv, d, cv, rb;
+    double w;
+    bool in;
+};
+template <typename TI, bool COARSE>
+struct PassLoader {
+    const TI* rin;
+    const TI* qin;
+    const TI* pin;
+    const TI* c;
+    const TI* dinv;
+    const TI* robin;
+    unsigned long long plain;
+    TI alpha, beta, keep;
+    const double* cw;
+    int crows, ccols, cblock, R, C;
+    int ccol;
+    const int* crow;  // patch row of each strip row (two-level slots)
+
+    __device__ auto fetch(const long long pb, const unsigned off, const bool owned, const int row,
+                          const int layer) const {
+        const TI rv = ldt(rin + pb, off), qv = ldt(qin + pb, off), pv = ldt(pin + pb, off);
+        const TI d = ldt(dinv + pb, off);
+        const TI cv = ldt(c + pb, owned ? off : kOutside);
+        const TI rb = ldt(robin + pb, mask_bit(plain, layer) ? kOutside : off);
+        if constexpr (COARSE) {
+            // The patch index does not depend on loaded data and is always in
+            // range, so w is fetched with the node's other loads.
+            int patch;
+            if (row >= 0) {
+                patch = (layer * crows + crow[row]) * ccols + ccol;
+            } else {
+                // A via's far end (always inside the grid): its layer, row and
+                // column from the index.
+                const long long plane = (long long)(R + 1) * (C + 1);
+                const long long n = pb + off;
+                const int l = (int)(n / plane);
+                const int rem = (int)(n - l * plane);
+                const int y = rem / (C + 1), x = rem - y * (C + 1);
+                patch = (l * crows + y / cblock) * ccols + x / cblock;
+            }
+            return PassRawZ<TI>{rv, qv, pv, d, cv, rb, cw[patch], off != kOutside};
+        } else {
+            return PassRaw<TI>{rv, qv, pv, d, cv, rb};
+        }
+    }
+    __device__ InnerVal<TI> make(const PassRaw<TI>& x) const {
+        const TI r = x.rv - alpha * x.qv;
+        const TI p = r * x.d + beta * x.pv;
+        return InnerVal<TI>{p, x.rb, r, x.d, keep * x.cv + alpha * x.pv};
+    }
+    __device__ InnerValZ<TI> make(const PassRawZ<TI>& x) const {
+        const TI r = x.rv - alpha * x.qv;
+        // The free mask is applied after the load.
+        const TI wv = (x.in && x.rb >= TI(0)) ? (TI)x.w : TI(0);
+        const TI z = r * x.d + wv;
+        const TI p = z + beta * x.pv;
+        return InnerValZ<TI>{p, x.rb, r, x.d, keep * x.cv + alpha * x.pv, z};
+    }
+};
+
 template <typename TI, int ROWS, int MFMA, bool COARSE = false>
 __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotState& st, const Tile& tile) {
     const OperatorDev& op = sd.op;
@@ -1138,43 +1322,11 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
     }
     // The correction is loaded at owned nodes only (elsewhere the offset is
     // out of range and reads zero).
-    auto load = [&](const long long pb, const unsigned off, const bool owned, const int row, const int layer) {
-        const TI rv = ldt(rin + pb, off), qv = ldt(qin + pb, off), pv = ldt(pin + pb, off);
-        const TI d = ldt(dinv + pb, off);
-        const TI cv = ldt(c + pb, owned ? off : kOutside);
-        const TI rb = ldt(robin + pb, mask_bit(op.plain, layer) ? kOutside : off);
-        const TI r = rv - alpha * qv;
-        if constexpr (COARSE) {
-            // The patch index does not depend on loaded data and is always in
-            // range, so w is fetched with the node's other loads and the free
-            // mask applied afterwards.
-            int patch;
-            if (row >= 0) {
-                patch = (layer * op.crows + crow[row]) * op.ccols + ccol;
-            } else {
-                // A via's far end (always inside the grid): its layer, row and
-                // column from the index.
-                const long long plane = (long long)(op.R + 1) * (op.C + 1);
-                const long long n = pb + off;
-                const int l = (int)(n / plane);
-                const int rem = (int)(n - l * plane);
-                const int y = rem / (op.C + 1), x = rem - y * (op.C + 1);
-                patch = (l * op.crows + y / op.cblock) * op.ccols + x / op.cblock;
-            }
-            const double wraw = sd.cw[patch];
-            const TI wv = (off != kOutside && rb >= TI(0)) ? (TI)wraw : TI(0);
-            const TI z = r * d + wv;
-            const TI p = z + beta * pv;
-            return InnerValZ<TI>{p, rb, r, d, keep * cv + alpha * pv, z};
-        } else {
-            const TI p = r * d + beta * pv;
-            return InnerVal<TI>{p, rb, r, d, keep * cv + alpha * pv};
-        }
-    };
+    const PassLoader<TI, COARSE> load{rin, qin, pin, c, dinv, robin, op.plain, alpha, beta, keep, sd.cw, op.crows,
+                                      op.ccols, op.cblock, op.R, op.C, ccol, crow};
     // Stores wait for their plane's emit, so a plane's loads issue back to back.
     auto emit = [&](const long long pb, const unsigned off, const int row, const TI conduction, const auto& v,
                     auto parity) {
-        constexpr int P = decltype(parity)::value;
         const TI ap = v.robin >= TI(0) ? conduction + v.robin * v.u : v.u;
         stt(pout + pb, off, v.u);
         stt(qout + pb, off, ap);
@@ -1186,11 +1338,11 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
         else
             z = v.r * v.d;
         const int g = row / kCellRows;
-        acc.v[0][g][P] += (double)v.u * (double)ap;
-        acc.v[1][g][P] += (double)v.r * (double)z;
-        acc.v[2][g][P] += (double)ap * (double)z;
-        acc.v[3][g][P] += (double)ap * (double)(ap * v.d);
-        acc.v[4][g][P] += (double)v.r * (double)v.r;
+        acc.t[0][g] = dot_add(acc.t[0][g], v.u, ap);
+        acc.t[1][g] = dot_add(acc.t[1][g], v.r, z);
+        acc.t[2][g] = dot_add(acc.t[2][g], ap, z);
+        acc.t[3][g] = dot_add(acc.t[3][g], ap, ap * v.d);
+        acc.t[4][g] = dot_add(acc.t[4][g], v.r, v.r);
     };
     if (op.kind == kSheet) {
         // Sheet tiles are one layer of 4-row wave strips.
@@ -1204,7 +1356,8 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
         return;
     }
     if (y0 <= op.R) {
-        strip_sweep<TI, ROWS, MFMA>(tile.sz, (op.S + tile.zb) / tile.zb, InnerOf<TI>::coefs(op), op.S, op.R, op.C,
+        strip_sweep<TI, ROWS, MFMA>(
+            tile.sz, (op.S + tile.zb) / tile.zb, InnerOf<TI>::coefs(op), op.S, op.R, op.C,
                                     x0, y0, zlo, zhi, load, emit, [&](const int z) { acc.plane_done(z, zlo, op.S); });
     }
     acc.flush_hex(zlo, zhi, tile.sy * kWavesPerGroup * ROWS, tile.sx, op.R, op.C, sd.part, sd.part_stride);
