@@ -623,6 +623,11 @@ def run_serve(args, torch) -> dict:
             "peak_admitted_gb": server.peak_bytes / 1e9,
             "replays": info["replays"],
             "graph_builds": info["graph_builds"],
+            "pumps": server.pumps,
+            "pump_s": server.pump_s,
+            "native_step_s": server.step_s,
+            "blocked_on_device_s": info["wait_ms"] / 1e3,
+            "device_replay_s": info["replay_ms"] / 1e3,
         }
         log(json.dumps(line))
         lines.append(line)

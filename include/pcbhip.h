@@ -251,6 +251,8 @@ typedef struct pcbhip_server_info {
     int64_t replays;
     int64_t graph_builds;
     int steps_per_replay;
+    double wait_ms;        /* host time step() spent blocked on a replay in flight */
+    double replay_ms;      /* device time of the retired replays (event to event) */
 } pcbhip_server_info;
 
 int pcbhip_server_create(int device, int max_slots, int inner_fp64, int steps_per_replay, pcbhip_server **out);

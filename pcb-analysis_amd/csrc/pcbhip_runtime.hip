@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -716,7 +717,7 @@ void build_graph(pcbhip_batch* b, int steps) {
                                      b->d_counts, b->n_outer, b->d_active, b->coarse(), b->stream);
         if (err == hipSuccess)
             err = launch::inner_step(b->inner64 != 0, b->rows_inner, n_slots, b->d_slots, b->d_states, b->d_configs,
-                                     b->d_inner, b->d_counts, std::max(b->n_inner, 1), false, b->d_block,
+                                     b->d_inner, b->d_counts, std::max(b->n_inner, 1), false, nullptr, b->d_block,
                                      b->d_groups, b->n_block, b->block_mfma, b->d_active, b->coarse(), b->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
@@ -737,7 +738,11 @@ void build_graph(pcbhip_batch* b, int steps) {
 
 namespace {
 
-enum SlotStatus : int { kSlotFree = 0, kSlotQueued = 1, kSlotRunning = 2, kSlotFinished = 3 };
+// A slot whose solve has ended is first "copying": its solution and history
+// are copied into pinned host memory on the replay stream, ahead of the next
+// replay, and it becomes "finished" (collectable without any device wait)
+// once a replay launched after those copies has been retired.
+enum SlotStatus : int { kSlotFree = 0, kSlotQueued = 1, kSlotRunning = 2, kSlotFinished = 3, kSlotCopying = 4 };
 
 struct ServerSlot {
     SlotHost h;
@@ -747,6 +752,12 @@ struct ServerSlot {
     long long first_replay = 0;  // sequence number of the first replay that runs it
     long long cap_steps = 0;     // steps after which the slot must have ended
     SlotState final_state{};
+    // Pinned result buffer (solution, then history rows) from the server's
+    // pinned pool, while copying or finished.
+    double* host = nullptr;
+    size_t host_bytes = 0;
+    bool copy_enqueued = false;
+    long long copy_seq = 0;  // the replay launched right after the result copies
 };
 
 // Pinned sources of one set of table uploads.  Two sets are used in turn;
@@ -783,6 +794,7 @@ struct pcbhip_server {
     hipStream_t copy_stream = nullptr;  // admissions and results
     std::vector<ServerSlot> slots;
     std::vector<std::pair<void*, size_t>> pool;  // released slot blocks
+    std::vector<std::pair<void*, size_t>> pinned_pool;  // released pinned result buffers
     size_t bytes_in_use = 0, bytes_pooled = 0;
     SlotDev* d_slots = nullptr;
     SlotState* d_states = nullptr;
@@ -791,15 +803,19 @@ struct pcbhip_server {
     int* d_fresh_index = nullptr;
     int* d_active = nullptr;  // decremented by the finish kernels; the server reads states instead
     int* d_counts = nullptr;  // outer, inner, (unused), (unused), patches, coarse rows
+    int* d_tickets = nullptr;  // tile tickets of the looping pass (kPassTickets)
     Tile* d_tiles = nullptr;  // outer | inner | patches | coarse rows tables at their capacities
     int cap_outer = 0, cap_inner = 0, cap_patches = 0, cap_rows = 0;
     Staging stage[2];
     int stage_next = 0;
     SlotState* h_states = nullptr;  // pinned, two snapshots of max_slots states
     hipEvent_t replay_done[2] = {nullptr, nullptr};
+    hipEvent_t replay_start[2] = {nullptr, nullptr};  // timing pairs of the replays in flight
     std::vector<Replay> inflight;  // oldest first, at most two
     long long seq = 0;             // replays launched
     long long graph_builds = 0;
+    double wait_s = 0.0;           // host time blocked on replays in flight (the device was the bottleneck)
+    double replay_s = 0.0;         // device time of the retired replays, start to end
     bool tables_dirty = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -808,11 +824,14 @@ struct pcbhip_server {
         if (stream) (void)hipStreamSynchronize(stream);
         if (copy_stream) (void)hipStreamSynchronize(copy_stream);
         (void)graph_release(exec, graph);
-        for (ServerSlot& s : slots)
+        for (ServerSlot& s : slots) {
             if (s.h.block) (void)device_free(s.h.block);
+            if (s.host) (void)pinned_free(s.host);
+        }
         for (auto& b : pool) (void)device_free(b.first);
+        for (auto& b : pinned_pool) (void)pinned_free(b.first);
         for (void* p : {(void*)d_slots, (void*)d_states, (void*)d_configs, (void*)d_fresh, (void*)d_fresh_index,
-                        (void*)d_active, (void*)d_counts, (void*)d_tiles})
+                        (void*)d_active, (void*)d_counts, (void*)d_tickets, (void*)d_tiles})
             if (p) (void)device_free(p);
         for (Staging& st : stage) {
             for (void* p : {(void*)st.counts, (void*)st.tiles, (void*)st.devs, (void*)st.cfgs, (void*)st.fresh,
@@ -822,6 +841,8 @@ struct pcbhip_server {
         }
         if (h_states) (void)pinned_free(h_states);
         for (hipEvent_t e : replay_done)
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : replay_start)
             if (e) (void)hipEventDestroy(e);
         if (copy_stream) (void)stream_destroy(copy_stream);
         if (stream) (void)stream_destroy(stream);
@@ -847,19 +868,75 @@ T* pinned_alloc(size_t count) {
 void server_retire_oldest(pcbhip_server* sv) {
     const Replay r = sv->inflight.front();
     sv->inflight.erase(sv->inflight.begin());
+    const auto t0 = std::chrono::steady_clock::now();
     check(hipEventSynchronize(sv->replay_done[r.parity]), "replay");
+    sv->wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, sv->replay_start[r.parity], sv->replay_done[r.parity]) == hipSuccess)
+        sv->replay_s += 1e-3 * ms;
     const SlotState* snap = sv->h_states + (size_t)r.parity * sv->max_slots;
     for (int i = 0; i < sv->max_slots; ++i) {
         ServerSlot& s = sv->slots[(size_t)i];
+        // Result copies enqueued before this replay are complete.
+        if (s.status == kSlotCopying && s.copy_enqueued && s.copy_seq <= r.seq) s.status = kSlotFinished;
         if (s.status != kSlotRunning || s.first_replay > r.seq) continue;
         if (snap[i].phase == kPhaseDone) {
-            s.status = kSlotFinished;
+            s.status = kSlotCopying;
+            s.copy_enqueued = false;
             s.final_state = snap[i];
             sv->tables_dirty = true;
         } else if ((r.seq - s.first_replay + 1) * sv->steps > s.cap_steps) {
             fail(PCBHIP_RUNTIME, "MPIR iteration did not terminate within its limits");
         }
     }
+}
+
+// A pinned host buffer of at least ``bytes`` from the server's pool (best
+// fit), else a new one.
+double* server_take_pinned(pcbhip_server* sv, size_t bytes, size_t* got) {
+    size_t best = sv->pinned_pool.size();
+    for (size_t i = 0; i < sv->pinned_pool.size(); ++i)
+        if (sv->pinned_pool[i].second >= bytes &&
+            (best == sv->pinned_pool.size() || sv->pinned_pool[i].second < sv->pinned_pool[best].second))
+            best = i;
+    if (best < sv->pinned_pool.size()) {
+        void* p = sv->pinned_pool[best].first;
+        *got = sv->pinned_pool[best].second;
+        sv->pinned_pool.erase(sv->pinned_pool.begin() + (long)best);
+        return static_cast<double*>(p);
+    }
+    *got = bytes;
+    return pinned_alloc<double>(bytes / sizeof(double));
+}
+
+// Enqueues, on the replay stream, the result copies of the slots that ended
+// since the last call; they complete before the next replay starts.
+void server_enqueue_copies(pcbhip_server* sv) {
+    for (ServerSlot& s : sv->slots) {
+        if (s.status != kSlotCopying || s.copy_enqueued) continue;
+        const size_t nx = sizeof(double) * (size_t)s.h.op->N;
+        const size_t nh = sizeof(double) * 4 * (size_t)std::max(s.h.history_rows, 1);
+        if (!s.host) s.host = server_take_pinned(sv, nx + nh, &s.host_bytes);
+        check(hipMemcpyAsync(s.host, s.h.x[s.final_state.xcur], nx, hipMemcpyDeviceToHost, sv->stream), "result copy");
+        if (s.final_state.outer > 0)
+            check(hipMemcpyAsync(s.host + s.h.op->N, s.h.history, sizeof(double) * 4 * (size_t)s.final_state.outer,
+                                 hipMemcpyDeviceToHost, sv->stream),
+                  "history copy");
+        s.copy_enqueued = true;
+        s.copy_seq = sv->seq;
+    }
+}
+
+// Copies enqueued with no replay after them: wait for them here.
+void server_settle_copies(pcbhip_server* sv) {
+    bool pending = false;
+    for (const ServerSlot& s : sv->slots) pending = pending || (s.status == kSlotCopying && s.copy_enqueued);
+    if (!pending) return;
+    const auto t0 = std::chrono::steady_clock::now();
+    check(hipStreamSynchronize(sv->stream), "result copies");
+    sv->wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (ServerSlot& s : sv->slots)
+        if (s.status == kSlotCopying && s.copy_enqueued) s.status = kSlotFinished;
 }
 
 void server_drain(pcbhip_server* sv) {
@@ -889,7 +966,7 @@ void server_build_graph(pcbhip_server* sv) {
         if (err == hipSuccess)
             err = launch::inner_step(sv->inner64 != 0, sv->rows_inner, sv->max_slots, sv->d_slots, sv->d_states,
                                      sv->d_configs, inner, sv->d_counts, std::min(sv->cap_inner, kPassGroups), true,
-                                     nullptr, nullptr, 0, 0, sv->d_active, coarse, sv->stream);
+                                     sv->d_tickets, nullptr, nullptr, 0, 0, sv->d_active, coarse, sv->stream);
         if (err != hipSuccess) {
             hipGraph_t dropped = nullptr;
             (void)hipStreamEndCapture(sv->stream, &dropped);
@@ -979,7 +1056,7 @@ void server_upload(pcbhip_server* sv, const std::vector<int>& admitted) {
     }
     for (int i = 0; i < sv->max_slots; ++i) {
         const ServerSlot& s = sv->slots[(size_t)i];
-        const bool bound = s.status == kSlotRunning || s.status == kSlotFinished;
+        const bool bound = s.status == kSlotRunning || s.status == kSlotFinished || s.status == kSlotCopying;
         st.devs[i] = bound ? slot_dev(s.h) : SlotDev{};
         st.cfgs[i] = bound ? slot_config(s.h.config) : SlotConfig{};
     }
@@ -1017,6 +1094,7 @@ void server_upload(pcbhip_server* sv, const std::vector<int>& admitted) {
 
 void server_launch(pcbhip_server* sv) {
     const int parity = (int)(sv->seq & 1);
+    check(hipEventRecord(sv->replay_start[parity], sv->stream), "event");
     check(hipGraphLaunch(sv->exec, sv->stream), "graph launch");
     check(hipMemcpyAsync(sv->h_states + (size_t)parity * sv->max_slots, sv->d_states,
                          sizeof(SlotState) * sv->max_slots, hipMemcpyDeviceToHost, sv->stream),
@@ -1835,7 +1913,7 @@ int pcbhip_batch_time_pass(pcbhip_batch* b, int launches, double* avg_us) {
         if (err == hipSuccess) err = hipEventRecord(ev[0], b->stream);
         for (int k = 0; k < launches && err == hipSuccess; ++k)
             err = launch::inner_pass(b->inner64 != 0, b->rows_inner, b->d_slots, b->d_states, b->d_inner, b->d_counts,
-                                     std::max(b->n_inner, 1), false, b->d_block, b->d_groups, b->n_block,
+                                     std::max(b->n_inner, 1), false, nullptr, b->d_block, b->d_groups, b->n_block,
                                      b->block_mfma, b->coarse(), b->stream);
         if (err == hipSuccess) err = hipEventRecord(ev[1], b->stream);
         if (err == hipSuccess) err = hipEventSynchronize(ev[1]);
@@ -1938,6 +2016,7 @@ int pcbhip_server_create(int device, int max_slots, int inner_fp64, int steps_pe
         sv->d_fresh_index = static_cast<int*>(device_alloc(sizeof(int) * n));
         sv->d_active = static_cast<int*>(device_alloc(sizeof(int)));
         sv->d_counts = static_cast<int*>(device_alloc(6 * sizeof(int)));
+        sv->d_tickets = static_cast<int*>(device_alloc(kPassTickets * sizeof(int)));
         for (Staging& st : sv->stage) {
             st.counts = pinned_alloc<int>(6);
             st.devs = pinned_alloc<SlotDev>(n);
@@ -1947,7 +2026,8 @@ int pcbhip_server_create(int device, int max_slots, int inner_fp64, int steps_pe
             check(hipEventCreateWithFlags(&st.used, hipEventDisableTiming), "event create");
         }
         sv->h_states = pinned_alloc<SlotState>(2 * n);
-        for (hipEvent_t& e : sv->replay_done) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        for (hipEvent_t& e : sv->replay_done) check(hipEventCreate(&e), "event");
+        for (hipEvent_t& e : sv->replay_start) check(hipEventCreate(&e), "event");
         // Every slot starts finished, so the finish kernels skip empty slots.
         std::vector<SlotState> done(n);
         for (SlotState& st : done) {
@@ -1959,6 +2039,7 @@ int pcbhip_server_create(int device, int max_slots, int inner_fp64, int steps_pe
         check(hipMemsetAsync(sv->d_slots, 0, sizeof(SlotDev) * n, sv->stream), "memset");
         check(hipMemsetAsync(sv->d_configs, 0, sizeof(SlotConfig) * n, sv->stream), "memset");
         check(hipMemsetAsync(sv->d_counts, 0, 6 * sizeof(int), sv->stream), "memset");
+        check(hipMemsetAsync(sv->d_tickets, 0, kPassTickets * sizeof(int), sv->stream), "memset");
         check(hipMemsetAsync(sv->d_active, 0, sizeof(int), sv->stream), "memset");
         check(hipStreamSynchronize(sv->stream), "server create");
         *out = sv.release();
@@ -2073,17 +2154,20 @@ int pcbhip_server_step(pcbhip_server* sv, int* running, int* finished) {
         for (const ServerSlot& s : sv->slots) n_running += s.status == kSlotRunning ? 1 : 0;
         if (n_running == 0) {
             server_drain(sv);
+            server_enqueue_copies(sv);
+            server_settle_copies(sv);
         } else {
             if (sv->tables_dirty || !sv->exec) {
                 server_upload(sv, admitted);
                 // server_upload may have drained: slots it finished leave at the next call.
                 for (int i : admitted) sv->slots[(size_t)i].first_replay = sv->seq;
             }
+            server_enqueue_copies(sv);
             server_launch(sv);
         }
         int r = 0, f = 0;
         for (const ServerSlot& s : sv->slots) {
-            r += s.status == kSlotRunning ? 1 : 0;
+            r += (s.status == kSlotRunning || s.status == kSlotCopying) ? 1 : 0;
             f += s.status == kSlotFinished ? 1 : 0;
         }
         if (running) *running = r;
@@ -2113,17 +2197,10 @@ int pcbhip_server_collect(pcbhip_server* sv, int slot, double* x, pcbhip_mpir_re
         if (s.status != kSlotFinished) fail(PCBHIP_INVALID_INPUT, "slot has no finished result");
         set_device(sv->device);
         const SlotState& st = s.final_state;
-        // The slot's vectors are final once its state reads done: no later
-        // kernel touches them, so the copy does not wait for the replays.
-        hipError_t err = hipSuccess;
-        if (x)
-            err = hipMemcpyAsync(x, s.h.x[st.xcur], sizeof(double) * (size_t)s.h.op->N, hipMemcpyDeviceToHost,
-                                 sv->copy_stream);
-        if (err == hipSuccess && history && st.outer > 0)
-            err = hipMemcpyAsync(history, s.h.history, sizeof(double) * 4 * (size_t)st.outer, hipMemcpyDeviceToHost,
-                                 sv->copy_stream);
-        if (err == hipSuccess) err = hipStreamSynchronize(sv->copy_stream);
-        check(err, "result copy");
+        // The result copies completed before a replay that has been retired,
+        // so the pinned buffer holds the final solution and history.
+        if (x) std::memcpy(x, s.host, sizeof(double) * (size_t)s.h.op->N);
+        if (history && st.outer > 0) std::memcpy(history, s.host + s.h.op->N, sizeof(double) * 4 * (size_t)st.outer);
         if (result) {
             result->converged = st.converged;
             result->outer_iterations = st.outer;
@@ -2138,6 +2215,7 @@ int pcbhip_server_collect(pcbhip_server* sv, int slot, double* x, pcbhip_mpir_re
         sv->pool.emplace_back(s.h.block, s.block_bytes);
         sv->bytes_pooled += s.block_bytes;
         sv->bytes_in_use -= s.block_bytes;
+        sv->pinned_pool.emplace_back(s.host, s.host_bytes);
         const bool not_spd = st.status == kStatusNotSpd;
         s = ServerSlot{};
         if (not_spd) fail(PCBHIP_RUNTIME, "inner PCG requires a finite symmetric positive-definite operator");
@@ -2150,7 +2228,7 @@ int pcbhip_server_stats(const pcbhip_server* sv, pcbhip_server_info* out) {
         pcbhip_server_info st{};
         for (const ServerSlot& s : sv->slots) {
             st.queued += s.status == kSlotQueued ? 1 : 0;
-            st.running += s.status == kSlotRunning ? 1 : 0;
+            st.running += (s.status == kSlotRunning || s.status == kSlotCopying) ? 1 : 0;
             st.finished += s.status == kSlotFinished ? 1 : 0;
             st.free_slots += s.status == kSlotFree ? 1 : 0;
         }
@@ -2159,6 +2237,8 @@ int pcbhip_server_stats(const pcbhip_server* sv, pcbhip_server_info* out) {
         st.replays = sv->seq;
         st.graph_builds = sv->graph_builds;
         st.steps_per_replay = sv->steps;
+        st.wait_ms = 1e3 * sv->wait_s;
+        st.replay_ms = 1e3 * sv->replay_s;
         *out = st;
     });
 }
@@ -2170,6 +2250,8 @@ int pcbhip_server_trim(pcbhip_server* sv) {
         for (auto& b : sv->pool) check(device_free(b.first), "hipFree");
         sv->pool.clear();
         sv->bytes_pooled = 0;
+        for (auto& b : sv->pinned_pool) check(pinned_free(b.first), "pinned free");
+        sv->pinned_pool.clear();
     });
 }
 

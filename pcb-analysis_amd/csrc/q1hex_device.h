@@ -36,8 +36,11 @@ constexpr int kTargetGroups = 1024;
 // Workgroups of the persistent outer-residual kernel.
 constexpr int kOuterGroups = 512;
 // Workgroups a server launches for the fused PCG pass over its tile table
-// (two per CU; the pass loops over the live entries).
+// (two per CU; the pass takes the live entries by ticket, eight ticket
+// counters per pass kernel, one per XCD, for the Jacobi and the two-level
+// pass).
 constexpr int kPassGroups = 512;
+constexpr int kPassTickets = 16;
 // Threads of the one-workgroup-per-slot finish kernels.
 constexpr int kFinishThreads = 512;
 
@@ -226,7 +229,9 @@ hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t st
 // row tables (entries 4 and 5).  The grids may be longer than the live
 // tables (workgroups past the count return at once), so a captured graph
 // survives changes of the tables; with ``loop`` the inner grid may also be
-// shorter and its workgroups loop over the live inner tiles.  Block tiles (a
+// shorter and its workgroups take the live inner tiles by ticket from
+// ``tickets`` (kPassTickets device ints, zero before the first pass; the
+// finish kernel of every step zeroes them again).  Block tiles (a
 // slot group per tile, ``groups``) are launched when grid_block > 0, with
 // the element action block_mfma (0, 2, 3 as in apply).
 // Two-level slots: ``coarse`` describes the coarse kernels' tables (counts
@@ -243,14 +248,14 @@ hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState
                       const CoarseLaunch& coarse, hipStream_t stream);
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
                       const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner, bool loop,
-                      const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma, int* active,
-                      const CoarseLaunch& coarse, hipStream_t stream);
+                      int* tickets, const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma,
+                      int* active, const CoarseLaunch& coarse, hipStream_t stream);
 // The fused PCG pass alone (inner_step without its coarse kernels and its
 // finish kernel).
 hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const SlotState* states,
-                      const Tile* inner_tiles, const int* counts, int grid_inner, bool loop, const Tile* block_tiles,
-                      const SlotGroup* groups, int grid_block, int block_mfma, const CoarseLaunch& coarse,
-                      hipStream_t stream);
+                      const Tile* inner_tiles, const int* counts, int grid_inner, bool loop, int* tickets,
+                      const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma,
+                      const CoarseLaunch& coarse, hipStream_t stream);
 // z = M^-1 r of one operator in FP32 (Jacobi, plus the coarse correction
 // when the operator has one); g and w are nc-long FP64 scratch vectors.
 hipError_t precondition(const OperatorDev& op, const float* r, float* z, double* g, double* w, hipStream_t stream);

@@ -1374,11 +1374,29 @@ template <typename TI, int ROWS, bool LOOP, bool COARSE>
 __global__ __launch_bounds__(kGroupThreads) void k_inner_step(const SlotDev* __restrict__ slots,
                                                               const SlotState* __restrict__ states,
                                                               const Tile* __restrict__ tiles,
-                                                              const int* __restrict__ counts) {
+                                                              const int* __restrict__ counts, int* tickets) {
     const int count = counts[1];
     if constexpr (LOOP) {
-        for (int i = blockIdx.x; i < count; i += gridDim.x) {
-            const Tile tile = tiles[xcd_tile(i, count)];
+        // Workgroups take tiles by ticket, so a workgroup that drew short
+        // tiles takes more of them and none waits on a precomputed share.
+        // Workgroup b draws from the contiguous range of XCD b % 8 (the
+        // dispatcher deals workgroups to the XCDs in turn), so neighbouring
+        // tiles still meet in one L2.  The finish kernel that follows every
+        // pass resets the tickets.
+        constexpr int kXcds = 8;
+        const int xcd = blockIdx.x % kXcds;
+        const int q = count / kXcds, rem = count % kXcds;
+        const int start = xcd * q + (xcd < rem ? xcd : rem);
+        const int len = q + (xcd < rem ? 1 : 0);
+        int* ticket = tickets + (COARSE ? kXcds : 0) + xcd;
+        __shared__ int drawn;
+        for (;;) {
+            if (threadIdx.x == 0) drawn = atomicAdd(ticket, 1);
+            __syncthreads();
+            const int j = drawn;
+            __syncthreads();
+            if (j >= len) break;
+            const Tile tile = tiles[start + j];
             const SlotState& st = states[tile.slot];
             if (st.phase != kPhaseInner) continue;
             const SlotDev& sd = slots[tile.slot];
@@ -1453,7 +1471,11 @@ __device__ double block_dot(const double* __restrict__ a, const double* __restri
 // cinv Z^T r_i for r_i = r_{i-1} - alpha_i q_i.
 template <typename TI>
 __global__ __launch_bounds__(kFinishThreads) void k_finish_step(const SlotDev* __restrict__ slots, SlotState* states,
-                                                                const SlotConfig* __restrict__ configs, int* active) {
+                                                                const SlotConfig* __restrict__ configs, int* active,
+                                                                int* tickets) {
+    // The tile tickets of the looping pass that ran before this kernel start
+    // again from zero for the next pass.
+    if (tickets && blockIdx.x == 0 && threadIdx.x < kPassTickets) tickets[threadIdx.x] = 0;
     // As in k_finish_outer, the descriptor is read with the phase, and thread
     // 0's scalars are fetched before the sum, so their latency hides behind it.
     const int slot = blockIdx.x;
@@ -1854,9 +1876,11 @@ hipError_t outer_step(bool inner64, int n_slots, const SlotDev* slots, SlotState
 }
 
 hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const SlotState* states,
-                      const Tile* inner_tiles, const int* counts, int grid_inner, bool loop, const Tile* block_tiles,
-                      const SlotGroup* groups, int grid_block, int block_mfma, const CoarseLaunch& coarse,
-                      hipStream_t stream) {
+                      const Tile* inner_tiles, const int* counts, int grid_inner, bool loop, int* tickets,
+                      const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma,
+                      const CoarseLaunch& coarse, hipStream_t stream) {
+    // A looping grid needs a ticket set and every XCD range drawn by some workgroup.
+    if (loop && (!tickets || grid_inner < 8)) return hipErrorInvalidValue;
     const dim3 block(kGroupThreads);
     const dim3 grid(grid_inner);
     const bool two_level = coarse.grid_patches > 0;
@@ -1864,10 +1888,10 @@ hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const 
     do {                                                                                                           \
         if (loop)                                                                                                  \
             hipLaunchKernelGGL((k_inner_step<TI, ROWS, true, COARSE>), grid, block, 0, stream, slots, states,        \
-                               inner_tiles, counts);                                                               \
+                               inner_tiles, counts, tickets);                                                      \
         else                                                                                                       \
             hipLaunchKernelGGL((k_inner_step<TI, ROWS, false, COARSE>), grid, block, 0, stream, slots, states,       \
-                               inner_tiles, counts);                                                               \
+                               inner_tiles, counts, nullptr);                                                      \
     } while (0)
     if (inner64) {
         if (coarse.plain) PCBHIP_PASS(double, kRowsInner64, false);
@@ -1899,18 +1923,20 @@ hipError_t inner_pass(bool inner64, int rows_inner, const SlotDev* slots, const 
 
 hipError_t inner_step(bool inner64, int rows_inner, int n_slots, const SlotDev* slots, SlotState* states,
                       const SlotConfig* configs, const Tile* inner_tiles, const int* counts, int grid_inner, bool loop,
-                      const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma, int* active,
-                      const CoarseLaunch& coarse, hipStream_t stream) {
+                      int* tickets, const Tile* block_tiles, const SlotGroup* groups, int grid_block, int block_mfma,
+                      int* active, const CoarseLaunch& coarse, hipStream_t stream) {
     const hipError_t err = inner_pass(inner64, rows_inner, slots, states, inner_tiles, counts, grid_inner, loop,
-                                      block_tiles, groups, grid_block, block_mfma, coarse, stream);
+                                      tickets, block_tiles, groups, grid_block, block_mfma, coarse, stream);
     if (err != hipSuccess) return err;
     const dim3 block(kFinishThreads);
     if (inner64) {
         coarse_kernels<double>(slots, states, counts, coarse, 1, stream);
-        hipLaunchKernelGGL(k_finish_step<double>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
+        hipLaunchKernelGGL(k_finish_step<double>, dim3(n_slots), block, 0, stream, slots, states, configs, active,
+                           tickets);
     } else {
         coarse_kernels<float>(slots, states, counts, coarse, 1, stream);
-        hipLaunchKernelGGL(k_finish_step<float>, dim3(n_slots), block, 0, stream, slots, states, configs, active);
+        hipLaunchKernelGGL(k_finish_step<float>, dim3(n_slots), block, 0, stream, slots, states, configs, active,
+                           tickets);
     }
     return hipGetLastError();
 }

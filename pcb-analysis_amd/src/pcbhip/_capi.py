@@ -99,6 +99,8 @@ class ServerInfoC(ctypes.Structure):
         ("replays", ctypes.c_int64),
         ("graph_builds", ctypes.c_int64),
         ("steps_per_replay", ctypes.c_int),
+        ("wait_ms", ctypes.c_double),
+        ("replay_ms", ctypes.c_double),
     ]
 
 
@@ -555,7 +557,8 @@ class Server:
     def info(self) -> dict:
         st = ServerInfoC()
         check(self._lib.pcbhip_server_stats(self.handle, ctypes.byref(st)))
-        return {name: int(getattr(st, name)) for name, _ in ServerInfoC._fields_}
+        return {name: (float if kind is ctypes.c_double else int)(getattr(st, name))
+                for name, kind in ServerInfoC._fields_}
 
     def trim(self) -> None:
         check(self._lib.pcbhip_server_trim(self.handle))

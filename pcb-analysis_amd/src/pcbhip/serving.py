@@ -116,6 +116,9 @@ class SolveServer:
         self.peak_running_requests = 0
         self.peak_running_slots = 0
         self.completed: list[ServeResult] = []
+        self.pumps = 0
+        self.pump_s = 0.0  # wall time inside pump(), the native step included
+        self.step_s = 0.0  # wall time inside the native step (it blocks on the oldest replay in flight)
 
     # -- admission ---------------------------------------------------------------
 
@@ -193,12 +196,22 @@ class SolveServer:
         """Admit, run one replay boundary, collect; returns the requests that
         completed in this call."""
 
+        start = time.perf_counter()
+        try:
+            return self._pump()
+        finally:
+            self.pumps += 1
+            self.pump_s += time.perf_counter() - start
+
+    def _pump(self) -> list[ServeResult]:
         now = self.clock()
         while self.queue and self._fits(self.queue[0]):
             self._admit(self.queue.popleft(), now)
         if not self.inflight:
             return []
+        t0 = time.perf_counter()
         self.backend.step()
+        self.step_s += time.perf_counter() - t0
         done: list[ServeResult] = []
         for slot, _tag in self.backend.finished():
             request_id, k = self._slot_of.pop(slot)
