@@ -1,32 +1,91 @@
-"""Convection boundaries and heat sources (reference:
-src/thermal/matrix_free_mpir_fem/boundaries.py).
+"""Convection boundaries and heat sources.
 
-Each boundary lumps its own nodal conductance ``R`` and load ``R T_amb``
-(a quarter of every face's ``h A`` on each face node), so the operator only
-ever sees one Robin array.
+The reference interface is src/thermal/matrix_free_mpir_fem/boundaries.py:27-206:
+``ConvectionBoundary`` (top or bottom face of the stack), ``ExposedFaceConvection``
+(every exposed face of the active elements in chosen directions) and
+``HeatSource``, with the reference's arguments, stored attribute forms and
+error messages.
+
+Both convection kinds are film faces: a set of element faces, each with a
+film coefficient ``h`` and an ambient.  A boundary only says which faces it
+covers (``face_sets``); the lumping onto nodes -- a quarter of every face's
+``h A`` on each face node into the Robin conductance ``R`` and likewise
+``h A T_amb`` into the load -- is shared, so the operator only ever sees one
+Robin array.
 """
 
 from __future__ import annotations
 
 from dataclasses import dataclass
+from typing import Iterator
 
 import numpy as np
 
-from .mesh import FACE_DIRECTIONS, FaceDirection, LayeredThermalMesh, Node, Side, face_area_m2, lump_faces_onto_nodes
+from ._fields import Field, require_shape, unique_directions
+from .mesh import FACE_DIRECTIONS, FaceDirection, LayeredThermalMesh, Node, Side, face_area_m2, lump_face_weights
 
 
-def _checked(value, ndims: tuple[int, ...], message: str, nonneg: bool, what: str) -> np.ndarray:
-    array = np.asarray(value, dtype=np.float64)
-    if array.ndim not in ndims:
-        raise ValueError(message)
-    if not np.all(np.isfinite(array)) or (nonneg and np.any(array < 0.0)):
-        raise ValueError(what)
-    return array
+class FilmFaces:
+    """Shared behaviour of boundaries made of element faces with a film
+    coefficient and an ambient.  Subclasses yield ``(direction, faces)``
+    pairs, ``faces`` a boolean ``(slabs, rows, cols)`` array of the element
+    faces facing ``direction`` that the boundary covers, and broadcast their
+    coefficient and ambient over the element grid."""
+
+    def face_sets(self, mesh: LayeredThermalMesh) -> Iterator[tuple[str, np.ndarray]]:
+        raise NotImplementedError
+
+    def element_values(self, mesh: LayeredThermalMesh, value) -> np.ndarray:
+        raise NotImplementedError
+
+    def face_conductance(self, mesh: LayeredThermalMesh, direction: str, faces: np.ndarray,
+                         coefficient) -> np.ndarray:
+        """``h A`` of every covered face facing ``direction`` (zero elsewhere)."""
+
+        return self.element_values(mesh, coefficient) * face_area_m2(mesh, direction) * faces
+
+    def lumped_weights(self, mesh: LayeredThermalMesh, coefficient, ambient) -> tuple[np.ndarray, np.ndarray]:
+        amb = self.element_values(mesh, ambient)
+        conductance = np.zeros(mesh.node_shape)
+        load = np.zeros(mesh.node_shape)
+        for direction, faces in self.face_sets(mesh):
+            ha = self.face_conductance(mesh, direction, faces, coefficient)
+            conductance += lump_face_weights(ha, direction, mesh.node_shape)
+            load += lump_face_weights(ha * amb, direction, mesh.node_shape)
+        return conductance.reshape(-1), load.reshape(-1)
+
+    def covered_area_m2(self, mesh: LayeredThermalMesh) -> float:
+        return float(sum(np.sum(face_area_m2(mesh, d) * faces) for d, faces in self.face_sets(mesh)))
+
+
+# Inputs of the two convection kinds (ranks: scalar or face array).
+_TOP_COEFFICIENT = Field((0, 2), "coefficient must be a scalar or a (rows, cols) array",
+                         "film coefficients must be finite and non-negative", low=0.0)
+_TOP_AMBIENT = Field((0, 2), "ambient must be a scalar or a (rows, cols) array", "ambient temperature must be finite")
+_EXPOSED_COEFFICIENT = Field((0, 3), "film coefficient must be a scalar or a (slabs, rows, cols) array",
+                             "film coefficient must be finite and non-negative", low=0.0)
+_EXPOSED_AMBIENT = Field((0, 3), "ambient must be a scalar or a (slabs, rows, cols) array",
+                         "ambient temperature must be finite")
+
+
+def check_side(side: str) -> None:
+    if side not in ("top", "bottom"):
+        raise ValueError("side must be 'top' or 'bottom'")
+
+
+def side_faces(mesh: LayeredThermalMesh, side: str) -> tuple[str, np.ndarray]:
+    """The ``+z`` faces of the top slab's active elements, or the ``-z``
+    faces of the bottom slab's."""
+
+    faces = np.zeros(mesh.element_grid_shape, dtype=bool)
+    slab = -1 if side == "top" else 0
+    faces[slab] = np.asarray(mesh.active)[slab]
+    return ("+z" if side == "top" else "-z"), faces
 
 
 @dataclass(frozen=True)
-class ConvectionBoundary:
-    """Newton cooling on the top or bottom face of the stack (boundaries.py:27-102).
+class ConvectionBoundary(FilmFaces):
+    """Newton cooling on the top or bottom face of the stack.
 
     Coefficient and ambient are scalars or ``(rows, cols)`` arrays; a zero
     coefficient switches a face off; faces of inactive elements do not cool.
@@ -37,22 +96,24 @@ class ConvectionBoundary:
     ambient_temperature_k: float | np.ndarray
 
     def __post_init__(self) -> None:
-        if self.side not in ("top", "bottom"):
-            raise ValueError("side must be 'top' or 'bottom'")
-        coefficient = _checked(self.coefficient_w_per_m2_k, (0, 2), "coefficient must be a scalar or a (rows, cols) array",
-                               True, "film coefficients must be finite and non-negative")
-        ambient = _checked(self.ambient_temperature_k, (0, 2), "ambient must be a scalar or a (rows, cols) array",
-                           False, "ambient temperature must be finite")
-        object.__setattr__(self, "coefficient_w_per_m2_k", coefficient.copy())
-        object.__setattr__(self, "ambient_temperature_k", float(ambient) if ambient.ndim == 0 else ambient.copy())
+        check_side(self.side)
+        object.__setattr__(self, "coefficient_w_per_m2_k", _TOP_COEFFICIENT.read(self.coefficient_w_per_m2_k).copy())
+        object.__setattr__(self, "ambient_temperature_k", _TOP_AMBIENT.read_compact(self.ambient_temperature_k))
 
     def check_shape(self, mesh: LayeredThermalMesh) -> None:
-        shape = mesh.element_grid_shape[1:]
-        if self.coefficient_w_per_m2_k.ndim == 2 and self.coefficient_w_per_m2_k.shape != shape:
-            raise ValueError("convection coefficient array must match (rows, cols) of the mesh")
-        ambient = np.asarray(self.ambient_temperature_k)
-        if ambient.ndim == 2 and ambient.shape != shape:
-            raise ValueError("convection ambient array must match (rows, cols) of the mesh")
+        plane = mesh.element_grid_shape[1:]
+        require_shape(self.coefficient_w_per_m2_k, plane,
+                      "convection coefficient array must match (rows, cols) of the mesh", when_rank=2)
+        require_shape(self.ambient_temperature_k, plane,
+                      "convection ambient array must match (rows, cols) of the mesh", when_rank=2)
+
+    def face_sets(self, mesh: LayeredThermalMesh) -> Iterator[tuple[str, np.ndarray]]:
+        yield side_faces(mesh, self.side)
+
+    def element_values(self, mesh: LayeredThermalMesh, value) -> np.ndarray:
+        # A (rows, cols) face array holds for every slab; only the side slab's
+        # faces are covered.
+        return np.array(np.broadcast_to(np.asarray(value, dtype=np.float64), mesh.element_grid_shape))
 
     @property
     def cools(self) -> bool:
@@ -62,54 +123,41 @@ class ConvectionBoundary:
         return float(np.mean(self.ambient_temperature_k))
 
     def lumped_nodal_weights(self, mesh: LayeredThermalMesh) -> tuple[np.ndarray, np.ndarray]:
-        """Nodal conductance ``R`` and load ``R T_amb`` of this face, flat."""
+        """Nodal conductance ``R`` and load ``R T_amb`` of this face, flat
+        (the reference's boundaries.py:82-102)."""
 
-        slab = mesh.element_grid_shape[0] - 1 if self.side == "top" else 0
-        direction = "+z" if self.side == "top" else "-z"
-        plane = mesh.element_grid_shape[1:]
-        h = np.broadcast_to(self.coefficient_w_per_m2_k, plane)
-        amb = np.broadcast_to(self.ambient_temperature_k, plane)
-        face = np.zeros(mesh.element_grid_shape)
-        face[slab] = h * mesh.active[slab]  # type: ignore[index]
-        face *= face_area_m2(mesh, direction)
-        load = face.copy()
-        load[slab] *= amb
-        return (
-            lump_faces_onto_nodes(face, direction, mesh.node_shape).reshape(-1),
-            lump_faces_onto_nodes(load, direction, mesh.node_shape).reshape(-1),
-        )
+        return self.lumped_weights(mesh, self.coefficient_w_per_m2_k, self.ambient_temperature_k)
 
 
 @dataclass(frozen=True)
-class ExposedFaceConvection:
-    """Newton cooling on every exposed face of the active elements
-    (boundaries.py:105-182)."""
+class ExposedFaceConvection(FilmFaces):
+    """Newton cooling on every exposed face of the active elements facing one
+    of ``directions``; coefficient and ambient are scalars or one value per
+    element."""
 
     coefficient_w_per_m2_k: float | np.ndarray
     ambient_temperature_k: float | np.ndarray
     directions: tuple[FaceDirection, ...] = FACE_DIRECTIONS
 
     def __post_init__(self) -> None:
-        coefficient = _checked(self.coefficient_w_per_m2_k, (0, 3),
-                               "film coefficient must be a scalar or a (slabs, rows, cols) array", True,
-                               "film coefficient must be finite and non-negative")
-        ambient = _checked(self.ambient_temperature_k, (0, 3), "ambient must be a scalar or a (slabs, rows, cols) array",
-                           False, "ambient temperature must be finite")
-        directions = tuple(self.directions)
-        if not directions or any(d not in FACE_DIRECTIONS for d in directions):
-            raise ValueError(f"directions must be drawn from {FACE_DIRECTIONS}")
-        if len(set(directions)) != len(directions):
-            raise ValueError("directions must be unique")
-        object.__setattr__(self, "coefficient_w_per_m2_k",
-                           float(coefficient) if coefficient.ndim == 0 else coefficient.copy())
-        object.__setattr__(self, "ambient_temperature_k", float(ambient) if ambient.ndim == 0 else ambient.copy())
-        object.__setattr__(self, "directions", directions)
+        coefficient = _EXPOSED_COEFFICIENT.read_compact(self.coefficient_w_per_m2_k)
+        ambient = _EXPOSED_AMBIENT.read_compact(self.ambient_temperature_k)
+        object.__setattr__(self, "directions", unique_directions(self.directions, FACE_DIRECTIONS))
+        object.__setattr__(self, "coefficient_w_per_m2_k", coefficient)
+        object.__setattr__(self, "ambient_temperature_k", ambient)
 
     def check_shape(self, mesh: LayeredThermalMesh) -> None:
         for name in ("coefficient_w_per_m2_k", "ambient_temperature_k"):
-            value = np.asarray(getattr(self, name))
-            if value.ndim == 3 and value.shape != mesh.element_grid_shape:
-                raise ValueError(f"{name} array must match (slabs, rows, cols) of the mesh")
+            require_shape(getattr(self, name), mesh.element_grid_shape,
+                          f"{name} array must match (slabs, rows, cols) of the mesh", when_rank=3)
+
+    def face_sets(self, mesh: LayeredThermalMesh) -> Iterator[tuple[str, np.ndarray]]:
+        exposed = mesh.exposed_faces()
+        for direction in self.directions:
+            yield direction, exposed[direction]
+
+    def element_values(self, mesh: LayeredThermalMesh, value) -> np.ndarray:
+        return np.array(np.broadcast_to(np.asarray(value, dtype=np.float64), mesh.element_grid_shape))
 
     @property
     def cools(self) -> bool:
@@ -119,20 +167,13 @@ class ExposedFaceConvection:
         return float(np.mean(self.ambient_temperature_k))
 
     def exposed_area_m2(self, mesh: LayeredThermalMesh) -> float:
-        faces = mesh.exposed_faces()
-        return float(sum(np.sum(face_area_m2(mesh, d) * faces[d]) for d in self.directions))
+        return self.covered_area_m2(mesh)
 
     def lumped_nodal_weights(self, mesh: LayeredThermalMesh) -> tuple[np.ndarray, np.ndarray]:
-        faces = mesh.exposed_faces()
-        h = np.broadcast_to(self.coefficient_w_per_m2_k, mesh.element_grid_shape)
-        amb = np.broadcast_to(self.ambient_temperature_k, mesh.element_grid_shape)
-        weights = np.zeros(mesh.node_shape)
-        rhs = np.zeros(mesh.node_shape)
-        for d in self.directions:
-            face = h * face_area_m2(mesh, d) * faces[d]
-            weights += lump_faces_onto_nodes(face, d, mesh.node_shape)
-            rhs += lump_faces_onto_nodes(face * amb, d, mesh.node_shape)
-        return weights.reshape(-1), rhs.reshape(-1)
+        """Nodal conductance and load of every covered face, flat (the
+        reference's boundaries.py:170-182)."""
+
+        return self.lumped_weights(mesh, self.coefficient_w_per_m2_k, self.ambient_temperature_k)
 
 
 Convection = ConvectionBoundary | ExposedFaceConvection
@@ -140,7 +181,8 @@ Convection = ConvectionBoundary | ExposedFaceConvection
 
 @dataclass(frozen=True)
 class HeatSource:
-    """A total power spread uniformly over a set of mesh nodes (boundaries.py:188-206)."""
+    """A total power spread uniformly over a set of mesh nodes (the
+    reference's boundaries.py:188-206)."""
 
     nodes: tuple[Node, ...]
     power_w: float
@@ -148,12 +190,13 @@ class HeatSource:
 
     def __post_init__(self) -> None:
         nodes = tuple(tuple(int(i) for i in node) for node in self.nodes)
-        if not nodes:
-            raise ValueError("a heat source needs at least one node")
-        if any(len(node) != 3 for node in nodes):
-            raise ValueError("heat source nodes must be (slab_face, row, column)")
-        if len(set(nodes)) != len(nodes):
-            raise ValueError("heat source nodes must be unique")
-        if not np.isfinite(self.power_w):
-            raise ValueError("heat source power must be finite")
+        problems = (
+            (not nodes, "a heat source needs at least one node"),
+            (any(len(node) != 3 for node in nodes), "heat source nodes must be (slab_face, row, column)"),
+            (len(set(nodes)) != len(nodes), "heat source nodes must be unique"),
+            (not np.isfinite(self.power_w), "heat source power must be finite"),
+        )
+        for failed, message in problems:
+            if failed:
+                raise ValueError(message)
         object.__setattr__(self, "nodes", nodes)

@@ -1,9 +1,16 @@
-"""Structured hexahedral Q1 mesh of a layered board (reference:
-src/thermal/matrix_free_mpir_fem/mesh.py).
+"""Structured hexahedral Q1 mesh of a layered board.
 
-Same constructor, validation, properties and node/element layout as the
-reference: nodes ``(slabs + 1, rows + 1, cols + 1)`` in C order with x
-fastest, local corners ``4 dz + 2 dy + dx``.
+The reference interface is ``LayeredThermalMesh`` of
+src/thermal/matrix_free_mpir_fem/mesh.py:126-330: the same constructor
+arguments, accepted shapes, properties and error messages, and the same
+node and element layout (nodes ``(slabs + 1, rows + 1, cols + 1)`` in C order
+with x fastest, local corners ``4 dz + 2 dy + dx``), since arrays cross to the
+device in that layout.
+
+Geometry is kept as three broadcastable cell-length arrays (z, y, x).  Face
+areas, volumes and coefficients are products of them; exposed faces come
+from first differences of the padded activity mask; quantities that live on
+nodes are spread from cells with the separable ``spread_to_nodes``.
 """
 
 from __future__ import annotations
@@ -15,6 +22,8 @@ import numpy as np
 
 from electrical.matrix_free_mpir_fem.grid import check_pitch_axis
 
+from ._fields import Field, per_element, spread_to_nodes
+
 COPPER_THERMAL_CONDUCTIVITY_W_PER_M_K = 385.0
 FR4_IN_PLANE_THERMAL_CONDUCTIVITY_W_PER_M_K = 0.8
 FR4_THROUGH_PLANE_THERMAL_CONDUCTIVITY_W_PER_M_K = 0.3
@@ -25,86 +34,91 @@ Preconditioner = Literal["two-level", "jacobi"]
 FaceDirection = Literal["-x", "+x", "-y", "+y", "-z", "+z"]
 FACE_DIRECTIONS: tuple[FaceDirection, ...] = ("-x", "+x", "-y", "+y", "-z", "+z")
 
-# Local corners of each element face (mesh.py:45-53).
-FACE_CORNERS: dict[str, tuple[int, int, int, int]] = {
-    "-z": (0, 1, 2, 3),
-    "+z": (4, 5, 6, 7),
-    "-y": (0, 1, 4, 5),
-    "+y": (2, 3, 6, 7),
-    "-x": (0, 2, 4, 6),
-    "+x": (1, 3, 5, 7),
-}
+# Array axis of each coordinate in (slabs, rows, cols) and node arrays.
+AXIS = {"z": 0, "y": 1, "x": 2}
 
 
 def corner_views(grid: Any) -> tuple[Any, ...]:
     """The eight ``(slabs, rows, cols)`` corner views of a node grid, ordered
-    ``4 dz + 2 dy + dx`` (mesh.py:28-37)."""
+    ``4 dz + 2 dy + dx`` (the reference's local corner order, mesh.py:28-37);
+    views, so they may be written through."""
 
     s, r, c = (n - 1 for n in grid.shape)
-    return tuple(grid[dz : dz + s, dy : dy + r, dx : dx + c] for dz in (0, 1) for dy in (0, 1) for dx in (0, 1))
+    return tuple(grid[k >> 2 : (k >> 2) + s, (k >> 1) & 1 : ((k >> 1) & 1) + r, k & 1 : (k & 1) + c]
+                 for k in range(8))
 
 
 def exposed_element_faces(active: np.ndarray) -> dict[FaceDirection, np.ndarray]:
-    """Faces of active elements bordering a void element or the grid edge (mesh.py:53-77)."""
+    """Faces of active elements that border a void element or the grid edge
+    (the reference's mesh.py:53-77): along each axis, the activity of the
+    padded grid steps down (+ face) or up (- face) across the face."""
 
     mask = np.asarray(active, dtype=bool)
     if mask.ndim != 3:
         raise ValueError("active must have shape (slabs, rows, cols)")
     out: dict[FaceDirection, np.ndarray] = {}
-    for direction in FACE_DIRECTIONS:
-        axis = "zyx".index(direction[1])
-        neighbour = np.zeros_like(mask)
+    for name, axis in AXIS.items():
+        pad = [(0, 0)] * 3
+        pad[axis] = (1, 1)
+        step = np.diff(np.pad(mask.astype(np.int8), pad), axis=axis)
         head = [slice(None)] * 3
         tail = [slice(None)] * 3
         head[axis], tail[axis] = slice(None, -1), slice(1, None)
-        if direction[0] == "+":
-            neighbour[tuple(head)] = mask[tuple(tail)]
-        else:
-            neighbour[tuple(tail)] = mask[tuple(head)]
-        out[direction] = mask & ~neighbour
-    return out
+        out[f"-{name}"] = step[tuple(head)] == 1  # type: ignore[index]
+        out[f"+{name}"] = step[tuple(tail)] == -1  # type: ignore[index]
+    return {d: out[d] for d in FACE_DIRECTIONS}
 
 
-def lump_faces_onto_nodes(face_weight: np.ndarray, direction: str, node_shape) -> np.ndarray:
-    """A quarter of every face weight onto each of the face's four nodes (mesh.py:80-92)."""
+def cell_lengths(mesh: "LayeredThermalMesh") -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Edge lengths of the elements along z, y and x, shaped to broadcast over
+    ``(slabs, rows, cols)``."""
 
-    weights = np.zeros(node_shape)
-    views = corner_views(weights)
-    quarter = face_weight / 4.0
-    for corner in FACE_CORNERS[direction]:
-        views[corner][...] += quarter
-    return weights
+    return (
+        np.asarray(mesh.slab_thickness_m, dtype=np.float64)[:, None, None],
+        np.asarray(mesh.pitch_y_m)[None, :, None],
+        np.asarray(mesh.pitch_x_m)[None, None, :],
+    )
 
 
 def face_area_m2(mesh: "LayeredThermalMesh", direction: str) -> np.ndarray:
-    """Area of every element face facing ``direction`` (mesh.py:95-106)."""
+    """Area of every element face facing ``direction``: the product of the
+    two edge lengths across the face normal."""
 
-    shape = mesh.element_grid_shape
-    hz = np.asarray(mesh.slab_thickness_m, dtype=np.float64)[:, None, None]
-    hx = mesh.pitch_x_m[None, None, :]
-    hy = mesh.pitch_y_m[None, :, None]
-    area = {"z": hx * hy, "y": hx * hz, "x": hy * hz}[direction[1]]
-    return np.broadcast_to(area, shape).copy()
+    lengths = cell_lengths(mesh)
+    normal = AXIS[direction[1]]
+    a, b = (lengths[i] for i in range(3) if i != normal)
+    return np.array(np.broadcast_to(a * b, mesh.element_grid_shape))
 
 
-def _element_field(value: Any, shape: tuple[int, int, int], name: str) -> np.ndarray:
-    array = np.asarray(value, dtype=np.float64)
-    if array.ndim == 0:
-        return np.full(shape, float(array))
-    if array.shape == (shape[0],):
-        return np.broadcast_to(array[:, None, None], shape).copy()
-    if array.shape == shape:
-        return array.copy()
-    raise ValueError(f"{name} must be a scalar, one value per slab, or match shape {shape}")
+def lump_face_weights(face_weight: np.ndarray, direction: str, node_shape) -> np.ndarray:
+    """Node values of per-face weights: each face's weight split in quarters
+    onto its four nodes (the reference's lumping, mesh.py:80-92).  The faces
+    facing ``+a`` lie on the node plane above their element along a, those
+    facing ``-a`` on the plane below."""
+
+    normal = AXIS[direction[1]]
+    across = tuple(i for i in range(3) if i != normal)
+    quarters = spread_to_nodes(np.asarray(face_weight, dtype=np.float64) / 4.0, across)
+    pad = [(0, 0)] * 3
+    pad[normal] = (1, 0) if direction[0] == "+" else (0, 1)
+    nodes = np.pad(quarters, pad)
+    assert nodes.shape == tuple(node_shape)
+    return nodes
+
+
+_THICKNESS = Field((1,), "slab_thickness_m must list at least one slab", "slab thicknesses must be finite and positive",
+                   low=0.0, low_open=True)
 
 
 @dataclass(frozen=True)
 class LayeredThermalMesh:
-    """Structured hexahedral Q1 mesh of a PCB stack (mesh.py:126-330).
+    """Structured hexahedral Q1 mesh of a PCB stack.
 
     ``slab_thickness_m`` lists element slabs bottom to top; pitches are one
     width or one value per column / row; conductivities are a scalar, one
-    value per slab or one per element; ``active`` carves void elements.
+    value per slab or one per element; ``active`` carves void elements;
+    ``volumetric_heat_capacity_j_per_m3_k`` (same forms) is needed by
+    transient solves only.
     """
 
     slab_thickness_m: Sequence[float]
@@ -117,18 +131,46 @@ class LayeredThermalMesh:
     volumetric_heat_capacity_j_per_m3_k: float | Sequence[float] | np.ndarray | None = None
 
     def __post_init__(self) -> None:
-        thickness = np.asarray(self.slab_thickness_m, dtype=np.float64)
-        if thickness.ndim != 1 or thickness.size < 1:
-            raise ValueError("slab_thickness_m must list at least one slab")
-        if not np.all(np.isfinite(thickness)) or np.any(thickness <= 0.0):
-            raise ValueError("slab thicknesses must be finite and positive")
-        k_in = np.asarray(self.conductivity_w_per_m_k, dtype=np.float64)
-        px = np.asarray(self.pitch_x_m, dtype=np.float64)
-        py = np.asarray(self.pitch_y_m, dtype=np.float64)
+        thickness = _THICKNESS.read(self.slab_thickness_m)
+        if thickness.size < 1:
+            raise ValueError(_THICKNESS.rank_message)
+        rows, cols = self._footprint()
+        shape = (int(thickness.size), rows, cols)
+        active = self._activity(shape)
+        k_in = per_element(self.conductivity_w_per_m_k, shape, "conductivity_w_per_m_k")
+        k_z = k_in.copy() if self.through_plane_conductivity_w_per_m_k is None else per_element(
+            self.through_plane_conductivity_w_per_m_k, shape, "through_plane_conductivity_w_per_m_k")
+        capacity = None
+        if self.volumetric_heat_capacity_j_per_m3_k is not None:
+            capacity = per_element(self.volumetric_heat_capacity_j_per_m3_k, shape,
+                                   "volumetric_heat_capacity_j_per_m3_k")
+        for name, values in (("conductivity_w_per_m_k", k_in), ("through_plane_conductivity_w_per_m_k", k_z),
+                             ("volumetric_heat_capacity_j_per_m3_k", capacity)):
+            if values is not None and not (np.all(np.isfinite(values)) and np.all(values[active] > 0.0)):
+                raise ValueError(f"{name} must be finite and positive on active elements")
+        fields = {
+            "slab_thickness_m": tuple(float(t) for t in thickness),
+            "element_shape": (rows, cols),
+            "pitch_x_m": check_pitch_axis(np.asarray(self.pitch_x_m, dtype=np.float64), cols, "pitch_x_m"),
+            "pitch_y_m": check_pitch_axis(np.asarray(self.pitch_y_m, dtype=np.float64), rows, "pitch_y_m"),
+            "conductivity_w_per_m_k": np.where(active, k_in, 0.0),
+            "through_plane_conductivity_w_per_m_k": np.where(active, k_z, 0.0),
+            "active": active,
+            "volumetric_heat_capacity_j_per_m3_k": None if capacity is None else np.where(active, capacity, 0.0),
+        }
+        for name, value in fields.items():
+            object.__setattr__(self, name, value)
+
+    def _footprint(self) -> tuple[int, int]:
+        """(rows, cols): given, else read off a full conductivity array, else
+        off two per-cell pitch arrays."""
+
+        k = np.asarray(self.conductivity_w_per_m_k)
+        px, py = np.asarray(self.pitch_x_m), np.asarray(self.pitch_y_m)
         if self.element_shape is not None:
             rows, cols = (int(v) for v in self.element_shape)
-        elif k_in.ndim == 3:
-            rows, cols = int(k_in.shape[1]), int(k_in.shape[2])
+        elif k.ndim == 3:
+            rows, cols = (int(v) for v in k.shape[1:])
         elif px.ndim == 1 and py.ndim == 1:
             rows, cols = int(py.size), int(px.size)
         else:
@@ -136,43 +178,19 @@ class LayeredThermalMesh:
                 "element_shape is required unless conductivity has shape "
                 "(slabs, rows, cols) or both pitches are per-cell arrays"
             )
-        if rows < 1 or cols < 1:
+        if min(rows, cols) < 1:
             raise ValueError("element_shape must have positive axes")
-        shape = (int(thickness.size), rows, cols)
-        object.__setattr__(self, "pitch_x_m", check_pitch_axis(px, cols, "pitch_x_m"))
-        object.__setattr__(self, "pitch_y_m", check_pitch_axis(py, rows, "pitch_y_m"))
-        k_in = _element_field(k_in, shape, "conductivity_w_per_m_k")
-        if self.through_plane_conductivity_w_per_m_k is None:
-            k_z = k_in.copy()
-        else:
-            k_z = _element_field(self.through_plane_conductivity_w_per_m_k, shape,
-                                 "through_plane_conductivity_w_per_m_k")
+        return rows, cols
+
+    def _activity(self, shape: tuple[int, int, int]) -> np.ndarray:
         if self.active is None:
-            active = np.ones(shape, dtype=bool)
-        else:
-            active = np.asarray(self.active, dtype=bool)
-            if active.shape != shape:
-                raise ValueError("active must match (slabs, rows, cols)")
-            if not np.any(active):
-                raise ValueError("at least one element must be active")
-            active = active.copy()
-        for name, array in (("conductivity_w_per_m_k", k_in), ("through_plane_conductivity_w_per_m_k", k_z)):
-            if not np.all(np.isfinite(array)) or np.any(array[active] <= 0.0):
-                raise ValueError(f"{name} must be finite and positive on active elements")
-        capacity = None
-        if self.volumetric_heat_capacity_j_per_m3_k is not None:
-            capacity = _element_field(self.volumetric_heat_capacity_j_per_m3_k, shape,
-                                      "volumetric_heat_capacity_j_per_m3_k")
-            if not np.all(np.isfinite(capacity)) or np.any(capacity[active] <= 0.0):
-                raise ValueError(
-                    "volumetric_heat_capacity_j_per_m3_k must be finite and positive on active elements")
-            capacity = np.where(active, capacity, 0.0)
-        object.__setattr__(self, "slab_thickness_m", tuple(thickness.tolist()))
-        object.__setattr__(self, "element_shape", (rows, cols))
-        object.__setattr__(self, "conductivity_w_per_m_k", np.where(active, k_in, 0.0))
-        object.__setattr__(self, "through_plane_conductivity_w_per_m_k", np.where(active, k_z, 0.0))
-        object.__setattr__(self, "active", active)
-        object.__setattr__(self, "volumetric_heat_capacity_j_per_m3_k", capacity)
+            return np.ones(shape, dtype=bool)
+        active = np.array(self.active, dtype=bool)
+        if active.shape != shape:
+            raise ValueError("active must match (slabs, rows, cols)")
+        if not active.any():
+            raise ValueError("at least one element must be active")
+        return active
 
     @property
     def element_grid_shape(self) -> tuple[int, int, int]:
@@ -181,8 +199,7 @@ class LayeredThermalMesh:
 
     @property
     def node_shape(self) -> tuple[int, int, int]:
-        s, r, c = self.element_grid_shape
-        return s + 1, r + 1, c + 1
+        return tuple(n + 1 for n in self.element_grid_shape)  # type: ignore[return-value]
 
     @property
     def size(self) -> int:
@@ -190,32 +207,31 @@ class LayeredThermalMesh:
 
     @property
     def uniform_pitch(self) -> bool:
-        return bool(np.all(self.pitch_x_m == self.pitch_x_m[0]) and np.all(self.pitch_y_m == self.pitch_y_m[0]))
+        return bool(np.ptp(self.pitch_x_m) == 0.0 and np.ptp(self.pitch_y_m) == 0.0)
 
     @property
     def cell_area_m2(self) -> np.ndarray:
-        return self.pitch_y_m[:, None] * self.pitch_x_m[None, :]
+        return np.multiply.outer(self.pitch_y_m, self.pitch_x_m)
 
     @property
     def x_edges_m(self) -> np.ndarray:
-        return np.concatenate(([0.0], np.cumsum(self.pitch_x_m)))
+        return np.cumsum(np.r_[0.0, self.pitch_x_m])
 
     @property
     def y_edges_m(self) -> np.ndarray:
-        return np.concatenate(([0.0], np.cumsum(self.pitch_y_m)))
+        return np.cumsum(np.r_[0.0, self.pitch_y_m])
 
     @property
     def element_volume_m3(self) -> np.ndarray:
-        hz = np.asarray(self.slab_thickness_m, dtype=np.float64)
-        return hz[:, None, None] * self.cell_area_m2[None, :, :]
+        hz, hy, hx = cell_lengths(self)
+        return hz * (hy * hx)
 
     def element_coefficients(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         """``a_x = k_in hy hz / hx``, ``a_y = k_in hx hz / hy``, ``a_z = k_z hx hy / hz``
-        per element, zero on void (mesh.py:276-293)."""
+        per element, zero on void (the reference's mesh.py:276-293, evaluated
+        left to right as there, so the bits agree)."""
 
-        hz = np.asarray(self.slab_thickness_m, dtype=np.float64)[:, None, None]
-        hx = self.pitch_x_m[None, None, :]
-        hy = self.pitch_y_m[None, :, None]
+        hz, hy, hx = cell_lengths(self)
         k_in = np.asarray(self.conductivity_w_per_m_k)
         k_z = np.asarray(self.through_plane_conductivity_w_per_m_k)
         return (
@@ -229,13 +245,11 @@ class LayeredThermalMesh:
         return self.volumetric_heat_capacity_j_per_m3_k is not None
 
     def nodal_heat_capacity_j_per_k(self) -> np.ndarray:
+        """An eighth of every element's heat capacity on each of its corners."""
+
         if self.volumetric_heat_capacity_j_per_m3_k is None:
             raise ValueError("the mesh has no volumetric_heat_capacity_j_per_m3_k; a transient solve needs one")
-        element = self.volumetric_heat_capacity_j_per_m3_k * self.element_volume_m3 / 8.0
-        nodal = np.zeros(self.node_shape)
-        for view in corner_views(nodal):
-            view[...] += element
-        return nodal
+        return spread_to_nodes(self.volumetric_heat_capacity_j_per_m3_k * self.element_volume_m3 / 8.0, (0, 1, 2))
 
     @property
     def is_full(self) -> bool:
@@ -243,29 +257,37 @@ class LayeredThermalMesh:
 
     @property
     def active_nodes(self) -> np.ndarray:
-        nodes = np.zeros(self.node_shape, dtype=bool)
-        for view in corner_views(nodes):
-            view[...] |= self.active  # type: ignore[operator]
-        return nodes
+        return spread_to_nodes(np.asarray(self.active, dtype=np.float64), (0, 1, 2)) > 0.0
 
     def exposed_faces(self) -> dict[FaceDirection, np.ndarray]:
         return exposed_element_faces(self.active)  # type: ignore[arg-type]
 
 
 def flat_index(node: Node, shape: tuple[int, int, int]) -> int:
-    if len(node) != 3 or any(index < 0 for index in node):
+    """Flat C-order index of a (slab face, row, column) node, refusing nodes
+    outside the grid."""
+
+    if len(node) != 3 or min(node) < 0:
         raise ValueError(f"invalid node {node!r}")
-    try:
-        return int(np.ravel_multi_index(node, shape))
-    except ValueError as exc:
-        raise ValueError(f"node {node!r} lies outside mesh shape {shape}") from exc
+    if any(i >= n for i, n in zip(node, shape)):
+        raise ValueError(f"node {node!r} lies outside mesh shape {shape}")
+    z, y, x = (int(i) for i in node)
+    return (z * shape[1] + y) * shape[2] + x
 
 
 def unit_hexahedron_matrices() -> tuple[np.ndarray, np.ndarray, np.ndarray]:
-    """The three (8, 8) unit trilinear stiffness matrices (mesh.py:346-359):
-    ``U_x = M(z) M(y) S(x)``, ``U_y = M S M``, ``U_z = S M M`` with
-    ``S = [[1, -1], [-1, 1]]`` and ``M = [[2, 1], [1, 2]] / 6``."""
+    """The three (8, 8) unit trilinear stiffness matrices: with the 1-D
+    stiffness ``S = [[1, -1], [-1, 1]]`` and mass ``M = [[2, 1], [1, 2]] / 6``,
+    ``U_x = M(z) M(y) S(x)``, ``U_y = M S M`` and ``U_z = S M M`` over the
+    corners ``4 dz + 2 dy + dx`` (the reference's mesh.py:346-359)."""
 
-    s = np.array([[1.0, -1.0], [-1.0, 1.0]])
-    m = np.array([[2.0, 1.0], [1.0, 2.0]]) / 6.0
-    return np.kron(m, np.kron(m, s)), np.kron(m, np.kron(s, m)), np.kron(s, np.kron(m, m))
+    one_d = {"S": np.array([[1.0, -1.0], [-1.0, 1.0]]), "M": np.array([[2.0, 1.0], [1.0, 2.0]]) / 6.0}
+
+    def tensor(z: str, y: str, x: str) -> np.ndarray:
+        # z (y x): the factors are associated as in the oracle, so every
+        # entry rounds the same way.
+        yx = np.einsum("be,cf->becf", one_d[y], one_d[x])
+        t = np.einsum("ad,becf->abcdef", one_d[z], yx)
+        return t.reshape(8, 8)
+
+    return tensor("M", "M", "S"), tensor("M", "S", "M"), tensor("S", "M", "M")

@@ -1,4 +1,11 @@
-"""The steady conduction problem (reference: src/thermal/matrix_free_mpir_fem/problem.py)."""
+"""The steady conduction problem: mesh, boundaries, fixed nodes and loads.
+
+The reference interface is ``ThermalConductionProblem`` of
+src/thermal/matrix_free_mpir_fem/problem.py:15-124: the same fields,
+normalised forms (a full fixed mask and fixed values, zero-filled loads,
+tuples) and error messages.  Here the boundary kinds and the two load
+fields are tables that one loop checks.
+"""
 
 from __future__ import annotations
 
@@ -10,13 +17,57 @@ from .boundaries import Convection, ConvectionBoundary, ExposedFaceConvection, H
 from .mesh import LayeredThermalMesh, flat_index
 from .radiation import ExposedFaceRadiation, RadiationBoundary
 
+# (attribute, accepted kinds, message for anything else, the property that
+# says whether an entry sets the temperature level)
+_BOUNDARY_GROUPS = (
+    ("convection", (ConvectionBoundary, ExposedFaceConvection),
+     "convection entries must be ConvectionBoundary or ExposedFaceConvection", "cools"),
+    ("radiation", (RadiationBoundary, ExposedFaceRadiation),
+     "radiation entries must be RadiationBoundary or ExposedFaceRadiation", "radiates"),
+)
+
+# (attribute, grid, shape message, finiteness message, inactive-support message)
+_LOADS = (
+    ("element_heat_w", "element", "element_heat_w must match (slabs, rows, cols)", "element heat must be finite",
+     "element heat must be zero on inactive elements"),
+    ("nodal_heat_w", "node", "nodal_heat_w must match mesh.node_shape", "nodal heat must be finite",
+     "nodal heat must be zero on inactive nodes"),
+)
+
+
+def fixed_nodes(mask, values, node_shape) -> tuple[np.ndarray, np.ndarray]:
+    """The fixed-temperature mask and a full array of fixed values (zero
+    where not fixed); no mask means no fixed node."""
+
+    if mask is None:
+        if values is not None:
+            raise ValueError("fixed_temperature_k needs a fixed_temperature_mask")
+        return np.zeros(node_shape, dtype=bool), np.zeros(node_shape)
+    mask = np.asarray(mask, dtype=bool)
+    if mask.shape != node_shape:
+        raise ValueError("fixed_temperature_mask must match mesh.node_shape")
+    if values is None:
+        raise ValueError("fixed_temperature_mask needs fixed_temperature_k")
+    values = np.asarray(values, dtype=np.float64)
+    if values.ndim != 0 and values.shape != node_shape:
+        raise ValueError("fixed_temperature_k must be a scalar or match mesh.node_shape")
+    values = np.broadcast_to(values, node_shape)
+    if not np.isfinite(values[mask]).all():
+        raise ValueError("fixed temperatures must be finite")
+    return mask.copy(), np.where(mask, values, 0.0)
+
 
 @dataclass(frozen=True)
 class ThermalConductionProblem:
     """Steady conduction with convective and radiating faces and
-    fixed-temperature nodes (problem.py:15-124).  ``element_heat_w`` is lumped equally onto each
-    element's eight corners; ``nodal_heat_w`` is per node; at least one
-    positive film coefficient or one fixed node is required."""
+    fixed-temperature nodes.
+
+    ``element_heat_w`` is each element's power, lumped equally onto its eight
+    corners; ``nodal_heat_w`` is power already on nodes.  At least one
+    positive film coefficient, one radiating face or one fixed node must set
+    the temperature level.  Radiation makes the problem nonlinear;
+    ``solve_thermal_conduction`` iterates its Newton linearisation.
+    """
 
     mesh: LayeredThermalMesh
     convection: tuple[Convection, ...] = ()
@@ -28,75 +79,44 @@ class ThermalConductionProblem:
     radiation: tuple = ()
 
     def __post_init__(self) -> None:
-        node_shape = self.mesh.node_shape
-        element_shape = self.mesh.element_grid_shape
-        convection = tuple(self.convection)
-        for boundary in convection:
-            if not isinstance(boundary, (ConvectionBoundary, ExposedFaceConvection)):
-                raise TypeError("convection entries must be ConvectionBoundary or ExposedFaceConvection")
-            boundary.check_shape(self.mesh)
-        radiation = tuple(self.radiation)
-        for boundary in radiation:
-            if not isinstance(boundary, (RadiationBoundary, ExposedFaceRadiation)):
-                raise TypeError("radiation entries must be RadiationBoundary or ExposedFaceRadiation")
-            boundary.check_shape(self.mesh)
-        cools = any(b.cools for b in convection) or any(b.radiates for b in radiation)
+        mesh = self.mesh
+        normalised: dict[str, object] = {}
+        sets_level = False
+        for name, kinds, message, flag in _BOUNDARY_GROUPS:
+            entries = tuple(getattr(self, name))
+            for entry in entries:
+                if not isinstance(entry, kinds):
+                    raise TypeError(message)
+                entry.check_shape(mesh)
+            sets_level = sets_level or any(getattr(e, flag) for e in entries)
+            normalised[name] = entries
 
-        if self.fixed_temperature_mask is None:
-            if self.fixed_temperature_k is not None:
-                raise ValueError("fixed_temperature_k needs a fixed_temperature_mask")
-            mask = np.zeros(node_shape, dtype=bool)
-            values = np.zeros(node_shape)
-        else:
-            mask = np.asarray(self.fixed_temperature_mask, dtype=bool)
-            if mask.shape != node_shape:
-                raise ValueError("fixed_temperature_mask must match mesh.node_shape")
-            if self.fixed_temperature_k is None:
-                raise ValueError("fixed_temperature_mask needs fixed_temperature_k")
-            values = np.asarray(self.fixed_temperature_k, dtype=np.float64)
-            if values.ndim == 0:
-                values = np.full(node_shape, float(values))
-            elif values.shape != node_shape:
-                raise ValueError("fixed_temperature_k must be a scalar or match mesh.node_shape")
-            if not np.all(np.isfinite(values[mask])):
-                raise ValueError("fixed temperatures must be finite")
-            values = np.where(mask, values, 0.0)
-        active_nodes = self.mesh.active_nodes
-        if not cools and not np.any(mask & active_nodes):
+        mask, values = fixed_nodes(self.fixed_temperature_mask, self.fixed_temperature_k, mesh.node_shape)
+        active_nodes = mesh.active_nodes
+        if not (sets_level or (mask & active_nodes).any()):
             raise ValueError("the problem needs a positive film coefficient, a radiating face or a fixed node")
+        normalised["fixed_temperature_mask"] = mask
+        normalised["fixed_temperature_k"] = values
 
-        if self.element_heat_w is None:
-            element_heat = np.zeros(element_shape)
-        else:
-            element_heat = np.asarray(self.element_heat_w, dtype=np.float64)
-            if element_heat.shape != element_shape:
-                raise ValueError("element_heat_w must match (slabs, rows, cols)")
-            if not np.all(np.isfinite(element_heat)):
-                raise ValueError("element heat must be finite")
-            if np.any(element_heat[~self.mesh.active] != 0.0):  # type: ignore[index]
-                raise ValueError("element heat must be zero on inactive elements")
-
-        if self.nodal_heat_w is None:
-            nodal_heat = np.zeros(node_shape)
-        else:
-            nodal_heat = np.asarray(self.nodal_heat_w, dtype=np.float64)
-            if nodal_heat.shape != node_shape:
-                raise ValueError("nodal_heat_w must match mesh.node_shape")
-            if not np.all(np.isfinite(nodal_heat)):
-                raise ValueError("nodal heat must be finite")
-            if np.any(nodal_heat[~active_nodes] != 0.0):
-                raise ValueError("nodal heat must be zero on inactive nodes")
+        support = {"element": np.asarray(mesh.active, dtype=bool), "node": active_nodes}
+        shapes = {"element": mesh.element_grid_shape, "node": mesh.node_shape}
+        for name, grid, shape_message, finite_message, support_message in _LOADS:
+            value = getattr(self, name)
+            load = np.zeros(shapes[grid]) if value is None else np.array(value, dtype=np.float64)
+            if load.shape != shapes[grid]:
+                raise ValueError(shape_message)
+            if not np.isfinite(load).all():
+                raise ValueError(finite_message)
+            if (load[~support[grid]] != 0.0).any():
+                raise ValueError(support_message)
+            normalised[name] = load
 
         sources = tuple(self.heat_sources)
         for source in sources:
             for node in source.nodes:
-                if not active_nodes.flat[flat_index(node, node_shape)]:
+                if not active_nodes.flat[flat_index(node, mesh.node_shape)]:
                     raise ValueError(f"heat source {source.name!r} touches inactive node {node!r}")
+        normalised["heat_sources"] = sources
 
-        object.__setattr__(self, "convection", convection)
-        object.__setattr__(self, "radiation", radiation)
-        object.__setattr__(self, "fixed_temperature_mask", mask.copy())
-        object.__setattr__(self, "fixed_temperature_k", values.copy())
-        object.__setattr__(self, "heat_sources", sources)
-        object.__setattr__(self, "element_heat_w", element_heat.copy())
-        object.__setattr__(self, "nodal_heat_w", nodal_heat.copy())
+        for name, value in normalised.items():
+            object.__setattr__(self, name, value)
