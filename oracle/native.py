@@ -45,6 +45,14 @@ def lib():
                                         _u8p, _int, _int, _int, _f64p, _f64p, _dbl, _dbl, _dbl, _int, _int,
                                         _int, _int, _i32p, _f64p]
         L.oracle_solve_mpir.restype = _dbl
+        L.oracle_pcg_two_level_f32.argtypes = [_f32p, _f32p, _f32p, _u8p, _f32p, _int, _int, _int, _f64p, _f32p,
+                                               _int, _f32p, _dbl, _int, _f32p, ctypes.POINTER(_int),
+                                               ctypes.POINTER(_int), ctypes.POINTER(_dbl), _int]
+        L.oracle_pcg_two_level_f32.restype = _int
+        L.oracle_solve_mpir_two_level.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f64p, _f64p, _f64p, _f64p,
+                                                  _f64p, _u8p, _int, _int, _int, _int, _f32p, _f64p, _f64p, _dbl,
+                                                  _dbl, _dbl, _int, _int, _int, _i32p, _f64p]
+        L.oracle_solve_mpir_two_level.restype = _dbl
         L.oracle_lane_sum.argtypes = [_f64p, ctypes.c_int64]
         L.oracle_lane_sum.restype = _dbl
         L.oracle_openmp.restype = _int
@@ -96,29 +104,59 @@ class OracleOperator:
                                np.ascontiguousarray(x, dtype=np.float32), y, threads or threads_default())
         return y
 
-    def pcg_f32(self, rhs, tol, max_its, threads=None):
+    @property
+    def node_shape(self):
+        s, r, c = self.dims
+        return (s + 1, r + 1, c + 1)
+
+    def two_level(self, block=None, threads=None) -> dict:
+        """The two-level coarse stage of this operator: patch width (the
+        default rule unless given), Galerkin matrix from 27 FP64 applies and
+        its dense inverse, rounded to FP32 for the inner solve."""
+
+        from . import two_level as tl
+
+        block = tl.block_size(self.node_shape) if block is None else int(block)
+        matrix = tl.coloured_galerkin(lambda v: self.apply_f64(v, threads), self.node_shape,
+                                      self.free_u8.astype(bool), block)
+        inverse = tl.spd_inverse(matrix)
+        return dict(block=block, matrix=matrix, inverse32=np.ascontiguousarray(inverse, dtype=np.float32))
+
+    def pcg_f32(self, rhs, tol, max_its, threads=None, coarse=None):
         x = np.empty(self.size, dtype=np.float32)
         its, apps, rel = _int(), _int(), _dbl()
-        bad = lib().oracle_pcg_f32(self.coef32, self.unit32, self.robin32, self.free_u8, self.mask32, *self.dims,
-                                   np.ascontiguousarray(rhs, dtype=np.float64), self.diag32, tol, max_its, x,
-                                   ctypes.byref(its), ctypes.byref(apps), ctypes.byref(rel),
-                                   threads or threads_default())
+        args = (self.coef32, self.unit32, self.robin32, self.free_u8, self.mask32, *self.dims,
+                np.ascontiguousarray(rhs, dtype=np.float64), self.diag32)
+        tail = (tol, max_its, x, ctypes.byref(its), ctypes.byref(apps), ctypes.byref(rel),
+                threads or threads_default())
+        if coarse is None:
+            bad = lib().oracle_pcg_f32(*args, *tail)
+        else:
+            bad = lib().oracle_pcg_two_level_f32(*args, coarse["block"], coarse["inverse32"], *tail)
         if bad:
             raise RuntimeError("inner PCG requires a finite symmetric positive-definite operator")
         return x, its.value, rel.value, apps.value
 
     def solve_mpir(self, rhs, x0=None, *, rtol=1e-10, atol=0.0, inner_tol=2e-3, max_outer=8, max_inner=200,
-                   inner_f64=False, threads=None):
-        """Returns (solution, dict of the MPIR result fields)."""
+                   inner_f64=False, threads=None, coarse=None):
+        """Returns (solution, dict of the MPIR result fields); ``coarse``
+        (from ``two_level``) makes the FP32 inner PCG two-level."""
 
         x = np.zeros(self.size) if x0 is None else np.array(x0, dtype=np.float64).reshape(-1)
         stats = np.zeros(6, dtype=np.int32)
         history = np.zeros(4 * max(1, max_outer))
-        rel = lib().oracle_solve_mpir(self.coef32, self.unit32, self.robin32, self.mask32, self.diag32,
-                                      self.coef64, self.unit64, self.robin64, self.mask64, self.diag64,
-                                      self.free_u8, *self.dims, np.ascontiguousarray(rhs, dtype=np.float64), x,
-                                      rtol, atol, inner_tol, max_outer, max_inner, int(inner_f64),
-                                      threads or threads_default(), stats, history)
+        ops = (self.coef32, self.unit32, self.robin32, self.mask32, self.diag32, self.coef64, self.unit64,
+               self.robin64, self.mask64, self.diag64, self.free_u8, *self.dims)
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        team = threads or threads_default()
+        if coarse is None:
+            rel = lib().oracle_solve_mpir(*ops, rhs, x, rtol, atol, inner_tol, max_outer, max_inner, int(inner_f64),
+                                          team, stats, history)
+        else:
+            if inner_f64:
+                raise ValueError("the two-level inner solve is FP32")
+            rel = lib().oracle_solve_mpir_two_level(*ops, coarse["block"], coarse["inverse32"], rhs, x, rtol, atol,
+                                                    inner_tol, max_outer, max_inner, team, stats, history)
         if stats[5]:
             raise RuntimeError("inner PCG requires a finite SPD operator")
         rows = history.reshape(-1, 4)[: int(stats[1])] if not stats[0] else history.reshape(-1, 4)[: int(stats[1])]

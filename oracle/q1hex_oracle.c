@@ -10,8 +10,10 @@
  *     (generic gather for the two end nodes of a line, the two-element
  *      x-gather with a free-mask multiply for the interior)
  *   - static line partition       hex_q1.cpp:195-205
- *   - FP32 Jacobi PCG in one SPMD region with per-line FP64 lane-sum
- *     partials, FTZ/DAZ           hex_q1.cpp:323-499 (Jacobi branch)
+ *   - FP32 PCG in one SPMD region with per-line FP64 lane-sum
+ *     partials, FTZ/DAZ           hex_q1.cpp:323-499
+ *   - its two-level preconditioner (Jacobi plus a patch-constant coarse
+ *     correction through a dense FP32 inverse)   hex_q1.cpp:388-432
  *   - FP64 MPIR outer loop        hex_q1.cpp:837-925
  *   - plain FP64 Jacobi PCG       the same PCG in double precision (the
  *                                 "plain FP64" solve of the 64x64x8 case)
@@ -131,13 +133,17 @@ static void thread_lines(int lines, int *b, int *e) {
                     const int lr0 = 4 * (z - ez) + 2 * (y - ey);                                     \
                     const idx_t erow = ((idx_t)ez * d->rows + ey) * d->cols;                         \
                     const idx_t crow = ((idx_t)ez * nr + ey) * nc;                                   \
+                    /* x lanes are independent: vectorising them keeps every sum's order */          \
+                    _Pragma("omp simd")                                                              \
                     for (int xi = 1; xi < d->cols; ++xi) {                                           \
                         T acc = 0;                                                                   \
+                        _Pragma("GCC unroll 2")                                                      \
                         for (int dx = 0; dx < 2; ++dx) {                                             \
                             const int ex = xi - 1 + dx, lr = lr0 + (1 - dx);                         \
                             const T a = coef[erow + ex], b = coef[ne + erow + ex],                   \
                                     c = coef[2 * ne + erow + ex];                                    \
                             const idx_t corner = crow + ex;                                          \
+                            _Pragma("GCC unroll 8")                                                  \
                             for (int k = 0; k < 8; ++k) {                                            \
                                 const idx_t cn = corner + (idx_t)(k >> 2) * plane + ((k >> 1) & 1) * nc + (k & 1); \
                                 const T w = (a * unit[8 * lr + k]) + (b * unit[64 + 8 * lr + k]) +   \
@@ -148,6 +154,7 @@ static void thread_lines(int lines, int *b, int *e) {
                         o[xi] += acc;                                                                \
                     }                                                                                \
                 }                                                                                    \
+            _Pragma("omp simd")                                                                      \
             for (int xi = 1; xi < d->cols; ++xi) {                                                   \
                 const idx_t n = base + xi;                                                           \
                 const T f = free_mask[n];                                                            \
@@ -177,17 +184,112 @@ DEFINE_OPERATOR(double, f64)
 DEFINE_APPLY(float, f32)
 DEFINE_APPLY(double, f64)
 
+/* ----------------------------------------------------- preconditioners */
+
+/* The coarse stage of the two-level preconditioner (hex_q1.cpp:388-432):
+ * patch-constant aggregates of ``block`` x ``block`` nodes in each node
+ * layer, and the dense FP32 inverse of their Galerkin matrix, row-major. */
+#define DEFINE_COARSE(T, SUF)                                                                        \
+    typedef struct {                                                                                 \
+        int block, coarse_rows, coarse_cols;                                                         \
+        idx_t size;                                                                                  \
+        const T *inverse;                                                                            \
+        T *restricted, *corrected;                                                                   \
+    } coarse_##SUF;                                                                                  \
+                                                                                                     \
+    /* z = r / diag on this thread's lines, then, with a coarse stage,         \
+     * z += P C^-1 P^T r: each coarse value is summed by the thread that owns  \
+     * its index (patch rows, then columns, in node order, accumulated in     \
+     * FP64 from FP32 products) so the result is the same at any thread       \
+     * count.  Called by every thread of the team: the barriers are the       \
+     * team's. */                                                                                    \
+    static void precondition_##SUF(const dims_t *d, const coarse_##SUF *cs, const T *r, const T *diag, \
+                                   const T *free_mask, T *z, int lb, int le) {                       \
+        const int nr = node_rows(d), nc = node_cols(d);                                              \
+        for (idx_t i = (idx_t)lb * nc; i < (idx_t)le * nc; ++i) z[i] = r[i] / diag[i];               \
+        if (cs == NULL) return;                                                                      \
+        int nt = 1, tid = 0;                                                                         \
+        OMP_TEAM(nt, tid);                                                                           \
+        const idx_t cb = (cs->size * tid) / nt, ce = (cs->size * (tid + 1)) / nt;                    \
+        const idx_t per_layer = (idx_t)cs->coarse_rows * cs->coarse_cols;                            \
+        const int b = cs->block;                                                                     \
+        for (idx_t c = cb; c < ce; ++c) {                                                            \
+            const int layer = (int)(c / per_layer);                                                  \
+            const int yc = (int)((c - layer * per_layer) / cs->coarse_cols);                         \
+            const int xc = (int)(c - layer * per_layer - (idx_t)yc * cs->coarse_cols);               \
+            const int y_end = (yc + 1) * b < nr ? (yc + 1) * b : nr;                                 \
+            const int x_end = (xc + 1) * b < nc ? (xc + 1) * b : nc;                                 \
+            double acc = 0.0;                                                                        \
+            for (int y = yc * b; y < y_end; ++y) {                                                   \
+                const idx_t row = ((idx_t)layer * nr + y) * nc;                                      \
+                for (int x = xc * b; x < x_end; ++x) acc += (double)(r[row + x] * free_mask[row + x]); \
+            }                                                                                        \
+            cs->restricted[c] = (T)acc;                                                              \
+        }                                                                                            \
+        _Pragma("omp barrier")                                                                       \
+        for (idx_t c = cb; c < ce; ++c) {                                                            \
+            const T *row = cs->inverse + c * cs->size;                                               \
+            const T *v = cs->restricted;                                                             \
+            double s;                                                                                \
+            LANE_SUM(s, cs->size, (double)row[i] * (double)v[i]);                                    \
+            cs->corrected[c] = (T)s;                                                                 \
+        }                                                                                            \
+        _Pragma("omp barrier")                                                                       \
+        for (int line = lb; line < le; ++line) {                                                     \
+            const int layer = line / nr, y = line - layer * nr;                                      \
+            const idx_t cbase = ((idx_t)layer * cs->coarse_rows + y / b) * cs->coarse_cols;          \
+            T *zl = z + (idx_t)line * nc;                                                            \
+            const T *ml = free_mask + (idx_t)line * nc;                                              \
+            for (int x = 0; x < nc; ++x) zl[x] += ml[x] * cs->corrected[cbase + x / b];              \
+        }                                                                                            \
+    }                                                                                                \
+                                                                                                     \
+    static int coarse_init_##SUF(coarse_##SUF *cs, const dims_t *d, int block, const T *inverse) {   \
+        if (inverse == NULL) return 0;                                                               \
+        if (block < 1) return -1;                                                                    \
+        cs->block = block;                                                                           \
+        cs->coarse_rows = (node_rows(d) + block - 1) / block;                                        \
+        cs->coarse_cols = (node_cols(d) + block - 1) / block;                                        \
+        cs->size = (idx_t)(d->slabs + 1) * cs->coarse_rows * cs->coarse_cols;                        \
+        cs->inverse = inverse;                                                                       \
+        cs->restricted = (T *)calloc((size_t)cs->size, sizeof(T));                                   \
+        cs->corrected = (T *)calloc((size_t)cs->size, sizeof(T));                                    \
+        return 1;                                                                                    \
+    }                                                                                                \
+                                                                                                     \
+    static void coarse_free_##SUF(coarse_##SUF *cs) {                                                \
+        free(cs->restricted);                                                                        \
+        free(cs->corrected);                                                                         \
+    }
+
+#ifdef _OPENMP
+#define OMP_TEAM(nt, tid)             \
+    do {                              \
+        (nt) = omp_get_num_threads(); \
+        (tid) = omp_get_thread_num(); \
+    } while (0)
+#else
+#define OMP_TEAM(nt, tid) \
+    do {                  \
+    } while (0)
+#endif
+
+DEFINE_COARSE(float, f32)
+DEFINE_COARSE(double, f64)
+
 /* ----------------------------------------------------------------- PCG */
 
-/* Inner Jacobi PCG of hex_q1.cpp:323-499, for T = float (MPIR inner solve,
- * FTZ/DAZ on) or T = double (the plain FP64 solve).  ``rhs`` is FP64 and is
+/* Inner PCG of hex_q1.cpp:323-499, for T = float (MPIR inner solve, FTZ/DAZ
+ * on) or T = double (the plain FP64 solve), with the Jacobi preconditioner
+ * or, given a coarse stage, the two-level one.  ``rhs`` is FP64 and is
  * rounded to T; the solution ``xsol`` starts at zero.  Returns 0, or 1 when
  * the curvature was not finite and positive. */
 #define DEFINE_PCG(T, SUF, FLUSH)                                                                     \
-    int oracle_pcg_##SUF(const T *coef, const T *unit, const T *robin, const uint8_t *free_nodes,     \
-                         const T *free_mask, int slabs, int rows, int cols, const double *rhs_in,     \
-                         const T *diag, double tol, int max_its, T *xsol, int *iterations_out,        \
-                         int *applications_out, double *relative_out, int threads) {                  \
+    static int pcg_core_##SUF(const T *coef, const T *unit, const T *robin, const uint8_t *free_nodes, \
+                              const T *free_mask, int slabs, int rows, int cols, const double *rhs_in, \
+                              const T *diag, const coarse_##SUF *cs, double tol, int max_its, T *xsol, \
+                              int *iterations_out, int *applications_out, double *relative_out,       \
+                              int threads) {                                                          \
         dims_t d = {slabs, rows, cols, threads < 1 ? 1 : threads};                                    \
         const int nc = node_cols(&d);                                                                 \
         const int lines = line_count(&d);                                                             \
@@ -228,7 +330,7 @@ DEFINE_APPLY(double, f64)
             if (rhs_norm == 0.0) {                                                                    \
                 l_rel = 0.0;                                                                          \
             } else {                                                                                  \
-                for (idx_t i = lo; i < hi; ++i) z[i] = r[i] / diag[i];                                \
+                precondition_##SUF(&d, cs, r, diag, free_mask, z, lb, le);                            \
                 for (idx_t i = lo; i < hi; ++i) p[i] = z[i];                                          \
                 for (int line = lb; line < le; ++line) {                                              \
                     const T *al = r + (idx_t)line * nc, *bl = z + (idx_t)line * nc;                   \
@@ -273,7 +375,7 @@ DEFINE_APPLY(double, f64)
                     l_rel = sqrt(rr) / rhs_norm;                                                      \
                     ++l_its;                                                                          \
                     if (l_rel <= tol) break;                                                          \
-                    for (idx_t i = lo; i < hi; ++i) z[i] = r[i] / diag[i];                            \
+                    precondition_##SUF(&d, cs, r, diag, free_mask, z, lb, le);                        \
                     for (int line = lb; line < le; ++line) {                                          \
                         const T *al = r + (idx_t)line * nc, *bl = z + (idx_t)line * nc;               \
                         double s;                                                                     \
@@ -312,6 +414,39 @@ DEFINE_APPLY(double, f64)
 DEFINE_PCG(float, f32, 1)
 DEFINE_PCG(double, f64, 0)
 
+#define DEFINE_PCG_EXPORTS(T, SUF)                                                                    \
+    int oracle_pcg_##SUF(const T *coef, const T *unit, const T *robin, const uint8_t *free_nodes,     \
+                         const T *free_mask, int slabs, int rows, int cols, const double *rhs_in,     \
+                         const T *diag, double tol, int max_its, T *xsol, int *iterations_out,        \
+                         int *applications_out, double *relative_out, int threads) {                  \
+        return pcg_core_##SUF(coef, unit, robin, free_nodes, free_mask, slabs, rows, cols, rhs_in,    \
+                              diag, NULL, tol, max_its, xsol, iterations_out, applications_out,       \
+                              relative_out, threads);                                                 \
+    }                                                                                                 \
+                                                                                                      \
+    /* The same PCG with the two-level preconditioner: ``inverse`` is the     \
+     * dense inverse of the Galerkin matrix of ``block`` x ``block`` patches  \
+     * (row-major, (slabs + 1) * ceil(rows + 1, block) * ceil(cols + 1, block) \
+     * squared).  Returns -1 for a block below one. */                                                \
+    int oracle_pcg_two_level_##SUF(const T *coef, const T *unit, const T *robin,                      \
+                                   const uint8_t *free_nodes, const T *free_mask, int slabs,          \
+                                   int rows, int cols, const double *rhs_in, const T *diag,           \
+                                   int block, const T *inverse, double tol, int max_its, T *xsol,     \
+                                   int *iterations_out, int *applications_out,                        \
+                                   double *relative_out, int threads) {                               \
+        dims_t d = {slabs, rows, cols, threads};                                                      \
+        coarse_##SUF cs;                                                                              \
+        if (coarse_init_##SUF(&cs, &d, block, inverse) != 1) return -1;                               \
+        const int bad = pcg_core_##SUF(coef, unit, robin, free_nodes, free_mask, slabs, rows, cols,   \
+                                       rhs_in, diag, &cs, tol, max_its, xsol, iterations_out,         \
+                                       applications_out, relative_out, threads);                      \
+        coarse_free_##SUF(&cs);                                                                       \
+        return bad;                                                                                   \
+    }
+
+DEFINE_PCG_EXPORTS(float, f32)
+DEFINE_PCG_EXPORTS(double, f64)
+
 /* ---------------------------------------------------------------- MPIR */
 
 /* The FP64 outer loop of hex_q1.cpp:837-925: r = b - A64 x, lane-sum norm,
@@ -321,13 +456,14 @@ DEFINE_PCG(double, f64, 0)
  * relative residual) per correction, 4 doubles each, at most max_outer rows.
  * ``stats`` receives converged, outer, inner, high applications, low
  * applications, not_spd.  Returns the final relative residual. */
-double oracle_solve_mpir(const float *coef32, const float *unit32, const float *robin32,
-                         const float *mask32, const float *diag32, const double *coef64,
-                         const double *unit64, const double *robin64, const double *mask64,
-                         const double *diag64, const uint8_t *free_nodes, int slabs, int rows,
-                         int cols, const double *rhs, double *x, double rtol, double atol,
-                         double inner_tol, int max_outer, int max_inner, int inner_f64,
-                         int threads, int *stats, double *history) {
+static double solve_mpir_core(const float *coef32, const float *unit32, const float *robin32,
+                              const float *mask32, const float *diag32, const double *coef64,
+                              const double *unit64, const double *robin64, const double *mask64,
+                              const double *diag64, const uint8_t *free_nodes, int slabs, int rows,
+                              int cols, const coarse_f32 *cs, const double *rhs, double *x,
+                              double rtol, double atol, double inner_tol, int max_outer,
+                              int max_inner, int inner_f64, int threads, int *stats,
+                              double *history) {
     dims_t d = {slabs, rows, cols, threads < 1 ? 1 : threads};
     const idx_t n = (idx_t)line_count(&d) * node_cols(&d);
     double bn;
@@ -363,9 +499,8 @@ double oracle_solve_mpir(const float *coef32, const float *unit32, const float *
                                  res, diag64, inner_tol, max_inner, corr64, &its, &apps, &irel,
                                  threads);
         } else {
-            bad = oracle_pcg_f32(coef32, unit32, robin32, free_nodes, mask32, slabs, rows, cols,
-                                 res, diag32, inner_tol, max_inner, corr32, &its, &apps, &irel,
-                                 threads);
+            bad = pcg_core_f32(coef32, unit32, robin32, free_nodes, mask32, slabs, rows, cols, res,
+                               diag32, cs, inner_tol, max_inner, corr32, &its, &apps, &irel, threads);
         }
         if (bad) {
             not_spd = 1;
@@ -393,6 +528,42 @@ double oracle_solve_mpir(const float *coef32, const float *unit32, const float *
     stats[3] = high;
     stats[4] = low;
     stats[5] = not_spd;
+    return rel;
+}
+
+/* The FP64 outer loop of hex_q1.cpp:837-925 over Jacobi-preconditioned
+ * inner solves (see solve_mpir_core for the arguments and outputs). */
+double oracle_solve_mpir(const float *coef32, const float *unit32, const float *robin32,
+                         const float *mask32, const float *diag32, const double *coef64,
+                         const double *unit64, const double *robin64, const double *mask64,
+                         const double *diag64, const uint8_t *free_nodes, int slabs, int rows,
+                         int cols, const double *rhs, double *x, double rtol, double atol,
+                         double inner_tol, int max_outer, int max_inner, int inner_f64,
+                         int threads, int *stats, double *history) {
+    return solve_mpir_core(coef32, unit32, robin32, mask32, diag32, coef64, unit64, robin64, mask64,
+                           diag64, free_nodes, slabs, rows, cols, NULL, rhs, x, rtol, atol, inner_tol,
+                           max_outer, max_inner, inner_f64, threads, stats, history);
+}
+
+/* The same outer loop with the FP32 two-level inner PCG (hex_q1.cpp:837-925
+ * with the preconditioner of :388-432); ``inverse`` as for
+ * oracle_pcg_two_level_f32.  Returns -1.0 for a block below one. */
+double oracle_solve_mpir_two_level(const float *coef32, const float *unit32, const float *robin32,
+                                   const float *mask32, const float *diag32, const double *coef64,
+                                   const double *unit64, const double *robin64,
+                                   const double *mask64, const double *diag64,
+                                   const uint8_t *free_nodes, int slabs, int rows, int cols,
+                                   int block, const float *inverse, const double *rhs, double *x,
+                                   double rtol, double atol, double inner_tol, int max_outer,
+                                   int max_inner, int threads, int *stats, double *history) {
+    dims_t d = {slabs, rows, cols, threads};
+    coarse_f32 cs;
+    if (coarse_init_f32(&cs, &d, block, inverse) != 1) return -1.0;
+    const double rel = solve_mpir_core(coef32, unit32, robin32, mask32, diag32, coef64, unit64,
+                                       robin64, mask64, diag64, free_nodes, slabs, rows, cols, &cs,
+                                       rhs, x, rtol, atol, inner_tol, max_outer, max_inner, 0,
+                                       threads, stats, history);
+    coarse_free_f32(&cs);
     return rel;
 }
 

@@ -12,14 +12,28 @@ expectation the same way) and the preconditioner is
 ``M^-1 r = r / diag + P C^-1 P^T r``.
 
 Pinned by: the Galerkin product against a brute-force double sum over node
-pairs on tiny grids, SPD of ``M^-1``, and the reduction to Jacobi when every
-patch is empty (tests/test_two_level.py).
+pairs on tiny grids, SPD of ``M^-1``, the reduction to Jacobi when every
+patch is empty, the coloured assembly against the sparse product, and the C
+port's two-level solve of the reference's 4-slab fixture against the
+iteration counts the reference publishes for it (tests/test_two_level.py).
 """
 
 from __future__ import annotations
 
 import numpy as np
 import scipy.sparse as sp
+
+
+def block_size(node_shape, max_coarse_size: int = 2048, minimum: int = 4) -> int:
+    """The default patch width (two_level.py:37-51): the smallest width from
+    ``minimum`` whose coarse space holds at most ``max_coarse_size`` patches,
+    or the width of the wider node side."""
+
+    layers, rows, cols = (int(v) for v in node_shape)
+    block = max(1, int(minimum))
+    while layers * -(-rows // block) * -(-cols // block) > max_coarse_size and block < max(rows, cols):
+        block += 1
+    return block
 
 
 def coarse_shape(node_shape, block):
@@ -75,6 +89,56 @@ def galerkin_brute(a_dense: np.ndarray, node_shape, free, block) -> np.ndarray:
             coarse[:, k] = 0.0
             coarse[k, k] = 1.0
     return coarse
+
+
+def coloured_galerkin(apply_f64, node_shape, free, block) -> np.ndarray:
+    """``P^T A P`` as the native path assembles it (hex_q1.cpp:703-835): 27
+    FP64 applications of ``A`` to the indicator of the free nodes in patches
+    of one colour (node layer mod 3, patch row mod 3, patch column mod 3),
+    each restricted to patch sums in node order; patches of one colour are
+    three apart, so a patch's sum is its entry against the one neighbour of
+    that colour.  Empty patches get a unit diagonal."""
+
+    shape = coarse_shape(node_shape, block)
+    free = np.asarray(free, dtype=bool).reshape(node_shape)
+    layer, row, col = np.indices(node_shape)
+    patch = np.ravel_multi_index((layer, row // block, col // block), shape)
+    colour = (layer % 3) * 9 + ((row // block) % 3) * 3 + (col // block) % 3
+    size = int(np.prod(shape))
+    free_patches = patch[free]
+    coarse_layer, coarse_row, coarse_col = np.indices(shape)
+    coarse_colour = (coarse_layer % 3) * 9 + (coarse_row % 3) * 3 + coarse_col % 3
+    matrix = np.zeros((size, size))
+    for c in range(27):
+        probe = free & (colour == c)
+        if not probe.any():
+            continue
+        action = np.asarray(apply_f64(probe.astype(np.float64).reshape(-1)), dtype=np.float64).reshape(node_shape)
+        restricted = np.bincount(free_patches, weights=action[free], minlength=size)
+        for dl in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    nl, ny, nx = coarse_layer + dl, coarse_row + dy, coarse_col + dx
+                    ok = ((nl >= 0) & (nl < shape[0]) & (ny >= 0) & (ny < shape[1]) & (nx >= 0) & (nx < shape[2]))
+                    i = np.ravel_multi_index((coarse_layer[ok], coarse_row[ok], coarse_col[ok]), shape)
+                    j = np.ravel_multi_index((nl[ok], ny[ok], nx[ok]), shape)
+                    hit = coarse_colour.reshape(-1)[j] == c
+                    matrix[i[hit], j[hit]] = restricted[i[hit]]
+    empty = np.bincount(free_patches, minlength=size) == 0
+    matrix[empty, :] = 0.0
+    matrix[:, empty] = 0.0
+    matrix[empty, empty] = 1.0
+    return matrix
+
+
+def spd_inverse(matrix: np.ndarray) -> np.ndarray:
+    """Inverse through the Cholesky factor, symmetrised (the native path's
+    LLT solve of the identity, hex_q1.cpp:818-823)."""
+
+    factor = np.linalg.cholesky(matrix)
+    lower_inverse = np.linalg.solve(factor, np.eye(matrix.shape[0]))
+    inverse = lower_inverse.T @ lower_inverse
+    return 0.5 * (inverse + inverse.T)
 
 
 def preconditioner(a: sp.spmatrix, diag, node_shape, free, block):

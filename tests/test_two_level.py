@@ -147,3 +147,95 @@ def test_oracle_two_level_pcg_cuts_iterations_on_a_copper_stack():
     x, two = T.pcg(a, b, T.preconditioner(a, ops["diag"], shape, ops["free"], block), 2e-3, 20000)
     assert two * 5 < jacobi
     assert np.linalg.norm(a @ x - b) <= 2e-3 * np.linalg.norm(b)
+
+
+def _port(problem):
+    from oracle.native import OracleOperator
+
+    a, ops = _system(problem)
+    return a, ops, OracleOperator(ops)
+
+
+def test_oracle_block_rule_is_the_reference_rule():
+    for shape, width in (((4, 41, 41), 4), ((4, 201, 201), 10), ((2, 3, 3), 4), ((5, 51, 51), 4), ((5, 101, 101), 6),
+                         ((5, 201, 201), 11)):
+        assert T.block_size(shape) == width == choose_block_size(shape)
+    assert T.block_size((2, 5, 9), max_coarse_size=1) == 9
+
+
+@pytest.mark.parametrize("block", [1, 2, 3, 5])
+def test_c_port_coloured_assembly_is_the_galerkin_product(block):
+    # The 27-colour probing of the native path (hex_q1.cpp:703-835) against
+    # the sparse P^T A P, which the brute-force double sum pins; scattered
+    # fixed nodes leave some patches empty at width 1.
+    problem = stack(7, 9, fixed=True)
+    a, ops, port = _port(problem)
+    expected = T.galerkin(a, problem.mesh.node_shape, ops["free"], block)
+    coarse = port.two_level(block, threads=2)
+    np.testing.assert_allclose(coarse["matrix"], expected, rtol=1e-12, atol=1e-12 * np.abs(expected).max())
+    inverse = T.spd_inverse(coarse["matrix"])
+    np.testing.assert_allclose(inverse @ expected, np.eye(len(expected)), atol=1e-9)
+    assert coarse["inverse32"].tobytes() == inverse.astype(np.float32).tobytes()
+
+
+@pytest.mark.parametrize("block", [1, 2, 3])
+def test_c_port_two_level_first_step_is_along_the_preconditioned_residual(block):
+    # One FP32 PCG step from zero is x1 = alpha z with z = M^-1 r: the
+    # restriction, the coarse inverse and the prolongation of the C port
+    # against the FP64 action through the explicit sparse P.  The direction
+    # is pinned node by node; alpha = r.z / z.Az only loosely, because the
+    # FP32 apply cancels heavily on the smooth coarse part of z (about 1e-3
+    # of the curvature).  The expectation uses the same FP32-rounded coarse
+    # inverse, applied in FP64.
+    problem = stack(6, 8, fixed=True)
+    a, ops, port = _port(problem)
+    shape = problem.mesh.node_shape
+    rhs = (ops["load"] + np.random.default_rng(1).uniform(0.0, 1e-3, port.size)) * ops["free"]
+    coarse = port.two_level(block)
+    prolong = T.prolongation(shape, ops["free"], block)
+    z = rhs / ops["diag"] + prolong @ (coarse["inverse32"].astype(np.float64) @ (prolong.T @ rhs))
+    expected = (rhs @ z) / (z @ (a @ z)) * z
+    x1, its, _, _ = port.pcg_f32(rhs, 1e-30, 1, threads=2, coarse=coarse)
+    assert its == 1
+    moved = expected != 0.0
+    assert not x1[~moved].any()
+    ratio = x1[moved] / expected[moved]
+    assert np.ptp(ratio) <= 1e-5 * abs(ratio.mean())
+    assert abs(ratio.mean() - 1.0) <= 1e-2
+    # A plain Jacobi step points elsewhere.
+    jacobi = port.pcg_f32(rhs, 1e-30, 1, threads=2)[0][moved] / expected[moved]
+    assert np.ptp(jacobi) > 0.1
+
+
+def test_c_port_two_level_solve_is_independent_of_the_thread_count():
+    problem = copper_stack(16, 20)
+    _, ops, port = _port(problem)
+    coarse = port.two_level()
+    rhs = ops["load"] * ops["free"]
+    one = port.solve_mpir(rhs, max_outer=16, threads=1, coarse=coarse)
+    three = port.solve_mpir(rhs, max_outer=16, threads=3, coarse=coarse)
+    assert one[1]["converged"] and one[1]["inner_iterations"] == three[1]["inner_iterations"]
+    assert one[0].tobytes() == three[0].tobytes()
+
+
+def test_c_port_two_level_solve_reproduces_the_published_fixture_counts():
+    # The reference's 4-slab fixture at 50 x 50 elements (13,005 nodes;
+    # experiments/thermal_native_benchmark.py:43-63) with its MPIR limits
+    # (16 outer steps, inner 2e-3 / 200) takes 7 outer and 584 inner
+    # iterations with the native two-level solve
+    # (docs/THERMAL_MPIR_FEM.md:472-474).
+    from pcbhip import workloads
+
+    problem = workloads.four_slab_fixture(50)
+    p = to_oracle(problem)
+    ops = P.operator_arrays(p)
+    from oracle.native import OracleOperator
+
+    port = OracleOperator(ops)
+    stiffness = OracleOperator(ops, stiffness=True)
+    rhs = P.build_rhs(ops, P.default_reference(ops, p), stiffness.apply_f64)
+    coarse = port.two_level()
+    assert coarse["block"] == 4 and coarse["matrix"].shape == (845, 845)
+    _, result = port.solve_mpir(rhs, rtol=1e-10, max_outer=16, max_inner=200, coarse=coarse)
+    assert result["converged"]
+    assert (result["outer_iterations"], result["inner_iterations"]) == (7, 584)
