@@ -4,9 +4,13 @@ dominant kernel (the fused Jacobi-PCG pass) and of the Q1 apply, and the
 reference algorithm timed on the host's cores.
 
 One step is one complete MPIR solve (FP64 outer residual, update and
-convergence check; FP32 Jacobi-PCG inner solves) of the seeded 512 x 512 x 16
-copper/FR-4 board with its right-hand side already resident in HBM.  Prints
-one JSON line.
+convergence check; FP32 Jacobi-PCG inner solves) of the seeded board
+(``--size`` x ``--size`` x ``--slabs``, copper/FR-4) with its right-hand side
+already resident in HBM.  Beside the headline the default line carries, on
+short schedules, the 200-board uneven batch, the 64-excitation DC board and
+the served stream at 50 requests/s, each against the single-solve path on
+the same work.  Prints one JSON line; ``--workload`` selects one of the
+workloads as a line of its own.
 """
 
 from __future__ import annotations
@@ -27,7 +31,6 @@ for p in (ROOT / "pcb-analysis_amd" / "src", ROOT):
         sys.path.insert(0, str(p))
 
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = "board 512x512x16 elements (4,473,873 nodes), seeded copper/FR-4, 8 sources"
 
 
 def log(msg: str) -> None:
@@ -157,7 +160,7 @@ def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
     )
 
 
-def _oracle_error(op, threads: int, rhs, gpu_solution) -> dict:
+def _oracle_error(op, threads: int, rhs, gpu_solution) -> tuple[dict, np.ndarray]:
     """The FP64 oracle's solve of the GPU's right-hand side (FP64 inner PCG,
     relative residual 1e-12) and the GPU solution's error against it,
     relative to the largest value of the oracle's rise."""
@@ -171,148 +174,199 @@ def _oracle_error(op, threads: int, rhs, gpu_solution) -> dict:
                 fp64_oracle_converged=bool(res64["converged"]), fp64_oracle_solve_s=seconds), x64
 
 
-def cpu_baseline(problem, gpu_result, gpu_solve_s: float, *, full_rhs=None, gpu_solution=None,
-                 oracle_rhs=None) -> dict:
-    """The reference algorithm (oracle C restatement of hex_q1.cpp's FP32
-    Jacobi PCG) on the host cores: a bounded sample of inner iterations on the
-    same board, scaled by the GPU solve's own iteration counts.
+def _iteration_sample(op, threads: int) -> tuple[float, float]:
+    """Seconds per FP32 Jacobi-PCG iteration and per FP64 apply of the port on
+    ``threads`` threads: the difference of a 2- and a 6-iteration PCG (it
+    drops the per-call set-up) and two FP64 applies."""
 
-    With ``full_rhs`` (``--cpu-full``, minutes of host time) it instead runs
-    the whole reference solve of the GPU's right-hand side, and an FP64
-    oracle solve (FP64 inner PCG, relative residual 1e-12) against which it
-    states the relative error of the GPU solution and of the reference
-    algorithm's.  With ``oracle_rhs`` (and ``gpu_solution``) the sample is
-    followed by that FP64 oracle solve and the GPU solution's error."""
-
-    from oracle import problem as oracle_problem
-    from oracle.native import OracleOperator, threads_default
-
-    ops = oracle_problem.operator_arrays(oracle_problem.from_problem(problem))
-    op = OracleOperator(ops)
-    threads = threads_default()
-    if full_rhs is not None:
-        return _cpu_full(op, threads, full_rhs, gpu_solution)
     rhs = np.random.default_rng(0).standard_normal(op.size)
-    # Two sample lengths: their difference drops the per-call set-up (the
-    # oracle allocates and first-touches its vectors on every call).
-    short, its = 10, 40
-    op.pcg_f32(rhs, 1e-30, 2, threads=threads)  # warm-up
+    op.pcg_f32(rhs, 1e-30, 1, threads=threads)
     t0 = time.perf_counter()
-    _, done_short, _, _ = op.pcg_f32(rhs, 1e-30, short, threads=threads)
+    _, short, _, _ = op.pcg_f32(rhs, 1e-30, 2, threads=threads)
     t1 = time.perf_counter()
-    _, done, _, _ = op.pcg_f32(rhs, 1e-30, its, threads=threads)
-    per_iter = ((time.perf_counter() - t1) - (t1 - t0)) / max(done - done_short, 1)
-    x = rhs.copy()
+    _, long, _, _ = op.pcg_f32(rhs, 1e-30, 6, threads=threads)
+    per_iteration = ((time.perf_counter() - t1) - (t1 - t0)) / max(long - short, 1)
     t0 = time.perf_counter()
-    for _ in range(3):
-        op.apply_f64(x, threads=threads)
-    per_f64 = (time.perf_counter() - t0) / 3
-    inner = gpu_result["inner_iterations"]
-    outer = gpu_result["high_operator_applications"]
-    solve_s = inner * per_iter + outer * per_f64
-    out = dict(value=1.0 / solve_s, unit="solves/s", cores=threads, kind="port",
-               sample=f"{its} less {short} FP32 Jacobi-PCG iterations + 3 FP64 applies of the same board on "
-                      f"{threads} host threads (oracle C restatement of hex_q1.cpp), scaled by the GPU solve's "
-                      f"{inner} inner / {outer} outer steps: {per_iter * 1e3:.2f} ms/iteration, "
-                      f"est. {solve_s:.1f} s/solve")
-    if oracle_rhs is not None:
-        out.update(_oracle_error(op, threads, oracle_rhs, gpu_solution)[0])
+    for _ in range(2):
+        op.apply_f64(rhs, threads=threads)
+    return per_iteration, (time.perf_counter() - t0) / 2
+
+
+# The reference's published two-level solve of its 4-slab fixture at 50 x 50
+# elements (13,005 nodes), C++ on a Xeon Platinum 8581C, construction
+# excluded (docs/THERMAL_MPIR_FEM.md:472-474).
+FIXTURE_PUBLISHED = {"elements": 50, "nodes": 13005, "outer": 7, "inner": 584, "ms_1_thread": 216.5,
+                     "ms_16_threads": 26.6}
+
+
+def _fixture_calibration(threads: int) -> dict:
+    """The port's two-level solve of the reference's 4-slab fixture, timed as
+    the reference times it (median of 5 after a warm-up), on one thread and
+    on ``threads``, beside the published figures."""
+
+    from oracle import native as N
+    from oracle import problem as P
+    from pcbhip import workloads
+
+    p = P.from_problem(workloads.four_slab_fixture(FIXTURE_PUBLISHED["elements"]))
+    ops = P.operator_arrays(p)
+    op = N.OracleOperator(ops)
+    rhs = P.build_rhs(ops, P.default_reference(ops, p), N.OracleOperator(ops, stiffness=True).apply_f64)
+    t0 = time.perf_counter()
+    coarse = op.two_level()
+    construction_ms = (time.perf_counter() - t0) * 1e3
+    out = {"published": FIXTURE_PUBLISHED, "construction_ms": construction_ms}
+    for team in sorted({1, threads}):
+        result, ms = _timed_ms(lambda: op.solve_mpir(rhs, rtol=1e-10, max_outer=16, max_inner=200, threads=team,
+                                                     coarse=coarse)[1], repeats=5, warmups=1)
+        out[f"{team}_threads"] = {"solve_ms": ms, "outer": result["outer_iterations"],
+                                  "inner": result["inner_iterations"], "converged": result["converged"]}
+    out["ratio_1_thread_vs_published"] = out["1_threads"]["solve_ms"] / FIXTURE_PUBLISHED["ms_1_thread"]
+    log(f"cpu fixture calibration: {out}")
     return out
 
 
-def _cpu_full(op, threads: int, rhs, gpu_solution) -> dict:
-    limits = dict(max_outer=60, max_inner=100000, threads=threads)
+def cpu_baseline(problem, rhs, gpu_solution, *, oracle_error: bool = True, calibrate: bool = True) -> dict:
+    """The reference algorithm on the host's cores: the oracle's C
+    restatement of hex_q1.cpp (FP32 Jacobi-PCG inner, FP64 outer, the GPU
+    line's limits), built for this host with -march=native, solving the
+    GPU's right-hand side whole on every thread the process may use.  Beside
+    it: a one-thread figure (per-iteration sample on one thread, scaled by
+    the whole solve's own counts), the port's two-level solve of the
+    reference's 4-slab fixture against the published times, and the FP64
+    oracle's solve, against which the GPU's and the port's answers are
+    stated."""
+
+    from oracle import native as N
+    from oracle import problem as P
+
+    build = N.use_native_build()
+    op = N.OracleOperator(P.operator_arrays(P.from_problem(problem)))
+    threads = N.threads_default()
     t0 = time.perf_counter()
-    x_port, res = op.solve_mpir(rhs, rtol=1e-10, **limits)
+    x_port, res = op.solve_mpir(rhs, rtol=1e-10, max_outer=60, max_inner=100000, threads=threads)
     port_s = time.perf_counter() - t0
-    log(f"cpu reference algorithm: {port_s:.1f} s, {res['inner_iterations']} inner iterations")
-    errors, x64 = _oracle_error(op, threads, rhs, gpu_solution)
-    scale = float(np.max(np.abs(x64)))
-    return dict(value=1.0 / port_s, unit="solves/s", cores=threads, kind="port",
-                sample=f"the whole solve on {threads} host threads (oracle C restatement of hex_q1.cpp): "
-                       f"{res['outer_iterations']} outer / {res['inner_iterations']} inner, {port_s:.1f} s",
-                fp64_oracle_rel_error_cpu_reference=float(np.max(np.abs(x_port - x64))) / scale, **errors)
+    log(f"cpu reference algorithm: {port_s:.1f} s on {threads} threads, {res['outer_iterations']} outer / "
+        f"{res['inner_iterations']} inner")
+    per_iteration, per_f64 = _iteration_sample(op, 1)
+    one_s = res["inner_iterations"] * per_iteration + res["high_operator_applications"] * per_f64
+    out = dict(
+        value=1.0 / port_s, unit="solves/s", cores=threads, kind="port",
+        sample=f"the whole solve of the GPU's right-hand side on {threads} host threads (oracle C restatement of "
+               f"hex_q1.cpp, {build}): {res['outer_iterations']} outer / {res['inner_iterations']} inner, "
+               f"{port_s:.1f} s",
+        build=build, converged=bool(res["converged"]), outer_iterations=res["outer_iterations"],
+        inner_iterations=res["inner_iterations"],
+        one_thread={"solves_per_s": 1.0 / one_s, "est_solve_s": one_s, "ms_per_iteration": per_iteration * 1e3,
+                    "ms_per_fp64_apply": per_f64 * 1e3,
+                    "sample": "4 FP32 Jacobi-PCG iterations and 2 FP64 applies on one thread, scaled by the "
+                              "whole solve's counts"},
+    )
+    if calibrate:
+        out["fixture_calibration"] = _fixture_calibration(threads)
+    if oracle_error:
+        errors, x64 = _oracle_error(op, threads, rhs, gpu_solution)
+        scale = float(np.max(np.abs(x64)))
+        out.update(errors, fp64_oracle_rel_error_cpu_reference=float(np.max(np.abs(x_port - x64))) / scale)
+    return out
 
 
-def run_batch200(args, torch) -> dict:
-    """200 uneven boards solved together in one batch (every launch covers
-    every board still iterating) against the same 200 solved one at a time
-    through the single-solve path."""
+def batch_leg(args, torch, *, count: int, warmup: int, steps: int) -> dict:
+    """``count`` uneven boards solved together in one batch (every launch
+    covers every board still iterating; tile shape from the autotuner) and
+    the same boards one at a time through the single-solve path."""
 
     from electrical.matrix_free_mpir_fem import MPIRConfig
-    from pcbhip import _capi, workloads
+    from pcbhip import _capi, autotune, workloads
     from thermal.matrix_free_mpir_fem import MatrixFreeThermalOperator
 
-    count = args.count
-    log(f"building {count} uneven boards")
+    log(f"batch: building {count} uneven boards")
     boards = workloads.uneven_boards(count)
     t0 = time.perf_counter()
     ops = [MatrixFreeThermalOperator(b, preconditioner=args.preconditioner) for b in boards]
-    log(f"operators ({args.preconditioner}) built in {time.perf_counter() - t0:.1f} s")
+    log(f"batch: operators ({args.preconditioner}) built in {time.perf_counter() - t0:.1f} s")
     rhss = [o.build_rhs(o.default_reference_temperature()) for o in ops]
     nodes = sum(o.size for o in ops)
-    config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000)
+    config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000).to_c()
     batch = _capi.Batch([o.device_operator for o in ops])
-    batch.set_shape(args.rows, 1024)
+    slots = [(b, None, config) for b in rhss]
     for i, b in enumerate(rhss):
-        batch.set(i, b, None, config.to_c())
-    log("batched warm-up")
-    for _ in range(args.warmup):
+        batch.set(i, b, None, config)
+    key = autotune.board_set_key("batch", [o.mesh.element_grid_shape for o in ops])
+    shape = autotune.tune_batch(batch, key, slots=slots)
+    log(f"batch: tile shape rows={shape['rows']} target={shape['target']} (cached {shape['cached']})")
+    for _ in range(warmup):
         batch.run()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         batch.run()
-    batched_s = (time.perf_counter() - t0) / args.steps
+    batched_s = (time.perf_counter() - t0) / steps
     results = [batch.result(i, solution=False)[1] for i in range(count)]
     converged = sum(r["converged"] for r in results)
-    log(f"batched: {batched_s * 1e3:.1f} ms for {count} solves ({converged} converged)")
+    work = sum(r["inner_iterations"] * _pass_bytes(o.device_operator) for r, o in zip(results, ops))
+    log(f"batch: {batched_s * 1e3:.1f} ms for {count} solves ({converged} converged), "
+        f"{work / batched_s / 1e9:.0f} GB/s of pass bytes")
     singles = [_capi.Batch([o.device_operator]) for o in ops]
     for sb, b in zip(singles, rhss):
         sb.set_shape(args.rows, 1024)
-        sb.set(0, b, None, config.to_c())
+        sb.set(0, b, None, config)
         sb.run()  # warm-up: graph capture of every single-solve batch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for sb in singles:
         sb.run()
     single_s = time.perf_counter() - t0
-    log(f"one at a time: {single_s * 1e3:.1f} ms")
+    log(f"batch: one at a time {single_s * 1e3:.1f} ms")
     # The batched answers are the single answers bit for bit.
     same = all(batch.result(i)[0].tobytes() == singles[i].result(0)[0].tobytes() for i in range(0, count, 17))
+    for sb in singles:
+        sb.close()
+    batch.close()
     return {
-        "metric": "MPIR solves/s at 1e-10 rel. residual (200 uneven boards, one batch)",
-        "value": count / batched_s,
+        "workload": f"{count} boards, sides log-uniform 32..384, depth 4..16, {nodes:,} nodes in total",
+        "solves_per_s": count / batched_s,
+        "ms_per_batch": batched_s * 1e3,
+        "pass_gbs": work / batched_s / 1e9,
+        "single_solve_path_pass_gbs": work / single_s / 1e9,
+        "single_solve_path_solves_per_s": count / single_s,
+        "speedup_vs_single_solve_path": single_s / batched_s,
+        "converged": converged,
+        "bitwise_equal_alone_and_in_batch": bool(same),
+        "tile_shape": {k: shape[k] for k in ("rows", "target", "us", "cached")},
+        "preconditioner": args.preconditioner,
+        "total_inner_iterations": int(sum(r["inner_iterations"] for r in results)),
+        "max_inner_iterations": int(max(r["inner_iterations"] for r in results)),
+    }
+
+
+def run_batch200(args, torch) -> dict:
+    """The batch of uneven boards as its own line (``--workload batch200``)."""
+
+    leg = batch_leg(args, torch, count=args.count, warmup=args.warmup, steps=args.steps)
+    return {
+        "metric": f"MPIR solves/s at 1e-10 rel. residual ({args.count} uneven boards, one batch)",
+        "value": leg["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": batched_s * 1e3,
+        "ms_per_step": leg["ms_per_batch"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 inner / f64 outer",
         "data": "synthetic (seeded uneven-board generator, seed 4)",
-        "config": {
-            "workload": f"{count} boards, sides log-uniform 32..384, depth 4..16, {nodes:,} nodes in total",
-            "preconditioner": args.preconditioner,
-            "converged": converged,
-            "single_solve_path_solves_per_s": count / single_s,
-            "speedup_vs_single_solve_path": single_s / batched_s,
-            "bitwise_equal_alone_and_in_batch": bool(same),
-            "total_inner_iterations": int(sum(r["inner_iterations"] for r in results)),
-            "max_inner_iterations": int(max(r["inner_iterations"] for r in results)),
-            "parallelism": "single GPU",
-        },
+        "config": dict(leg, parallelism="single GPU"),
     }
 
 
-def run_multirhs64(args, torch) -> dict:
+def multirhs_leg(args, torch, *, warmup: int, steps: int, variants=("slots", "block_mfma", "block_vector")) -> dict:
     """64 terminal-pair excitations of the 256 x 256 x 12 copper/dielectric
-    DC board on one device-resident operator, solved together with the block
-    apply (groups of right-hand sides sharing the operator's reads, matrix-core
-    element action), against the same 64 as plain slots of one batch and
-    solved one at a time."""
+    DC board on one device-resident operator: as plain slots of one batch
+    (the default), with the block apply (groups of right-hand sides sharing
+    the operator's reads; matrix-core or vector element action) and one at a
+    time through the single-solve path."""
 
     from electrical.matrix_free_mpir_fem import MPIRConfig
     from electrical.matrix_free_mpir_fem.hex_dc import MatrixFreeHexDCOperator
@@ -321,85 +375,99 @@ def run_multirhs64(args, torch) -> dict:
     board = workloads.dc_board(size=args.dc_size, slabs=12, pairs=64)
     operator = MatrixFreeHexDCOperator(board.mesh, reference_node=board.reference_node)
     rhss = [operator.build_rhs(t) for t in board.excitations]
-    config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000)
+    config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000).to_c()
+    plans = {"slots": (0, 0), "block_mfma": (args.block, args.engine), "block_vector": (args.block, 0)}
 
     def timed_batch(group: int, engine: int):
         batch = _capi.Batch([operator.device_operator] * len(rhss))
         batch.set_shape(args.rows if group == 0 else 4, 1024)
         batch.set_block(group, engine)
         for i, b in enumerate(rhss):
-            batch.set(i, b, None, config.to_c())
-        for _ in range(args.warmup):
+            batch.set(i, b, None, config)
+        for _ in range(warmup):
             batch.run()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             batch.run()
-        seconds = (time.perf_counter() - t0) / args.steps
+        seconds = (time.perf_counter() - t0) / steps
         results = [batch.result(i, solution=False)[1] for i in range(len(rhss))]
         return batch, seconds, results
 
-    variants = {}
-    for name, group, engine in (("block_mfma", args.block, args.engine), ("block_vector", args.block, 0),
-                                ("slots", 0, 0)):
-        batch, seconds, results = timed_batch(group, engine)
-        variants[name] = (batch, seconds, results)
+    runs = {}
+    for name in variants:
+        batch, seconds, results = timed_batch(*plans[name])
+        runs[name] = (batch, seconds, results)
         log(f"64 rhs, {name}: {seconds * 1e3:.1f} ms, {sum(r['converged'] for r in results)} converged, "
             f"max inner {max(r['inner_iterations'] for r in results)}")
     single = _capi.Batch([operator.device_operator])
     single.set_shape(args.rows, 1024)
-    single.set(0, rhss[0], None, config.to_c())
+    single.set(0, rhss[0], None, config)
     single.run()
     t0 = time.perf_counter()
     for b in rhss:
         single.set(0, b, None, None)
         single.run()
     single_s = time.perf_counter() - t0
+    single.close()
     log(f"64 rhs one at a time: {single_s * 1e3:.1f} ms")
-    # Answers of the matrix-core block path against the plain FP32 slots (both
-    # refined to 1e-10 in FP64): relative difference in potential.
-    # (Potential on the terminals' copper cluster, where the 1e-10 residual
-    # determines it; floating islands couple only through the 1e-12 S/m
-    # dielectric.)
-    mf_batch, mf_s, mf_results = variants["block_mfma"]
-    pl_batch = variants["slots"][0]
+    plain_batch, plain_s, plain_results = runs["slots"]
+    out = {
+        "workload": f"{board.mesh.size:,} nodes, 64 terminal pairs in one copper cluster",
+        "path": "plain FP32 slots of one batch sharing the device operator",
+        "solves_per_s": len(rhss) / plain_s,
+        "ms_per_batch": plain_s * 1e3,
+        "single_solve_path_solves_per_s": len(rhss) / single_s,
+        "speedup_vs_single_solve_path": single_s / plain_s,
+        "converged": int(sum(r["converged"] for r in plain_results)),
+        "total_inner_iterations": int(sum(r["inner_iterations"] for r in plain_results)),
+        "max_inner_iterations": int(max(r["inner_iterations"] for r in plain_results)),
+    }
+    # The block apply's answers against the plain FP32 slots (both refined
+    # to 1e-10 in FP64): relative difference in potential on the terminals'
+    # copper cluster, where the 1e-10 residual determines it (floating
+    # islands couple only through the 1e-12 S/m dielectric).
     top = board.mesh.node_shape[0] - 1
     on = np.zeros(board.mesh.node_shape[1:], dtype=bool)
     for dy in (0, 1):
         for dx in (0, 1):
             on[dy:dy + board.cluster.shape[0], dx:dx + board.cluster.shape[1]] |= board.cluster
-    diff = 0.0
-    for i in range(0, len(rhss), 9):
-        a = mf_batch.result(i)[0].reshape(board.mesh.node_shape)[top][on]
-        b = pl_batch.result(i)[0].reshape(board.mesh.node_shape)[top][on]
-        diff = max(diff, float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
+    for name in variants:
+        if name == "slots":
+            continue
+        batch, seconds, results = runs[name]
+        diff = 0.0
+        for i in range(0, len(rhss), 9):
+            a = batch.result(i)[0].reshape(board.mesh.node_shape)[top][on]
+            b = plain_batch.result(i)[0].reshape(board.mesh.node_shape)[top][on]
+            diff = max(diff, float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
+        out[name] = {"solves_per_s": len(rhss) / seconds, "vs_plain_slots": plain_s / seconds,
+                     "group": args.block, "engine": plans[name][1],
+                     "converged": int(sum(r["converged"] for r in results)),
+                     "max_rel_difference_vs_fp32_slots_on_cluster": diff}
+    for batch, _, _ in runs.values():
+        batch.close()
+    return out
+
+
+def run_multirhs64(args, torch) -> dict:
+    """The 64-excitation DC board as its own line (``--workload multirhs64``)."""
+
+    leg = multirhs_leg(args, torch, warmup=args.warmup, steps=args.steps)
     return {
         "metric": "MPIR solves/s at 1e-10 rel. residual (64 RHS, 256x256x12 DC board)",
-        "value": len(rhss) / mf_s,
+        "value": leg["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": mf_s * 1e3,
+        "ms_per_step": leg["ms_per_batch"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "split-bf16 matrix-core apply, f32 inner / f64 outer",
+        "dtype": "f32 inner / f64 outer",
         "data": "synthetic (seeded DC board generator, seed 3)",
-        "config": {
-            "workload": f"{board.mesh.size:,} nodes, 64 terminal pairs in one copper cluster",
-            "path": f"block apply, groups of {args.block} right-hand sides, matrix-core element action "
-                    f"({args.engine}-term bf16 split)",
-            "converged": int(sum(r["converged"] for r in mf_results)),
-            "block_vector_solves_per_s": len(rhss) / variants["block_vector"][1],
-            "plain_slots_solves_per_s": len(rhss) / variants["slots"][1],
-            "single_solve_path_solves_per_s": len(rhss) / single_s,
-            "speedup_vs_single_solve_path": single_s / mf_s,
-            "max_rel_difference_vs_fp32_slots_on_cluster": diff,
-            "total_inner_iterations": int(sum(r["inner_iterations"] for r in mf_results)),
-            "max_inner_iterations": int(max(r["inner_iterations"] for r in mf_results)),
-            "parallelism": "single GPU",
-        },
+        "config": dict(leg, parallelism="single GPU"),
     }
 
 
@@ -541,24 +609,32 @@ def run_pass(args, torch) -> dict:
     }
 
 
-def run_serve(args, torch) -> dict:
+def _pass_bytes(op) -> int:
+    """Algorithmic bytes of one fused PCG pass over one slot of ``op``."""
+
+    s, r, c = op.element_shape
+    return 40 * op.size + 12 * s * r * c
+
+
+def serve_leg(args, torch, *, rates, duration: float, drain_limit: float) -> dict:
     """The serving stream: Poisson arrivals of thermal (70 %) and DC (30 %)
     requests of 1..16 right-hand sides on pool boards of the uneven-batch
     size distribution, admitted into a 200 GB device budget and iterated by
-    continuous batching; against the same request mix solved one right-hand
-    side at a time through the single-solve path."""
+    continuous batching (tile shape tuned on the pool); against the same
+    request mix solved one right-hand side at a time through the
+    single-solve path."""
 
     from electrical.matrix_free_mpir_fem import MPIRConfig
     from pcbhip import _capi, serving, workloads
 
-    log("building the serving pool")
+    log("serve: building the pool")
     pool = workloads.serving_pool(seed=5, thermal_boards=args.pool_thermal, dc_boards=args.pool_dc)
     thermal = [b for b in pool if b.kind == "thermal"]
     dc = [b for b in pool if b.kind == "dc"]
     config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000).to_c()
     rng = np.random.default_rng(5)
 
-    def make_requests(rate: float, duration: float) -> list:
+    def make_requests(rate: float) -> list:
         out = []
         for i, t in enumerate(serving.poisson_arrivals(rng, rate, duration)):
             group = thermal if (rng.random() < 0.7 or not dc) else dc
@@ -588,15 +664,18 @@ def run_serve(args, torch) -> dict:
             times.append(time.perf_counter() - t0)
         single_s[id(b.operator)] = float(np.mean(times))
         sb.close()
-    log(f"single-solve path: {len(pool)} boards, mean {1e3 * np.mean(list(single_s.values())):.1f} ms per solve")
+    log(f"serve: single-solve path, {len(pool)} boards, mean {1e3 * np.mean(list(single_s.values())):.1f} ms per solve")
     lines = []
-    for rate in args.rates:
-        requests = make_requests(rate, args.duration)
+    shape = None
+    for rate in rates:
+        requests = make_requests(rate)
         server = serving.SolveServer(max_slots=args.max_slots, memory_budget_bytes=200e9,
                                      steps_per_replay=args.replay_steps, max_running_nodes=args.max_running_nodes,
-                                     rows_inner=args.rows)
-        log(f"rate {rate}/s: {len(requests)} requests, {sum(len(r.rhs) for r in requests)} right-hand sides")
-        elapsed = serving.run_arrivals(server, requests, log=log, time_limit_s=args.duration + args.drain_limit)
+                                     tune_with=[b.operator for b in pool])
+        shape = server.shape
+        log(f"serve: rate {rate}/s, {len(requests)} requests, {sum(len(r.rhs) for r in requests)} right-hand sides, "
+            f"tile shape {shape}")
+        elapsed = serving.run_arrivals(server, requests, log=log, time_limit_s=duration + drain_limit)
         done = server.completed
         solves = sum(len(r.results) for r in done)
         unfinished = len(requests) - len(done)
@@ -605,11 +684,15 @@ def run_serve(args, torch) -> dict:
         single_rate = solves / single_time if single_time > 0 else None
         summary = serving.latency_summary(done) if unfinished == 0 else {"p50_ms": None, "p99_ms": None}
         info = server.backend.info()
+        # Algorithmic pass bytes of the inner iterations the served solves did,
+        # over the device time of the server's replays.
+        work = sum(x["inner_iterations"] * _pass_bytes(by_id[r.request_id].operator) for r in done for x in r.results)
+        replay_s = info["replay_ms"] / 1e3
         line = {
             "rate_per_s": rate,
             "requests": len(requests),
             "unfinished": unfinished,
-            "offered_solves_per_s": sum(len(r.rhs) for r in requests) / args.duration,
+            "offered_solves_per_s": sum(len(r.rhs) for r in requests) / duration,
             "solves_per_s": solves / elapsed,
             "single_solve_path_solves_per_s": single_rate,
             "speedup_vs_single_solve_path": (solves / elapsed) / single_rate if single_rate else None,
@@ -618,6 +701,10 @@ def run_serve(args, torch) -> dict:
             "p50_ms_thermal": serving.latency_summary([r for r in done if r.kind == "thermal"])["p50_ms"],
             "p50_ms_dc": serving.latency_summary([r for r in done if r.kind == "dc"])["p50_ms"],
             "converged": int(sum(sum(x["converged"] for x in r.results) for r in done)),
+            "inner_iterations_thermal": int(sum(x["inner_iterations"] for r in done if r.kind == "thermal"
+                                                for x in r.results)),
+            "inner_iterations_dc": int(sum(x["inner_iterations"] for r in done if r.kind == "dc" for x in r.results)),
+            "pass_gbs_over_replays": work / replay_s / 1e9 if replay_s > 0 else None,
             "peak_requests_in_flight": server.peak_running_requests,
             "peak_slots": server.peak_running_slots,
             "peak_admitted_gb": server.peak_bytes / 1e9,
@@ -627,12 +714,25 @@ def run_serve(args, torch) -> dict:
             "pump_s": server.pump_s,
             "native_step_s": server.step_s,
             "blocked_on_device_s": info["wait_ms"] / 1e3,
-            "device_replay_s": info["replay_ms"] / 1e3,
+            "device_replay_s": replay_s,
+            "elapsed_s": elapsed,
         }
-        log(json.dumps(line))
+        log("serve: " + json.dumps(line))
         lines.append(line)
         server.close()
-    best = max(lines, key=lambda l: l["solves_per_s"])
+    return {
+        "workload": f"Poisson arrivals, 70 % thermal / 30 % DC, 1..16 right-hand sides, "
+                    f"{len(thermal)} + {len(dc)} pool boards, {duration:g} s of arrivals per rate",
+        "tile_shape": shape,
+        "rates": lines,
+    }
+
+
+def run_serve(args, torch) -> dict:
+    """The serving stream as its own line (``--workload serve``)."""
+
+    leg = serve_leg(args, torch, rates=args.rates, duration=args.duration, drain_limit=args.drain_limit)
+    best = max(leg["rates"], key=lambda l: l["solves_per_s"])
     return {
         "metric": "served MPIR solves/s at 1e-10 rel. residual with p50/p99 latency",
         "value": best["solves_per_s"],
@@ -646,14 +746,8 @@ def run_serve(args, torch) -> dict:
         "vs_baseline": None,
         "dtype": "f32 inner / f64 outer",
         "data": "synthetic (seeded serving pool, seed 5)",
-        "config": {
-            "workload": f"Poisson arrivals, 70 % thermal / 30 % DC, 1..16 right-hand sides, "
-                        f"{len(thermal)} + {len(dc)} pool boards, {args.duration:.0f} s per rate",
-            "single_solve_path_solves_per_s": best["single_solve_path_solves_per_s"],
-            "speedup_vs_single_solve_path": best["speedup_vs_single_solve_path"],
-            "rates": lines,
-            "parallelism": "single GPU",
-        },
+        "config": dict(leg, single_solve_path_solves_per_s=best["single_solve_path_solves_per_s"],
+                       speedup_vs_single_solve_path=best["speedup_vs_single_solve_path"], parallelism="single GPU"),
     }
 
 
@@ -825,15 +919,17 @@ def main() -> int:
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--no-oracle", action="store_true",
                         help="skip the FP64 oracle solve that states the GPU solution's error")
-    parser.add_argument("--cpu-full", action="store_true",
-                        help="time the whole reference solve on the host and state the FP64-oracle errors (minutes)")
+    parser.add_argument("--no-legs", action="store_true",
+                        help="the board solve alone: no batch, multi-excitation or served summaries")
+    parser.add_argument("--leg-duration", type=float, default=3.0,
+                        help="seconds of arrivals of the served summary in the default line")
     parser.add_argument("--rows", type=int, default=4, help="rows per wave strip of the FP32 inner stencil")
     parser.add_argument("--target", type=int, default=1024, help="stencil workgroups per launch before z blocks split")
     parser.add_argument("--workload", default="board512",
                         choices=["board512", "batch200", "multirhs64", "serve", "apply_engines", "coupling",
                                  "two_level", "pass"])
     parser.add_argument("--board", default="standard", choices=["standard", "general"],
-                        help="pass: the benchmark board or the board with no stencil shortcuts")
+                        help="board512, pass: the benchmark board or the board with no stencil shortcuts")
     parser.add_argument("--targets", type=int, nargs="+", default=[512, 1024, 2048, 4096],
                         help="pass: stencil workgroup targets to sweep")
     parser.add_argument("--no-board", action="store_true", help="two_level: the fixture sizes only")
@@ -859,35 +955,36 @@ def main() -> int:
 
     import torch
 
-    from electrical.matrix_free_mpir_fem import MPIRConfig
-    from pcbhip import _capi, workloads
-    from thermal.matrix_free_mpir_fem import MatrixFreeThermalOperator
+    from pcbhip import _capi
 
     _capi.require_device(0)
     torch.cuda.init()
-    if args.workload == "batch200":
-        print(json.dumps(run_batch200(args, torch)), flush=True)
-        return 0
-    if args.workload == "multirhs64":
-        print(json.dumps(run_multirhs64(args, torch)), flush=True)
-        return 0
-    if args.workload == "serve":
-        print(json.dumps(run_serve(args, torch)), flush=True)
-        return 0
-    if args.workload == "apply_engines":
-        print(json.dumps(run_apply_engines(args, torch)), flush=True)
-        return 0
-    if args.workload == "coupling":
-        print(json.dumps(run_coupling(args, torch)), flush=True)
-        return 0
-    if args.workload == "two_level":
-        print(json.dumps(run_two_level(args, torch)), flush=True)
-        return 0
-    if args.workload == "pass":
-        print(json.dumps(run_pass(args, torch)), flush=True)
-        return 0
+    runs = {"board512": run_board, "batch200": run_batch200, "multirhs64": run_multirhs64, "serve": run_serve,
+            "apply_engines": run_apply_engines, "coupling": run_coupling, "two_level": run_two_level,
+            "pass": run_pass}
+    print(json.dumps(runs[args.workload](args, torch)), flush=True)
+    return 0
+
+
+def run_board(args, torch) -> dict:
+    """The default line: one board's MPIR solve (the headline), the roofline
+    of its dominant kernel, and -- unless ``--no-legs`` -- the batched,
+    served and many-excitation workloads as summaries beside it, and the CPU
+    baseline."""
+
+    from electrical.matrix_free_mpir_fem import MPIRConfig
+    from pcbhip import _capi
+    from thermal.matrix_free_mpir_fem import MatrixFreeThermalOperator
+
     log("building board")
-    problem = workloads.board_512x512x16(size=args.size, slabs=args.slabs)
+    problem = _board(args)
+    s, r, c = problem.mesh.element_grid_shape
+    nodes = problem.mesh.size
+    if args.board == "general":
+        workload = (f"general board {c}x{r}x{s} elements ({nodes:,} nodes): jittered per-element k, "
+                    "films on both faces and the sides")
+    else:
+        workload = f"board {c}x{r}x{s} elements ({nodes:,} nodes), seeded copper/FR-4, 8 sources"
     operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
     reference = operator.default_reference_temperature()
     rhs = operator.build_rhs(reference)
@@ -924,8 +1021,9 @@ def main() -> int:
     copy_at_apply = copy_bandwidth(torch, stream, roof["roofline"]["algorithmic_bytes"] // 2 // 16 * 16)
     copy_at_pass = copy_bandwidth(torch, stream, fused["roofline"]["algorithmic_bytes"] // 2 // 16 * 16)
     log(f"copy at apply bytes: {copy_at_apply:.0f} GB/s, at pass bytes: {copy_at_pass:.0f} GB/s")
+    batch.close()
     line = {
-        "metric": "MPIR solves/s at 1e-10 rel. residual (512x512x16 board)",
+        "metric": f"MPIR solves/s at 1e-10 rel. residual ({c}x{r}x{s} board)",
         "value": args.steps / elapsed,
         "unit": "solves/s",
         "n_gpus": 1,
@@ -936,9 +1034,9 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 inner / f64 outer",
-        "data": "synthetic (seeded board generator)",
+        "data": f"synthetic (seeded {args.board} board generator)",
         "config": {
-            "workload": WORKLOAD,
+            "workload": workload,
             "solver": "MPIR, FP32 Jacobi-PCG inner (rtol 2e-3), FP64 outer to 1e-10",
             "converged": result["converged"],
             "relative_residual": result["relative_residual"],
@@ -969,17 +1067,22 @@ def main() -> int:
     convective = float(np.sum(operator.convective_heat(temperature)))
     line["config"]["heat_input_w"] = heat_input
     line["config"]["heat_balance_error_w"] = heat_input - convective - fixed_heat
+    if not args.no_legs:
+        # The batched, multi-excitation and served workloads on short
+        # schedules: their speed-ups over the single-solve path on the same
+        # work, and the served latency at 50 requests/s.
+        batched = batch_leg(args, torch, count=args.count, warmup=1, steps=1)
+        many = multirhs_leg(args, torch, warmup=1, steps=1, variants=("slots", "block_mfma"))
+        served = serve_leg(args, torch, rates=[50.0], duration=args.leg_duration, drain_limit=args.drain_limit)
+        line["config"]["batch_200_uneven_boards"] = batched
+        line["config"]["dc_board_64_excitations"] = many
+        line["config"]["served_stream_50_per_s"] = dict(served["rates"][0], workload=served["workload"],
+                                                       tile_shape=served["tile_shape"])
     if not args.no_cpu:
         log("cpu baseline")
-        if args.cpu_full:
-            extra = dict(full_rhs=rhs, gpu_solution=x)
-        elif args.no_oracle:
-            extra = {}
-        else:
-            extra = dict(oracle_rhs=rhs, gpu_solution=x)
-        line["cpu_baseline"] = cpu_baseline(problem, result, elapsed / args.steps, **extra)
-    print(json.dumps(line), flush=True)
-    return 0
+        line["cpu_baseline"] = cpu_baseline(problem, rhs, x, oracle_error=not args.no_oracle,
+                                            calibrate=not args.no_legs)
+    return line
 
 
 if __name__ == "__main__":

@@ -27,6 +27,23 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
 
 
+def use_native_build() -> str:
+    """Build the port for this host's ISA (``make native``, -march=native as
+    the reference's default build) and load that build; returns which build
+    is in use.  Falls back to the portable build when the host cannot
+    compile it.  Call before the first use of the library."""
+
+    global LIB_PATH
+    if _lib is not None:
+        return "portable x86-64-v3 (already loaded)" if LIB_PATH.parent.name == "_build" else "-march=native"
+    try:
+        subprocess.run(["make", "-s", "-C", str(_HERE), "native"], check=True, capture_output=True, timeout=120)
+    except (OSError, subprocess.SubprocessError):
+        return "portable x86-64-v3 (the native build failed on this host)"
+    LIB_PATH = _HERE / "_build" / "native" / "libq1hex_oracle.so"
+    return "-march=native"
+
+
 def lib():
     global _lib
     if _lib is None:

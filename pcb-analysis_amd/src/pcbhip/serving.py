@@ -92,13 +92,24 @@ class SolveServer:
     bounds the nodes times right-hand sides in flight: past the point where
     the device is full, more work in flight only stretches every request's
     latency.  ``backend`` is the native server (a ``_capi.Server`` unless a
-    test passes a stand-in with the same methods)."""
+    test passes a stand-in with the same methods).  ``tune_with``, a list of
+    device operators typical of the traffic, picks the tile shape by timing
+    the fused pass over them (``autotune.tune_server``, cached per set of
+    board shapes) instead of ``rows_inner`` / ``target_groups``."""
 
     def __init__(self, device: int = 0, *, max_slots: int = 512, memory_budget_bytes: float = 200e9,
                  steps_per_replay: int = 16, max_running_nodes: int | None = None, keep_solutions: bool = True,
-                 rows_inner: int = 4, target_groups: int = 1024, backend=None, clock=time.perf_counter):
+                 rows_inner: int = 4, target_groups: int = 1024, tune_with=None, backend=None,
+                 clock=time.perf_counter):
         self.backend = backend if backend is not None else _capi.Server(device, max_slots, False, steps_per_replay)
-        if backend is None:
+        self.shape = {"rows": int(rows_inner), "target": int(target_groups), "tuned": False}
+        if tune_with:
+            from . import autotune
+
+            chosen = autotune.tune_server(self.backend, tune_with)
+            self.shape = {"rows": chosen["rows"], "target": chosen["target"], "tuned": True,
+                          "cached": chosen["cached"]}
+        elif backend is None:
             self.backend.set_shape(rows_inner, target_groups)
         self.max_slots = int(max_slots)
         self.memory_budget_bytes = float(memory_budget_bytes)
