@@ -280,6 +280,37 @@ int pcbhip_server_trim(pcbhip_server *server);
 /* Bandwidth reference: copy n_floats (multiple of 4) device floats. */
 int pcbhip_copy_dev(const float *src, float *dst, int64_t n_floats, void *stream);
 
+/* FP32 vectors of the inner-solve runtime on one device: the vector
+ * operations of the LowPrecisionRuntime contract
+ * (src/electrical/matrix_free_mpir_fem/runtime.py:20-44; the device runtime
+ * CupyFloat32Runtime, runtime.py:121-170) for callers that drive an inner
+ * PCG loop from the host.  Each call returns when its work is done (it runs
+ * on a stream of the vector runtime's own and waits for it).  n is the
+ * length in floats; n = 0 is a no-op.  Outputs must not overlap inputs.
+ *   create   a zeroed vector of n floats (at least one is allocated);
+ *   destroy  release one (NULL is a no-op);
+ *   write    host -> device, read  device -> host, copy  device -> device;
+ *   dot      sum of x[i] * y[i] accumulated in FP64 in an order fixed by n
+ *            (the same bits on every call; the reference accumulates in FP32);
+ *   axpy     out = y + alpha * x with FP32 rounding of the product and the sum,
+ *            as runtime.py:74-82 computes it;
+ *   divide   out = num / den, correctly rounded (runtime.py:84-90). */
+int pcbhip_vec_create(int device, int64_t n, float **v);
+int pcbhip_vec_destroy(int device, float *v);
+int pcbhip_vec_write(int device, float *dst, const float *src, int64_t n);
+int pcbhip_vec_read(int device, const float *src, float *dst, int64_t n);
+int pcbhip_vec_copy(int device, const float *src, float *dst, int64_t n);
+int pcbhip_vec_dot(int device, const float *x, const float *y, int64_t n, double *out);
+int pcbhip_vec_axpy(int device, float alpha, const float *x, const float *y, float *out, int64_t n);
+int pcbhip_vec_divide(int device, const float *num, const float *den, float *out, int64_t n);
+/* z = M^-1 r in FP32 on device vectors of pcbhip_operator_size() floats,
+ * returning when done (replaces MatrixFreeThermalOperator.precondition_low on
+ * runtime arrays, operator.py:366-371). */
+int pcbhip_operator_precondition_f32_dev(const pcbhip_operator *op, const float *r, float *z);
+/* Wait for the work queued on the operator's own stream (pcbhip_apply_f32_dev
+ * with a NULL stream runs there). */
+int pcbhip_operator_synchronize(const pcbhip_operator *op);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1191,6 +1191,52 @@ std::pair<int, int> coarse_grid(const pcbhip_operator* op, int block, int nc) {
     return {(int)crows, (int)ccols};
 }
 
+// Per-device state of the FP32 vector runtime: a stream of its own
+// (non-blocking, default priority: no implicit waits on other streams, so it
+// never disturbs a server capturing its graph in another thread) and the dot
+// product's partial sums and result (kDotGroupsMax + 1 doubles), made on
+// first use with the device current and kept.  Every call returns when its
+// work is done, so the runtime's calls order with every other call of this
+// library; a call holds the lock from its first launch until it has waited.
+struct VectorRuntime {
+    std::mutex lock;
+    hipStream_t stream = nullptr;
+    double* scratch = nullptr;
+};
+
+VectorRuntime& vector_runtime(int device) {
+    static std::mutex table_lock;
+    static std::vector<std::unique_ptr<VectorRuntime>> table;
+    std::lock_guard<std::mutex> guard(table_lock);
+    if ((size_t)device >= table.size()) table.resize((size_t)device + 1);
+    if (!table[(size_t)device]) {
+        auto vr = std::make_unique<VectorRuntime>();
+        {
+            DeviceLock lock(device_mutex());
+            check(hipStreamCreateWithFlags(&vr->stream, hipStreamNonBlocking), "vector stream");
+        }
+        vr->scratch = static_cast<double*>(device_alloc(sizeof(double) * (kDotGroupsMax + 1)));
+        table[(size_t)device] = std::move(vr);
+    }
+    return *table[(size_t)device];
+}
+
+void check_vector_args(int device, const void* a, const void* b, int64_t n) {
+    if (device < 0) fail(PCBHIP_INVALID_INPUT, "device must be non-negative");
+    if (n < 0) fail(PCBHIP_INVALID_INPUT, "vector length must be non-negative");
+    if (n > 0 && (!a || !b)) fail(PCBHIP_INVALID_INPUT, "null vector");
+}
+
+// Runs ``body(runtime)`` on the device's vector stream and waits for it.
+template <class F>
+void on_vector_stream(int device, const char* what, F&& body) {
+    set_device(device);
+    VectorRuntime& vr = vector_runtime(device);
+    std::lock_guard<std::mutex> guard(vr.lock);
+    body(vr);
+    check(hipStreamSynchronize(vr.stream), what);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1972,6 +2018,128 @@ int pcbhip_solve_mpir(const pcbhip_operator* op, const double* rhs, double* x, c
     (void)pcbhip_batch_destroy(b);
     g_last_error = message;
     return rc;
+}
+
+int pcbhip_vec_create(int device, int64_t n, float** v) {
+    return guarded([&] {
+        if (!v) fail(PCBHIP_INVALID_INPUT, "null output");
+        check_vector_args(device, v, v, n);
+        set_device(device);
+        VectorRuntime& vr = vector_runtime(device);
+        *v = nullptr;
+        const size_t bytes = sizeof(float) * (size_t)std::max<int64_t>(n, 1);
+        float* p = static_cast<float*>(device_alloc(bytes));
+        std::lock_guard<std::mutex> guard(vr.lock);
+        hipError_t err = hipMemsetAsync(p, 0, bytes, vr.stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(vr.stream);
+        if (err != hipSuccess) {
+            (void)device_free(p);
+            check(err, "vector clear");
+        }
+        *v = p;
+    });
+}
+
+int pcbhip_vec_destroy(int device, float* v) {
+    return guarded([&] {
+        if (device < 0) fail(PCBHIP_INVALID_INPUT, "device must be non-negative");
+        if (!v) return;
+        set_device(device);
+        check(device_free(v), "hipFree");
+    });
+}
+
+int pcbhip_vec_write(int device, float* dst, const float* src, int64_t n) {
+    return guarded([&] {
+        check_vector_args(device, dst, src, n);
+        if (n == 0) return;
+        on_vector_stream(device, "vector write", [&](VectorRuntime& vr) {
+            check(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, vr.stream),
+                  "vector write");
+        });
+    });
+}
+
+int pcbhip_vec_read(int device, const float* src, float* dst, int64_t n) {
+    return guarded([&] {
+        check_vector_args(device, dst, src, n);
+        if (n == 0) return;
+        on_vector_stream(device, "vector read", [&](VectorRuntime& vr) {
+            check(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, vr.stream),
+                  "vector read");
+        });
+    });
+}
+
+int pcbhip_vec_copy(int device, const float* src, float* dst, int64_t n) {
+    return guarded([&] {
+        check_vector_args(device, dst, src, n);
+        if (n == 0) return;
+        on_vector_stream(device, "vector copy", [&](VectorRuntime& vr) {
+            check(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, vr.stream),
+                  "vector copy");
+        });
+    });
+}
+
+int pcbhip_vec_dot(int device, const float* x, const float* y, int64_t n, double* out) {
+    return guarded([&] {
+        if (!out) fail(PCBHIP_INVALID_INPUT, "null output");
+        check_vector_args(device, x, y, n);
+        if (n == 0) {
+            *out = 0.0;
+            return;
+        }
+        on_vector_stream(device, "dot", [&](VectorRuntime& vr) {
+            check(launch::dot(x, y, n, vr.scratch, vr.scratch + kDotGroupsMax, vr.stream), "dot launch");
+            check(hipMemcpyAsync(out, vr.scratch + kDotGroupsMax, sizeof(double), hipMemcpyDeviceToHost, vr.stream),
+                  "dot read");
+        });
+    });
+}
+
+int pcbhip_vec_axpy(int device, float alpha, const float* x, const float* y, float* out, int64_t n) {
+    return guarded([&] {
+        check_vector_args(device, x, y, n);
+        check_vector_args(device, out, out, n);
+        if (n == 0) return;
+        on_vector_stream(device, "axpy", [&](VectorRuntime& vr) {
+            check(launch::axpy(alpha, x, y, out, n, vr.stream), "axpy launch");
+        });
+    });
+}
+
+int pcbhip_vec_divide(int device, const float* num, const float* den, float* out, int64_t n) {
+    return guarded([&] {
+        check_vector_args(device, num, den, n);
+        check_vector_args(device, out, out, n);
+        if (n == 0) return;
+        on_vector_stream(device, "divide", [&](VectorRuntime& vr) {
+            check(launch::divide(num, den, out, n, vr.stream), "divide launch");
+        });
+    });
+}
+
+int pcbhip_operator_precondition_f32_dev(const pcbhip_operator* op, const float* r, float* z) {
+    return guarded([&] {
+        if (!op || !r || !z) fail(PCBHIP_INVALID_INPUT, "null argument");
+        set_device(op->device);
+        const size_t coarse = align256(sizeof(double) * (size_t)std::max(op->nc, 1));
+        std::lock_guard<std::mutex> guard(op->staging_mutex);
+        char* base = staging_of(op, std::max(2 * coarse, 2 * align256(sizeof(double) * (size_t)op->N)));
+        double* g = reinterpret_cast<double*>(base);
+        double* w = reinterpret_cast<double*>(base + coarse);
+        check(launch::precondition(op->view(), r, z, g, w, op->stream), "precondition launch");
+        check(hipStreamSynchronize(op->stream), "precondition");
+    });
+}
+
+int pcbhip_operator_synchronize(const pcbhip_operator* op) {
+    return guarded([&] {
+        if (!op) fail(PCBHIP_INVALID_INPUT, "null operator");
+        set_device(op->device);
+        check(hipStreamSynchronize(op->stream), "operator stream");
+    });
 }
 
 int pcbhip_copy_dev(const float* src, float* dst, int64_t n_floats, void* stream) {

@@ -198,6 +198,9 @@ constexpr int kCoarseGroups = 1024;
 // Largest coarse space a slot may carry (its inverse is nc^2 FP32 values).
 constexpr int kMaxCoarse = 16384;
 
+// Workgroups of the first stage of the vector runtime's dot product.
+constexpr int kDotGroupsMax = 1024;
+
 // Slots that share one operator, applied together by the block apply
 // (at most kGroupMax right-hand sides per group).
 constexpr int kGroupMax = 64;
@@ -222,6 +225,12 @@ hipError_t sheet_apply(const OperatorDev& op, const T* x, T* y, int mode, hipStr
 hipError_t sheet_diagonal(const OperatorDev& op, double* diag, double* dinv64, float* dinv32, int* bad,
                           hipStream_t stream);
 hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream);
+// FP32 vector runtime: dot (FP64, an order fixed by n; ``partial`` holds
+// kDotGroupsMax doubles, ``out`` one), out = y + alpha x, out = num / den.
+int dot_groups(long long n);
+hipError_t dot(const float* x, const float* y, long long n, double* partial, double* out, hipStream_t stream);
+hipError_t axpy(float alpha, const float* x, const float* y, float* out, long long n, hipStream_t stream);
+hipError_t divide(const float* num, const float* den, float* out, long long n, hipStream_t stream);
 // One outer step (FP64 residual and decision) for the slots in their outer
 // phase, and one fused Jacobi-PCG pass for the slots in their inner phase.
 // ``counts`` (device) holds the live lengths of the outer, inner and block

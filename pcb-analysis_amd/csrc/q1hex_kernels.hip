@@ -1033,6 +1033,66 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
     if (i < n4) dst[i] = src[i];
 }
 
+// ------------------------------------------------------- vector runtime
+
+// The FP32 vector operations of the inner-solve runtime, for callers that
+// drive a PCG loop from the host.  The dot product accumulates FP32 products
+// (exact in FP64) in FP64 in an order fixed by the length alone: workgroup g
+// sums the contiguous range g of kDotGroupsMax at most (threads stride it,
+// then a fixed LDS tree), and one workgroup adds the partials in the same
+// tree, so a vector gives the same bits on every call.
+constexpr int kVecThreads = 256;
+
+__device__ double block_tree_sum(double v, double* lds) {
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kVecThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    return lds[0];
+}
+
+__global__ __launch_bounds__(kVecThreads) void k_dot_partial(const float* __restrict__ x, const float* __restrict__ y,
+                                                             const long long n, const long long per,
+                                                             double* __restrict__ partial) {
+    __shared__ double lds[kVecThreads];
+    const long long lo = (long long)blockIdx.x * per;
+    const long long hi = lo + per < n ? lo + per : n;
+    double acc = 0.0;
+    for (long long i = lo + threadIdx.x; i < hi; i += kVecThreads) acc += (double)x[i] * (double)y[i];
+    const double total = block_tree_sum(acc, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kVecThreads) void k_dot_final(const double* __restrict__ partial, const int groups,
+                                                           double* __restrict__ out) {
+    __shared__ double lds[kVecThreads];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < groups; i += kVecThreads) acc += partial[i];
+    const double total = block_tree_sum(acc, lds);
+    if (threadIdx.x == 0) out[0] = total;
+}
+
+// out = y + alpha x with the two roundings of FP32 arithmetic (product, then
+// sum); no contraction into an FMA (the library is built without it).
+__global__ __launch_bounds__(kVecThreads) void k_axpy(const float alpha, const float* __restrict__ x,
+                                                      const float* __restrict__ y, float* __restrict__ out,
+                                                      const long long n) {
+    const long long i = (long long)blockIdx.x * kVecThreads + threadIdx.x;
+    if (i < n) {
+        const float scaled = alpha * x[i];
+        out[i] = y[i] + scaled;
+    }
+}
+
+// out = num / den, correctly rounded FP32 division.
+__global__ __launch_bounds__(kVecThreads) void k_divide(const float* __restrict__ num, const float* __restrict__ den,
+                                                        float* __restrict__ out, const long long n) {
+    const long long i = (long long)blockIdx.x * kVecThreads + threadIdx.x;
+    if (i < n) out[i] = num[i] / den[i];
+}
+
 // ---------------------------------------------------------- MPIR kernels
 
 template <typename TI>
@@ -1830,6 +1890,33 @@ hipError_t diagonal(const double* coef, const double* robin, double* diag, doubl
     const int blocks = (int)((N + 255) / 256);
     hipLaunchKernelGGL(k_diagonal, dim3(blocks), dim3(256), 0, stream, coef, robin, diag, dinv64, dinv32, S, R, C,
                        bad);
+    return hipGetLastError();
+}
+
+int dot_groups(long long n) {
+    const long long groups = (n + 4095) / 4096;
+    return (int)(groups < 1 ? 1 : (groups > kDotGroupsMax ? kDotGroupsMax : groups));
+}
+
+hipError_t dot(const float* x, const float* y, long long n, double* partial, double* out, hipStream_t stream) {
+    const int groups = dot_groups(n);
+    const long long per = (n + groups - 1) / groups;
+    hipLaunchKernelGGL(k_dot_partial, dim3(groups), dim3(kVecThreads), 0, stream, x, y, n, per, partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kVecThreads), 0, stream, partial, groups, out);
+    return hipGetLastError();
+}
+
+hipError_t axpy(float alpha, const float* x, const float* y, float* out, long long n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const long long blocks = (n + kVecThreads - 1) / kVecThreads;
+    hipLaunchKernelGGL(k_axpy, dim3((unsigned)blocks), dim3(kVecThreads), 0, stream, alpha, x, y, out, n);
+    return hipGetLastError();
+}
+
+hipError_t divide(const float* num, const float* den, float* out, long long n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const long long blocks = (n + kVecThreads - 1) / kVecThreads;
+    hipLaunchKernelGGL(k_divide, dim3((unsigned)blocks), dim3(kVecThreads), 0, stream, num, den, out, n);
     return hipGetLastError();
 }
 

@@ -120,8 +120,19 @@ class MatrixFreeHexDCOperator:
     def apply_high(self, vector) -> np.ndarray:
         return self._device.apply_f64(vector)
 
-    def apply_low(self, vector) -> np.ndarray:
+    def apply_low(self, vector):
+        """``A x`` in FP32: device vector to device vector, or host to host."""
+
+        if isinstance(vector, _capi.DeviceVector):
+            return self._device.apply_f32_vec(vector)
         return self._device.apply_f32(vector)
+
+    def precondition_low(self, vector):
+        """``M^-1 r`` in FP32 with the operator's Jacobi diagonal."""
+
+        if isinstance(vector, _capi.DeviceVector):
+            return self._device.precondition_f32_vec(vector)
+        return self._device.precondition_f32(np.asarray(vector, dtype=np.float32).reshape(-1))
 
     def diagonal_low(self) -> np.ndarray:
         return self._device.diagonal().astype(np.float32)

@@ -320,7 +320,11 @@ class MatrixFreePCBOperator:
     def apply_high(self, vector) -> np.ndarray:
         return self._device.apply_f64(self._flat(vector))
 
-    def apply_low(self, vector) -> np.ndarray:
+    def apply_low(self, vector):
+        """``A x`` in FP32: device vector to device vector, or host to host."""
+
+        if isinstance(vector, _capi.DeviceVector):
+            return self._device.apply_f32_vec(vector)
         return self._device.apply_f32(np.asarray(vector, dtype=np.float32).reshape(-1))
 
     def apply_full_high(self, vector) -> np.ndarray:
@@ -333,8 +337,11 @@ class MatrixFreePCBOperator:
         return self._device.diagonal().astype(np.float32)
 
     def precondition_low(self, vector) -> np.ndarray:
-        """``M^-1 r`` in FP32 on the device (pcb.py:712-717)."""
+        """``M^-1 r`` in FP32 on the device (pcb.py:712-717); device vector
+        in, device vector out, or host array in and out."""
 
+        if isinstance(vector, _capi.DeviceVector):
+            return self._device.precondition_f32_vec(vector)
         return self._device.precondition_f32(np.asarray(vector, dtype=np.float32).reshape(-1))
 
     def native_solve_mpir(self, rhs, config: MPIRConfig, initial_guess=None):

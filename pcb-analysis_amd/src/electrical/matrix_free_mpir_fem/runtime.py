@@ -1,15 +1,28 @@
-"""The low-precision runtime of the HIP path (reference contract:
-src/electrical/matrix_free_mpir_fem/runtime.py:20-44, 277-294).
+"""The FP32 inner-solve runtime of the HIP path.
 
-On MI355X the whole inner solve runs in fused device kernels, so the runtime
-is a descriptor of the device the operator lives on rather than a vector
-library.  ``backend`` "hip", "cuda" (the reference's name for its GPU path)
-and "auto" select the HIP device; "cpu" is refused: this package has no host
-fallback.
+The reference interface is the ``LowPrecisionRuntime`` contract of
+src/electrical/matrix_free_mpir_fem/runtime.py:20-44 with the device runtime
+of :121-170: ``from_host``, ``to_host``, ``zeros_like``, ``copy``, ``dot``,
+``norm``, ``axpy``, ``divide`` and ``synchronize``, on arrays that live on the
+device.  Here the arrays are ``pcbhip._capi.DeviceVector`` (FP32 on one HIP
+device) and every operation is a kernel of ``libpcbhip.so`` that returns when
+it is done, so a caller that drives a PCG loop from the host -- the reference's
+``_pcg_low`` shape, with an operator's ``apply_low`` / ``precondition_low`` on
+the same vectors -- runs it on the device.  The product's own solves do not
+go through here: they run the whole iteration in fused device kernels.
+
+Differences from the reference's device runtime, both deliberate: ``dot`` and
+``norm`` accumulate in FP64 in an order fixed by the length (the same bits
+on every call) where CuPy accumulates in FP32; and there is no array
+library behind the vectors, so ``namespace`` is None.  ``backend`` "hip",
+"cuda" (the reference's name for its GPU path) and "auto" select the HIP
+device; "cpu" is refused: this package has no host fallback.
 """
 
 from __future__ import annotations
 
+import ctypes
+import math
 from typing import Literal
 
 import numpy as np
@@ -18,9 +31,10 @@ RuntimeBackend = Literal["hip", "cuda", "auto", "cpu"]
 
 
 class HipFloat32Runtime:
-    """FP32 inner solves on one HIP device."""
+    """FP32 vectors and their operations on one HIP device."""
 
     name = "hip-fp32"
+    namespace = None
     dtype = np.float32
     is_cuda = True  # the reference's flag for "device runtime"
     is_hip = True
@@ -30,14 +44,80 @@ class HipFloat32Runtime:
 
         _capi.require_device(int(device_id))
         self.device_id = int(device_id)
+        self._capi = _capi
+        self._lib = _capi.load()
 
-    def from_host(self, value: np.ndarray) -> np.ndarray:
-        return np.asarray(value, dtype=self.dtype)
+    # -- arrays ----------------------------------------------------------------
+
+    def _device(self, value):
+        """``value`` as a device vector of this device (host arrays are uploaded)."""
+
+        if isinstance(value, self._capi.DeviceVector):
+            if value.device != self.device_id:
+                raise ValueError(f"vector lives on device {value.device}, the runtime on {self.device_id}")
+            return value
+        return self._capi.DeviceVector.from_host(self.device_id, value)
+
+    @staticmethod
+    def _same_size(a, b) -> int:
+        if a.size != b.size:
+            raise ValueError(f"vectors of sizes {a.size} and {b.size}")
+        return a.size
+
+    def from_host(self, value: np.ndarray):
+        return self._capi.DeviceVector.from_host(self.device_id, value)
 
     def to_host(self, value) -> np.ndarray:
+        if isinstance(value, self._capi.DeviceVector):
+            return value.to_host().astype(np.float64)
         return np.asarray(value, dtype=np.float64)
 
+    def zeros_like(self, value):
+        size = value.size if isinstance(value, self._capi.DeviceVector) else int(np.size(value))
+        return self._capi.DeviceVector(self.device_id, size)
+
+    def copy(self, value):
+        src = self._device(value)
+        out = self._capi.DeviceVector(self.device_id, src.size)
+        self._capi.check(self._lib.pcbhip_vec_copy(self.device_id, src.ptr, out.ptr, src.size))
+        return out
+
+    # -- reductions --------------------------------------------------------------
+
+    def dot(self, left, right) -> float:
+        a, b = self._device(left), self._device(right)
+        n = self._same_size(a, b)
+        out = ctypes.c_double()
+        self._capi.check(self._lib.pcbhip_vec_dot(self.device_id, a.ptr, b.ptr, n, ctypes.byref(out)))
+        return float(out.value)
+
+    def norm(self, value) -> float:
+        v = self._device(value)
+        return math.sqrt(max(self.dot(v, v), 0.0))
+
+    # -- element-wise --------------------------------------------------------------
+
+    def axpy(self, alpha: float, x, y):
+        """A new vector ``y + alpha x`` in FP32."""
+
+        a, b = self._device(x), self._device(y)
+        n = self._same_size(a, b)
+        out = self._capi.DeviceVector(self.device_id, n)
+        self._capi.check(self._lib.pcbhip_vec_axpy(self.device_id, float(np.float32(alpha)), a.ptr, b.ptr, out.ptr, n))
+        return out
+
+    def divide(self, numerator, denominator):
+        """A new vector ``numerator / denominator`` in FP32 (a host
+        denominator, such as an operator's FP32 diagonal, is uploaded)."""
+
+        a, b = self._device(numerator), self._device(denominator)
+        n = self._same_size(a, b)
+        out = self._capi.DeviceVector(self.device_id, n)
+        self._capi.check(self._lib.pcbhip_vec_divide(self.device_id, a.ptr, b.ptr, out.ptr, n))
+        return out
+
     def synchronize(self) -> None:
+        # Every operation has finished when it returns.
         return None
 
 

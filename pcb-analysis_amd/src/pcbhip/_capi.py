@@ -75,6 +75,16 @@ EXPORTS = (
     "pcbhip_server_stats",
     "pcbhip_server_trim",
     "pcbhip_copy_dev",
+    "pcbhip_vec_create",
+    "pcbhip_vec_destroy",
+    "pcbhip_vec_write",
+    "pcbhip_vec_read",
+    "pcbhip_vec_copy",
+    "pcbhip_vec_dot",
+    "pcbhip_vec_axpy",
+    "pcbhip_vec_divide",
+    "pcbhip_operator_precondition_f32_dev",
+    "pcbhip_operator_synchronize",
 )
 
 
@@ -177,6 +187,16 @@ def load() -> ctypes.CDLL:
     lib.pcbhip_server_collect.argtypes = [_vp, _int, _vp, ctypes.POINTER(MpirResultC), _vp]
     lib.pcbhip_server_stats.argtypes = [_vp, ctypes.POINTER(ServerInfoC)]
     lib.pcbhip_server_trim.argtypes = [_vp]
+    _i64 = ctypes.c_int64
+    lib.pcbhip_vec_create.argtypes = [_int, _i64, ctypes.POINTER(_vp)]
+    lib.pcbhip_vec_destroy.argtypes = [_int, _vp]
+    for name in ("pcbhip_vec_write", "pcbhip_vec_read", "pcbhip_vec_copy"):
+        getattr(lib, name).argtypes = [_int, _vp, _vp, _i64]
+    lib.pcbhip_vec_dot.argtypes = [_int, _vp, _vp, _i64, ctypes.POINTER(ctypes.c_double)]
+    lib.pcbhip_vec_axpy.argtypes = [_int, ctypes.c_float, _vp, _vp, _vp, _i64]
+    lib.pcbhip_vec_divide.argtypes = [_int, _vp, _vp, _vp, _i64]
+    lib.pcbhip_operator_precondition_f32_dev.argtypes = [_vp, _vp, _vp]
+    lib.pcbhip_operator_synchronize.argtypes = [_vp]
     _lib = lib
     return lib
 
@@ -208,6 +228,58 @@ def _ptr(a: np.ndarray) -> int:
 def config_c(relative_tolerance, absolute_tolerance, inner_relative_tolerance, max_outer, max_inner) -> MpirConfigC:
     return MpirConfigC(float(relative_tolerance), float(absolute_tolerance), float(inner_relative_tolerance),
                        int(max_outer), int(max_inner))
+
+
+class DeviceVector:
+    """An FP32 vector on one HIP device, owned by this object (released by
+    ``close`` or when collected): the array type of the FP32 inner-solve
+    runtime.  ``np.asarray(v)`` copies it to the host."""
+
+    dtype = np.dtype(np.float32)
+
+    def __init__(self, device: int, size: int):
+        lib = load()
+        self._lib = lib
+        self.device = int(device)
+        self.size = int(size)
+        self.ptr = None
+        p = _vp()
+        check(lib.pcbhip_vec_create(self.device, self.size, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, device: int, values) -> "DeviceVector":
+        host = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+        v = cls(device, host.size)
+        check(v._lib.pcbhip_vec_write(v.device, v.ptr, _ptr(host), host.size))
+        return v
+
+    @property
+    def shape(self) -> tuple[int]:
+        return (self.size,)
+
+    def __len__(self) -> int:
+        return self.size
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.size, dtype=np.float32)
+        check(self._lib.pcbhip_vec_read(self.device, self.ptr, _ptr(out), self.size))
+        return out
+
+    def __array__(self, dtype=None, copy=None):
+        host = self.to_host()
+        return host if dtype is None else host.astype(dtype)
+
+    def close(self) -> None:
+        if self.ptr is not None:
+            ptr, self.ptr = self.ptr, None
+            check(self._lib.pcbhip_vec_destroy(self.device, ptr))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Operator:
@@ -361,6 +433,28 @@ class Operator:
         two- or three-term bf16 split."""
 
         check(self._lib.pcbhip_operator_set_engine(self.handle, int(engine)))
+
+    def _vector_of_size(self, v: "DeviceVector") -> None:
+        if v.size != self.size or v.device != self.device:
+            raise ValueError(f"device vector of size {v.size} on device {v.device}, expected size {self.size} "
+                             f"on device {self.device}")
+
+    def apply_f32_vec(self, x: "DeviceVector") -> "DeviceVector":
+        """``A x`` in FP32 from one device vector into a new one."""
+
+        self._vector_of_size(x)
+        y = DeviceVector(self.device, self.size)
+        check(self._lib.pcbhip_apply_f32_dev(self.handle, x.ptr, y.ptr, None))
+        check(self._lib.pcbhip_operator_synchronize(self.handle))
+        return y
+
+    def precondition_f32_vec(self, r: "DeviceVector") -> "DeviceVector":
+        """``M^-1 r`` in FP32 (Jacobi or two-level) from one device vector into a new one."""
+
+        self._vector_of_size(r)
+        z = DeviceVector(self.device, self.size)
+        check(self._lib.pcbhip_operator_precondition_f32_dev(self.handle, r.ptr, z.ptr))
+        return z
 
     def apply_f32_dev(self, x_ptr: int, y_ptr: int, stream: int | None = None) -> None:
         check(self._lib.pcbhip_apply_f32_dev(self.handle, x_ptr, y_ptr, stream))

@@ -125,7 +125,12 @@ class MatrixFreeThermalOperator:
     def apply_high(self, vector: np.ndarray) -> np.ndarray:
         return self._device.apply_f64(self._flat(vector))
 
-    def apply_low(self, vector: Any) -> np.ndarray:
+    def apply_low(self, vector: Any):
+        """``A x`` in FP32: a device vector of the runtime gives a new device
+        vector, a host array a host array."""
+
+        if isinstance(vector, _capi.DeviceVector):
+            return self._device.apply_f32_vec(vector)
         v = np.ascontiguousarray(vector, dtype=np.float32).reshape(-1)
         if v.size != self.size:
             raise ValueError(f"vector has size {v.size}, expected {self.size}")
@@ -139,10 +144,13 @@ class MatrixFreeThermalOperator:
     def diagonal_low(self) -> np.ndarray:
         return self._diagonal_low
 
-    def precondition_low(self, vector: Any) -> np.ndarray:
+    def precondition_low(self, vector: Any):
         """``M^-1 r`` in FP32 on the device: Jacobi, plus the coarse
-        correction with the two-level preconditioner (operator.py:366-371)."""
+        correction with the two-level preconditioner (operator.py:366-371);
+        device vector in, device vector out, or host array in and out."""
 
+        if isinstance(vector, _capi.DeviceVector):
+            return self._device.precondition_f32_vec(vector)
         v = np.ascontiguousarray(vector, dtype=np.float32).reshape(-1)
         if v.size != self.size:
             raise ValueError(f"vector has size {v.size}, expected {self.size}")
