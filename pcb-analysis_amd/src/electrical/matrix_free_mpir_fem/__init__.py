@@ -2,6 +2,7 @@
 front end (the reference's ``electrical.matrix_free_mpir_fem`` solver,
 config and result types, ``pcb`` and ``ports`` modules)."""
 
+from .grid import TensorGrid, graded_edges, refined_grid
 from .pcb import (
     COPPER_CONDUCTIVITY_S_PER_M,
     CurrentTerminal,
@@ -35,12 +36,15 @@ __all__ = [
     "MPIRResult",
     "MPIRStep",
     "RuntimeBackend",
+    "TensorGrid",
     "DEFAULT_MAX_COARSE_SIZE",
     "choose_block_size",
     "dc_port_basis",
+    "graded_edges",
     "make_float32_runtime",
     "mpir_result_from_core",
     "reduce_correlation",
+    "refined_grid",
     "solve_mpir",
     "solve_mpir_device",
     "solve_pcb_dc",
