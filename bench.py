@@ -66,6 +66,34 @@ def copy_bandwidth(torch, stream, nbytes: int = 1 << 30) -> float:
     return 2 * nbytes / per / 1e9
 
 
+def stream_ceilings(torch, stream, n: int, shapes=((1, 1), (3, 1), (6, 2), (6, 4), (9, 4), (13, 4))) -> list:
+    """GB/s of ``reads`` + ``writes`` arrays of ``n`` floats streamed at once
+    (``pcbhip_streams_dev``), HIP events on ``stream``: the ceiling of a
+    kernel that moves that many streams.  The fused PCG pass reads six node
+    vectors and three element-coefficient arrays and writes four node
+    vectors: the (9, 4) row."""
+
+    import ctypes
+
+    from pcbhip import _capi
+
+    lib = _capi.load()
+    n = -(-n // 4) * 4
+    pool = [torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(max(r + w for r, w in shapes))]
+    for t in pool:
+        t.fill_(1.0)
+    rows = []
+    for reads, writes in shapes:
+        ins = (ctypes.c_void_p * reads)(*[pool[k].data_ptr() for k in range(reads)])
+        outs = (ctypes.c_void_p * max(writes, 1))(*[pool[reads + k].data_ptr() for k in range(writes)] or [None])
+        per = _time_launches(torch, stream, lambda: _capi.check(
+            lib.pcbhip_streams_dev(ins, reads, outs, writes, n, stream.cuda_stream)), 30)
+        gbs = (reads + writes) * n * 4 / per / 1e9
+        rows.append(dict(reads=reads, writes=writes, us=per * 1e6, gbs=gbs, frac=gbs / HBM_PEAK_GBS))
+        log(f"streams {reads} reads + {writes} writes of {n:,} floats: {per * 1e6:.1f} us, {gbs:.0f} GB/s")
+    return rows
+
+
 def profiled_traffic(element_shape, rows: int, zb: int):
     """HBM bytes per launch of the FP32 apply at this tile shape, from the
     committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or
@@ -81,12 +109,13 @@ def profiled_traffic(element_shape, rows: int, zb: int):
     return None
 
 
-def profiled_pass_traffic(element_shape, rows: int, target: int):
+def profiled_pass_traffic(element_shape, rows: int, target: int, board: str = "standard"):
     """HBM bytes per launch of the fused PCG pass at this tile shape, from the
-    committed rocprofv3 summary of the MPIR solve, or None."""
+    committed rocprofv3 summary of that board (standard or general), or None."""
 
     s, r, c = element_shape
-    path = ROOT / "profiles" / f"q1hex_pcg_pass_board_{c}x{r}x{s}.md"
+    stem = "board" if board == "standard" else board
+    path = ROOT / "profiles" / f"q1hex_pcg_pass_{stem}_{c}x{r}x{s}.md"
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -95,7 +124,7 @@ def profiled_pass_traffic(element_shape, rows: int, target: int):
     return None
 
 
-def pass_roofline(operator, batch, rows: int, target: int, launches: int = 100) -> dict:
+def pass_roofline(operator, batch, rows: int, target: int, launches: int = 100, board: str = "standard") -> dict:
     """The dominant kernel of the solve, the fused Jacobi-PCG pass, timed with
     HIP events on the batch's stream (``pcbhip_batch_time_pass``)."""
 
@@ -107,7 +136,7 @@ def pass_roofline(operator, batch, rows: int, target: int, launches: int = 100) 
     # (12 B/element); writes p, q, r and c (4 B/node each).
     algorithmic = 40 * n + 12 * e
     achieved = algorithmic / (per_us * 1e-6) / 1e9
-    traffic = profiled_pass_traffic(operator.mesh.element_grid_shape, rows, target)
+    traffic = profiled_pass_traffic(operator.mesh.element_grid_shape, rows, target, board)
     return dict(
         kernel=f"k_inner_step<float,{rows}> (fused FP32 Jacobi-PCG pass), target {target} workgroups",
         per_launch_us=per_us,
@@ -586,6 +615,7 @@ def run_pass(args, torch) -> dict:
     stream = torch.cuda.Stream()
     apply = apply_roofline(operator, torch, stream)
     copy = copy_bandwidth(torch, stream, pass_bytes // 2 // 16 * 16)
+    ceilings = stream_ceilings(torch, stream, n)
     best = min(sweep, key=lambda row: row["us"])
     return {
         "metric": f"fused Jacobi-PCG pass GB/s (algorithmic 40N+12E), {args.board} board {c}x{r}x{s}",
@@ -603,6 +633,7 @@ def run_pass(args, torch) -> dict:
         "config": {"workload": f"{args.board} board {c}x{r}x{s} elements ({n:,} nodes)", "pass_sweep": sweep,
                    "pass_bytes": pass_bytes, "apply_us": apply["per_launch_us"], "apply_sweep": apply["sweep"],
                    "apply_algorithmic_gbs": apply["roofline"]["achieved"], "copy_gbs_at_pass_bytes": copy,
+                   "stream_ceilings": ceilings,
                    "solve_ms": solve_s * 1e3, "inner_iterations": result["inner_iterations"],
                    "solution_sha256_16": digest,
                    "parallelism": "single GPU"},
@@ -1009,7 +1040,7 @@ def run_board(args, torch) -> dict:
     log(f"solve: {result['outer_iterations']} outer / {result['inner_iterations']} inner, "
         f"rel {result['relative_residual']:.2e}, {elapsed / args.steps * 1e3:.1f} ms/solve")
 
-    fused = pass_roofline(operator, batch, args.rows, args.target)
+    fused = pass_roofline(operator, batch, args.rows, args.target, board=args.board)
     log(f"pcg pass: {fused['per_launch_us']:.1f} us, {fused['roofline']['achieved']:.0f} GB/s")
     stream = torch.cuda.Stream()
     roof = apply_roofline(operator, torch, stream)

@@ -279,6 +279,13 @@ int pcbhip_server_trim(pcbhip_server *server);
 
 /* Bandwidth reference: copy n_floats (multiple of 4) device floats. */
 int pcbhip_copy_dev(const float *src, float *dst, int64_t n_floats, void *stream);
+/* Bandwidth reference for kernels that stream many arrays at once: reads
+ * (1..16) device arrays and writes (0..16) device arrays of n_floats
+ * (multiple of 4) each, one 16-byte element of every array per thread,
+ * the inputs summed into every output.  No counterpart in the reference:
+ * the ceiling against which the fused PCG pass's bytes are judged. */
+int pcbhip_streams_dev(const float *const *in, int reads, float *const *out, int writes, int64_t n_floats,
+                       void *stream);
 
 /* FP32 vectors of the inner-solve runtime on one device: the vector
  * operations of the LowPrecisionRuntime contract

@@ -2020,6 +2020,21 @@ int pcbhip_solve_mpir(const pcbhip_operator* op, const double* rhs, double* x, c
     return rc;
 }
 
+int pcbhip_streams_dev(const float* const* in, int reads, float* const* out, int writes, int64_t n_floats,
+                       void* stream) {
+    return guarded([&] {
+        if (!in || reads < 1 || reads > kStreamsMax || writes < 0 || writes > kStreamsMax || (writes > 0 && !out))
+            fail(PCBHIP_INVALID_INPUT, "streams needs 1.." + std::to_string(kStreamsMax) + " inputs and 0.." +
+                                           std::to_string(kStreamsMax) + " outputs");
+        if (n_floats < 4 || n_floats % 4 != 0) fail(PCBHIP_INVALID_INPUT, "streams needs a multiple of 4 floats");
+        for (int k = 0; k < reads; ++k)
+            if (!in[k]) fail(PCBHIP_INVALID_INPUT, "null input array");
+        for (int k = 0; k < writes; ++k)
+            if (!out[k]) fail(PCBHIP_INVALID_INPUT, "null output array");
+        check(launch::streams(in, reads, out, writes, n_floats, static_cast<hipStream_t>(stream)), "streams launch");
+    });
+}
+
 int pcbhip_vec_create(int device, int64_t n, float** v) {
     return guarded([&] {
         if (!v) fail(PCBHIP_INVALID_INPUT, "null output");

@@ -198,6 +198,9 @@ constexpr int kCoarseGroups = 1024;
 // Largest coarse space a slot may carry (its inverse is nc^2 FP32 values).
 constexpr int kMaxCoarse = 16384;
 
+// Arrays the multi-stream bandwidth reference takes on each side.
+constexpr int kStreamsMax = 16;
+
 // Workgroups of the first stage of the vector runtime's dot product.
 constexpr int kDotGroupsMax = 1024;
 
@@ -225,6 +228,10 @@ hipError_t sheet_apply(const OperatorDev& op, const T* x, T* y, int mode, hipStr
 hipError_t sheet_diagonal(const OperatorDev& op, double* diag, double* dinv64, float* dinv32, int* bad,
                           hipStream_t stream);
 hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream);
+// Bandwidth reference: 1..kStreamsMax input and 0..kStreamsMax output arrays
+// of n_floats (a multiple of 4) streamed together.
+hipError_t streams(const float* const* in, int reads, float* const* out, int writes, long long n_floats,
+                   hipStream_t stream);
 // FP32 vector runtime: dot (FP64, an order fixed by n; ``partial`` holds
 // kDotGroupsMax doubles, ``out`` one), out = y + alpha x, out = num / den.
 int dot_groups(long long n);

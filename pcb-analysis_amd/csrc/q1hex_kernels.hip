@@ -1033,6 +1033,31 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
     if (i < n4) dst[i] = src[i];
 }
 
+// A bandwidth reference for kernels that stream several arrays at once:
+// ``reads`` input arrays and ``writes`` output arrays of one length, one
+// 16-byte element of every array per thread (the inputs summed, the sum
+// written to every output), so the only cost is the traffic of that many
+// concurrent streams.
+struct StreamSet {
+    const float4* in[kStreamsMax];
+    float4* out[kStreamsMax];
+};
+
+__global__ __launch_bounds__(256) void k_streams(const StreamSet set, const int reads, const int writes,
+                                                 const long long n4) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n4) return;
+    float4 sum = set.in[0][i];
+    for (int k = 1; k < reads; ++k) {
+        const float4 v = set.in[k][i];
+        sum.x += v.x;
+        sum.y += v.y;
+        sum.z += v.z;
+        sum.w += v.w;
+    }
+    for (int k = 0; k < writes; ++k) set.out[k][i] = sum;
+}
+
 // ------------------------------------------------------- vector runtime
 
 // The FP32 vector operations of the inner-solve runtime, for callers that
@@ -1917,6 +1942,17 @@ hipError_t divide(const float* num, const float* den, float* out, long long n, h
     if (n <= 0) return hipSuccess;
     const long long blocks = (n + kVecThreads - 1) / kVecThreads;
     hipLaunchKernelGGL(k_divide, dim3((unsigned)blocks), dim3(kVecThreads), 0, stream, num, den, out, n);
+    return hipGetLastError();
+}
+
+hipError_t streams(const float* const* in, int reads, float* const* out, int writes, long long n_floats,
+                   hipStream_t stream) {
+    StreamSet set{};
+    for (int k = 0; k < reads; ++k) set.in[k] = reinterpret_cast<const float4*>(in[k]);
+    for (int k = 0; k < writes; ++k) set.out[k] = reinterpret_cast<float4*>(out[k]);
+    const long long n4 = n_floats / 4;
+    const int blocks = (int)((n4 + 255) / 256);
+    hipLaunchKernelGGL(k_streams, dim3(blocks), dim3(256), 0, stream, set, reads, writes, n4);
     return hipGetLastError();
 }
 

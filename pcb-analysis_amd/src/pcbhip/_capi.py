@@ -75,6 +75,7 @@ EXPORTS = (
     "pcbhip_server_stats",
     "pcbhip_server_trim",
     "pcbhip_copy_dev",
+    "pcbhip_streams_dev",
     "pcbhip_vec_create",
     "pcbhip_vec_destroy",
     "pcbhip_vec_write",
@@ -174,6 +175,7 @@ def load() -> ctypes.CDLL:
     lib.pcbhip_batch_result.argtypes = [_vp, _int, _vp, ctypes.POINTER(MpirResultC), _vp]
     lib.pcbhip_batch_time_pass.argtypes = [_vp, _int, ctypes.POINTER(ctypes.c_double)]
     lib.pcbhip_copy_dev.argtypes = [_vp, _vp, ctypes.c_int64, _vp]
+    lib.pcbhip_streams_dev.argtypes = [ctypes.POINTER(_vp), _int, ctypes.POINTER(_vp), _int, ctypes.c_int64, _vp]
     _i64p = ctypes.POINTER(ctypes.c_int64)
     lib.pcbhip_operator_bytes.argtypes = [_vp, _i64p]
     lib.pcbhip_server_create.argtypes = [_int, _int, _int, _int, ctypes.POINTER(_vp)]

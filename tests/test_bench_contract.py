@@ -48,3 +48,10 @@ def test_profile_summary_has_traffic_for_the_default_pcg_pass():
     traffic = bench.profiled_pass_traffic((16, 512, 512), 4, 1024)
     assert isinstance(traffic, int) and 100e6 < traffic < 400e6, traffic
     assert bench.profiled_pass_traffic((16, 512, 512), 8, 1024) is None
+
+
+def test_general_board_profile_has_the_pass_traffic_of_its_best_shape():
+    bench = _bench()
+    traffic = bench.profiled_pass_traffic((16, 1024, 1024), 4, 4096, "general")
+    # Above the algorithmic 915.7 MB (halo re-reads), below twice it.
+    assert isinstance(traffic, int) and 915.7e6 < traffic < 1.9e9, traffic
