@@ -124,6 +124,25 @@ def profiled_pass_traffic(element_shape, rows: int, target: int, board: str = "s
     return None
 
 
+def skipped_bytes(operator) -> int:
+    """Bytes of the algorithmic count that the stencils legitimately skip on
+    this operator: the Robin word (4 B) of every node of a plain node plane
+    (all nodes free, Robin exactly +0) and the coefficient rows (12 B per
+    element) of every uniform element layer, among the first 64 of each (the
+    device's own criteria)."""
+
+    s, r, c = operator.mesh.element_grid_shape
+    per_plane, per_layer = (r + 1) * (c + 1), r * c
+    robin = np.asarray(operator._robin_total_high).reshape(s + 1, per_plane)
+    free = np.asarray(operator.free_nodes).reshape(s + 1, per_plane)
+    plain = [bool(free[z].all() and (robin[z] == 0.0).all() and not np.signbit(robin[z]).any())
+             for z in range(min(s + 1, 64))]
+    coef = np.asarray(operator._coefficients_high).reshape(3, s, per_layer)
+    uniform = [bool(all((coef[d, ez] == coef[d, ez, 0]).all() and not np.signbit(coef[d, ez]).any() for d in range(3)))
+               for ez in range(min(s, 64))]
+    return 4 * per_plane * sum(plain) + 12 * per_layer * sum(uniform)
+
+
 def pass_roofline(operator, batch, rows: int, target: int, launches: int = 100, board: str = "standard") -> dict:
     """The dominant kernel of the solve, the fused Jacobi-PCG pass, timed with
     HIP events on the batch's stream (``pcbhip_batch_time_pass``)."""
@@ -137,11 +156,14 @@ def pass_roofline(operator, batch, rows: int, target: int, launches: int = 100, 
     algorithmic = 40 * n + 12 * e
     achieved = algorithmic / (per_us * 1e-6) / 1e9
     traffic = profiled_pass_traffic(operator.mesh.element_grid_shape, rows, target, board)
+    needed = algorithmic - skipped_bytes(operator)
     return dict(
         kernel=f"k_inner_step<float,{rows}> (fused FP32 Jacobi-PCG pass), target {target} workgroups",
         per_launch_us=per_us,
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=traffic, algorithmic_bytes=algorithmic, hbm_gbs=_hbm_rate(traffic, per_us * 1e-6)),
+                      traffic=traffic, algorithmic_bytes=algorithmic, hbm_gbs=_hbm_rate(traffic, per_us * 1e-6),
+                      board_needed_bytes=needed, board_needed_gbs=needed / (per_us * 1e-6) / 1e9,
+                      board_needed_frac=needed / (per_us * 1e-6) / 1e9 / HBM_PEAK_GBS),
     )
 
 
@@ -177,6 +199,7 @@ def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
     traffic = profiled_traffic(operator.mesh.element_grid_shape, rows, zb)
     algorithmic = 13 * n + 12 * e  # x, y, robin (4 B/node each), free mask (1 B/node), coefficients (12 B/element)
     achieved = algorithmic / per / 1e9
+    needed = algorithmic - skipped_bytes(operator)
     return dict(
         kernel=f"k_apply<float,{rows},0,0> (FP32 Q1 apply), {zb} planes per z block",
         rows=rows,
@@ -185,7 +208,9 @@ def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
         gdof_s=n / per / 1e9,
         sweep=[dict(rows=r_, zb=z_, us=p_ * 1e6) for p_, r_, z_ in table],
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=traffic, algorithmic_bytes=algorithmic, hbm_gbs=_hbm_rate(traffic, per)),
+                      traffic=traffic, algorithmic_bytes=algorithmic, hbm_gbs=_hbm_rate(traffic, per),
+                      board_needed_bytes=needed, board_needed_gbs=needed / per / 1e9,
+                      board_needed_frac=needed / per / 1e9 / HBM_PEAK_GBS),
     )
 
 
