@@ -325,6 +325,44 @@ def cpu_baseline(problem, rhs, gpu_solution, *, oracle_error: bool = True, calib
     return out
 
 
+def _oracle_operator(front):
+    """The oracle port's operator for a front-end operator: a thermal one from
+    its problem, a hexahedral DC one from its mesh's coefficients and fixed
+    nodes (no Robin term)."""
+
+    from oracle import native as N
+    from oracle import problem as P
+
+    if hasattr(front, "problem"):
+        return N.OracleOperator(P.operator_arrays(P.from_problem(front.problem)))
+    coef = np.stack(front.mesh.element_coefficients())
+    unit = P.unit_matrices()
+    free = np.asarray(front.free_nodes).reshape(-1)
+    robin = np.zeros(front.size)
+    return N.OracleOperator(dict(element_shape=front.mesh.element_grid_shape, coef=coef, unit=unit,
+                                 robin_total=robin, free=free,
+                                 diag=P.diagonal(coef, unit, robin, free, front.mesh.node_shape)))
+
+
+def cpu_node_iteration_cost(pairs, threads: int) -> dict:
+    """Whole reference-algorithm solves (the oracle's C port: FP32 Jacobi-PCG
+    inner, FP64 outer, the GPU's limits) of ``(front operator, rhs)`` pairs
+    on ``threads`` host threads: their time, and the seconds per node-
+    iteration (nodes x inner iterations) that scale it to a whole workload."""
+
+    seconds, work, iterations = 0.0, 0, 0
+    for front, rhs in pairs:
+        op = _oracle_operator(front)
+        t0 = time.perf_counter()
+        _, res = op.solve_mpir(np.asarray(rhs, dtype=np.float64), rtol=1e-10, max_outer=60, max_inner=100000,
+                               threads=threads)
+        seconds += time.perf_counter() - t0
+        work += op.size * max(res["inner_iterations"], 1)
+        iterations += res["inner_iterations"]
+    return dict(solves=len(pairs), seconds=seconds, inner_iterations=iterations, node_iterations=work,
+                seconds_per_node_iteration=seconds / work, threads=threads)
+
+
 def batch_leg(args, torch, *, count: int, warmup: int, steps: int) -> dict:
     """``count`` uneven boards solved together in one batch (every launch
     covers every board still iterating; tile shape from the autotuner) and
@@ -377,8 +415,23 @@ def batch_leg(args, torch, *, count: int, warmup: int, steps: int) -> dict:
     for sb in singles:
         sb.close()
     batch.close()
+    cpu = None
+    if not args.no_cpu:
+        from oracle.native import threads_default
+
+        # Six boards spread over the batch, solved whole on the host; the
+        # whole batch's host time is their cost per node-iteration times the
+        # batch's node-iterations.
+        sample = cpu_node_iteration_cost([(ops[i], rhss[i]) for i in range(0, count, max(1, count // 6))][:6],
+                                         threads_default())
+        total_work = sum(o.size * max(r["inner_iterations"], 1) for o, r in zip(ops, results))
+        cpu_s = total_work * sample["seconds_per_node_iteration"]
+        cpu = dict(sample, solves_per_s=count / cpu_s, est_batch_s=cpu_s,
+                   sample_note="whole solves of 6 of the boards on the host, scaled by node-iterations")
+        log(f"batch: cpu sample {sample['seconds']:.1f} s, est. {cpu_s:.1f} s for the batch")
     return {
         "workload": f"{count} boards, sides log-uniform 32..384, depth 4..16, {nodes:,} nodes in total",
+        "cpu": cpu,
         "solves_per_s": count / batched_s,
         "ms_per_batch": batched_s * 1e3,
         "pass_gbs": work / batched_s / 1e9,
@@ -392,6 +445,49 @@ def batch_leg(args, torch, *, count: int, warmup: int, steps: int) -> dict:
         "total_inner_iterations": int(sum(r["inner_iterations"] for r in results)),
         "max_inner_iterations": int(max(r["inner_iterations"] for r in results)),
     }
+
+
+def plain_fp64_leg(args, torch, *, steps: int = 10) -> dict:
+    """The 64 x 64 x 8 board as plain FP64 Jacobi PCG to 1e-10: on the device
+    (FP64 inner solve, one refinement step), and the same algorithm on the
+    host (the oracle's C port, FP64 inner PCG), whole solves on every thread
+    and on one."""
+
+    from electrical.matrix_free_mpir_fem import MPIRConfig
+    from pcbhip import _capi, workloads
+    from thermal.matrix_free_mpir_fem import MatrixFreeThermalOperator
+
+    problem = workloads.board_64x64x8()
+    operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
+    rhs = operator.build_rhs(operator.default_reference_temperature())
+    plain = MPIRConfig(relative_tolerance=1e-10, inner_relative_tolerance=1e-10, max_outer_iterations=4,
+                       max_inner_iterations=20000)
+    batch = _capi.Batch([operator.device_operator], inner_fp64=True)
+    batch.set(0, rhs, None, plain.to_c())
+    batch.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+    gpu_s = (time.perf_counter() - t0) / steps
+    x, result = batch.result(0)
+    batch.close()
+    out = {"workload": f"board 64x64x8 elements ({operator.size:,} nodes), plain FP64 Jacobi PCG to 1e-10",
+           "solves_per_s": 1.0 / gpu_s, "ms_per_solve": gpu_s * 1e3, "converged": result["converged"],
+           "inner_iterations": result["inner_iterations"], "relative_residual": result["relative_residual"]}
+    if not args.no_cpu:
+        from oracle.native import threads_default
+
+        op = _oracle_operator(operator)
+        for team in (threads_default(), 1):
+            (x_cpu, res), ms = _timed_ms(lambda: op.solve_mpir(rhs, rtol=1e-10, inner_tol=1e-10, max_outer=4,
+                                                               max_inner=20000, inner_f64=True, threads=team),
+                                         repeats=3, warmups=1)
+            out[f"cpu_{team}_threads"] = {"solves_per_s": 1e3 / ms, "ms_per_solve": ms,
+                                          "inner_iterations": res["inner_iterations"]}
+        out["rel_difference_gpu_vs_cpu"] = float(np.max(np.abs(x - x_cpu)) / np.max(np.abs(x_cpu)))
+    log(f"plain fp64 64x64x8: {out}")
+    return out
 
 
 def run_batch200(args, torch) -> dict:
@@ -466,7 +562,16 @@ def multirhs_leg(args, torch, *, warmup: int, steps: int, variants=("slots", "bl
     single.close()
     log(f"64 rhs one at a time: {single_s * 1e3:.1f} ms")
     plain_batch, plain_s, plain_results = runs["slots"]
+    cpu = None
+    if not args.no_cpu:
+        from oracle.native import threads_default
+
+        sample = cpu_node_iteration_cost([(operator, rhss[0])], threads_default())
+        cpu = dict(sample, solves_per_s=1.0 / sample["seconds"],
+                   sample_note="one of the 64 excitations solved whole on the host")
+        log(f"64 rhs: cpu {sample['seconds']:.1f} s per solve")
     out = {
+        "cpu": cpu,
         "workload": f"{board.mesh.size:,} nodes, 64 terminal pairs in one copper cluster",
         "path": "plain FP32 slots of one batch sharing the device operator",
         "solves_per_s": len(rhss) / plain_s,
@@ -721,6 +826,13 @@ def serve_leg(args, torch, *, rates, duration: float, drain_limit: float) -> dic
         single_s[id(b.operator)] = float(np.mean(times))
         sb.close()
     log(f"serve: single-solve path, {len(pool)} boards, mean {1e3 * np.mean(list(single_s.values())):.1f} ms per solve")
+    cpu_sample = None
+    if not args.no_cpu:
+        from oracle.native import threads_default
+
+        picks = thermal[:4] + dc[:2]
+        cpu_sample = cpu_node_iteration_cost([(b.front_end, b.rhs[0]) for b in picks], threads_default())
+        log(f"serve: cpu sample {cpu_sample['seconds']:.1f} s for {len(picks)} pool boards")
     lines = []
     shape = None
     for rate in rates:
@@ -744,6 +856,14 @@ def serve_leg(args, torch, *, rates, duration: float, drain_limit: float) -> dic
         # over the device time of the server's replays.
         work = sum(x["inner_iterations"] * _pass_bytes(by_id[r.request_id].operator) for r in done for x in r.results)
         replay_s = info["replay_ms"] / 1e3
+        cpu_line = None
+        if cpu_sample is not None:
+            node_its = sum(by_id[r.request_id].operator.size * max(x["inner_iterations"], 1)
+                           for r in done for x in r.results)
+            cpu_s = node_its * cpu_sample["seconds_per_node_iteration"]
+            cpu_line = dict(cpu_sample, solves_per_s=solves / cpu_s,
+                            sample_note="whole solves of 4 thermal and 2 DC pool boards on the host, scaled by the "
+                                        "served solves' node-iterations")
         line = {
             "rate_per_s": rate,
             "requests": len(requests),
@@ -772,6 +892,7 @@ def serve_leg(args, torch, *, rates, duration: float, drain_limit: float) -> dic
             "blocked_on_device_s": info["wait_ms"] / 1e3,
             "device_replay_s": replay_s,
             "elapsed_s": elapsed,
+            "cpu": cpu_line,
         }
         log("serve: " + json.dumps(line))
         lines.append(line)
@@ -1123,10 +1244,16 @@ def run_board(args, torch) -> dict:
     convective = float(np.sum(operator.convective_heat(temperature)))
     line["config"]["heat_input_w"] = heat_input
     line["config"]["heat_balance_error_w"] = heat_input - convective - fixed_heat
+    if not args.no_cpu:
+        # Every host figure below runs the port built for this host.
+        from oracle import native
+
+        native.use_native_build()
     if not args.no_legs:
         # The batched, multi-excitation and served workloads on short
         # schedules: their speed-ups over the single-solve path on the same
         # work, and the served latency at 50 requests/s.
+        line["config"]["board_64x64x8_plain_fp64"] = plain_fp64_leg(args, torch)
         batched = batch_leg(args, torch, count=args.count, warmup=1, steps=1)
         many = multirhs_leg(args, torch, warmup=1, steps=1, variants=("slots", "block_mfma"))
         served = serve_leg(args, torch, rates=[50.0], duration=args.leg_duration, drain_limit=args.drain_limit)

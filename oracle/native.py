@@ -35,7 +35,7 @@ def use_native_build() -> str:
 
     global LIB_PATH
     if _lib is not None:
-        return "portable x86-64-v3 (already loaded)" if LIB_PATH.parent.name == "_build" else "-march=native"
+        return "-march=native" if LIB_PATH.parent.name == "native" else "portable x86-64-v3 (already loaded)"
     try:
         subprocess.run(["make", "-s", "-C", str(_HERE), "native"], check=True, capture_output=True, timeout=120)
     except (OSError, subprocess.SubprocessError):
