@@ -313,8 +313,8 @@ __device__ __forceinline__ void element_row_mfma(const bf16x8 unit, const unsign
 // element layer below and the element layer above are each summed from zero
 // (their element rows in order, the corner terms by fused multiply-adds) and
 // the two partials then added, a missing layer contributing +0.
-template <typename T, int ROWS, int MFMA = 0, bool DOWN = false, bool TABLE = false, class Load, class Emit,
-          class PlaneDone>
+template <typename T, int ROWS, int MFMA = 0, bool DOWN = false, bool TABLE = false, bool GROUPED = true, class Load,
+          class Emit, class PlaneDone>
 __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S, const int R, const int C,
                                             const int x0, const int y0, const int zlo, const int zhi, Load&& load,
                                             Emit&& emit, PlaneDone&& plane_done) {
@@ -533,10 +533,12 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
 
     // Output planes [zlo, zhi) need element layers zlo - 1 .. zhi - 1 (those
     // that exist); the layer outside the block only completes a plane.  Each
-    // step fetches one plane and one layer's coefficients, and every load of
-    // a step is issued before the step's arithmetic: the scheduling barrier
-    // keeps the compiler from sinking loads towards their uses, which left
-    // three dependent round trips to memory per step.
+    // step fetches one plane and one layer's coefficients.  GROUPED: every
+    // load of a step is issued before the step's arithmetic: the scheduling
+    // barrier keeps the compiler from sinking loads towards their uses, which
+    // left the two-phase loaders of the MPIR kernels three dependent round
+    // trips to memory per step.  The standalone apply, whose loader is one
+    // phase, measured about 2 % faster without the barrier.
     const int first_layer = zlo > 0 ? zlo - 1 : 0;
     const int last_layer = zhi - 1 < S - 1 ? zhi - 1 : S - 1;
 #pragma unroll
@@ -562,7 +564,7 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
         T ca[ROWS + 1], cb[ROWS + 1], cc[ROWS + 1];
         fetch_layer(ez, ca, cb, cc);
         fetch_plane(DOWN ? ez : ez + 1, raw);
-        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (GROUPED) __builtin_amdgcn_sched_barrier(0);
         T aL[ROWS], aR[ROWS], bL[ROWS], bR[ROWS];
         if constexpr (!DOWN) {
             make_plane(raw, mhi, nhi, vhi, qhi);
@@ -719,14 +721,15 @@ __device__ __forceinline__ int xcd_tile(const int block, const int count) {
 __device__ __forceinline__ bool sweep_down(const int sz, const int nsz) { return nsz > 1 && (sz & 1) == 0; }
 
 // strip_apply in the direction of z block sz of nsz.
-template <typename T, int ROWS, int MFMA = 0, bool TABLE = false, class Load, class Emit, class PlaneDone>
+template <typename T, int ROWS, int MFMA = 0, bool TABLE = false, bool GROUPED = true, class Load, class Emit,
+          class PlaneDone>
 __device__ __forceinline__ void strip_sweep(const int sz, const int nsz, const CoefSource<T>& coef, const int S,
                                             const int R, const int C, const int x0, const int y0, const int zlo,
                                             const int zhi, Load&& load, Emit&& emit, PlaneDone&& plane_done) {
     if (sweep_down(sz, nsz))
-        strip_apply<T, ROWS, MFMA, true, TABLE>(coef, S, R, C, x0, y0, zlo, zhi, load, emit, plane_done);
+        strip_apply<T, ROWS, MFMA, true, TABLE, GROUPED>(coef, S, R, C, x0, y0, zlo, zhi, load, emit, plane_done);
     else
-        strip_apply<T, ROWS, MFMA, false, TABLE>(coef, S, R, C, x0, y0, zlo, zhi, load, emit, plane_done);
+        strip_apply<T, ROWS, MFMA, false, TABLE, GROUPED>(coef, S, R, C, x0, y0, zlo, zhi, load, emit, plane_done);
 }
 
 
@@ -910,7 +913,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_apply(const CoefSource<T> coe
     const int zlo = sz * zb;
     const int zhi = zlo + zb < S + 1 ? zlo + zb : S + 1;
     if (y0 > R) return;
-    strip_sweep<T, ROWS, MFMA, true>(
+    strip_sweep<T, ROWS, MFMA, true, false>(
         sz, nsz, coef, S, R, C, x0, y0, zlo, zhi,
         [&](const long long pb, const unsigned off, const bool, const int, const int z) -> NodeVal<T> {
             if (MODE == 1) return {ldt(x + pb, off), T(0)};
