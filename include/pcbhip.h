@@ -78,7 +78,7 @@ int pcbhip_device_count(int *count);
  *   robin       (nodes) FP64 lumped Robin conductance plus C/dt
  *   free_nodes  (nodes) 1 free, 0 fixed (fixed rows are the identity)
  * The Jacobi diagonal is formed on the device; a free node without positive
- * conductance is PCBHIP_INVALID_INPUT, and so is a node plane of 2^27 nodes
+ * conductance is PCBHIP_INVALID_INPUT, and so is a node plane of 2^25 nodes
  * or more. */
 int pcbhip_operator_create(int device, int slabs, int rows, int cols, const double *coef, const double *robin,
                            const uint8_t *free_nodes, pcbhip_operator **out);
@@ -91,7 +91,7 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
  * (pcb.py:173-186); free_nodes as for the hexahedral operator; via k joins
  * flat nodes via_a[k] and via_b[k] with conductance via_g[k] (pcb.py:40-64).
  * The returned handle works with every apply, solve, batch and server call.
- * An operator of 2^27 nodes or more is PCBHIP_INVALID_INPUT. */
+ * An operator of 2^25 nodes or more is PCBHIP_INVALID_INPUT. */
 int pcbhip_sheet_operator_create(int device, int layers, int rows, int cols, const double *coef,
                                  const uint8_t *free_nodes, int64_t via_count, const int64_t *via_a,
                                  const int64_t *via_b, const double *via_g, pcbhip_operator **out);

@@ -280,10 +280,7 @@ struct SlotHost {
     double* b = nullptr;
     double* x[2] = {nullptr, nullptr};
     double* x0 = nullptr;
-    void* r[2] = {nullptr, nullptr};
-    void* p[2] = {nullptr, nullptr};
-    void* q[2] = {nullptr, nullptr};
-    void* c = nullptr;
+    void* state[2] = {nullptr, nullptr};
     double* part = nullptr;
     double* cg = nullptr;
     double* ch = nullptr;
@@ -434,12 +431,8 @@ SlotDev slot_dev(const SlotHost& s) {
     d.b64 = s.b;
     d.x64[0] = s.x[0];
     d.x64[1] = s.x[1];
-    for (int k = 0; k < 2; ++k) {
-        d.r[k] = s.r[k];
-        d.p[k] = s.p[k];
-        d.q[k] = s.q[k];
-    }
-    d.c = s.c;
+    d.state[0] = s.state[0];
+    d.state[1] = s.state[1];
     d.part = s.part;
     d.cg = s.cg;
     d.ch = s.ch;
@@ -455,12 +448,12 @@ SlotConfig slot_config(const pcbhip_mpir_config& c) {
                       c.max_inner_iterations};
 }
 
-// Device bytes of one slot's vectors: b and the x pair in FP64, the r, p and
-// q pairs and c in the inner precision, the partial sums and, for a
-// two-level slot, its three coarse vectors.
+// Device bytes of one slot's vectors: b and the x pair in FP64, the pair of
+// KNode state arrays (r, q, p, c per node) in the inner precision, the
+// partial sums and, for a two-level slot, its three coarse vectors.
 size_t slot_vector_bytes(const SlotHost& s, int inner64) {
     const size_t ti = inner64 ? sizeof(double) : sizeof(float);
-    return 3 * align256(sizeof(double) * s.op->N) + 7 * align256(ti * s.op->N) +
+    return 3 * align256(sizeof(double) * s.op->N) + 2 * align256(4 * ti * s.op->N) +
            align256(sizeof(double) * kInnerSums * (size_t)s.part_stride) +
            3 * align256(sizeof(double) * (size_t)s.view.nc);
 }
@@ -469,20 +462,16 @@ size_t slot_vector_bytes(const SlotHost& s, int inner64) {
 // counts them (the batch adds an x0 buffer behind its own block).
 void carve_slot(SlotHost& s, char* base, int inner64) {
     const size_t ti = inner64 ? sizeof(double) : sizeof(float);
-    const size_t n64 = align256(sizeof(double) * s.op->N), nti = align256(ti * s.op->N);
+    const size_t n64 = align256(sizeof(double) * s.op->N), nst = align256(4 * ti * s.op->N);
     s.block = base;
     s.b = reinterpret_cast<double*>(base);
     s.x[0] = reinterpret_cast<double*>(base + n64);
     s.x[1] = reinterpret_cast<double*>(base + 2 * n64);
     base += 3 * n64;
-    for (int k = 0; k < 2; ++k) {
-        s.r[k] = base + k * nti;
-        s.p[k] = base + (2 + k) * nti;
-        s.q[k] = base + (4 + k) * nti;
-    }
-    s.c = base + 6 * nti;
-    s.part = reinterpret_cast<double*>(base + 7 * nti);
-    base += 7 * nti + align256(sizeof(double) * kInnerSums * (size_t)s.part_stride);
+    s.state[0] = base;
+    s.state[1] = base + nst;
+    s.part = reinterpret_cast<double*>(base + 2 * nst);
+    base += 2 * nst + align256(sizeof(double) * kInnerSums * (size_t)s.part_stride);
     const size_t ncb = align256(sizeof(double) * (size_t)s.view.nc);
     s.cg = s.view.nc ? reinterpret_cast<double*>(base) : nullptr;
     s.ch = s.view.nc ? reinterpret_cast<double*>(base + ncb) : nullptr;
@@ -1261,10 +1250,11 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
         const long long E = (long long)slabs * rows * cols;
         const long long N = (long long)(slabs + 1) * (rows + 1) * (cols + 1);
         if (N > (1LL << 31) - 1) fail(PCBHIP_INVALID_INPUT, "node count exceeds 2^31 - 1");
-        // Stencil offsets inside a node plane are 32-bit byte offsets of FP64
-        // values below the buffer range (2^30) and the out-of-grid offset.
-        if ((long long)(rows + 1) * (cols + 1) >= (1LL << 27))
-            fail(PCBHIP_INVALID_INPUT, "a node plane must hold fewer than 2^27 nodes");
+        // Stencil offsets inside a node plane are 32-bit byte offsets below
+        // the buffer range (2^30) of the widest per-node record, the FP64
+        // inner state of 32 bytes.
+        if ((long long)(rows + 1) * (cols + 1) >= (1LL << 25))
+            fail(PCBHIP_INVALID_INPUT, "a node plane must hold fewer than 2^25 nodes");
         std::vector<double> coef64(3 * (size_t)E), robin64((size_t)N);
         std::vector<float> coef32(3 * (size_t)E), robin32((size_t)N);
         for (long long i = 0; i < 3 * E; ++i) {
@@ -1364,8 +1354,9 @@ int pcbhip_sheet_operator_create(int device, int layers, int rows, int cols, con
         if (N > (1LL << 31) - 1 || 2 * via_count > (1LL << 31) - 1)
             fail(PCBHIP_INVALID_INPUT, "node or via count exceeds 2^31 - 1");
         // A via's far end is loaded at its node index as a 32-bit byte offset
-        // of an FP64 value below the buffer range (2^30).
-        if (N >= (1LL << 27)) fail(PCBHIP_INVALID_INPUT, "a sheet operator must hold fewer than 2^27 nodes");
+        // below the buffer range (2^30) of the widest per-node record, the
+        // FP64 inner state of 32 bytes.
+        if (N >= (1LL << 25)) fail(PCBHIP_INVALID_INPUT, "a sheet operator must hold fewer than 2^25 nodes");
         std::vector<double> coef64(2 * (size_t)E), robin64((size_t)N);
         std::vector<float> coef32(2 * (size_t)E), robin32((size_t)N);
         for (long long i = 0; i < 2 * E; ++i) {

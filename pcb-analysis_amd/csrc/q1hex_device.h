@@ -140,20 +140,25 @@ __host__ __device__ inline int sheet_cell_count(const int layers, const int R, c
     return layers * cell_bands(R) * nsx;
 }
 
-// One right-hand side.  ``x64``, ``r``, ``p`` and ``q`` are ping-pong
-// pairs: a stencil kernel reads one buffer at its halo while neighbouring
-// workgroups write the other, so no workgroup ever reads a value another
-// rewrites in the same launch.  r/p/q/c are the inner-precision residual,
-// direction, operator-times-direction and correction; c is read and written
-// only at owned nodes, so one buffer serves.
+// The inner Krylov vectors of one node side by side: the inner-precision
+// residual r, operator-times-direction q, direction p and correction c.  A
+// pass moves a node's four words with one 16-byte access each way (two for
+// FP64), so it streams one state array in and one out instead of four and
+// four.
+template <typename TI>
+struct KNode {
+    TI r, q, p, c;
+};
+
+// One right-hand side.  ``x64`` and ``state`` are ping-pong pairs: a stencil
+// kernel reads one buffer at its halo while neighbouring workgroups write
+// the other, so no workgroup ever reads a value another rewrites in the same
+// launch.  ``state`` holds (N) KNode<TI> in the inner precision.
 struct SlotDev {
     OperatorDev op;
     const double* b64;  // (N) right-hand side
     double* x64[2];     // (N) solution pair
-    void* r[2];
-    void* p[2];
-    void* q[2];
-    void* c;
+    void* state[2];
     double* part;       // kInnerSums * part_stride partial sums
     double* history;    // 4 * max_outer: (outer, fp64 rel before, inner its, inner rel)
     // Coarse vectors of a two-level slot (nc each): g the patch sums of the
@@ -172,7 +177,8 @@ struct SlotConfig {
 
 // Mutable scalars of one slot.  Written only by the one-workgroup "finish"
 // kernels that follow each compute kernel; read by every later launch.
-// ``pcur`` selects the current buffer of the r, p and q pairs.
+// ``pcur`` selects the current buffer of the state pair; when an inner solve
+// ends, its final correction is the c of the other buffer.
 struct SlotState {
     int phase, first, pending, outer;
     int inner_its, total_inner, high_apps, low_apps;

@@ -66,6 +66,51 @@ __device__ __forceinline__ void stt(T* base, const unsigned off, const T v) {
     store(v, rsrc(base), off);
 }
 
+// Byte offset of the KNode at node offset ``off``: node offsets at or past
+// 2^30 / sizeof(KNode) (kOutside among them) are clamped to that bound, which
+// lands at byte 2^30, past kRange, so they read zero as the scalar loads do
+// (kOutside times 16 or 32 would wrap to 0 in 32 bits).  Operator creation
+// keeps real offsets below the bound.
+template <typename TI>
+__device__ __forceinline__ unsigned knode_bytes(const unsigned off) {
+    constexpr unsigned kSize = sizeof(KNode<TI>);
+    constexpr unsigned kBound = (1u << 30) / kSize;
+    return (off < kBound ? off : kBound) * kSize;
+}
+// The loaded vectors are bit-cast whole: this compiler's __builtin_bit_cast
+// of a single vector element (v.y) reads element 0.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ KNode<float> ld_knode(const KNode<float>* base, const unsigned off) {
+    const f32x4 v = __builtin_bit_cast(
+        f32x4, __builtin_amdgcn_raw_buffer_load_b128(rsrc(base), knode_bytes<float>(off), 0, 0));
+    return {v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ KNode<double> ld_knode(const KNode<double>* base, const unsigned off) {
+    const Rsrc r = rsrc(base);
+    const unsigned b = knode_bytes<double>(off);
+    const f64x2 lo = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, b, 0, 0));
+    const f64x2 hi = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, b + 16u, 0, 0));
+    return {lo.x, lo.y, hi.x, hi.y};
+}
+__device__ __forceinline__ void st_knode(KNode<float>* base, const unsigned off, const KNode<float>& v) {
+    const f32x4 w{v.r, v.q, v.p, v.c};
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, w), rsrc(base), knode_bytes<float>(off), 0, 0);
+}
+__device__ __forceinline__ void st_knode(KNode<double>* base, const unsigned off, const KNode<double>& v) {
+    const f64x2 lo{v.r, v.q}, hi{v.p, v.c};
+    const Rsrc d = rsrc(base);
+    const unsigned b = knode_bytes<double>(off);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, lo), d, b, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, hi), d, b + 16u, 0, 0);
+}
+// One word of a KNode (0 r, 1 q, 2 p, 3 c).
+template <typename TI>
+__device__ __forceinline__ TI ld_knode_word(const KNode<TI>* base, const unsigned off, const int word) {
+    return ld(rsrc(reinterpret_cast<const TI*>(base) + word), knode_bytes<TI>(off) / (unsigned)sizeof(TI), TI());
+}
+
 __device__ __forceinline__ float fmadd(const float a, const float b, const float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmadd(const double a, const double b, const double c) {
     return __builtin_fma(a, b, c);
@@ -1172,8 +1217,11 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
         const double cw = pending == kPendingCorrection ? 1.0 : 0.0;
         const double* xin = sd.x64[xcur];
         double* xout = sd.x64[1 - xcur];
-        const TI* __restrict__ cin = static_cast<const TI*>(sd.c);
-        TI* __restrict__ rout = static_cast<TI*>(sd.r[st.pcur]);
+        // The pending correction is the c of the buffer that the inner
+        // solve's last pass wrote; the new inner residual starts the other
+        // buffer with zero q, p and c.
+        const KNode<TI>* __restrict__ cin = static_cast<const KNode<TI>*>(sd.state[1 - st.pcur]);
+        KNode<TI>* __restrict__ sout = static_cast<KNode<TI>*>(sd.state[st.pcur]);
         const TI* __restrict__ dinv = InnerOf<TI>::dinv(op);
         const double* __restrict__ robin = op.robin64;
         const double* __restrict__ b = sd.b64;
@@ -1182,7 +1230,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
         acc.clear();
         auto load = [&](const long long pb, const unsigned off, const bool, const int,
                         const int layer) -> NodeVal<double> {
-            const double xv = ldt(xin + pb, off) + cw * (double)ldt(cin + pb, off);
+            const double xv = ldt(xin + pb, off) + cw * (double)ld_knode_word(cin + pb, off, 3);
             return {xv, ldt(robin + pb, mask_bit(op.plain, layer) ? kOutside : off)};
         };
         auto emit = [&](const long long pb, const unsigned off, const int row, const double conduction,
@@ -1192,7 +1240,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_outer(const SlotDev* __restri
             const double bv = ldt(b + pb, off);
             const double r = bv - ax;
             const TI ri = (TI)r;
-            stt(rout + pb, off, ri);
+            st_knode(sout + pb, off, KNode<TI>{ri, TI(0), TI(0), TI(0)});
             const int g = row / kCellRows;
             acc.t[0][g] += bv * bv;
             acc.t[1][g] += r * r;
@@ -1298,27 +1346,26 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish_outer(const SlotDev* 
 // beta_{i+1} from r_i.z_i = r.z - 2 alpha q.z + alpha^2 q.D^-1 q, the one
 // value the pass cannot yet have summed.
 // The fused pass's node loader, in two phases (see fetch_node): fetch issues
-// the node's loads of r, q, p, 1/diag, the correction (owned nodes only;
-// elsewhere the offset is out of range and reads zero) and Robin (through
-// the out-of-range offset on a plain plane), and for a two-level slot the
-// coarse term w at its patch; make forms
-//   r = r - alpha q,  z = r D^-1 (+ w),  p = z + beta p,  c = keep c + alpha p.
+// the node's loads of its state (r, q, p, c in one access), 1/diag and Robin
+// (through the out-of-range offset on a plain plane), and for a two-level
+// slot the coarse term w at its patch; make forms
+//   r = r - alpha q,  z = r D^-1 (+ w),  p = z + beta p,  c = keep c + alpha p
+// (c is formed at halo nodes too, and only owned nodes store it).
 template <typename TI>
 struct PassRaw {
-    TI rv, qv, pv, d, cv, rb;
+    KNode<TI> s;
+    TI d, rb;
 };
 template <typename TI>
 struct PassRawZ {
-    TI rv, qv, pv, d, cv, rb;
+    KNode<TI> s;
+    TI d, rb;
     double w;
     bool in;
 };
 template <typename TI, bool COARSE>
 struct PassLoader {
-    const TI* rin;
-    const TI* qin;
-    const TI* pin;
-    const TI* c;
+    const KNode<TI>* sin;
     const TI* dinv;
     const TI* robin;
     unsigned long long plain;
@@ -1330,9 +1377,8 @@ struct PassLoader {
 
     __device__ auto fetch(const long long pb, const unsigned off, const bool owned, const int row,
                           const int layer) const {
-        const TI rv = ldt(rin + pb, off), qv = ldt(qin + pb, off), pv = ldt(pin + pb, off);
+        const KNode<TI> s = ld_knode(sin + pb, off);
         const TI d = ldt(dinv + pb, off);
-        const TI cv = ldt(c + pb, owned ? off : kOutside);
         const TI rb = ldt(robin + pb, mask_bit(plain, layer) ? kOutside : off);
         if constexpr (COARSE) {
             // The patch index does not depend on loaded data and is always in
@@ -1350,23 +1396,23 @@ struct PassLoader {
                 const int y = rem / (C + 1), x = rem - y * (C + 1);
                 patch = (l * crows + y / cblock) * ccols + x / cblock;
             }
-            return PassRawZ<TI>{rv, qv, pv, d, cv, rb, cw[patch], off != kOutside};
+            return PassRawZ<TI>{s, d, rb, cw[patch], off != kOutside};
         } else {
-            return PassRaw<TI>{rv, qv, pv, d, cv, rb};
+            return PassRaw<TI>{s, d, rb};
         }
     }
     __device__ InnerVal<TI> make(const PassRaw<TI>& x) const {
-        const TI r = x.rv - alpha * x.qv;
-        const TI p = r * x.d + beta * x.pv;
-        return InnerVal<TI>{p, x.rb, r, x.d, keep * x.cv + alpha * x.pv};
+        const TI r = x.s.r - alpha * x.s.q;
+        const TI p = r * x.d + beta * x.s.p;
+        return InnerVal<TI>{p, x.rb, r, x.d, keep * x.s.c + alpha * x.s.p};
     }
     __device__ InnerValZ<TI> make(const PassRawZ<TI>& x) const {
-        const TI r = x.rv - alpha * x.qv;
+        const TI r = x.s.r - alpha * x.s.q;
         // The free mask is applied after the load.
         const TI wv = (x.in && x.rb >= TI(0)) ? (TI)x.w : TI(0);
         const TI z = r * x.d + wv;
-        const TI p = z + beta * x.pv;
-        return InnerValZ<TI>{p, x.rb, r, x.d, keep * x.cv + alpha * x.pv, z};
+        const TI p = z + beta * x.s.p;
+        return InnerValZ<TI>{p, x.rb, r, x.d, keep * x.s.c + alpha * x.s.p, z};
     }
 };
 
@@ -1383,13 +1429,8 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
     const TI beta = first ? TI(0) : (TI)st.beta;
     const TI keep = first ? TI(0) : TI(1);
     const int cur = st.pcur;
-    const TI* __restrict__ rin = static_cast<const TI*>(sd.r[cur]);
-    const TI* __restrict__ qin = static_cast<const TI*>(sd.q[cur]);
-    const TI* __restrict__ pin = static_cast<const TI*>(sd.p[cur]);
-    TI* __restrict__ rout = static_cast<TI*>(sd.r[1 - cur]);
-    TI* __restrict__ qout = static_cast<TI*>(sd.q[1 - cur]);
-    TI* __restrict__ pout = static_cast<TI*>(sd.p[1 - cur]);
-    TI* __restrict__ c = static_cast<TI*>(sd.c);
+    const KNode<TI>* __restrict__ sin = static_cast<const KNode<TI>*>(sd.state[cur]);
+    KNode<TI>* __restrict__ sout = static_cast<KNode<TI>*>(sd.state[1 - cur]);
     const TI* __restrict__ dinv = InnerOf<TI>::dinv(op);
     const TI* __restrict__ robin = InnerOf<TI>::robin(op);
     CellSums<kInnerSums, ROWS> acc;
@@ -1408,18 +1449,13 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
 #pragma unroll
         for (int i = 0; i < ROWS + 2; ++i) crow[i] = min(max(ybase - 1 + i, 0), op.R) / cb;
     }
-    // The correction is loaded at owned nodes only (elsewhere the offset is
-    // out of range and reads zero).
-    const PassLoader<TI, COARSE> load{rin, qin, pin, c, dinv, robin, op.plain, alpha, beta, keep, sd.cw, op.crows,
+    const PassLoader<TI, COARSE> load{sin, dinv, robin, op.plain, alpha, beta, keep, sd.cw, op.crows,
                                       op.ccols, op.cblock, op.R, op.C, ccol, crow};
     // Stores wait for their plane's emit, so a plane's loads issue back to back.
     auto emit = [&](const long long pb, const unsigned off, const int row, const TI conduction, const auto& v,
                     auto parity) {
         const TI ap = v.robin >= TI(0) ? conduction + v.robin * v.u : v.u;
-        stt(pout + pb, off, v.u);
-        stt(qout + pb, off, ap);
-        stt(rout + pb, off, v.r);
-        stt(c + pb, off, v.c);
+        st_knode(sout + pb, off, KNode<TI>{v.r, ap, v.u, v.c});
         TI z;
         if constexpr (COARSE)
             z = v.z;
@@ -1690,10 +1726,11 @@ __device__ __forceinline__ double coarse_row(const float* __restrict__ cinv, con
 
 // Z^T v at one patch by one wave: lane l takes the patch's nodes l, l + 64,
 // ... in row-major order (free nodes only), then the butterfly.  Every lane
-// returns the sum.
+// returns the sum.  Node n's value is v[n * stride] (a word of a KNode array
+// has stride 4).
 template <typename TI>
-__device__ double patch_sum_wave(const OperatorDev& op, const TI* __restrict__ v, const TI* __restrict__ robin,
-                                 const int layer, const int cy, const int cx) {
+__device__ double patch_sum_wave(const OperatorDev& op, const TI* __restrict__ v, const int stride,
+                                 const TI* __restrict__ robin, const int layer, const int cy, const int cx) {
     const int lane = threadIdx.x & (kWave - 1);
     const int b = op.cblock;
     const long long NC = op.C + 1, NR = op.R + 1;
@@ -1705,7 +1742,7 @@ __device__ double patch_sum_wave(const OperatorDev& op, const TI* __restrict__ v
     for (int k = lane; k < count; k += kWave) {
         const int yy = k / w, xx = k - yy * w;
         const long long idx = ((long long)layer * NR + y_lo + yy) * NC + x_lo + xx;
-        const TI value = v[idx];
+        const TI value = v[idx * stride];
         if (robin[idx] >= TI(0)) s += (double)value;
     }
     return wave_sum(s);
@@ -1730,8 +1767,10 @@ __global__ __launch_bounds__(kGroupThreads) void k_coarse_restrict(const SlotDev
         if (st.phase != kPhaseInner || (which == 0 && st.first == 0)) continue;
         const SlotDev& sd = slots[tile.slot];
         const OperatorDev& op = sd.op;
-        const TI* v = static_cast<const TI*>(which == 0 ? sd.r[st.pcur] : sd.q[1 - st.pcur]);
-        const double s = patch_sum_wave<TI>(op, v, InnerOf<TI>::robin(op), tile.sz, tile.sy, tile.sx);
+        // r of the current state, or q of the state the pass just wrote.
+        const KNode<TI>* kn = static_cast<const KNode<TI>*>(which == 0 ? sd.state[st.pcur] : sd.state[1 - st.pcur]);
+        const TI* v = which == 0 ? &kn->r : &kn->q;
+        const double s = patch_sum_wave<TI>(op, v, 4, InnerOf<TI>::robin(op), tile.sz, tile.sy, tile.sx);
         if ((threadIdx.x & (kWave - 1)) == 0)
             sd.cg[((long long)tile.sz * op.crows + tile.sy) * op.ccols + tile.sx] = s;
     }
