@@ -214,6 +214,8 @@ struct pcbhip_operator {
     float* dinv32 = nullptr;
     double* dinv64 = nullptr;
     double* diag64 = nullptr;
+    float* nrec32 = nullptr;  // (N, 2) records (1 / diagonal, robin)
+    double* nrec64 = nullptr;
     int kind = kHex;
     int* via_ptr = nullptr;  // sheet operators: node-ordered via CSR (null without vias)
     int* via_nbr = nullptr;
@@ -248,6 +250,8 @@ struct pcbhip_operator {
         v.uniform = uniform;
         v.dinv32 = dinv32;
         v.dinv64 = dinv64;
+        v.nrec32 = nrec32;
+        v.nrec64 = nrec64;
         v.kind = kind;
         v.via_ptr = via_ptr;
         v.via_nbr = via_nbr;
@@ -1288,12 +1292,20 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
         std::vector<double> uvalues64;
         op->uniform = uniform_layers(coef64, slabs, (long long)rows * cols, uvalues64);
         const std::vector<float> uvalues32(uvalues64.begin(), uvalues64.end());
+        // On the device an element's coefficients are one record (a, b, c).
+        std::vector<double> rec64(3 * (size_t)E);
+        std::vector<float> rec32(3 * (size_t)E);
+        for (long long e = 0; e < E; ++e)
+            for (int d = 0; d < 3; ++d) {
+                rec64[(size_t)(3 * e + d)] = coef64[(size_t)(d * E + e)];
+                rec32[(size_t)(3 * e + d)] = coef32[(size_t)(d * E + e)];
+            }
         const size_t s_c32 = align256(sizeof(float) * 3 * E), s_c64 = align256(sizeof(double) * 3 * E);
         const size_t s_n32 = align256(sizeof(float) * N), s_n64 = align256(sizeof(double) * N);
         const size_t s_u32 = align256(sizeof(float) * uvalues32.size());
         const size_t s_u64 = align256(sizeof(double) * uvalues64.size());
-        char* base = static_cast<char*>(
-            stream_alloc(op->stream, s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + s_u32 + s_u64 + 256));
+        char* base = static_cast<char*>(stream_alloc(
+            op->stream, s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + 2 * s_n32 + 2 * s_n64 + s_u32 + s_u64 + 256));
         op->block = base;
         op->coef32 = reinterpret_cast<float*>(base);
         base += s_c32;
@@ -1309,6 +1321,10 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
         base += s_n64;
         op->diag64 = reinterpret_cast<double*>(base);
         base += s_n64;
+        op->nrec32 = reinterpret_cast<float*>(base);
+        base += 2 * s_n32;
+        op->nrec64 = reinterpret_cast<double*>(base);
+        base += 2 * s_n64;
         op->ucoef32 = reinterpret_cast<float*>(base);
         base += s_u32;
         op->ucoef64 = reinterpret_cast<double*>(base);
@@ -1320,9 +1336,9 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
         check(hipMemcpyAsync(op->ucoef64, uvalues64.data(), sizeof(double) * uvalues64.size(), hipMemcpyHostToDevice,
                              op->stream),
               "upload");
-        check(hipMemcpyAsync(op->coef32, coef32.data(), sizeof(float) * 3 * E, hipMemcpyHostToDevice, op->stream),
+        check(hipMemcpyAsync(op->coef32, rec32.data(), sizeof(float) * 3 * E, hipMemcpyHostToDevice, op->stream),
               "upload");
-        check(hipMemcpyAsync(op->coef64, coef64.data(), sizeof(double) * 3 * E, hipMemcpyHostToDevice, op->stream),
+        check(hipMemcpyAsync(op->coef64, rec64.data(), sizeof(double) * 3 * E, hipMemcpyHostToDevice, op->stream),
               "upload");
         check(hipMemcpyAsync(op->robin32, robin32.data(), sizeof(float) * N, hipMemcpyHostToDevice, op->stream),
               "upload");
@@ -1332,6 +1348,9 @@ int pcbhip_operator_create(int device, int slabs, int rows, int cols, const doub
         check(launch::diagonal(op->coef64, op->robin64, op->diag64, op->dinv64, op->dinv32, slabs, rows, cols, d_bad,
                                op->stream),
               "diagonal launch");
+        check(launch::pack_nodes(op->dinv32, op->robin32, op->dinv64, op->robin64, op->nrec32, op->nrec64, N,
+                                 op->stream),
+              "node records");
         int bad = 0;
         check(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, op->stream), "copy");
         check(hipStreamSynchronize(op->stream), "operator create");
@@ -1410,7 +1429,8 @@ int pcbhip_sheet_operator_create(int device, int layers, int rows, int cols, con
         op->plain = plain_planes(robin64, layers, (long long)(rows + 1) * (cols + 1));
         const size_t s_c32 = align256(sizeof(float) * 2 * E), s_c64 = align256(sizeof(double) * 2 * E);
         const size_t s_n32 = align256(sizeof(float) * N), s_n64 = align256(sizeof(double) * N);
-        char* base = static_cast<char*>(stream_alloc(op->stream, s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + 256));
+        char* base = static_cast<char*>(
+            stream_alloc(op->stream, s_c32 + s_c64 + 2 * s_n32 + 3 * s_n64 + 2 * s_n32 + 2 * s_n64 + 256));
         op->block = base;
         op->coef32 = reinterpret_cast<float*>(base);
         base += s_c32;
@@ -1426,6 +1446,10 @@ int pcbhip_sheet_operator_create(int device, int layers, int rows, int cols, con
         base += s_n64;
         op->diag64 = reinterpret_cast<double*>(base);
         base += s_n64;
+        op->nrec32 = reinterpret_cast<float*>(base);
+        base += 2 * s_n32;
+        op->nrec64 = reinterpret_cast<double*>(base);
+        base += 2 * s_n64;
         int* d_bad = reinterpret_cast<int*>(base);
         if (via_count > 0) {
             const size_t s_ptr = align256(sizeof(int) * ((size_t)N + 1)), s_nbr = align256(sizeof(int) * nbr.size());
@@ -1457,6 +1481,9 @@ int pcbhip_sheet_operator_create(int device, int layers, int rows, int cols, con
         check(hipMemsetAsync(d_bad, 0, sizeof(int), op->stream), "memset");
         check(launch::sheet_diagonal(op->view(), op->diag64, op->dinv64, op->dinv32, d_bad, op->stream),
               "diagonal launch");
+        check(launch::pack_nodes(op->dinv32, op->robin32, op->dinv64, op->robin64, op->nrec32, op->nrec64, N,
+                                 op->stream),
+              "node records");
         int bad = 0;
         check(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, op->stream), "copy");
         check(hipStreamSynchronize(op->stream), "operator create");
@@ -2161,8 +2188,9 @@ int pcbhip_operator_bytes(const pcbhip_operator* op, int64_t* bytes) {
         const size_t e = (size_t)op->E, n = (size_t)op->N;
         const size_t uniform_values = op->kind == kSheet ? 0 : align256(4 * 3 * kMaxUniformLayers) +
                                                                    align256(8 * 3 * kMaxUniformLayers);
-        *bytes = (int64_t)(align256(12 * e) + align256(24 * e) + align256(4 * n) + 2 * align256(8 * n) +
-                           align256(4 * n) + align256(8 * n) + uniform_values);
+        const size_t words = op->kind == kSheet ? 2 : 3;  // coefficients per element
+        *bytes = (int64_t)(align256(4 * words * e) + align256(8 * words * e) + align256(4 * n) + 2 * align256(8 * n) +
+                           align256(4 * n) + align256(8 * n) + align256(8 * n) + align256(16 * n) + uniform_values);
     });
 }
 

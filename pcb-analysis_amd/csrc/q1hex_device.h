@@ -75,8 +75,13 @@ enum Pending : int { kPendingNone = 0, kPendingCorrection = 1, kPendingZero = 2 
 // The operator of one problem, read-only on the device.  Coefficients are
 // stored pre-divided by 36: with M = W / 6, W = [[2, 1], [1, 2]], the element
 // action a U_x u is (a / 36) times integer-weighted sums of edge differences.
-// ``robin`` holds the lumped Robin conductance of a free node and -1 on a
-// fixed node (the free mask and the Robin term in one array).
+// A hexahedral element's three coefficients are one record (a, b, c), so a
+// stencil fetches an element row with one load instead of three loads from
+// three arrays.  ``robin`` holds the lumped Robin conductance of a free node
+// and -1 on a fixed node (the free mask and the Robin term in one array);
+// ``nrec`` repeats 1 / diag and robin per node as one record (d, robin), the
+// fused pass's one node-operator stream on planes that are not plain (plain
+// planes read ``dinv`` alone).
 //
 // A second operator kind, the layered-sheet DC operator of copper layers
 // joined by vias (src/electrical/matrix_free_mpir_fem/pcb.py:575-604): S node
@@ -103,7 +108,7 @@ constexpr int kMaxUniformLayers = 64;
 
 template <typename T>
 struct CoefSource {
-    const T* coef;               // (3, S, R, C)
+    const T* coef;               // (S, R, C, 3) element records (a, b, c)
     unsigned long long uniform;  // bit ez: layer ez is uniform
     const T* values;             // (kMaxUniformLayers, 3) a, b, c of the uniform layers, or null
 };
@@ -111,7 +116,7 @@ struct CoefSource {
 struct OperatorDev {
     int S, R, C, kind;
     long long N;  // nodes
-    const float* coef32;    // hex: (3, S, R, C) a/36, b/36, c/36; sheet: (2, S, R, C) c_x/6, c_y/6
+    const float* coef32;    // hex: (S, R, C, 3) records a/36, b/36, c/36; sheet: (2, S, R, C) c_x/6, c_y/6
     const double* coef64;
     const float* robin32;   // (N)
     const double* robin64;
@@ -122,6 +127,8 @@ struct OperatorDev {
     unsigned long long uniform;  // hex: bit ez, every element of layer ez has the same coefficients
     const float* dinv32;    // (N) 1 / diagonal, 1 on fixed rows
     const double* dinv64;
+    const float* nrec32;    // (N, 2) records (1 / diagonal, robin)
+    const double* nrec64;
     const int* via_ptr;     // sheet: (N + 1) CSR offsets, or null without vias
     const int* via_nbr;     // sheet: other end of each via entry
     const float* via_g32;   // sheet: via conductance of each entry
@@ -233,6 +240,10 @@ template <typename T>
 hipError_t sheet_apply(const OperatorDev& op, const T* x, T* y, int mode, hipStream_t stream);
 hipError_t sheet_diagonal(const OperatorDev& op, double* diag, double* dinv64, float* dinv32, int* bad,
                           hipStream_t stream);
+// The node records (1 / diagonal, robin) of either operator kind, in both
+// precisions, from the separate arrays.
+hipError_t pack_nodes(const float* dinv32, const float* robin32, const double* dinv64, const double* robin64,
+                      float* nrec32, double* nrec64, long long n, hipStream_t stream);
 hipError_t copy(const float* src, float* dst, long long n_floats, hipStream_t stream);
 // Bandwidth reference: 1..kStreamsMax input and 0..kStreamsMax output arrays
 // of n_floats (a multiple of 4) streamed together.

@@ -111,6 +111,43 @@ __device__ __forceinline__ TI ld_knode_word(const KNode<TI>* base, const unsigne
     return ld(rsrc(reinterpret_cast<const TI*>(base) + word), knode_bytes<TI>(off) / (unsigned)sizeof(TI), TI());
 }
 
+// Operator records: an element's (a, b, c) and a node's (1 / diag, robin),
+// one access each.  Record offsets are clamped as KNode offsets are, so
+// kOutside reads zero.
+template <typename T, int WORDS>
+__device__ __forceinline__ unsigned rec_bytes(const unsigned off) {
+    constexpr unsigned kSize = WORDS * sizeof(T);
+    constexpr unsigned kBound = ((1u << 30) + kSize - 1) / kSize;
+    return (off < kBound ? off : kBound) * kSize;
+}
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x3 __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ void ld_rec3(const Rsrc r, const unsigned off, float& a, float& b, float& c) {
+    const f32x3 v = __builtin_bit_cast(f32x3, __builtin_amdgcn_raw_buffer_load_b96(r, rec_bytes<float, 3>(off), 0, 0));
+    a = v.x;
+    b = v.y;
+    c = v.z;
+}
+__device__ __forceinline__ void ld_rec3(const Rsrc r, const unsigned off, double& a, double& b, double& c) {
+    const unsigned o = rec_bytes<double, 3>(off);
+    const f64x2 ab = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0));
+    c = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, o + 16u, 0, 0));
+    a = ab.x;
+    b = ab.y;
+}
+__device__ __forceinline__ void ld_rec2(const float* base, const unsigned off, float& d, float& rb) {
+    const f32x2 v =
+        __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(rsrc(base), rec_bytes<float, 2>(off), 0, 0));
+    d = v.x;
+    rb = v.y;
+}
+__device__ __forceinline__ void ld_rec2(const double* base, const unsigned off, double& d, double& rb) {
+    const f64x2 v = __builtin_bit_cast(
+        f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc(base), rec_bytes<double, 2>(off), 0, 0));
+    d = v.x;
+    rb = v.y;
+}
+
 __device__ __forceinline__ float fmadd(const float a, const float b, const float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmadd(const double a, const double b, const double c) {
     return __builtin_fma(a, b, c);
@@ -370,7 +407,6 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
     const bool out_lane = lane >= 1 && lane <= kWave - 2 && col_in;
     const long long NC = C + 1;
     const long long NR = R + 1;
-    const long long E = (long long)S * R * C;
 
     // Lane offsets of the strip's node rows inside a plane and of its element
     // rows inside a layer, fixed for the whole sweep.
@@ -450,8 +486,7 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
     // The three coefficient rows of element layer ez (loads only).
     auto fetch_layer = [&](const int ez, T (&ca)[ROWS + 1], T (&cb)[ROWS + 1], T (&cc)[ROWS + 1]) {
         // Uniform layers as described at CoefSource.
-        const T* layer_coef = cs.coef + (long long)ez * R * C;
-        const Rsrc ra = rsrc(layer_coef), rb = rsrc(layer_coef + E), rc = rsrc(layer_coef + 2 * E);
+        const Rsrc rl = rsrc(cs.coef + 3 * (long long)ez * R * C);
         const bool uniform = mask_bit(cs.uniform, ez);
         if constexpr (TABLE) {
             const T* uv = cs.values + 3 * (ez & (kMaxUniformLayers - 1));
@@ -459,7 +494,8 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
 #pragma unroll
             for (int j = 0; j < ROWS + 1; ++j) {
                 const unsigned off = uniform ? kOutside : eoff[j];
-                const T la = ld(ra, off, T()), lb = ld(rb, off, T()), lc = ld(rc, off, T());
+                T la, lb, lc;
+                ld_rec3(rl, off, la, lb, lc);
                 const bool in = eoff[j] != kOutside;
                 ca[j] = uniform ? (in ? ua : T(0)) : la;
                 cb[j] = uniform ? (in ? ub : T(0)) : lb;
@@ -470,11 +506,7 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
             // with it maps them to the first element and leaves kOutside.
             const unsigned keep = uniform ? kOutside : ~0u;
 #pragma unroll
-            for (int j = 0; j < ROWS + 1; ++j) {
-                ca[j] = ld(ra, eoff[j] & keep, T());
-                cb[j] = ld(rb, eoff[j] & keep, T());
-                cc[j] = ld(rc, eoff[j] & keep, T());
-            }
+            for (int j = 0; j < ROWS + 1; ++j) ld_rec3(rl, eoff[j] & keep, ca[j], cb[j], cc[j]);
         }
     };
 
@@ -1031,6 +1063,20 @@ __global__ __launch_bounds__(256) void k_sheet_diagonal(const OperatorDev op, do
     dinv32[n] = (float)(1.0 / value);
 }
 
+// The node records (1 / diagonal, robin) from the separate arrays.
+__global__ __launch_bounds__(256) void k_pack_nodes(const float* __restrict__ dinv32,
+                                                    const float* __restrict__ robin32,
+                                                    const double* __restrict__ dinv64,
+                                                    const double* __restrict__ robin64, float* __restrict__ nrec32,
+                                                    double* __restrict__ nrec64, const long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    nrec32[2 * i] = dinv32[i];
+    nrec32[2 * i + 1] = robin32[i];
+    nrec64[2 * i] = dinv64[i];
+    nrec64[2 * i + 1] = robin64[i];
+}
+
 // Jacobi diagonal: sum of the element diagonals ((a + b + c) / 9 for every
 // corner of the trilinear unit tensors) over the adjacent elements plus the
 // Robin conductance; 1 on fixed rows (operator.py:289-306).  ``bad`` counts
@@ -1041,7 +1087,6 @@ __global__ __launch_bounds__(256) void k_diagonal(const double* __restrict__ coe
                                                   int* bad) {
     const long long NC = C + 1, NR = R + 1;
     const long long N = (long long)(S + 1) * NR * NC;
-    const long long E = (long long)S * R * C;
     const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const int x = (int)(n % NC);
@@ -1055,7 +1100,7 @@ __global__ __launch_bounds__(256) void k_diagonal(const double* __restrict__ coe
             for (int ex = x - 1; ex <= x; ++ex) {
                 if (ex < 0 || ex >= C) continue;
                 const long long e = ((long long)ez * R + ey) * C + ex;
-                d += 4.0 * ((coef[e] + coef[E + e]) + coef[2 * E + e]);
+                d += 4.0 * ((coef[3 * e] + coef[3 * e + 1]) + coef[3 * e + 2]);
             }
         }
     }
@@ -1173,6 +1218,7 @@ struct InnerOf;
 template <>
 struct InnerOf<float> {
     __device__ static const float* dinv(const OperatorDev& op) { return op.dinv32; }
+    __device__ static const float* nrec(const OperatorDev& op) { return op.nrec32; }
     __device__ static const float* coef(const OperatorDev& op) { return op.coef32; }
     __device__ static CoefSource<float> coefs(const OperatorDev& op) { return {op.coef32, op.uniform, nullptr}; }
     __device__ static const float* robin(const OperatorDev& op) { return op.robin32; }
@@ -1181,6 +1227,7 @@ struct InnerOf<float> {
 template <>
 struct InnerOf<double> {
     __device__ static const double* dinv(const OperatorDev& op) { return op.dinv64; }
+    __device__ static const double* nrec(const OperatorDev& op) { return op.nrec64; }
     __device__ static const double* coef(const OperatorDev& op) { return op.coef64; }
     __device__ static CoefSource<double> coefs(const OperatorDev& op) { return {op.coef64, op.uniform, nullptr}; }
     __device__ static const double* robin(const OperatorDev& op) { return op.robin64; }
@@ -1346,8 +1393,9 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish_outer(const SlotDev* 
 // beta_{i+1} from r_i.z_i = r.z - 2 alpha q.z + alpha^2 q.D^-1 q, the one
 // value the pass cannot yet have summed.
 // The fused pass's node loader, in two phases (see fetch_node): fetch issues
-// the node's loads of its state (r, q, p, c in one access), 1/diag and Robin
-// (through the out-of-range offset on a plain plane), and for a two-level
+// the node's loads of its state (r, q, p, c in one access), its operator
+// record (1/diag and Robin in one access; on a plain plane 1/diag alone, and
+// the record's load goes to the out-of-range offset), and for a two-level
 // slot the coarse term w at its patch; make forms
 //   r = r - alpha q,  z = r D^-1 (+ w),  p = z + beta p,  c = keep c + alpha p
 // (c is formed at halo nodes too, and only owned nodes store it).
@@ -1367,7 +1415,7 @@ template <typename TI, bool COARSE>
 struct PassLoader {
     const KNode<TI>* sin;
     const TI* dinv;
-    const TI* robin;
+    const TI* nrec;
     unsigned long long plain;
     TI alpha, beta, keep;
     const double* cw;
@@ -1378,8 +1426,11 @@ struct PassLoader {
     __device__ auto fetch(const long long pb, const unsigned off, const bool owned, const int row,
                           const int layer) const {
         const KNode<TI> s = ld_knode(sin + pb, off);
-        const TI d = ldt(dinv + pb, off);
-        const TI rb = ldt(robin + pb, mask_bit(plain, layer) ? kOutside : off);
+        const bool plain_plane = mask_bit(plain, layer);
+        const TI dp = ldt(dinv + pb, plain_plane ? off : kOutside);
+        TI dr, rb;
+        ld_rec2(nrec + 2 * pb, plain_plane ? kOutside : off, dr, rb);
+        const TI d = plain_plane ? dp : dr;
         if constexpr (COARSE) {
             // The patch index does not depend on loaded data and is always in
             // range, so w is fetched with the node's other loads.
@@ -1432,7 +1483,7 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
     const KNode<TI>* __restrict__ sin = static_cast<const KNode<TI>*>(sd.state[cur]);
     KNode<TI>* __restrict__ sout = static_cast<KNode<TI>*>(sd.state[1 - cur]);
     const TI* __restrict__ dinv = InnerOf<TI>::dinv(op);
-    const TI* __restrict__ robin = InnerOf<TI>::robin(op);
+    const TI* __restrict__ nrec = InnerOf<TI>::nrec(op);
     CellSums<kInnerSums, ROWS> acc;
     acc.clear();
     // Two-level slots: the coarse term of z at a node is w at its patch
@@ -1449,7 +1500,7 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
 #pragma unroll
         for (int i = 0; i < ROWS + 2; ++i) crow[i] = min(max(ybase - 1 + i, 0), op.R) / cb;
     }
-    const PassLoader<TI, COARSE> load{sin, dinv, robin, op.plain, alpha, beta, keep, sd.cw, op.crows,
+    const PassLoader<TI, COARSE> load{sin, dinv, nrec, op.plain, alpha, beta, keep, sd.cw, op.crows,
                                       op.ccols, op.cblock, op.R, op.C, ccol, crow};
     // Stores wait for their plane's emit, so a plane's loads issue back to back.
     auto emit = [&](const long long pb, const unsigned off, const int row, const TI conduction, const auto& v,
@@ -1957,6 +2008,14 @@ hipError_t diagonal(const double* coef, const double* robin, double* diag, doubl
     const int blocks = (int)((N + 255) / 256);
     hipLaunchKernelGGL(k_diagonal, dim3(blocks), dim3(256), 0, stream, coef, robin, diag, dinv64, dinv32, S, R, C,
                        bad);
+    return hipGetLastError();
+}
+
+hipError_t pack_nodes(const float* dinv32, const float* robin32, const double* dinv64, const double* robin64,
+                      float* nrec32, double* nrec64, long long n, hipStream_t stream) {
+    const int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_pack_nodes, dim3(blocks), dim3(256), 0, stream, dinv32, robin32, dinv64, robin64, nrec32,
+                       nrec64, n);
     return hipGetLastError();
 }
 
