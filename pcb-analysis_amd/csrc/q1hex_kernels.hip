@@ -135,19 +135,17 @@ __device__ __forceinline__ void ld_rec3(const Rsrc r, const unsigned off, double
     a = ab.x;
     b = ab.y;
 }
-__device__ __forceinline__ void ld_rec2(const float* base, const unsigned off, float& d, float& rb) {
-    const f32x2 v =
-        __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(rsrc(base), rec_bytes<float, 2>(off), 0, 0));
-    d = v.x;
-    rb = v.y;
+// Two consecutive words at byte offset ``bytes`` (at or past kRange: zero).
+__device__ __forceinline__ void ld_pair(const float* base, const unsigned bytes, float& a, float& b) {
+    const f32x2 v = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(rsrc(base), bytes, 0, 0));
+    a = v.x;
+    b = v.y;
 }
-__device__ __forceinline__ void ld_rec2(const double* base, const unsigned off, double& d, double& rb) {
-    const f64x2 v = __builtin_bit_cast(
-        f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc(base), rec_bytes<double, 2>(off), 0, 0));
-    d = v.x;
-    rb = v.y;
+__device__ __forceinline__ void ld_pair(const double* base, const unsigned bytes, double& a, double& b) {
+    const f64x2 v = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc(base), bytes, 0, 0));
+    a = v.x;
+    b = v.y;
 }
-
 __device__ __forceinline__ float fmadd(const float a, const float b, const float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmadd(const double a, const double b, const double c) {
     return __builtin_fma(a, b, c);
@@ -454,9 +452,12 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
             m[i] = masked(nv);
             if (i >= 1 && i <= ROWS) v[i - 1] = nv;
         }
-#pragma unroll
-        for (int i = 0; i < ROWS + 2; ++i) n[i] = lane_next(m[i]);
+        // The x neighbours (lane_next) are kept only for the matrix-core
+        // operands; the vector action forms them where it uses them, so they
+        // hold no registers across steps.
         if constexpr (MFMA != 0) {
+#pragma unroll
+            for (int i = 0; i < ROWS + 2; ++i) n[i] = lane_next(m[i]);
 #pragma unroll
             for (int i = 0; i < ROWS + 2; ++i) plane_operands<MFMA>(m[i], n[i], q[i]);
         }
@@ -549,6 +550,12 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
             // terms with fused multiply-adds, so the bits do not depend on the
             // tile shape that computes an element.
             T dxl[ROWS + 2], dxh[ROWS + 2], zw0[ROWS + 2], zw1[ROWS + 2];
+            T nlo[ROWS + 2], nhi[ROWS + 2];
+#pragma unroll
+            for (int i = 0; i < ROWS + 2; ++i) {
+                nlo[i] = lane_next(mlo[i]);
+                nhi[i] = lane_next(mhi[i]);
+            }
 #pragma unroll
             for (int i = 0; i < ROWS + 2; ++i) {
                 dxl[i] = nlo[i] - mlo[i];
@@ -627,21 +634,11 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
     // ez = first_layer + k and pz = ez + 1; going down, ez = last_layer - k
     // and pz = ez.
     const int steps = last_layer - first_layer + 1;
-    {
-        Raw r0[ROWS + 2];
-        fetch_plane(DOWN ? last_layer + 1 : first_layer, r0);
-        if constexpr (DOWN)
-            make_plane(r0, mhi, nhi, vhi, qhi);
-        else
-            make_plane(r0, mlo, nlo, vlo, qlo);
-    }
-    for (int k = 0; k < steps; ++k) {
-        const int ez = DOWN ? last_layer - k : first_layer + k;
-        Raw raw[ROWS + 2];
-        T ca[ROWS + 1], cb[ROWS + 1], cc[ROWS + 1];
-        fetch_layer(ez, ca, cb, cc);
-        fetch_plane(DOWN ? ez : ez + 1, raw);
-        if constexpr (GROUPED) __builtin_amdgcn_sched_barrier(0);
+    // The arithmetic of step k: the plane it fetched becomes the new plane
+    // in registers, its layer's partials are formed, and the plane the layer
+    // completes is emitted.
+    auto step_body = [&](const int ez, const Raw (&raw)[ROWS + 2], const T (&ca)[ROWS + 1], const T (&cb)[ROWS + 1],
+                         const T (&cc)[ROWS + 1]) {
         T aL[ROWS], aR[ROWS], bL[ROWS], bR[ROWS];
         if constexpr (!DOWN) {
             make_plane(raw, mhi, nhi, vhi, qhi);
@@ -682,6 +679,23 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
                 heldR[i] = aR[i];
             }
         }
+    };
+    {
+        Raw r0[ROWS + 2];
+        fetch_plane(DOWN ? last_layer + 1 : first_layer, r0);
+        if constexpr (DOWN)
+            make_plane(r0, mhi, nhi, vhi, qhi);
+        else
+            make_plane(r0, mlo, nlo, vlo, qlo);
+    }
+    for (int k = 0; k < steps; ++k) {
+        const int ez = DOWN ? last_layer - k : first_layer + k;
+        Raw raw[ROWS + 2];
+        T ca[ROWS + 1], cb[ROWS + 1], cc[ROWS + 1];
+        fetch_layer(ez, ca, cb, cc);
+        fetch_plane(DOWN ? ez : ez + 1, raw);
+        if constexpr (GROUPED) __builtin_amdgcn_sched_barrier(0);
+        step_body(ez, raw, ca, cb, cc);
     }
     const T none[ROWS] = {};
     if constexpr (!DOWN) {
@@ -835,44 +849,48 @@ struct CellSums {
     static constexpr int kGroups = ROWS / kCellRows;  // 4-row groups per wave
     static constexpr int kEntries = kWavesPerGroup * kGroups;
     double t[NV][kGroups];  // per-lane sums of the current plane
-    double v[NV][kGroups];  // per-lane sums of the pair's planes folded so far
     int planes;             // planes of the current pair folded so far
 
     __device__ static double (&table())[kMaxTilePairs][NV][kEntries] {
         __shared__ double sums[kMaxTilePairs][NV][kEntries];
         return sums;
     }
+    // Per-lane sums of the pair's planes folded so far, kept in LDS (each
+    // lane its own words) so they hold no registers during the sweep.
+    __device__ static double (&pair())[NV][kGroups][kGroupThreads] {
+        __shared__ double held[NV][kGroups][kGroupThreads];
+        return held;
+    }
     __device__ void clear() {
 #pragma unroll
         for (int k = 0; k < NV; ++k)
 #pragma unroll
-            for (int g = 0; g < kGroups; ++g) {
-                t[k][g] = 0.0;
-                v[k][g] = 0.0;
-            }
+            for (int g = 0; g < kGroups; ++g) t[k][g] = 0.0;
         planes = 0;
     }
     // The plane's sums join the pair: the first plane's as they are, the
     // second's by one addition.
     __device__ void fold() {
+        const int lane = threadIdx.x;
 #pragma unroll
         for (int k = 0; k < NV; ++k)
 #pragma unroll
             for (int g = 0; g < kGroups; ++g) {
-                v[k][g] = planes == 0 ? t[k][g] : v[k][g] + t[k][g];
+                double& v = pair()[k][g][lane];
+                v = planes == 0 ? t[k][g] : v + t[k][g];
                 t[k][g] = 0.0;
             }
         ++planes;
     }
-    __device__ void store_groups(const int pair, const int groups) {
+    __device__ void store_groups(const int pair_index, const int groups) {
         const int wave = threadIdx.x / kWave;
 #pragma unroll
         for (int g = 0; g < kGroups; ++g) {
             if (g >= groups) break;
 #pragma unroll
             for (int k = 0; k < NV; ++k) {
-                const double s = wave_sum(v[k][g]);
-                if ((threadIdx.x & (kWave - 1)) == 0) table()[pair][k][wave * kGroups + g] = s;
+                const double s = wave_sum(pair()[k][g][threadIdx.x]);
+                if ((threadIdx.x & (kWave - 1)) == 0) table()[pair_index][k][wave * kGroups + g] = s;
             }
         }
         clear();
@@ -1426,11 +1444,16 @@ struct PassLoader {
     __device__ auto fetch(const long long pb, const unsigned off, const bool owned, const int row,
                           const int layer) const {
         const KNode<TI> s = ld_knode(sin + pb, off);
+        // One two-word load: the record (1/diag, robin), or on a plain plane
+        // (1/diag[n], 1/diag[n + 1]) from the 1/diag array, whose second word
+        // is not used (the array is followed by other operator arrays in its
+        // allocation, so the last node's load stays inside it).  Both byte
+        // offsets of kOutside are at least 2^30, past kRange.
         const bool plain_plane = mask_bit(plain, layer);
-        const TI dp = ldt(dinv + pb, plain_plane ? off : kOutside);
-        TI dr, rb;
-        ld_rec2(nrec + 2 * pb, plain_plane ? kOutside : off, dr, rb);
-        const TI d = plain_plane ? dp : dr;
+        TI d, rb;
+        ld_pair(plain_plane ? dinv + pb : nrec + 2 * pb,
+                plain_plane ? off * (unsigned)sizeof(TI) : rec_bytes<TI, 2>(off), d, rb);
+        rb = plain_plane ? TI(0) : rb;
         if constexpr (COARSE) {
             // The patch index does not depend on loaded data and is always in
             // range, so w is fetched with the node's other loads.
@@ -1532,8 +1555,8 @@ __device__ __forceinline__ void inner_step_slot(const SlotDev& sd, const SlotSta
     }
     if (y0 <= op.R) {
         strip_sweep<TI, ROWS, MFMA>(
-            tile.sz, (op.S + tile.zb) / tile.zb, InnerOf<TI>::coefs(op), op.S, op.R, op.C,
-                                    x0, y0, zlo, zhi, load, emit, [&](const int z) { acc.plane_done(z, zlo, op.S); });
+            tile.sz, (op.S + tile.zb) / tile.zb, InnerOf<TI>::coefs(op), op.S, op.R, op.C, x0, y0, zlo, zhi, load,
+            emit, [&](const int z) { acc.plane_done(z, zlo, op.S); });
     }
     acc.flush_hex(zlo, zhi, tile.sy * kWavesPerGroup * ROWS, tile.sx, op.R, op.C, sd.part, sd.part_stride);
 }
