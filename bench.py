@@ -401,7 +401,7 @@ def batch_leg(args, torch, *, count: int, warmup: int, steps: int) -> dict:
         f"{work / batched_s / 1e9:.0f} GB/s of pass bytes")
     singles = [_capi.Batch([o.device_operator]) for o in ops]
     for sb, b in zip(singles, rhss):
-        sb.set_shape(args.rows, 1024)
+        sb.set_shape(args.rows, args.target)
         sb.set(0, b, None, config)
         sb.run()  # warm-up: graph capture of every single-solve batch
     torch.cuda.synchronize()
@@ -530,7 +530,7 @@ def multirhs_leg(args, torch, *, warmup: int, steps: int, variants=("slots", "bl
 
     def timed_batch(group: int, engine: int):
         batch = _capi.Batch([operator.device_operator] * len(rhss))
-        batch.set_shape(args.rows if group == 0 else 4, 1024)
+        batch.set_shape(args.rows if group == 0 else 4, args.target)
         batch.set_block(group, engine)
         for i, b in enumerate(rhss):
             batch.set(i, b, None, config)
@@ -551,7 +551,7 @@ def multirhs_leg(args, torch, *, warmup: int, steps: int, variants=("slots", "bl
         log(f"64 rhs, {name}: {seconds * 1e3:.1f} ms, {sum(r['converged'] for r in results)} converged, "
             f"max inner {max(r['inner_iterations'] for r in results)}")
     single = _capi.Batch([operator.device_operator])
-    single.set_shape(args.rows, 1024)
+    single.set_shape(args.rows, args.target)
     single.set(0, rhss[0], None, config)
     single.run()
     t0 = time.perf_counter()
@@ -812,7 +812,7 @@ def serve_leg(args, torch, *, rates, duration: float, drain_limit: float) -> dic
     single_s = {}
     for b in pool:
         sb = _capi.Batch([b.operator])
-        sb.set_shape(args.rows, 1024)
+        sb.set_shape(args.rows, args.target)
         sb.set(0, b.rhs[0], None, config)
         sb.run()
         times = []
@@ -1101,7 +1101,7 @@ def main() -> int:
     parser.add_argument("--leg-duration", type=float, default=3.0,
                         help="seconds of arrivals of the served summary in the default line")
     parser.add_argument("--rows", type=int, default=4, help="rows per wave strip of the FP32 inner stencil")
-    parser.add_argument("--target", type=int, default=1024, help="stencil workgroups per launch before z blocks split")
+    parser.add_argument("--target", type=int, default=2048, help="stencil workgroups per launch before z blocks split")
     parser.add_argument("--workload", default="board512",
                         choices=["board512", "batch200", "multirhs64", "serve", "apply_engines", "coupling",
                                  "two_level", "pass"])

@@ -321,7 +321,7 @@ struct pcbhip_batch {
     int replay_steps = 32;   // steps per captured replay
     int compact_percent = 50;  // drop finished slots' tiles once this share of the tiled slots has ended
     int tiled_slots = 0;  // slots whose tiles are in the tables
-    long long target_groups = kTargetGroups;
+    long long target_groups = kPassTargetGroups;
     int* d_active = nullptr;
     int* h_active = nullptr;
     hipGraph_t graph = nullptr;
@@ -782,7 +782,7 @@ struct pcbhip_server {
     int outer_every = 8;
     int steps = 16;
     int max_slots = 0;
-    long long target_groups = kTargetGroups;
+    long long target_groups = kPassTargetGroups;
     hipStream_t stream = nullptr;       // the MPIR replays and table uploads
     hipStream_t copy_stream = nullptr;  // admissions and results
     std::vector<ServerSlot> slots;

@@ -31,8 +31,11 @@ constexpr int kGroupThreads = kWave * kWavesPerGroup;
 constexpr int kRowsOuter = 4;  // multiples of the 4-row reduction cell
 constexpr int kRowsInner64 = 4;
 // Stencil workgroups wanted per launch before the z sweep is split into
-// blocks (256 CUs, several waves each).
+// blocks (256 CUs, several waves each): the standalone apply's default, and
+// the fused pass's (three workgroups per CU fit at its 168 VGPRs, so about
+// three rounds of them).
 constexpr int kTargetGroups = 1024;
+constexpr int kPassTargetGroups = 2048;
 // Workgroups of the persistent outer-residual kernel.
 constexpr int kOuterGroups = 512;
 // Workgroups a server launches for the fused PCG pass over its tile table

@@ -407,18 +407,43 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
     const long long NR = R + 1;
 
     // Lane offsets of the strip's node rows inside a plane and of its element
-    // rows inside a layer, fixed for the whole sweep.
-    unsigned noff[ROWS + 2], eoff[ROWS + 1];
-#pragma unroll
-    for (int i = 0; i < ROWS + 2; ++i) {
+    // rows inside a layer.  Only the first row's offset is kept in a
+    // register (kOutside in a column outside the grid); row i adds i rows
+    // to it, and a row outside the grid (a wave-uniform test) is kOutside.
+    // Offsets of outside columns are then kOutside + i (C + 1), which every
+    // load treats as outside (at or past kOutside).
+    // A one-phase loader (the standalone apply) keeps every row's offset in
+    // registers instead, which its register budget allows.
+    constexpr bool kRebuildOffsets = HasFetch<std::decay_t<Load>>::value;
+    unsigned nbase = col_in ? (unsigned)((y0 - 1) * (int)NC + xc) : kOutside;
+    unsigned ebase = ecol_in ? (unsigned)((y0 - 1) * C + xc) : kOutside;
+    auto row_noff = [&](const int i) -> unsigned {
         const int y = y0 - 1 + i;
-        noff[i] = (col_in && y >= 0 && y <= R) ? (unsigned)(y * (int)NC + xc) : kOutside;
-    }
-#pragma unroll
-    for (int j = 0; j < ROWS + 1; ++j) {
+        return (y >= 0 && y <= R) ? nbase + (unsigned)(i * (int)NC) : kOutside;
+    };
+    auto row_eoff = [&](const int j) -> unsigned {
         const int ye = y0 - 1 + j;
-        eoff[j] = (ecol_in && ye >= 0 && ye < R) ? (unsigned)(ye * C + xc) : kOutside;
+        return (ye >= 0 && ye < R) ? ebase + (unsigned)(j * C) : kOutside;
+    };
+    [[maybe_unused]] unsigned noff_kept[ROWS + 2], eoff_kept[ROWS + 1];
+    if constexpr (!kRebuildOffsets) {
+#pragma unroll
+        for (int i = 0; i < ROWS + 2; ++i) noff_kept[i] = row_noff(i);
+#pragma unroll
+        for (int j = 0; j < ROWS + 1; ++j) eoff_kept[j] = row_eoff(j);
     }
+    auto noff = [&](const int i) -> unsigned {
+        if constexpr (kRebuildOffsets)
+            return row_noff(i);
+        else
+            return noff_kept[i];
+    };
+    auto eoff = [&](const int j) -> unsigned {
+        if constexpr (kRebuildOffsets)
+            return row_eoff(j);
+        else
+            return eoff_kept[j];
+    };
 
     // Raw loads of a node (what fetch returns) and the node value made from
     // them (NodeVal, InnerVal, InnerValZ).
@@ -441,7 +466,7 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
 #pragma unroll
         for (int i = 0; i < ROWS + 2; ++i) {
             const bool owned = out_plane && out_lane && i >= 1 && i <= ROWS && y0 - 1 + i <= R;
-            raw[i] = fetch_node(load, pb, noff[i], owned, i, z);
+            raw[i] = fetch_node(load, pb, noff(i), owned, i, z);
         }
     };
     auto make_plane = [&](const Raw (&raw)[ROWS + 2], T (&m)[ROWS + 2], T (&n)[ROWS + 2], V (&v)[ROWS],
@@ -474,7 +499,7 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
                 const T L = La[i] + Lb[i];
                 const T Rt = Ra[i] + Rb[i];
                 const T total = L + lane_prev(Rt);
-                if (out_lane && y0 + i <= R) emit(pb, noff[i + 1], i, total, v[i], parity);
+                if (out_lane && y0 + i <= R) emit(pb, noff(i + 1), i, total, v[i], parity);
             }
         };
         if (z & 1)
@@ -494,10 +519,10 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
             const T ua = uv[0], ub = uv[1], uc = uv[2];
 #pragma unroll
             for (int j = 0; j < ROWS + 1; ++j) {
-                const unsigned off = uniform ? kOutside : eoff[j];
+                const unsigned off = uniform ? kOutside : eoff(j);
                 T la, lb, lc;
                 ld_rec3(rl, off, la, lb, lc);
-                const bool in = eoff[j] != kOutside;
+                const bool in = eoff(j) < kOutside;
                 ca[j] = uniform ? (in ? ua : T(0)) : la;
                 cb[j] = uniform ? (in ? ub : T(0)) : lb;
                 cc[j] = uniform ? (in ? uc : T(0)) : lc;
@@ -507,7 +532,7 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
             // with it maps them to the first element and leaves kOutside.
             const unsigned keep = uniform ? kOutside : ~0u;
 #pragma unroll
-            for (int j = 0; j < ROWS + 1; ++j) ld_rec3(rl, eoff[j] & keep, ca[j], cb[j], cc[j]);
+            for (int j = 0; j < ROWS + 1; ++j) ld_rec3(rl, eoff(j) & keep, ca[j], cb[j], cc[j]);
         }
     };
 
@@ -696,6 +721,11 @@ __device__ __forceinline__ void strip_apply(const CoefSource<T>& cs, const int S
         fetch_plane(DOWN ? ez : ez + 1, raw);
         if constexpr (GROUPED) __builtin_amdgcn_sched_barrier(0);
         step_body(ez, raw, ca, cb, cc);
+        // Two-phase loaders (the MPIR kernels): the row offsets are formed
+        // again in each step from the two bases rather than held in
+        // registers across the sweep, which keeps the fused FP32 pass at
+        // three waves per SIMD.
+        if constexpr (kRebuildOffsets) asm volatile("" : "+v"(nbase), "+v"(ebase));
     }
     const T none[ROWS] = {};
     if constexpr (!DOWN) {
@@ -1470,7 +1500,7 @@ struct PassLoader {
                 const int y = rem / (C + 1), x = rem - y * (C + 1);
                 patch = (l * crows + y / cblock) * ccols + x / cblock;
             }
-            return PassRawZ<TI>{s, d, rb, cw[patch], off != kOutside};
+            return PassRawZ<TI>{s, d, rb, cw[patch], off < kOutside};
         } else {
             return PassRaw<TI>{s, d, rb};
         }

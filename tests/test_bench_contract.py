@@ -45,7 +45,7 @@ def test_more_than_one_gpu_is_refused():
 
 def test_profile_summary_has_traffic_for_the_default_pcg_pass():
     bench = _bench()
-    traffic = bench.profiled_pass_traffic((16, 512, 512), 4, 1024)
+    traffic = bench.profiled_pass_traffic((16, 512, 512), 4, 2048)
     assert isinstance(traffic, int) and 100e6 < traffic < 400e6, traffic
     assert bench.profiled_pass_traffic((16, 512, 512), 8, 1024) is None
 
