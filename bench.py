@@ -1118,7 +1118,7 @@ def main() -> int:
     parser.add_argument("--drain-limit", type=float, default=60.0)
     parser.add_argument("--pool-thermal", type=int, default=16)
     parser.add_argument("--pool-dc", type=int, default=8)
-    parser.add_argument("--max-slots", type=int, default=1024)
+    parser.add_argument("--max-slots", type=int, default=512)
     parser.add_argument("--replay-steps", type=int, default=16)
     parser.add_argument("--max-running-nodes", type=int, default=None)
     parser.add_argument("--dc-size", type=int, default=256)
