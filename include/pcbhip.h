@@ -310,6 +310,20 @@ int pcbhip_vec_copy(int device, const float *src, float *dst, int64_t n);
 int pcbhip_vec_dot(int device, const float *x, const float *y, int64_t n, double *out);
 int pcbhip_vec_axpy(int device, float alpha, const float *x, const float *y, float *out, int64_t n);
 int pcbhip_vec_divide(int device, const float *num, const float *den, float *out, int64_t n);
+/* Gram-Schmidt products of the restarted GMRES inner solve on basis rows
+ * held as device vectors (replaces the device basis products of
+ * src/electrical/matrix_free_mpir_fem/solver.py:400-577, which project a
+ * candidate on the stored basis with GEMV, and the dot / axpy loop of the
+ * host GMRES, solver.py:235-397).  count is 1..64, every row has n floats,
+ * and each call returns when done:
+ *   project  out[k] = rows[k] . w for k < count (host array of count
+ *            doubles), each in FP64 with the bits pcbhip_vec_dot gives;
+ *   combine  out = y + sum_k coef[k] rows[k] (coef a host array of count
+ *            finite doubles; y NULL: zero), accumulated in FP64 in row order
+ *            and rounded once to FP32.  out may be y or one of the rows. */
+int pcbhip_vec_project(int device, const float *const *rows, int count, const float *w, int64_t n, double *out);
+int pcbhip_vec_combine(int device, const float *const *rows, const double *coef, int count, const float *y,
+                       float *out, int64_t n);
 /* z = M^-1 r in FP32 on device vectors of pcbhip_operator_size() floats,
  * returning when done (replaces MatrixFreeThermalOperator.precondition_low on
  * runtime arrays, operator.py:366-371). */

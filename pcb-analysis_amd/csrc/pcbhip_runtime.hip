@@ -1195,6 +1195,9 @@ struct VectorRuntime {
     std::mutex lock;
     hipStream_t stream = nullptr;
     double* scratch = nullptr;
+    // The basis projections' partial sums and results (kBasisRowsMax rows),
+    // allocated at the first projection.
+    double* basis_scratch = nullptr;
 };
 
 VectorRuntime& vector_runtime(int device) {
@@ -2149,6 +2152,45 @@ int pcbhip_vec_divide(int device, const float* num, const float* den, float* out
         if (n == 0) return;
         on_vector_stream(device, "divide", [&](VectorRuntime& vr) {
             check(launch::divide(num, den, out, n, vr.stream), "divide launch");
+        });
+    });
+}
+
+int pcbhip_vec_project(int device, const float* const* rows, int count, const float* w, int64_t n, double* out) {
+    return guarded([&] {
+        if (!rows || !out) fail(PCBHIP_INVALID_INPUT, "null argument");
+        if (count < 1 || count > kBasisRowsMax)
+            fail(PCBHIP_INVALID_INPUT, "a projection takes 1.." + std::to_string(kBasisRowsMax) + " rows");
+        for (int k = 0; k < count; ++k) check_vector_args(device, rows[k], w, n);
+        if (n == 0) {
+            for (int k = 0; k < count; ++k) out[k] = 0.0;
+            return;
+        }
+        on_vector_stream(device, "project", [&](VectorRuntime& vr) {
+            if (!vr.basis_scratch)
+                vr.basis_scratch = static_cast<double*>(
+                    device_alloc(sizeof(double) * (size_t)kBasisRowsMax * (size_t)(kDotGroupsMax + 1)));
+            double* results = vr.basis_scratch + (size_t)kBasisRowsMax * kDotGroupsMax;
+            check(launch::project(rows, count, w, n, vr.basis_scratch, results, vr.stream), "project launch");
+            check(hipMemcpyAsync(out, results, sizeof(double) * (size_t)count, hipMemcpyDeviceToHost, vr.stream),
+                  "project read");
+        });
+    });
+}
+
+int pcbhip_vec_combine(int device, const float* const* rows, const double* coef, int count, const float* y,
+                       float* out, int64_t n) {
+    return guarded([&] {
+        if (!rows || !coef) fail(PCBHIP_INVALID_INPUT, "null argument");
+        if (count < 1 || count > kBasisRowsMax)
+            fail(PCBHIP_INVALID_INPUT, "a combination takes 1.." + std::to_string(kBasisRowsMax) + " rows");
+        for (int k = 0; k < count; ++k) {
+            check_vector_args(device, rows[k], out, n);
+            if (!std::isfinite(coef[k])) fail(PCBHIP_INVALID_INPUT, "combination coefficients must be finite");
+        }
+        if (n == 0) return;
+        on_vector_stream(device, "combine", [&](VectorRuntime& vr) {
+            check(launch::combine(rows, coef, count, y, out, n, vr.stream), "combine launch");
         });
     });
 }

@@ -220,6 +220,9 @@ constexpr int kStreamsMax = 16;
 // Workgroups of the first stage of the vector runtime's dot product.
 constexpr int kDotGroupsMax = 1024;
 
+// Basis rows one Gram-Schmidt projection or combination takes at once.
+constexpr int kBasisRowsMax = 64;
+
 // Slots that share one operator, applied together by the block apply
 // (at most kGroupMax right-hand sides per group).
 constexpr int kGroupMax = 64;
@@ -258,6 +261,14 @@ int dot_groups(long long n);
 hipError_t dot(const float* x, const float* y, long long n, double* partial, double* out, hipStream_t stream);
 hipError_t axpy(float alpha, const float* x, const float* y, float* out, long long n, hipStream_t stream);
 hipError_t divide(const float* num, const float* den, float* out, long long n, hipStream_t stream);
+// GMRES basis products on 1..kBasisRowsMax rows of n floats: out[k] = rows[k]
+// . w in FP64, each with the dot's bits (``partial`` holds count *
+// kDotGroupsMax doubles, ``out`` count); out = y + sum_k coef[k] rows[k] in
+// FP64, rounded once (y may be null, out may alias y or a row).
+hipError_t project(const float* const* rows, int count, const float* w, long long n, double* partial, double* out,
+                   hipStream_t stream);
+hipError_t combine(const float* const* rows, const double* coef, int count, const float* y, float* out, long long n,
+                   hipStream_t stream);
 // One outer step (FP64 residual and decision) for the slots in their outer
 // phase, and one fused Jacobi-PCG pass for the slots in their inner phase.
 // ``counts`` (device) holds the live lengths of the outer, inner and block

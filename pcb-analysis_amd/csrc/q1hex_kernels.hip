@@ -1259,6 +1259,61 @@ __global__ __launch_bounds__(kVecThreads) void k_divide(const float* __restrict_
     if (i < n) out[i] = num[i] / den[i];
 }
 
+// Gram-Schmidt products of a restarted GMRES basis: ``count`` stored rows
+// of n floats against one vector.  The rows travel as kernel arguments, so a
+// launch needs no device table.
+struct BasisRows {
+    const float* row[kBasisRowsMax];
+};
+struct BasisCoefs {
+    double c[kBasisRowsMax];
+};
+
+// partial[k * groups + g] = the FP64 sum of rows[k][i] * w[i] over workgroup
+// g's range, in exactly the order of k_dot_partial: one row's projection has
+// the bits of the dot product of the same two vectors.  The workgroup's
+// range of w is read once from HBM and again from cache for every further
+// row.
+__global__ __launch_bounds__(kVecThreads) void k_project_partial(const BasisRows rows, const int count,
+                                                                 const float* __restrict__ w, const long long n,
+                                                                 const long long per, double* __restrict__ partial) {
+    __shared__ double lds[kVecThreads];
+    const long long lo = (long long)blockIdx.x * per;
+    const long long hi = lo + per < n ? lo + per : n;
+    for (int k = 0; k < count; ++k) {
+        const float* __restrict__ v = rows.row[k];
+        double acc = 0.0;
+        for (long long i = lo + threadIdx.x; i < hi; i += kVecThreads) acc += (double)v[i] * (double)w[i];
+        const double total = block_tree_sum(acc, lds);
+        if (threadIdx.x == 0) partial[(long long)k * gridDim.x + blockIdx.x] = total;
+        __syncthreads();  // every thread has read lds[0] before the next row writes it
+    }
+}
+
+// One workgroup per row: out[k] sums that row's partials as k_dot_final does.
+__global__ __launch_bounds__(kVecThreads) void k_project_final(const double* __restrict__ partial, const int groups,
+                                                               double* __restrict__ out) {
+    __shared__ double lds[kVecThreads];
+    const double* __restrict__ mine = partial + (long long)blockIdx.x * groups;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < groups; i += kVecThreads) acc += mine[i];
+    const double total = block_tree_sum(acc, lds);
+    if (threadIdx.x == 0) out[blockIdx.x] = total;
+}
+
+// out[i] = y[i] + sum_k coef[k] rows[k][i], accumulated in FP64 in row order
+// and rounded once to FP32 (y null: zero).  Each element's inputs are read
+// before it is written, so out may be y or one of the rows.
+__global__ __launch_bounds__(kVecThreads) void k_combine(const BasisRows rows, const BasisCoefs coef, const int count,
+                                                         const float* y, float* out, const long long n) {
+    const long long i = (long long)blockIdx.x * kVecThreads + threadIdx.x;
+    if (i < n) {
+        double acc = y ? (double)y[i] : 0.0;
+        for (int k = 0; k < count; ++k) acc += coef.c[k] * (double)rows.row[k][i];
+        out[i] = (float)acc;
+    }
+}
+
 // ---------------------------------------------------------- MPIR kernels
 
 template <typename TI>
@@ -2096,6 +2151,33 @@ hipError_t divide(const float* num, const float* den, float* out, long long n, h
     if (n <= 0) return hipSuccess;
     const long long blocks = (n + kVecThreads - 1) / kVecThreads;
     hipLaunchKernelGGL(k_divide, dim3((unsigned)blocks), dim3(kVecThreads), 0, stream, num, den, out, n);
+    return hipGetLastError();
+}
+
+hipError_t project(const float* const* rows, int count, const float* w, long long n, double* partial, double* out,
+                   hipStream_t stream) {
+    if (count < 1 || count > kBasisRowsMax || n <= 0) return hipErrorInvalidValue;
+    BasisRows set{};
+    for (int k = 0; k < count; ++k) set.row[k] = rows[k];
+    const int groups = dot_groups(n);
+    const long long per = (n + groups - 1) / groups;
+    hipLaunchKernelGGL(k_project_partial, dim3(groups), dim3(kVecThreads), 0, stream, set, count, w, n, per, partial);
+    hipLaunchKernelGGL(k_project_final, dim3(count), dim3(kVecThreads), 0, stream, partial, groups, out);
+    return hipGetLastError();
+}
+
+hipError_t combine(const float* const* rows, const double* coef, int count, const float* y, float* out, long long n,
+                   hipStream_t stream) {
+    if (count < 1 || count > kBasisRowsMax) return hipErrorInvalidValue;
+    if (n <= 0) return hipSuccess;
+    BasisRows set{};
+    BasisCoefs c{};
+    for (int k = 0; k < count; ++k) {
+        set.row[k] = rows[k];
+        c.c[k] = coef[k];
+    }
+    const long long blocks = (n + kVecThreads - 1) / kVecThreads;
+    hipLaunchKernelGGL(k_combine, dim3((unsigned)blocks), dim3(kVecThreads), 0, stream, set, c, count, y, out, n);
     return hipGetLastError();
 }
 

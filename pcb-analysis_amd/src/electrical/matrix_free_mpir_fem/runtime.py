@@ -8,8 +8,10 @@ device.  Here the arrays are ``pcbhip._capi.DeviceVector`` (FP32 on one HIP
 device) and every operation is a kernel of ``libpcbhip.so`` that returns when
 it is done, so a caller that drives a PCG loop from the host -- the reference's
 ``_pcg_low`` shape, with an operator's ``apply_low`` / ``precondition_low`` on
-the same vectors -- runs it on the device.  The product's own solves do not
-go through here: they run the whole iteration in fused device kernels.
+the same vectors -- runs it on the device.  ``project`` and ``combine`` add
+the basis products of GMRES's Gram-Schmidt passes (the GEMV calls of the
+reference's device GMRES, solver.py:400-577).  The product's own solves do
+not go through here: they run the whole iteration in fused device kernels.
 
 Differences from the reference's device runtime, both deliberate: ``dot`` and
 ``norm`` accumulate in FP64 in an order fixed by the length (the same bits
@@ -114,6 +116,58 @@ class HipFloat32Runtime:
         n = self._same_size(a, b)
         out = self._capi.DeviceVector(self.device_id, n)
         self._capi.check(self._lib.pcbhip_vec_divide(self.device_id, a.ptr, b.ptr, out.ptr, n))
+        return out
+
+    # -- Gram-Schmidt products of a GMRES basis ------------------------------------
+
+    # Rows one library call takes; longer bases are taken in chunks.
+    basis_chunk = 64
+
+    def project(self, rows, vector) -> np.ndarray:
+        """``[row . vector for row in rows]`` as FP64 values, each with the
+        bits :meth:`dot` gives for the same pair, in one pass over
+        ``vector`` per chunk of rows."""
+
+        w = self._device(vector)
+        vectors = [self._device(row) for row in rows]
+        for row in vectors:
+            self._same_size(row, w)
+        out = np.zeros(len(vectors), dtype=np.float64)
+        for lo in range(0, len(vectors), self.basis_chunk):
+            chunk = vectors[lo : lo + self.basis_chunk]
+            table = (ctypes.c_void_p * len(chunk))(*[row.ptr for row in chunk])
+            part = np.zeros(len(chunk), dtype=np.float64)
+            self._capi.check(self._lib.pcbhip_vec_project(
+                self.device_id, table, len(chunk), w.ptr, w.size,
+                part.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+            out[lo : lo + len(chunk)] = part
+        return out
+
+    def combine(self, coefficients, rows, base=None):
+        """A new vector ``base + sum_k coefficients[k] rows[k]`` (``base``
+        None: zero), accumulated in FP64 per chunk of rows and rounded to
+        FP32 once per chunk."""
+
+        coefficients = np.ascontiguousarray(coefficients, dtype=np.float64).reshape(-1)
+        vectors = [self._device(row) for row in rows]
+        if coefficients.size != len(vectors) or not vectors:
+            raise ValueError("combine needs one coefficient per row and at least one row")
+        n = vectors[0].size
+        for row in vectors:
+            if row.size != n:
+                raise ValueError(f"vectors of sizes {row.size} and {n}")
+        out = self._capi.DeviceVector(self.device_id, n)
+        previous = None if base is None else self._device(base)
+        if previous is not None:
+            self._same_size(previous, out)
+        for lo in range(0, len(vectors), self.basis_chunk):
+            chunk = vectors[lo : lo + self.basis_chunk]
+            table = (ctypes.c_void_p * len(chunk))(*[row.ptr for row in chunk])
+            coef = np.ascontiguousarray(coefficients[lo : lo + len(chunk)])
+            self._capi.check(self._lib.pcbhip_vec_combine(
+                self.device_id, table, coef.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(chunk),
+                None if previous is None else previous.ptr, out.ptr, n))
+            previous = out
         return out
 
     def synchronize(self) -> None:

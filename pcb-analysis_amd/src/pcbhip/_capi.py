@@ -84,6 +84,8 @@ EXPORTS = (
     "pcbhip_vec_dot",
     "pcbhip_vec_axpy",
     "pcbhip_vec_divide",
+    "pcbhip_vec_project",
+    "pcbhip_vec_combine",
     "pcbhip_operator_precondition_f32_dev",
     "pcbhip_operator_synchronize",
 )
@@ -197,6 +199,10 @@ def load() -> ctypes.CDLL:
     lib.pcbhip_vec_dot.argtypes = [_int, _vp, _vp, _i64, ctypes.POINTER(ctypes.c_double)]
     lib.pcbhip_vec_axpy.argtypes = [_int, ctypes.c_float, _vp, _vp, _vp, _i64]
     lib.pcbhip_vec_divide.argtypes = [_int, _vp, _vp, _vp, _i64]
+    _pp = ctypes.POINTER(_vp)
+    _dp = ctypes.POINTER(ctypes.c_double)
+    lib.pcbhip_vec_project.argtypes = [_int, _pp, _int, _vp, _i64, _dp]
+    lib.pcbhip_vec_combine.argtypes = [_int, _pp, _dp, _int, _vp, _vp, _i64]
     lib.pcbhip_operator_precondition_f32_dev.argtypes = [_vp, _vp, _vp]
     lib.pcbhip_operator_synchronize.argtypes = [_vp]
     _lib = lib

@@ -85,7 +85,7 @@ def test_board_generators_have_the_stated_shapes_and_power():
     assert all(4 <= s <= 16 and 32 <= r <= 384 and 32 <= c <= 384 for s, r, c in shapes)
 
 
-def test_mpir_dispatch_refuses_complex_gmres_systems_and_validates_inputs():
+def test_mpir_dispatch_refuses_complex_and_host_systems_and_validates_inputs():
     from electrical.matrix_free_mpir_fem import MPIRConfig, solve_mpir
 
     class ComplexSystem:
@@ -98,7 +98,7 @@ def test_mpir_dispatch_refuses_complex_gmres_systems_and_validates_inputs():
         inner_solver = "pcg"
         high_dtype = np.float64
 
-    with pytest.raises(NotImplementedError, match="GMRES"):
+    with pytest.raises(NotImplementedError, match="complex"):
         solve_mpir(ComplexSystem(), np.ones(4), config=MPIRConfig())
     with pytest.raises(TypeError, match="native_solve_mpir"):
         solve_mpir(NoDevice(), np.ones(4))
