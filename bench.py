@@ -678,7 +678,7 @@ def run_apply_engines(args, torch) -> dict:
     dev.set_engine(0)
     best = engines["0"]
     return {
-        "metric": "Q1 apply GDoF/s (FP32, 512x512x16 board) per element action",
+        "metric": f"Q1 apply GDoF/s (FP32, {c}x{r}x{s} board) per element action",
         "value": best["gdof_s"],
         "unit": "GDoF/s",
         "n_gpus": 1,
