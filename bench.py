@@ -69,9 +69,9 @@ def copy_bandwidth(torch, stream, nbytes: int = 1 << 30) -> float:
 def stream_ceilings(torch, stream, n: int, shapes=((1, 1), (3, 1), (6, 2), (6, 4), (9, 4), (13, 4))) -> list:
     """GB/s of ``reads`` + ``writes`` arrays of ``n`` floats streamed at once
     (``pcbhip_streams_dev``), HIP events on ``stream``: the ceiling of a
-    kernel that moves that many streams.  The fused PCG pass reads six node
-    vectors and three element-coefficient arrays and writes four node
-    vectors: the (9, 4) row."""
+    kernel that moves that many streams.  The fused PCG pass reads three
+    record arrays (state, node operator, element coefficients) and writes
+    one: the (3, 1) row."""
 
     import ctypes
 
@@ -94,13 +94,14 @@ def stream_ceilings(torch, stream, n: int, shapes=((1, 1), (3, 1), (6, 2), (6, 4
     return rows
 
 
-def profiled_traffic(element_shape, rows: int, zb: int):
+def profiled_traffic(element_shape, rows: int, zb: int, board: str = "standard"):
     """HBM bytes per launch of the FP32 apply at this tile shape, from the
-    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or
-    None when that shape was not profiled."""
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of that board (standard
+    or general; copy-calibrated), or None when that shape was not profiled."""
 
     s, r, c = element_shape
-    path = ROOT / "profiles" / f"q1hex_apply_f32_board_{c}x{r}x{s}.md"
+    stem = "board" if board == "standard" else board
+    path = ROOT / "profiles" / f"q1hex_apply_f32_{stem}_{c}x{r}x{s}.md"
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -175,7 +176,7 @@ def _hbm_rate(traffic, seconds: float):
     return None if traffic is None else traffic / seconds / 1e9
 
 
-def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
+def apply_roofline(operator, torch, stream, launches: int = 100, board: str = "standard") -> dict:
     """Tile-shape sweep of the FP32 Q1 apply (the autotuner's candidates),
     timed with HIP events on the stream it runs on; the fastest shape is kept."""
 
@@ -196,7 +197,7 @@ def apply_roofline(operator, torch, stream, launches: int = 100) -> dict:
         log(f"apply rows={rows} zb={zb}: {per * 1e6:.1f} us")
     per, rows, zb = min(table)
     dev.set_shape(rows, zb)
-    traffic = profiled_traffic(operator.mesh.element_grid_shape, rows, zb)
+    traffic = profiled_traffic(operator.mesh.element_grid_shape, rows, zb, board)
     algorithmic = 13 * n + 12 * e  # x, y, robin (4 B/node each), free mask (1 B/node), coefficients (12 B/element)
     achieved = algorithmic / per / 1e9
     needed = algorithmic - skipped_bytes(operator)
@@ -631,14 +632,15 @@ def run_multirhs64(args, torch) -> dict:
 
 
 def run_apply_engines(args, torch) -> dict:
-    """The standalone FP32 Q1 apply of the 512 x 512 x 16 board with each
-    element action: FP32 vector FMAs, and the matrix cores with a two- or
-    three-term bf16 split (best tile shape of each, HIP-event timing)."""
+    """The standalone FP32 Q1 apply of the benchmark board (``--board
+    general``: the board with no stencil shortcuts) with each element action:
+    FP32 vector FMAs, and the matrix cores with a two- or three-term bf16
+    split (best tile shape of each, HIP-event timing)."""
 
-    from pcbhip import autotune, workloads
+    from pcbhip import autotune
     from thermal.matrix_free_mpir_fem import MatrixFreeThermalOperator
 
-    problem = workloads.board_512x512x16(size=args.size, slabs=args.slabs)
+    problem = _board(args)
     operator = MatrixFreeThermalOperator(problem, preconditioner="jacobi")
     dev = operator.device_operator
     n = operator.size
@@ -677,8 +679,12 @@ def run_apply_engines(args, torch) -> dict:
         log(f"engine {engine}: {entry}")
     dev.set_engine(0)
     best = engines["0"]
+    # The box's copy moving the apply's algorithmic bytes (also the byte
+    # counters' calibration kernel in a profiled run).
+    copy = copy_bandwidth(torch, stream, (13 * n + 12 * e) // 2 // 16 * 16)
+    board = "general board" if args.board == "general" else "board"
     return {
-        "metric": f"Q1 apply GDoF/s (FP32, {c}x{r}x{s} board) per element action",
+        "metric": f"Q1 apply GDoF/s (FP32, {c}x{r}x{s} {board}) per element action",
         "value": best["gdof_s"],
         "unit": "GDoF/s",
         "n_gpus": 1,
@@ -690,8 +696,8 @@ def run_apply_engines(args, torch) -> dict:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded board generator, seed 2), x ~ N(0, 1)",
-        "config": {"workload": f"board {c}x{r}x{s} elements ({n:,} nodes)", "engines": engines,
-                   "parallelism": "single GPU"},
+        "config": {"workload": f"{board} {c}x{r}x{s} elements ({n:,} nodes)", "engines": engines,
+                   "copy_gbs_at_apply_bytes": copy, "parallelism": "single GPU"},
     }
 
 
@@ -743,7 +749,7 @@ def run_pass(args, torch) -> dict:
             sweep.append(dict(rows=rows, target=target, us=us, algorithmic_gbs=pass_bytes / us / 1e3))
             log(f"pass rows={rows} target={target}: {us:.2f} us, {pass_bytes / us / 1e3:.0f} GB/s")
     stream = torch.cuda.Stream()
-    apply = apply_roofline(operator, torch, stream)
+    apply = apply_roofline(operator, torch, stream, board=args.board)
     copy = copy_bandwidth(torch, stream, pass_bytes // 2 // 16 * 16)
     ceilings = stream_ceilings(torch, stream, n)
     best = min(sweep, key=lambda row: row["us"])
@@ -1189,7 +1195,7 @@ def run_board(args, torch) -> dict:
     fused = pass_roofline(operator, batch, args.rows, args.target, board=args.board)
     log(f"pcg pass: {fused['per_launch_us']:.1f} us, {fused['roofline']['achieved']:.0f} GB/s")
     stream = torch.cuda.Stream()
-    roof = apply_roofline(operator, torch, stream)
+    roof = apply_roofline(operator, torch, stream, board=args.board)
     log(f"apply: {roof['per_launch_us']:.1f} us, {roof['roofline']['achieved']:.0f} GB/s")
     copy_gbs = copy_bandwidth(torch, stream)
     log(f"copy: {copy_gbs:.0f} GB/s")
