@@ -104,3 +104,27 @@ def test_host_driven_refinement_matches_the_fused_device_solve(inner, preconditi
     assert np.abs(result.solution - fused.solution).max() <= 1e-6 * scale
     assert result.low_runtime == operator.runtime.name
     assert result.inner_iterations == sum(step.inner_iterations for step in result.history)
+
+
+def test_gmres_refinement_of_a_dc_bar_gives_the_closed_form_resistance():
+    """The hexahedral DC operator refined through host-driven GMRES: a
+    uniform copper bar one element across, driven by +-1 A end faces,
+    drops exactly I L / (sigma w t) (the reference's bar check,
+    tests/test_matrix_free_mpir_fem.py:84-101)."""
+
+    from electrical.matrix_free_mpir_fem.hex_dc import CurrentTerminal, MatrixFreeHexDCOperator
+    from thermal.matrix_free_mpir_fem import LayeredThermalMesh
+
+    print("host-driven GMRES dc bar", flush=True)
+    sigma, cells = 5.8e7, 16
+    mesh = LayeredThermalMesh((35e-6,), 1.0e-3, 1.0e-3, sigma, element_shape=(1, cells))
+    face = lambda x: tuple((z, y, x) for z in range(mesh.node_shape[0]) for y in range(mesh.node_shape[1]))
+    operator = MatrixFreeHexDCOperator(mesh, reference_node=(0, 0, cells // 2))
+    rhs = operator.build_rhs((CurrentTerminal(face(0), 1.0), CurrentTerminal(face(cells), -1.0)))
+    result = solve_mpir(HostDrivenSystem(operator, "gmres"), rhs,
+                        config=MPIRConfig(max_inner_iterations=2000, gmres_restart=8))
+    assert result.converged and result.relative_residual <= 1e-10
+    v = result.solution.reshape(mesh.node_shape)
+    resistance = cells * 1.0e-3 / (sigma * 1.0e-3 * 35e-6)
+    np.testing.assert_allclose(v[..., 0] - v[..., cells], resistance, rtol=1e-9)
+    np.testing.assert_allclose(v[..., cells // 2], 0.0, atol=1e-9 * resistance)
