@@ -45,3 +45,21 @@ def test_errors_map_to_python_exceptions():
     assert code == _capi.INVALID_INPUT
     with pytest.raises(ValueError, match="null argument"):
         _capi.check(code)
+
+
+def test_basis_products_check_their_arguments_before_any_device_work():
+    import ctypes
+
+    lib = _capi.load()
+    out = (ctypes.c_double * 1)()
+    coef = (ctypes.c_double * 1)(1.0)
+    rows = (ctypes.c_void_p * 1)(None)
+    assert lib.pcbhip_vec_project(0, None, 1, None, 4, out) == _capi.INVALID_INPUT
+    assert lib.pcbhip_vec_project(0, rows, 0, None, 4, out) == _capi.INVALID_INPUT
+    assert lib.pcbhip_vec_project(0, rows, 65, None, 4, out) == _capi.INVALID_INPUT
+    assert lib.pcbhip_vec_combine(0, rows, coef, 65, None, None, 4) == _capi.INVALID_INPUT
+    assert lib.pcbhip_vec_combine(0, rows, None, 1, None, None, 4) == _capi.INVALID_INPUT
+    # A null row of a non-empty vector is refused.
+    assert lib.pcbhip_vec_combine(0, rows, coef, 1, None, None, 4) == _capi.INVALID_INPUT
+    with pytest.raises(ValueError, match="1..64 rows"):
+        _capi.check(lib.pcbhip_vec_project(0, rows, 65, None, 4, out))
