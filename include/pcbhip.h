@@ -182,8 +182,8 @@ int pcbhip_batch_set(pcbhip_batch *batch, int slot, const double *rhs, const dou
  * reference's CPU team splits node lines statically, hex_q1.cpp:195-205). */
 int pcbhip_batch_set_shape(pcbhip_batch *batch, int rows_inner, int target_groups);
 /* Schedule of pcbhip_batch_run: MPIR steps per captured replay (1..1024,
- * default 32; the host checks for finished slots between replays), inner
- * iterations between outer steps (default 8) and the share of the tiled
+ * default 64; the host checks for finished slots between replays), inner
+ * iterations between outer steps (default 32) and the share of the tiled
  * slots, in percent, that must have ended before their tiles leave the
  * launches (default 50).  Results do not depend on it.  No reference
  * counterpart. */
