@@ -382,6 +382,7 @@ def batch_leg(args, torch, *, count: int, warmup: int, steps: int) -> dict:
     nodes = sum(o.size for o in ops)
     config = MPIRConfig(relative_tolerance=1e-10, max_outer_iterations=60, max_inner_iterations=100000).to_c()
     batch = _capi.Batch([o.device_operator for o in ops])
+    batch.set_schedule(compact_percent=args.compact_percent)
     slots = [(b, None, config) for b in rhss]
     for i, b in enumerate(rhss):
         batch.set(i, b, None, config)
@@ -1131,6 +1132,9 @@ def main() -> int:
     parser.add_argument("--block", type=int, default=1, help="right-hand sides per block-apply group")
     parser.add_argument("--engine", type=int, default=2, choices=[2, 3], help="bf16 terms of the matrix-core action")
     parser.add_argument("--count", type=int, default=200, help="boards in the batch200 workload")
+    parser.add_argument("--compact-percent", type=int, default=50,
+                        help="batch200: share of finished boards (percent) after which their tiles leave the "
+                             "launches; 100 keeps every tile table for the whole run")
     args = parser.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only"}))
