@@ -1132,7 +1132,7 @@ def main() -> int:
     parser.add_argument("--block", type=int, default=1, help="right-hand sides per block-apply group")
     parser.add_argument("--engine", type=int, default=2, choices=[2, 3], help="bf16 terms of the matrix-core action")
     parser.add_argument("--count", type=int, default=200, help="boards in the batch200 workload")
-    parser.add_argument("--compact-percent", type=int, default=50,
+    parser.add_argument("--compact-percent", type=int, default=25,
                         help="batch200: share of finished boards (percent) after which their tiles leave the "
                              "launches; 100 keeps every tile table for the whole run")
     args = parser.parse_args()

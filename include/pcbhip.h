@@ -185,7 +185,7 @@ int pcbhip_batch_set_shape(pcbhip_batch *batch, int rows_inner, int target_group
  * default 64; the host checks for finished slots between replays), inner
  * iterations between outer steps (default 32) and the share of the tiled
  * slots, in percent, that must have ended before their tiles leave the
- * launches (default 50).  Results do not depend on it.  No reference
+ * launches (default 25).  Results do not depend on it.  No reference
  * counterpart. */
 int pcbhip_batch_set_schedule(pcbhip_batch *batch, int replay_steps, int outer_every, int compact_percent);
 /* Tile shape of the standalone applies: rows per wave strip (FP32: 4 or 8,

@@ -323,7 +323,7 @@ struct pcbhip_batch {
     // for one 4.5 M-node solve, the 200-board batch unchanged.
     int outer_every = 32;
     int replay_steps = 64;   // steps per captured replay
-    int compact_percent = 50;  // drop finished slots' tiles once this share of the tiled slots has ended
+    int compact_percent = 25;  // drop finished slots' tiles once this share of the tiled slots has ended
     int tiled_slots = 0;  // slots whose tiles are in the tables
     long long target_groups = kPassTargetGroups;
     int* d_active = nullptr;

@@ -531,7 +531,7 @@ class Batch:
     def set_shape(self, rows_inner: int, target_groups: int) -> None:
         check(self._lib.pcbhip_batch_set_shape(self.handle, int(rows_inner), int(target_groups)))
 
-    def set_schedule(self, replay_steps: int = 64, outer_every: int = 32, compact_percent: int = 50) -> None:
+    def set_schedule(self, replay_steps: int = 64, outer_every: int = 32, compact_percent: int = 25) -> None:
         """Steps per captured replay, inner iterations between outer steps and
         the finished share (percent) that triggers compaction."""
 
